@@ -1,0 +1,361 @@
+"""MI355X-native per-timestep SPH path of CCAKE.
+
+Python front-end over the C-ABI of `libccake_amd.so` (include/ccake_amd.h).
+The HIP library is the only compute path: importing works on any machine,
+but every call that computes requires the built library and a HIP device,
+and raises otherwise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from . import config  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(_HERE, "libccake_amd.so")
+HEADER = os.path.join(ROOT, "include", "ccake_amd.h")
+
+
+class CcakeError(RuntimeError):
+    def __init__(self, status, what):
+        self.status = status
+        super().__init__(f"{what}: status {status} ({STATUS.get(status, '?')}): {last_error()}")
+
+
+def _parse_enum(name):
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    body = text[text.index("enum " + name):]
+    body = body[body.index("{") + 1: body.index("}")]
+    out, cur = {}, -1
+    for item in body.split(","):
+        item = item.strip()
+        if not item:
+            continue
+        if "=" in item:
+            nm, expr = [x.strip() for x in item.split("=", 1)]
+            for k, v in sorted(out.items(), key=lambda kv: -len(kv[0])):
+                expr = re.sub(r"\b%s\b" % k, str(v), expr)
+            cur = int(eval(expr))
+        else:
+            nm = item
+            cur += 1
+        out[nm] = cur
+    return out
+
+
+FIELDS = _parse_enum("ccake_field")
+IFIELDS = _parse_enum("ccake_ifield")
+KERNELS = _parse_enum("ccake_kernel_id")
+REDUCTIONS = _parse_enum("ccake_reduction")
+STATUS = {v: k for k, v in _parse_enum("ccake_status").items()}
+
+PARAM_FIELDS = [
+    ("dim", C.c_int), ("rk_order", C.c_int),
+    ("hT", C.c_double), ("hEta", C.c_double),
+    ("t0", C.c_double), ("dt", C.c_double), ("e_freeze", C.c_double),
+    ("particlization", C.c_int), ("using_shear", C.c_int), ("using_vorticity", C.c_int),
+    ("input_initial_shear", C.c_int), ("bulk_from_trace", C.c_int),
+    ("prohibit_unstable_cs2", C.c_int), ("prohibit_acausal_cs2", C.c_int),
+    ("eta_mode", C.c_int), ("constant_eta_over_s", C.c_double),
+    ("shear_relax_mode", C.c_int),
+    ("zeta_mode", C.c_int), ("constant_zeta_over_s", C.c_double),
+    ("cs2_dependent_zeta_A", C.c_double), ("cs2_dependent_zeta_p", C.c_double),
+    ("modulate_zeta_with_tanh", C.c_int),
+    ("zeta_gauss_A", C.c_double), ("zeta_gauss_T0", C.c_double), ("zeta_gauss_width", C.c_double),
+    ("delta_pipi_mode", C.c_int), ("tau_pipi_mode", C.c_int), ("lambda_piPi_mode", C.c_int),
+    ("phi6_mode", C.c_int), ("phi7_mode", C.c_int),
+    ("delta_PiPi_mode", C.c_int), ("lambda_Pipi_mode", C.c_int), ("phi1_mode", C.c_int),
+    ("phi3_mode", C.c_int),
+    ("cells_per_rc", C.c_int),
+]
+
+
+class Params(C.Structure):
+    _fields_ = PARAM_FIELDS
+
+
+class EosDesc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("T_freeze_out", C.c_double), ("dims", C.c_int * 4),
+                ("mins", C.c_double * 4), ("maxs", C.c_double * 4),
+                ("table", C.POINTER(C.c_double))]
+
+
+# exported C-ABI: name -> (restype, argtypes)
+_vp, _dp, _ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)
+ABI = {
+    "ccake_dev_create": (C.c_int, [C.c_int, C.c_size_t, C.POINTER(_vp)]),
+    "ccake_dev_destroy": (C.c_int, [_vp]),
+    "ccake_dev_sync": (C.c_int, [_vp]),
+    "ccake_dev_memory": (C.c_int, [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "ccake_last_error": (C.c_char_p, []),
+    "ccake_eos_create": (C.c_int, [_vp, C.POINTER(EosDesc), C.POINTER(_vp)]),
+    "ccake_eos_destroy": (C.c_int, [_vp]),
+    "ccake_eos_efreeze": (C.c_int, [_vp, C.c_double, _dp]),
+    "ccake_eos_full_thermo": (C.c_int, [_vp, C.c_int, _dp, _dp]),
+    "ccake_event_create": (C.c_int, [_vp, C.POINTER(Params), _vp, C.c_int, C.POINTER(_vp)]),
+    "ccake_event_destroy": (C.c_int, [_vp]),
+    "ccake_event_size": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "ccake_event_set_time": (C.c_int, [_vp, C.c_double]),
+    "ccake_event_get_time": (C.c_int, [_vp, _dp]),
+    "ccake_event_upload": (C.c_int, [_vp, C.c_int, _dp, C.c_size_t]),
+    "ccake_event_download": (C.c_int, [_vp, C.c_int, _dp, C.c_size_t]),
+    "ccake_event_upload_int": (C.c_int, [_vp, C.c_int, _ip, C.c_size_t]),
+    "ccake_event_download_int": (C.c_int, [_vp, C.c_int, _ip, C.c_size_t]),
+    "ccake_build_neighbors": (C.c_int, [_vp]),
+    "ccake_export_pairs": (C.c_int, [_vp, _ip, _ip, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "ccake_gamma_velocity": (C.c_int, [_vp]),
+    "ccake_smooth_fields": (C.c_int, [_vp]),
+    "ccake_initial_sigma": (C.c_int, [_vp]),
+    "ccake_eos_invert": (C.c_int, [_vp, C.c_int]),
+    "ccake_reset_pi_tensor": (C.c_int, [_vp]),
+    "ccake_viscosities": (C.c_int, [_vp]),
+    "ccake_smooth_gradients": (C.c_int, [_vp]),
+    "ccake_eom": (C.c_int, [_vp]),
+    "ccake_get_time_derivatives": (C.c_int, [_vp]),
+    "ccake_cache_state": (C.c_int, [_vp]),
+    "ccake_rk_update": (C.c_int, [_vp, C.c_double, C.c_double]),
+    "ccake_advance_timestep": (C.c_int, [_vp, C.c_double, C.c_int]),
+    "ccake_run_steps": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    "ccake_initialize": (C.c_int, [_vp]),
+    "ccake_freeze_out_particles": (C.c_int, [_vp]),
+    "ccake_continue_evolution": (C.c_int, [_vp, C.c_double, C.POINTER(C.c_int)]),
+    "ccake_event_check": (C.c_int, [_vp]),
+    "ccake_reduce": (C.c_int, [_vp, C.c_int, _dp]),
+    "ccake_profile_enable": (C.c_int, [_vp, C.c_int]),
+    "ccake_profile_read": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_int64)]),
+    "ccake_profile_reset": (C.c_int, [_vp]),
+    "ccake_last_pair_count": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library.  There is no fallback: a missing library raises."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in ABI.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error():
+    try:
+        return lib().ccake_last_error().decode()
+    except Exception:  # pragma: no cover - only during failed loads
+        return ""
+
+
+def _ok(st, what):
+    if st != 0:
+        raise CcakeError(st, what)
+
+
+def make_params(cfg):
+    p = Params()
+    for name, _ in PARAM_FIELDS:
+        setattr(p, name, cfg[name])
+    return p
+
+
+class Device:
+    def __init__(self, gpu=0, hbm_budget=0):
+        h = C.c_void_p()
+        _ok(lib().ccake_dev_create(gpu, hbm_budget, C.byref(h)), "ccake_dev_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ccake_dev_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def sync(self):
+        _ok(lib().ccake_dev_sync(self.h), "ccake_dev_sync")
+
+    def memory(self):
+        u, b = C.c_size_t(), C.c_size_t()
+        _ok(lib().ccake_dev_memory(self.h, C.byref(u), C.byref(b)), "ccake_dev_memory")
+        return u.value, b.value
+
+    def profile(self, enable=True):
+        _ok(lib().ccake_profile_enable(self.h, int(enable)), "ccake_profile_enable")
+
+    def profile_reset(self):
+        _ok(lib().ccake_profile_reset(self.h), "ccake_profile_reset")
+
+    def profile_read(self, kernel):
+        k = KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+        ms, n = C.c_double(), C.c_int64()
+        _ok(lib().ccake_profile_read(self.h, k, C.byref(ms), C.byref(n)), "ccake_profile_read")
+        return ms.value, n.value
+
+
+class Eos:
+    def __init__(self, dev, kind=0, T_freeze_out=150.0 / 197.3269804, dims=None, mins=None, maxs=None,
+                 table=None):
+        d = EosDesc()
+        d.kind = kind
+        d.T_freeze_out = T_freeze_out
+        if table is not None:
+            tab = np.ascontiguousarray(table, dtype=np.float64)
+            for a in range(4):
+                d.dims[a] = int(dims[a]); d.mins[a] = float(mins[a]); d.maxs[a] = float(maxs[a])
+            d.table = tab.ctypes.data_as(C.POINTER(C.c_double))
+        h = C.c_void_p()
+        _ok(lib().ccake_eos_create(dev.h, C.byref(d), C.byref(h)), "ccake_eos_create")
+        self.h = h
+        self.dev = dev
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ccake_eos_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def efreeze(self, T):
+        e = C.c_double()
+        _ok(lib().ccake_eos_efreeze(self.h, T, C.byref(e)), "ccake_eos_efreeze")
+        return e.value
+
+    def full_thermo(self, member, tbqs):
+        x = (C.c_double * 4)(*tbqs)
+        out = (C.c_double * 17)()
+        _ok(lib().ccake_eos_full_thermo(self.h, member, x, out), "ccake_eos_full_thermo")
+        return np.array(out[:])
+
+
+class Event:
+    """One particle system (one SystemState) resident on the device."""
+
+    def __init__(self, dev, eos, cfg, n):
+        self.dev, self.eos, self.cfg, self.n = dev, eos, cfg, int(n)
+        self._p = make_params(cfg)
+        h = C.c_void_p()
+        _ok(lib().ccake_event_create(dev.h, C.byref(self._p), eos.h, self.n, C.byref(h)), "ccake_event_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ccake_event_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # ---- copies
+    def upload(self, field, values):
+        fid = FIELDS[field] if isinstance(field, str) else int(field)
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(values, dtype=np.float64), (self.n,)))
+        _ok(lib().ccake_event_upload(self.h, fid, a.ctypes.data_as(_dp), self.n), "upload")
+
+    def download(self, field):
+        fid = FIELDS[field] if isinstance(field, str) else int(field)
+        a = np.empty(self.n, dtype=np.float64)
+        _ok(lib().ccake_event_download(self.h, fid, a.ctypes.data_as(_dp), self.n), "download")
+        return a
+
+    def upload_int(self, field, values):
+        fid = IFIELDS[field] if isinstance(field, str) else int(field)
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(values, dtype=np.int32), (self.n,)))
+        _ok(lib().ccake_event_upload_int(self.h, fid, a.ctypes.data_as(_ip), self.n), "upload_int")
+
+    def download_int(self, field):
+        fid = IFIELDS[field] if isinstance(field, str) else int(field)
+        a = np.empty(self.n, dtype=np.int32)
+        _ok(lib().ccake_event_download_int(self.h, fid, a.ctypes.data_as(_ip), self.n), "download_int")
+        return a
+
+    @property
+    def t(self):
+        v = C.c_double()
+        _ok(lib().ccake_event_get_time(self.h, C.byref(v)), "get_time")
+        return v.value
+
+    @t.setter
+    def t(self, v):
+        _ok(lib().ccake_event_set_time(self.h, float(v)), "set_time")
+
+    # ---- phases
+    def _call(self, name, *args):
+        _ok(getattr(lib(), "ccake_" + name)(self.h, *args), name)
+
+    def build_neighbors(self): self._call("build_neighbors")
+    def gamma_velocity(self): self._call("gamma_velocity")
+    def smooth_fields(self): self._call("smooth_fields")
+    def initial_sigma(self): self._call("initial_sigma")
+    def eos_invert(self, mode): self._call("eos_invert", int(mode))
+    def reset_pi_tensor(self): self._call("reset_pi_tensor")
+    def viscosities(self): self._call("viscosities")
+    def smooth_gradients(self): self._call("smooth_gradients")
+    def eom(self): self._call("eom")
+    def get_time_derivatives(self): self._call("get_time_derivatives")
+    def cache_state(self): self._call("cache_state")
+    def rk_update(self, dt, t0): self._call("rk_update", float(dt), float(t0))
+    def advance_timestep(self, dt, rk_order=2): self._call("advance_timestep", float(dt), int(rk_order))
+    def initialize(self): self._call("initialize")
+    def freeze_out_particles(self): self._call("freeze_out_particles")
+    def check(self): self._call("event_check")
+
+    def run_steps(self, dt, n_steps, rk_order=2):
+        done = C.c_int()
+        _ok(lib().ccake_run_steps(self.h, float(dt), int(rk_order), int(n_steps), C.byref(done)), "run_steps")
+        return done.value
+
+    def continue_evolution(self, max_tau=50.0):
+        k = C.c_int()
+        _ok(lib().ccake_continue_evolution(self.h, float(max_tau), C.byref(k)), "continue_evolution")
+        return bool(k.value)
+
+    def reduce(self, which):
+        w = REDUCTIONS[which] if isinstance(which, str) else int(which)
+        v = C.c_double()
+        _ok(lib().ccake_reduce(self.h, w, C.byref(v)), "reduce")
+        return v.value
+
+    def pairs(self):
+        n = C.c_size_t()
+        st = lib().ccake_export_pairs(self.h, None, None, 0, C.byref(n))
+        if st not in (0, STATUS_CAPACITY):
+            raise CcakeError(st, "export_pairs")
+        pi = np.empty(n.value, dtype=np.int32)
+        pj = np.empty(n.value, dtype=np.int32)
+        _ok(lib().ccake_export_pairs(self.h, pi.ctypes.data_as(_ip), pj.ctypes.data_as(_ip), n.value,
+                                     C.byref(n)), "export_pairs")
+        return pi, pj
+
+    def last_pair_count(self):
+        v = C.c_int64()
+        _ok(lib().ccake_last_pair_count(self.h, C.byref(v)), "last_pair_count")
+        return v.value
+
+
+STATUS_CAPACITY = 7
+
+
+def load_event(ev, ic):
+    """Upload an initial condition dict (x, y, e, ux, uy, rhoB, rhoS, rhoQ)."""
+    ev.upload("CCAKE_F_X", ic["x"])
+    ev.upload("CCAKE_F_Y", ic["y"])
+    ev.upload("CCAKE_F_IN_E", ic["e"])
+    ev.upload("CCAKE_F_UX", ic["ux"])
+    ev.upload("CCAKE_F_UY", ic["uy"])
+    ev.upload("CCAKE_F_IN_B", ic["rhoB"])
+    ev.upload("CCAKE_F_IN_SS", ic["rhoS"])
+    ev.upload("CCAKE_F_IN_Q", ic["rhoQ"])

@@ -1,0 +1,577 @@
+// C-ABI of the MI355X SPH library: lifetime, copies, phase entry points,
+// the RK2 step and the initialisation sequence.  See include/ccake_amd.h.
+#include "ccake_internal.h"
+#include "eos_device.h"
+#include <cstring>
+#include <cstdio>
+#include <mutex>
+#include <vector>
+
+namespace ccake {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+int hip_fail(hipError_t e, const char* what)
+{
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return CCAKE_ERR_HIP;
+}
+
+// ---- profiling: HIP events on the library stream --------------------------
+struct PendingProf { int k; hipEvent_t a, b; };
+static std::mutex g_prof_mu;
+static std::vector<PendingProf> g_pending;
+static std::vector<hipEvent_t> g_free_events;
+static thread_local hipEvent_t t_open[CCAKE_NUM_KERNELS];
+
+static hipEvent_t get_event()
+{
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  if (!g_free_events.empty()) { hipEvent_t e = g_free_events.back(); g_free_events.pop_back(); return e; }
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+void prof_begin(ccake_dev* d, int k)
+{
+  if (!d->prof.enabled) return;
+  hipEvent_t e = get_event();
+  if (e && hipEventRecord(e, d->stream) == hipSuccess) t_open[k] = e;
+}
+void prof_end(ccake_dev* d, int k)
+{
+  if (!d->prof.enabled || !t_open[k]) return;
+  hipEvent_t e = get_event();
+  if (e && hipEventRecord(e, d->stream) == hipSuccess) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_pending.push_back({ k, t_open[k], e });
+  }
+  t_open[k] = nullptr;
+}
+static void prof_drain(ccake_dev* d)
+{
+  if (hipStreamSynchronize(d->stream) != hipSuccess) return;
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  for (auto& p : g_pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      d->prof.total_ms[p.k] += ms;
+      d->prof.launches[p.k] += 1;
+    }
+    g_free_events.push_back(p.a);
+    g_free_events.push_back(p.b);
+  }
+  g_pending.clear();
+}
+
+static int check_launch(int rc) { return rc; }
+
+// ---- one RK stage: SPHWorkstation::get_time_derivatives ------------------
+static int time_derivatives(ccake_event* ev, int fuse_rk, double dt_stage, double t0)
+{
+  ccake_dev* d = ev->dev;
+  int rc;
+  prof_begin(d, CCAKE_K_BIN);
+  if ((rc = launch_stage_begin(ev))) return rc;                 // 02 gamma, v (+ bbox)
+  prof_end(d, CCAKE_K_BIN);
+  prof_begin(d, CCAKE_K_SORT);
+  if ((rc = launch_neighbor_sort(ev))) return rc;               // 01 cell sort
+  prof_end(d, CCAKE_K_SORT);
+  prof_begin(d, CCAKE_K_SMOOTH);
+  if ((rc = launch_gather_smooth(ev))) return rc;
+  if ((rc = launch_smooth(ev, 0))) return rc;                   // 03 (+ btrack)
+  prof_end(d, CCAKE_K_SMOOTH);
+  prof_begin(d, CCAKE_K_EOS);
+  if ((rc = launch_eos(ev, 0, 1))) return rc;                   // 04 + 05 + 08 + gather
+  prof_end(d, CCAKE_K_EOS);
+  prof_begin(d, CCAKE_K_GRAD);
+  if ((rc = launch_grad(ev))) return rc;                        // 09
+  prof_end(d, CCAKE_K_GRAD);
+  prof_begin(d, CCAKE_K_EOM);
+  if ((rc = launch_eom(ev, fuse_rk, dt_stage, t0))) return rc;  // 10 (+ RK update)
+  prof_end(d, CCAKE_K_EOM);
+  return 0;
+}
+
+static int rk2_step(ccake_event* ev, double dt)
+{
+  int rc;
+  prof_begin(ev->dev, CCAKE_K_RK);
+  if ((rc = launch_cache(ev))) return rc;
+  prof_end(ev->dev, CCAKE_K_RK);
+  const double t0 = ev->t;
+  if ((rc = time_derivatives(ev, 1, 0.5 * dt, t0))) return rc;
+  ev->t = t0 + 0.5 * dt;
+  if ((rc = time_derivatives(ev, 1, dt, t0))) return rc;
+  ev->t = t0 + dt;
+  if (ev->p.particlization) {
+    prof_begin(ev->dev, CCAKE_K_MISC);
+    if ((rc = launch_freeze(ev, ev->frzc))) return rc;
+    prof_end(ev->dev, CCAKE_K_MISC);
+    if (ev->frzc < 2) ev->frzc++;
+  }
+  return 0;
+}
+
+template <class T>
+static int dalloc(ccake_event* ev, T** p, size_t count)
+{
+  const size_t bytes = count * sizeof(T) + 256;
+  ccake_dev* d = ev->dev;
+  if (d->budget && d->used + bytes > d->budget) { set_error("HBM budget exceeded"); return CCAKE_ERR_OOM; }
+  if (hipMalloc((void**)p, bytes) != hipSuccess) { set_error("hipMalloc failed"); return CCAKE_ERR_OOM; }
+  d->used += bytes;
+  ev->bytes += bytes;
+  return 0;
+}
+
+}  // namespace ccake
+
+using namespace ccake;
+
+extern "C" {
+
+const char* ccake_last_error(void) { return g_err.c_str(); }
+
+int ccake_dev_create(int gpu, size_t budget, ccake_dev** out)
+{
+  if (!out) return CCAKE_ERR_INVALID;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { set_error("no HIP device"); return CCAKE_ERR_HIP; }
+  if (gpu < 0 || gpu >= ndev) { set_error("bad device index"); return CCAKE_ERR_INVALID; }
+  CCAKE_HIP(hipSetDevice(gpu));
+  ccake_dev* d = new ccake_dev();
+  d->gpu = gpu;
+  d->budget = budget;
+  hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete d; return hip_fail(e, "hipStreamCreate"); }
+  *out = d;
+  return 0;
+}
+
+int ccake_dev_destroy(ccake_dev* d)
+{
+  if (!d) return CCAKE_ERR_INVALID;
+  if (hipStreamSynchronize(d->stream) != hipSuccess) { /* fall through: still release */ }
+  prof_drain(d);
+  hipError_t e = hipStreamDestroy(d->stream);
+  delete d;
+  return e == hipSuccess ? 0 : hip_fail(e, "hipStreamDestroy");
+}
+
+int ccake_dev_sync(ccake_dev* d)
+{
+  if (!d) return CCAKE_ERR_INVALID;
+  CCAKE_HIP(hipStreamSynchronize(d->stream));
+  return 0;
+}
+
+int ccake_dev_memory(ccake_dev* d, size_t* used, size_t* budget)
+{
+  if (!d) return CCAKE_ERR_INVALID;
+  if (used) *used = d->used;
+  if (budget) *budget = d->budget;
+  return 0;
+}
+
+// ---------------------------------------------------------------- EoS
+static void init_analytic(EosMember& m, int kind, double c, double T0, double muB0, double muQ0, double muS0,
+                          double Tc, double Ts)
+{
+  std::memset(&m, 0, sizeof(m));
+  m.kind = kind; m.c = c; m.T0 = T0; m.muB0 = muB0; m.muQ0 = muQ0; m.muS0 = muS0; m.Tc = Tc; m.Ts = Ts;
+  m.mins[0] = 0.0; m.maxs[0] = 1e10;
+  for (int a = 1; a < 4; ++a) { m.mins[a] = -1e10; m.maxs[a] = 1e10; }
+}
+
+int ccake_eos_create(ccake_dev* dev, const ccake_eos_desc* desc, ccake_eos** out)
+{
+  if (!dev || !desc || !out) return CCAKE_ERR_INVALID;
+  CCAKE_HIP(hipSetDevice(dev->gpu));
+  ccake_eos* e = new ccake_eos();
+  e->dev = dev;
+  EosChain& ch = e->chain;
+  ch.kind = desc->kind;
+  const double Nc = 3.0, Nf = 2.5;
+  const double c0 = kPi * kPi * (2.0 * (Nc * Nc - 1.0) + (7.0 / 2.0) * Nc * Nf) / 90.0;
+  if (desc->kind == CCAKE_EOS_CONFORMAL) {
+    ch.n = 2;
+    init_analytic(ch.m[0], M_CONF, c0, 1, 1, 1, 1, 0, 0);
+    init_analytic(ch.m[1], M_DIAG, c0, 1, 1, 1, 1, 0, 0);
+    *out = e;
+    return 0;
+  }
+  if (desc->kind != CCAKE_EOS_TABLE || !desc->table) { delete e; set_error("bad EoS description"); return CCAKE_ERR_INVALID; }
+  ch.n = 4;
+  EosMember& t = ch.m[0];
+  std::memset(&t, 0, sizeof(t));
+  t.kind = M_TABLE;
+  size_t nn = 1;
+  for (int a = 0; a < 4; ++a) {
+    t.dims[a] = desc->dims[a];
+    nn *= (size_t)desc->dims[a];
+    t.gmin[a] = desc->mins[a];
+    t.gdel[a] = desc->dims[a] <= 1 ? 0.0 : (desc->maxs[a] - desc->mins[a]) / (desc->dims[a] - 1.0);
+    t.mins[a] = desc->mins[a]; t.maxs[a] = desc->maxs[a];
+  }
+  e->h_tab = new double[nn * 17];
+  std::memcpy(e->h_tab, desc->table, nn * 17 * sizeof(double));
+  std::vector<double> t5(nn * 5);
+  for (size_t k = 0; k < nn; ++k) {
+    const double* c = desc->table + k * 17;
+    t5[k * 5 + 0] = c[X_S]; t5[k * 5 + 1] = c[X_B]; t5[k * 5 + 2] = c[X_SS]; t5[k * 5 + 3] = c[X_Q];
+    t5[k * 5 + 4] = c[X_E];
+  }
+  e->bytes = nn * 22 * sizeof(double);
+  if (dev->budget && dev->used + e->bytes > dev->budget) { delete[] e->h_tab; delete e; set_error("HBM budget exceeded"); return CCAKE_ERR_OOM; }
+  CCAKE_HIP(hipMalloc((void**)&e->d_tab, nn * 17 * sizeof(double)));
+  CCAKE_HIP(hipMalloc((void**)&e->d_tab4, nn * 5 * sizeof(double)));
+  dev->used += e->bytes;
+  CCAKE_HIP(hipMemcpy(e->d_tab, desc->table, nn * 17 * sizeof(double), hipMemcpyHostToDevice));
+  CCAKE_HIP(hipMemcpy(e->d_tab4, t5.data(), nn * 5 * sizeof(double), hipMemcpyHostToDevice));
+  ch.tab = e->d_tab;
+  ch.tab4 = e->d_tab4;
+  // matched fallbacks at Tmax = 1.1 T_FO (src/eos_initialization.cpp:270-488)
+  const double Tmax = 1.1 * desc->T_freeze_out;
+  double r[17], pt[4] = { Tmax, 0, 0, 0 };
+  member_fields(t, e->h_tab, pt, r);
+  const double pT = r[X_P];
+  const double c = pT / (Tmax * Tmax * Tmax * Tmax);
+  double mu0[4] = { 1, 1, 1, 1 }, mud[4] = { 1, 1, 1, 1 };
+  for (int a = 1; a < 4; ++a) {
+    const double mmax = t.maxs[a];
+    if (std::fabs(mmax) < 1e-30) continue;
+    double q[4] = { Tmax, 0, 0, 0 };
+    q[a] = mmax;
+    member_fields(t, e->h_tab, q, r);
+    mu0[a] = pow(c, 0.25) * mmax / sqrt(sqrt(r[X_P]) - sqrt(pT));
+    mud[a] = pow(c / (r[X_P] - pT), 0.25) * mmax;
+  }
+  const double Tc = 220.0 / kHbarcMeVfm, Ts = 120.0 / kHbarcMeVfm;
+  init_analytic(ch.m[1], M_TANH, c, 1.0, mu0[1], mu0[2], mu0[3], Tc, Ts);
+  init_analytic(ch.m[2], M_CONF, c, 1.0, mu0[1], mu0[2], mu0[3], 0, 0);
+  init_analytic(ch.m[3], M_DIAG, c, 1.0, mud[1], mud[2], mud[3], 0, 0);
+  *out = e;
+  return 0;
+}
+
+int ccake_eos_destroy(ccake_eos* e)
+{
+  if (!e) return CCAKE_ERR_INVALID;
+  if (e->d_tab) (void)hipFree(e->d_tab);
+  if (e->d_tab4) (void)hipFree(e->d_tab4);
+  if (e->dev) e->dev->used -= (e->dev->used >= e->bytes ? e->bytes : e->dev->used);
+  delete[] e->h_tab;
+  delete e;
+  return 0;
+}
+
+int ccake_eos_efreeze(ccake_eos* e, double T_fo, double* e_out)
+{
+  if (!e || !e_out) return CCAKE_ERR_INVALID;
+  double r[17], x[4] = { T_fo, 0, 0, 0 };
+  member_fields(e->chain.m[0], e->h_tab, x, r);
+  *e_out = r[X_E];
+  return 0;
+}
+
+int ccake_eos_full_thermo(ccake_eos* e, int member, const double tbqs[4], double out17[17])
+{
+  if (!e || !tbqs || !out17) return CCAKE_ERR_INVALID;
+  return host_eos_full(e, member, tbqs, out17);
+}
+
+// ---------------------------------------------------------------- events
+int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, int n, ccake_event** out)
+{
+  if (!dev || !p || !eos || !out || n <= 0) return CCAKE_ERR_INVALID;
+  if (p->dim != 2) { set_error("only D = 2 is built"); return CCAKE_ERR_UNSUPPORTED; }
+  CCAKE_HIP(hipSetDevice(dev->gpu));
+  ccake_event* ev = new ccake_event();
+  ev->dev = dev; ev->eos = eos; ev->p = *p; ev->n = n;
+  ev->npad = (n + 255) / 256 * 256;
+  ev->t = p->t0;
+  ev->cap_cells = 65536;
+  ev->radix_blocks = (n + 1023) / 1024;
+  int rc = 0;
+  const size_t np = (size_t)ev->npad;
+  const int nscan = 256 * ev->radix_blocks;
+  const int nscan_blocks = (nscan + 2047) / 2048 + 1;
+  rc |= dalloc(ev, &ev->F, (size_t)NUM_COLUMNS * np);
+  rc |= dalloc(ev, &ev->I, (size_t)CCAKE_NUM_IFIELDS * np);
+  rc |= dalloc(ev, &ev->grid, 1);
+  rc |= dalloc(ev, &ev->bbox, 4 * (np / 256 + 1));
+  rc |= dalloc(ev, &ev->key, np);
+  rc |= dalloc(ev, &ev->key_a, np);
+  rc |= dalloc(ev, &ev->key_b, np);
+  rc |= dalloc(ev, &ev->val_a, np);
+  rc |= dalloc(ev, &ev->val_b, np);
+  rc |= dalloc(ev, &ev->hist, (size_t)nscan);
+  rc |= dalloc(ev, &ev->scan_tmp, (size_t)nscan_blocks);
+  rc |= dalloc(ev, &ev->cell_off, (size_t)ev->cap_cells + 1);
+  rc |= dalloc(ev, &ev->pair_count, 1);
+  rc |= dalloc(ev, &ev->cand_s, 8 * np);
+  rc |= dalloc(ev, &ev->cand_g, 16 * np);
+  rc |= dalloc(ev, &ev->red, 1024);
+  rc |= dalloc(ev, &ev->err, 1);
+  if (rc) { ccake_event_destroy(ev); return CCAKE_ERR_OOM; }
+  hipStream_t st = dev->stream;
+  CCAKE_HIP(hipMemsetAsync(ev->F, 0, (size_t)NUM_COLUMNS * np * sizeof(double), st));
+  CCAKE_HIP(hipMemsetAsync(ev->I, 0, (size_t)CCAKE_NUM_IFIELDS * np * sizeof(int32_t), st));
+  CCAKE_HIP(hipMemsetAsync(ev->err, 0, sizeof(int32_t), st));
+  CCAKE_HIP(hipMemsetAsync(ev->pair_count, 0, sizeof(int64_t), st));
+  // sph_mass = 1 and gamma = 1 until the caller overrides them
+  std::vector<double> ones(n, 1.0);
+  for (int f : { (int)CCAKE_F_MASS_S, (int)CCAKE_F_MASS_B, (int)CCAKE_F_MASS_SS, (int)CCAKE_F_MASS_Q, (int)CCAKE_F_GAMMA })
+    CCAKE_HIP(hipMemcpyAsync(col(ev, f), ones.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+  CCAKE_HIP(hipStreamSynchronize(st));
+  ev->perm = ev->val_a;
+  ev->skey = ev->key_a;
+  *out = ev;
+  return 0;
+}
+
+int ccake_event_destroy(ccake_event* ev)
+{
+  if (!ev) return CCAKE_ERR_INVALID;
+  (void)hipStreamSynchronize(ev->dev->stream);
+  void* bufs[] = { ev->F, ev->I, ev->grid, ev->bbox, ev->key, ev->key_a, ev->key_b, ev->val_a, ev->val_b, ev->hist,
+                   ev->scan_tmp, ev->cell_off, ev->pair_count, ev->cand_s, ev->cand_g, ev->red, ev->err };
+  for (void* b : bufs) if (b) (void)hipFree(b);
+  ev->dev->used -= (ev->dev->used >= ev->bytes ? ev->bytes : ev->dev->used);
+  delete ev;
+  return 0;
+}
+
+int ccake_event_size(ccake_event* ev, int* n) { if (!ev || !n) return CCAKE_ERR_INVALID; *n = ev->n; return 0; }
+int ccake_event_set_time(ccake_event* ev, double t) { if (!ev) return CCAKE_ERR_INVALID; ev->t = t; return 0; }
+int ccake_event_get_time(ccake_event* ev, double* t) { if (!ev || !t) return CCAKE_ERR_INVALID; *t = ev->t; return 0; }
+
+int ccake_event_upload(ccake_event* ev, int field, const double* host, size_t n)
+{
+  if (!ev || !host || field < 0 || field >= CCAKE_NUM_FIELDS || n != (size_t)ev->n) return CCAKE_ERR_INVALID;
+  CCAKE_HIP(hipMemcpyAsync(col(ev, field), host, n * sizeof(double), hipMemcpyHostToDevice, ev->dev->stream));
+  CCAKE_HIP(hipStreamSynchronize(ev->dev->stream));
+  return 0;
+}
+
+int ccake_event_download(ccake_event* ev, int field, double* host, size_t n)
+{
+  if (!ev || !host || field < 0 || field >= CCAKE_NUM_FIELDS || n != (size_t)ev->n) return CCAKE_ERR_INVALID;
+  CCAKE_HIP(hipMemcpyAsync(host, col(ev, field), n * sizeof(double), hipMemcpyDeviceToHost, ev->dev->stream));
+  CCAKE_HIP(hipStreamSynchronize(ev->dev->stream));
+  return 0;
+}
+
+int ccake_event_upload_int(ccake_event* ev, int f, const int32_t* host, size_t n)
+{
+  if (!ev || !host || f < 0 || f >= CCAKE_NUM_IFIELDS || n != (size_t)ev->n) return CCAKE_ERR_INVALID;
+  CCAKE_HIP(hipMemcpyAsync(icol(ev, f), host, n * sizeof(int32_t), hipMemcpyHostToDevice, ev->dev->stream));
+  CCAKE_HIP(hipStreamSynchronize(ev->dev->stream));
+  return 0;
+}
+
+int ccake_event_download_int(ccake_event* ev, int f, int32_t* host, size_t n)
+{
+  if (!ev || !host || f < 0 || f >= CCAKE_NUM_IFIELDS || n != (size_t)ev->n) return CCAKE_ERR_INVALID;
+  CCAKE_HIP(hipMemcpyAsync(host, icol(ev, f), n * sizeof(int32_t), hipMemcpyDeviceToHost, ev->dev->stream));
+  CCAKE_HIP(hipStreamSynchronize(ev->dev->stream));
+  return 0;
+}
+
+// ---------------------------------------------------------------- phases
+int ccake_build_neighbors(ccake_event* ev)
+{
+  if (!ev) return CCAKE_ERR_INVALID;
+  int rc;
+  if ((rc = launch_bbox(ev))) return rc;
+  if ((rc = launch_neighbor_sort(ev))) return rc;
+  if ((rc = launch_gather_smooth(ev))) return rc;
+  return launch_smooth(ev, 2);
+}
+
+int ccake_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, size_t* n_pairs)
+{
+  if (!ev || !n_pairs) return CCAKE_ERR_INVALID;
+  int rc = launch_gather_smooth(ev);
+  if (rc) return rc;
+  return launch_export_pairs(ev, pi, pj, cap, n_pairs);
+}
+
+int ccake_gamma_velocity(ccake_event* ev) { return ev ? launch_stage_begin(ev) : CCAKE_ERR_INVALID; }
+
+int ccake_smooth_fields(ccake_event* ev)
+{
+  if (!ev) return CCAKE_ERR_INVALID;
+  int rc = launch_gather_smooth(ev);
+  return rc ? rc : launch_smooth(ev, 0);
+}
+
+int ccake_initial_sigma(ccake_event* ev)
+{
+  if (!ev) return CCAKE_ERR_INVALID;
+  int rc = launch_gather_smooth(ev);
+  return rc ? rc : launch_smooth(ev, 1);
+}
+
+int ccake_eos_invert(ccake_event* ev, int mode)
+{
+  if (!ev || (mode != 0 && mode != 1)) return CCAKE_ERR_INVALID;
+  int rc = launch_eos(ev, mode == 1 ? 1 : 2, 0);
+  if (rc) return rc;
+  return ccake_event_check(ev);
+}
+
+int ccake_reset_pi_tensor(ccake_event* ev) { return ev ? launch_reset_pi(ev) : CCAKE_ERR_INVALID; }
+int ccake_viscosities(ccake_event* ev) { return ev ? launch_viscosities(ev) : CCAKE_ERR_INVALID; }
+
+int ccake_smooth_gradients(ccake_event* ev)
+{
+  if (!ev) return CCAKE_ERR_INVALID;
+  int rc = launch_gather_grad(ev);
+  return rc ? rc : launch_grad(ev);
+}
+
+int ccake_eom(ccake_event* ev) { return ev ? launch_eom(ev, 0, 0.0, ev->t) : CCAKE_ERR_INVALID; }
+
+int ccake_get_time_derivatives(ccake_event* ev)
+{
+  if (!ev) return CCAKE_ERR_INVALID;
+  int rc = time_derivatives(ev, 0, 0.0, ev->t);
+  if (rc) return rc;
+  return ccake_event_check(ev);
+}
+
+int ccake_cache_state(ccake_event* ev) { return ev ? launch_cache(ev) : CCAKE_ERR_INVALID; }
+
+int ccake_rk_update(ccake_event* ev, double dt, double t0)
+{
+  if (!ev) return CCAKE_ERR_INVALID;
+  int rc = launch_rk(ev, dt, t0);
+  if (rc) return rc;
+  ev->t = t0 + dt;
+  return 0;
+}
+
+int ccake_advance_timestep(ccake_event* ev, double dt, int rk_order)
+{
+  if (!ev) return CCAKE_ERR_INVALID;
+  if (rk_order != 2) { set_error("only RK2 is built"); return CCAKE_ERR_UNSUPPORTED; }
+  int rc = rk2_step(ev, dt);
+  if (rc) return rc;
+  return ccake_event_check(ev);
+}
+
+int ccake_run_steps(ccake_event* ev, double dt, int rk_order, int n_steps, int* steps_done)
+{
+  if (!ev) return CCAKE_ERR_INVALID;
+  if (rk_order != 2) { set_error("only RK2 is built"); return CCAKE_ERR_UNSUPPORTED; }
+  int done = 0;
+  for (int k = 0; k < n_steps; ++k) {
+    int rc = rk2_step(ev, dt);
+    if (rc) { if (steps_done) *steps_done = done; return rc; }
+    ++done;
+  }
+  if (steps_done) *steps_done = done;
+  return ccake_event_check(ev);
+}
+
+int ccake_initialize(ccake_event* ev)
+{
+  if (!ev) return CCAKE_ERR_INVALID;
+  int rc;
+  ev->t = ev->p.t0;
+  ev->frzc = 0;
+  if ((rc = launch_init_particles(ev, 0))) return rc;       // process_initial_conditions
+  if ((rc = ccake_build_neighbors(ev))) return rc;          // initialize_linklist
+  if ((rc = launch_eos(ev, 1, 0))) return rc;               // e -> s inversion
+  if ((rc = launch_init_particles(ev, 1))) return rc;
+  if ((rc = ccake_initial_sigma(ev))) return rc;            // calculate_intial_sigma
+  if ((rc = launch_init_particles(ev, 2))) return rc;       // init_particles
+  if ((rc = launch_stage_begin(ev))) return rc;             // calculate_gamma_and_velocities
+  if ((rc = ccake_smooth_fields(ev))) return rc;            // initial_smoothing
+  if ((rc = launch_init_particles(ev, 3))) return rc;       // extensive shear, bulk
+  if ((rc = launch_reset_pi(ev))) return rc;
+  if ((rc = launch_eos(ev, 2, 0))) return rc;               // update_all_particle_thermodynamics
+  if ((rc = launch_freeze(ev, -1))) return rc;              // check_freeze_out_status
+  return ccake_event_check(ev);
+}
+
+int ccake_freeze_out_particles(ccake_event* ev)
+{
+  if (!ev) return CCAKE_ERR_INVALID;
+  int rc = launch_freeze(ev, ev->frzc);
+  if (rc) return rc;
+  if (ev->frzc < 2) ev->frzc++;
+  return 0;
+}
+
+int ccake_continue_evolution(ccake_event* ev, double max_tau, int* keep_going)
+{
+  if (!ev || !keep_going) return CCAKE_ERR_INVALID;
+  int keep = 1;
+  if (ev->p.particlization) {
+    double frozen = 0.0;
+    int rc = launch_reduce(ev, CCAKE_R_FROZEN, &frozen);
+    if (rc) return rc;
+    keep = (int)(frozen + 0.5) < ev->n;
+  }
+  *keep_going = keep && (ev->t <= max_tau);
+  return 0;
+}
+
+int ccake_event_check(ccake_event* ev)
+{
+  if (!ev) return CCAKE_ERR_INVALID;
+  int32_t e = 0;
+  hipStream_t st = ev->dev->stream;
+  CCAKE_HIP(hipMemcpyAsync(&e, ev->err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  CCAKE_HIP(hipStreamSynchronize(st));
+  if (e) CCAKE_HIP(hipMemsetAsync(ev->err, 0, sizeof(int32_t), st));
+  if (e & DE_EOS_FAIL) { set_error("no EoS solution"); return CCAKE_ERR_EOS_NO_SOLUTION; }
+  if (e & DE_NEG_ENTROPY) { set_error("negative entropy density"); return CCAKE_ERR_NEG_ENTROPY; }
+  if (e & DE_GRID_OVERFLOW) { set_error("non-finite positions"); return CCAKE_ERR_INVALID; }
+  return 0;
+}
+
+int ccake_reduce(ccake_event* ev, int which, double* out)
+{
+  if (!ev || !out || which < 0 || which >= CCAKE_NUM_REDUCTIONS) return CCAKE_ERR_INVALID;
+  return launch_reduce(ev, which, out);
+}
+
+int ccake_profile_enable(ccake_dev* d, int enable)
+{
+  if (!d) return CCAKE_ERR_INVALID;
+  d->prof.enabled = enable != 0;
+  return 0;
+}
+
+int ccake_profile_read(ccake_dev* d, int k, double* total_ms, int64_t* launches)
+{
+  if (!d || k < 0 || k >= CCAKE_NUM_KERNELS) return CCAKE_ERR_INVALID;
+  prof_drain(d);
+  if (total_ms) *total_ms = d->prof.total_ms[k];
+  if (launches) *launches = d->prof.launches[k];
+  return 0;
+}
+
+int ccake_profile_reset(ccake_dev* d)
+{
+  if (!d) return CCAKE_ERR_INVALID;
+  prof_drain(d);
+  for (int k = 0; k < CCAKE_NUM_KERNELS; ++k) { d->prof.total_ms[k] = 0; d->prof.launches[k] = 0; }
+  return 0;
+}
+
+int ccake_last_pair_count(ccake_event* ev, int64_t* n_pairs)
+{
+  if (!ev || !n_pairs) return CCAKE_ERR_INVALID;
+  int64_t v = 0;
+  CCAKE_HIP(hipMemcpyAsync(&v, ev->pair_count, sizeof(int64_t), hipMemcpyDeviceToHost, ev->dev->stream));
+  CCAKE_HIP(hipStreamSynchronize(ev->dev->stream));
+  *n_pairs = v;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,163 @@
+// Internal declarations of the MI355X SPH library (host + device).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include "../../include/ccake_amd.h"
+
+namespace ccake {
+
+constexpr double kHbarcMeVfm = 197.3269804;    // include/constants.h:16
+constexpr double kPi = 3.1415926535897932384626433;
+constexpr int kWave = 64;                       // CDNA wavefront width
+constexpr int kTile = 64;                       // targets per pair-kernel workgroup
+
+// Private per-particle columns stored after the public fields.
+enum PrivateField {
+  P_C_UX = CCAKE_NUM_FIELDS, P_C_UY, P_C_UETA,     // RK cache: u0
+  P_C_X, P_C_Y, P_C_ETA,                           // r0
+  P_C_S, P_C_B, P_C_SS, P_C_Q, P_C_BULK, P_C_EZ,   // extensive s, B, S, Q, Pi; Ez
+  P_C_SHV0,                                        // 5 extensive shear
+  P_FB_E0 = P_C_SHV0 + 5,                          // fback..fback4 energy
+  P_FB_G0 = P_FB_E0 + 4,                           // fback..fback4 gradP_x
+  P_FRZ1_G, P_FRZ2_G,                              // frz1 / frz2 gradP_x
+  NUM_COLUMNS
+};
+
+// Uniform cell grid of one neighbour build (written on the device).
+struct Grid {
+  double lo[2];       // grid origin
+  double cell;        // cell edge
+  double inv_cell;
+  int nc[2];          // cells per axis
+  int ncells;
+  int pad;
+};
+
+// Device error flags (sticky until ccake_event_check).
+enum DevErr { DE_NEG_ENTROPY = 1, DE_EOS_FAIL = 2, DE_GRID_OVERFLOW = 4 };
+
+// EoS chain on the device.
+struct EosMember {
+  int kind;                 // 0 table, 1 tanh, 2 conformal, 3 conformal_diagonal
+  int pad;
+  double c, T0, muB0, muS0, muQ0, Tc, Ts;
+  double mins[4], maxs[4];
+  int dims[4];
+  double gmin[4], gdel[4];
+};
+constexpr int kMaxMembers = 4;
+struct EosChain {
+  int n;
+  int kind;
+  EosMember m[kMaxMembers];
+  const double* tab;        // node-major, 17 fields per node (device)
+  const double* tab4;       // node-major (s, B, S, Q, e) packed per node for the solver
+};
+
+struct Profile {
+  bool enabled = false;
+  hipEvent_t ev0[CCAKE_NUM_KERNELS][2];
+  double total_ms[CCAKE_NUM_KERNELS] = {};
+  int64_t launches[CCAKE_NUM_KERNELS] = {};
+};
+
+}  // namespace ccake
+
+struct ccake_dev {
+  int gpu = 0;
+  hipStream_t stream = nullptr;
+  size_t budget = 0;
+  size_t used = 0;
+  ccake::Profile prof;
+};
+
+struct ccake_eos {
+  ccake_dev* dev = nullptr;
+  ccake::EosChain chain{};
+  double* d_tab = nullptr;
+  double* d_tab4 = nullptr;
+  size_t bytes = 0;
+  // host copy for host-side evaluation (efreeze / full_thermo)
+  double* h_tab = nullptr;
+};
+
+struct ccake_event {
+  ccake_dev* dev = nullptr;
+  ccake_eos* eos = nullptr;
+  ccake_params p{};
+  int n = 0;
+  int npad = 0;
+  double t = 0.0;
+  int frzc = 0;
+  // device buffers
+  double* F = nullptr;          // NUM_COLUMNS x npad
+  int32_t* I = nullptr;         // CCAKE_NUM_IFIELDS x npad
+  // neighbour build
+  int cap_cells = 0;
+  ccake::Grid* grid = nullptr;  // device
+  double* bbox = nullptr;       // device partials
+  int32_t* key = nullptr;       // cell key per particle (id order)
+  int32_t* key_a = nullptr, *key_b = nullptr;   // radix ping-pong keys
+  int32_t* val_a = nullptr, *val_b = nullptr;   // radix ping-pong ids
+  int32_t* hist = nullptr;      // radix block histograms
+  int32_t* scan_tmp = nullptr;  // device-wide scan partials
+  int32_t* cell_off = nullptr;  // cap_cells + 1
+  int32_t* perm = nullptr;      // sorted position -> particle id (alias of the final val buffer)
+  int32_t* skey = nullptr;      // sorted keys (alias)
+  int32_t* rank = nullptr;      // particle id -> sorted position
+  int64_t* pair_count = nullptr;  // device accumulators
+  int radix_blocks = 0;
+  // gathered candidates (sorted order)
+  double* cand_s = nullptr;     // 8 doubles per particle
+  double* cand_g = nullptr;     // 16 doubles per particle
+  // reductions
+  double* red = nullptr;
+  int32_t* err = nullptr;       // device error flags
+  int64_t last_pairs = 0;
+  size_t bytes = 0;
+};
+
+namespace ccake {
+
+// error reporting
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+#define CCAKE_HIP(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return ccake::hip_fail(e_, #x); } while (0)
+
+// profiling helpers
+void prof_begin(ccake_dev* d, int k);
+void prof_end(ccake_dev* d, int k);
+
+inline double* col(ccake_event* ev, int f) { return ev->F + (size_t)f * ev->npad; }
+inline int32_t* icol(ccake_event* ev, int f) { return ev->I + (size_t)f * ev->npad; }
+
+// ---- launchers (one per .hip unit) ----
+// neighbours
+int launch_stage_begin(ccake_event* ev);            // gamma/v, bbox
+int launch_bbox(ccake_event* ev);                   // bbox only
+int launch_neighbor_sort(ccake_event* ev);          // grid, keys, radix sort, cell offsets, btrack reset
+int launch_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, size_t* n);
+// smoothing
+int launch_gather_smooth(ccake_event* ev);
+int launch_smooth(ccake_event* ev, int mode);       // mode 0 fields, 1 initial sigma
+// EoS + per-particle physics
+int launch_eos(ccake_event* ev, int mode, int fuse_post);   // fuse_post: reset_pi + transport + gradient gather
+int launch_reset_pi(ccake_event* ev);
+int launch_viscosities(ccake_event* ev);
+int launch_gather_grad(ccake_event* ev);
+int host_eos_full(const ccake_eos* eos, int member, const double* x, double* r17);
+// gradients
+int launch_grad(ccake_event* ev);
+// EoM + RK
+int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0);
+int launch_cache(ccake_event* ev);
+int launch_rk(ccake_event* ev, double dt, double t0);
+// misc
+int launch_gamma_velocity(ccake_event* ev);
+int launch_init_particles(ccake_event* ev, int phase);
+int launch_freeze(ccake_event* ev, int phase);
+int launch_reduce(ccake_event* ev, int which, double* out);
+
+}  // namespace ccake

@@ -1,0 +1,433 @@
+// Equations of motion (Milne, Israel-Stewart, MRF form) and the RK update,
+// one thread per particle, D = 2.
+// Replaces EoM_default::evaluate_time_derivatives (src/eom_default.cpp:211-869)
+// with calculate_MRF_shear (882-1412) -- seven Kokkos kernels with a global
+// round trip between each -- and Evolver::step_rk's update (src/evolver.cpp:363-484).
+// Index conventions (include/milne.hpp): 2-vectors and 2x2 matrices are
+// lowered / raised by a sign flip; pi^{mu nu} (4x4) lowers slots 1,2 by a
+// sign and slot 3 by -tau^2; the 2+1D geometric factor and delta_{i eta}
+// vanish.
+#include "ccake_internal.h"
+
+namespace ccake {
+
+struct EomIn {
+  double t;
+  int shear, vort;
+};
+
+// Column accessor for particle i.
+struct Cols {
+  double* F; int npad; int i;
+  __device__ __forceinline__ double& operator()(int c) const { return F[(size_t)c * npad + i]; }
+};
+
+template <bool SHEAR>
+__device__ __forceinline__ void eom_d2(const Cols& col, double t, double use_vort, double* du_out,
+                                       double* d_s_out, double* dN, double* d_b_out, double* d_shv_out,
+                                       double* theta_out, double* snu_out, double* dez_out)
+{
+  const double t2 = t * t;
+  const double cmd3 = -1.0 / t / t;                      // contra metric (1,-1,-1,-1/t^2)
+  const double cmd[4] = { 1.0, -1.0, -1.0, cmd3 };
+  const double gamma = col(CCAKE_F_GAMMA), sigma = col(CCAKE_F_SIGMA), bulk = col(CCAKE_F_BULK);
+  const double T = col(CCAKE_F_T), w = col(CCAKE_F_W), s = col(CCAKE_F_S);
+  const double muB = col(CCAKE_F_MUB), muS = col(CCAKE_F_MUS), muQ = col(CCAKE_F_MUQ);
+  const double rB = col(CCAKE_F_RHOB), rS = col(CCAKE_F_RHOS), rQ = col(CCAKE_F_RHOQ);
+  const double dwds = col(CCAKE_F_DWDS), dwdB = col(CCAKE_F_DWDB), dwdS = col(CCAKE_F_DWDS_S), dwdQ = col(CCAKE_F_DWDQ);
+  const double eta_pi = col(CCAKE_F_ETA_PI), tau_pi = col(CCAKE_F_TAU_PI);
+  const double zeta = col(CCAKE_F_ZETA_PI), tau_Pi = col(CCAKE_F_TAU_BULK);
+  const double delta_PiPi = col(CCAKE_F_DELTA_BB), delta_pipi = col(CCAKE_F_DELTA_PIPI);
+  const double lambda_piPi = col(CCAKE_F_LAMBDA_PIB), lambda_Pipi = col(CCAKE_F_LAMBDA_BPI);
+  const double phi1 = col(CCAKE_F_PHI1), phi3 = col(CCAKE_F_PHI3), phi6 = col(CCAKE_F_PHI6), phi7 = col(CCAKE_F_PHI7);
+  const double tau_pipi = col(CCAKE_F_TAU_PIPI);
+  const double u[2] = { col(CCAKE_F_UX), col(CCAKE_F_UY) };
+  const double v[2] = { col(CCAKE_F_VX), col(CCAKE_F_VY) };
+  const double uc[2] = { -u[0], -u[1] };
+  const double fu[3] = { u[0], u[1], 0.0 };
+  const double fc[3] = { -u[0], -u[1], -t2 * 0.0 };
+  double gV[2][2] = { { col(CCAKE_F_GRADV0 + 0), col(CCAKE_F_GRADV0 + 1) },
+                      { col(CCAKE_F_GRADV0 + 3), col(CCAKE_F_GRADV0 + 4) } };
+  const double divV = gV[0][0] + gV[1][1];
+  double S[4][4], H[4][4];
+  #pragma unroll
+  for (int a = 0; a < 4; ++a)
+    #pragma unroll
+    for (int b = 0; b < 4; ++b) S[a][b] = col(CCAKE_F_SHV0 + 4 * a + b);
+  // H = pi^mu_nu (slot 1 lowered); C = pi_{mu nu}
+  #pragma unroll
+  for (int a = 0; a < 4; ++a) { H[a][0] = S[a][0]; H[a][1] = -S[a][1]; H[a][2] = -S[a][2]; H[a][3] = S[a][3] * -t2; }
+  double shv_shvcov = 0.0;
+  #pragma unroll
+  for (int a = 0; a < 4; ++a)
+    #pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const double c = (a == 0) ? H[a][b] : ((a == 3) ? H[a][b] * -t2 : -H[a][b]);
+      shv_shvcov += S[a][b] * c;
+    }
+  const double expf = gamma / t + gamma * divV;   // gf + gamma/t + gamma*divV, gf = 0
+
+  double M_snu[2] = { 0.0, 0.0 }, F_snu = 0.0;
+  double F_0i[2] = { 0.0, 0.0 }, M_0i[2][2] = { { 0.0, 0.0 }, { 0.0, 0.0 } };
+  double F_es[2][3] = { { 0, 0, 0 }, { 0, 0, 0 } };
+  double M_es[2][3][2] = { { { 0, 0 }, { 0, 0 }, { 0, 0 } }, { { 0, 0 }, { 0, 0 }, { 0, 0 } } };
+  if (SHEAR) {
+    // ---------------- calculate_MRF_shear ----------------
+    double gu[2][2];
+    #pragma unroll
+    for (int a = 0; a < 2; ++a)
+      #pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        gu[a][b] = gamma * gV[a][b];
+        gu[a][b] += -gamma * u[b] * uc[0] * gV[a][0];
+        gu[a][b] += -gamma * u[b] * uc[1] * gV[a][1];
+      }
+    double gu0[2];
+    #pragma unroll
+    for (int a = 0; a < 2; ++a) gu0[a] = -1. * (gu[a][0] * uc[0] + gu[a][1] * uc[1]) / gamma;
+    const double gu0c[2] = { -gu0[0], -gu0[1] };
+    double cg3[3][3] = { { -gu[0][0], -gu[0][1], 0.0 }, { -gu[1][0], -gu[1][1], 0.0 }, { 0.0, 0.0, 0.0 } };
+    double g3[3][3] = { { gu[0][0], gu[0][1], 0.0 }, { gu[1][0], gu[1][1], 0.0 }, { 0.0, 0.0, 0.0 } };
+    double guv[2];
+    #pragma unroll
+    for (int b = 0; b < 2; ++b) guv[b] = gu[0][b] * v[0] + gu[1][b] * v[1];
+    const double gu0v = gu0[0] * v[0] + gu0[1] * v[1];
+    double vgs[2], gugs[2], sgui[2], vsgu[2];
+    #pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      vgs[a] = guv[0] * H[a + 1][1] + guv[1] * H[a + 1][2] + gu0v * H[a + 1][0];
+      gugs[a] = cg3[a][0] * H[0][1] + cg3[a][1] * H[0][2] + cmd[a + 1] * gu0[a] * H[0][0];
+      sgui[a] = 0.0; vsgu[a] = 0.0;
+      #pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        sgui[a] += S[b + 1][0] * gu[b][a] + S[b + 1][a + 1] * gu0[b];
+        vsgu[a] += v[b] * S[0][0] * gu[b][a] + v[b] * S[0][a + 1] * gu0[b];
+      }
+    }
+    double sgu = 0.0;
+    #pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      #pragma unroll
+      for (int b = 0; b < 2; ++b) sgu += (H[a + 1][b + 1] - v[a] * H[0][b + 1]) * gu[a][b];
+      sgu += (H[a + 1][0] - v[a] * H[0][0]) * gu0[a];
+    }
+    double sl0l[2];
+    #pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      sl0l[a] = 0.0;
+      #pragma unroll
+      for (int b = 0; b < 4; ++b) sl0l[a] += (S[0][b] * H[a + 1][b] + S[a + 1][b] * H[0][b]) / 2.;
+    }
+    #pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const double Fsig = -(guv[a] - gu0c[a]) / 2. + u[a] * gamma * expf / 3.;
+      const double FD = 0.0;
+      const double Fds = -(vgs[a] - gugs[a] - 4. * gamma * u[a] * sgu / 3. - sgui[a] + vsgu[a]) / 4.
+                       + (4. * gamma * u[a] * (gamma * H[3][3] / t) / 3.) / 4.
+                       - S[0][a + 1] * expf / 3.;
+      const double Fdo = -(vgs[a] - gugs[a] + sgui[a] - vsgu[a]) / 4.;
+      const double Fdd = S[a + 1][0] * expf;
+      double Msig[2], MD[2], Mds[2], Mdo[2], Mdd[2];
+      #pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        Msig[b] = u[a] * uc[b] / (2. * 3.);
+        MD[b] = gamma * u[a] * (H[0][b + 1] - H[0][0] * uc[b] / gamma)
+              + gamma * gamma * (H[a + 1][b + 1] - H[a + 1][0] * uc[b] / gamma);
+        Mds[b] = (H[a + 1][b + 1] - H[a + 1][0] * uc[b] / gamma - gamma * gamma * H[a + 1][b + 1]
+                  - gamma * u[a] * H[0][b + 1] + uc[b] * gamma * H[a + 1][0] + uc[b] * u[a] * H[0][0]
+                  + 4. * gamma * u[a] * (H[0][b + 1] - H[0][0] * uc[b] / gamma) / 3.
+                  + (-S[0][a + 1] + 4. * S[0][a + 1] / 3.) * uc[b] / gamma) / 4.;
+        Mdo[b] = (H[a + 1][b + 1] - H[a + 1][0] * uc[b] / gamma - gamma * gamma * H[a + 1][b + 1]
+                  - gamma * u[a] * H[0][b + 1] + uc[b] * gamma * H[a + 1][0] + uc[b] * u[a] * H[0][0]
+                  + S[0][a + 1] * uc[b] / gamma) / 4.;
+        Mdd[b] = -S[a + 1][0] * uc[b] / gamma;
+      }
+      Msig[a] += (1. - gamma * gamma) / 2.;
+      Mdo[a] += -S[0][0] / 4.;
+      Mds[a] += +S[0][0] / 4.;
+      F_0i[a] = (-S[a + 1][0] - tau_pi * FD - delta_pipi * Fdd + 2. * tau_pi * Fdo * use_vort - tau_pipi * Fds
+                 + (2. * eta_pi + lambda_piPi * bulk) * Fsig + phi6 * bulk * S[a + 1][0]
+                 + phi7 * (sl0l[a] + gamma * u[a] * shv_shvcov / 3.)) / (tau_pi * gamma);
+      #pragma unroll
+      for (int b = 0; b < 2; ++b)
+        M_0i[a][b] = (-tau_pi * MD[b] - delta_pipi * Mdd[b] + 2. * tau_pi * Mdo[b] * use_vort - tau_pipi * Mds[b]
+                      + (2. * eta_pi + lambda_piPi * bulk) * Msig[b]) / (tau_pi * gamma);
+    }
+    #pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      M_snu[a] = H[0][a + 1] - H[0][0] * uc[a] / gamma;
+      #pragma unroll
+      for (int b = 0; b < 2; ++b) F_snu += (H[a + 1][b + 1] - v[a] * H[0][b + 1]) * gu[a][b];
+      F_snu += (H[a + 1][0] - v[a] * H[0][0]) * gu0[a];
+    }
+    F_snu += +H[3][3] * gamma / t;
+
+    const double tilde_delta = delta_pipi - tau_pi;
+    double Fdl[2][3], FDm[2][3], Fsm[2][3], Fdom[2][3], Fdsm[2][3], sls[2][3];
+    double MDm[2][3][2], Msm[2][3][2], Mdl[2][3][2], Mdom[2][3][2], Mdsm[2][3][2];
+    #pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      #pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        if (b < a) continue;
+        double gugj = 0.0, sgj = 0.0, vsg = 0.0, sl_ = 0.0;
+        #pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          gugj += cg3[a][k] * H[b + 1][k + 1] + cg3[b][k] * H[a + 1][k + 1];
+          sgj += S[k + 1][a + 1] * g3[k][b] + S[k + 1][b + 1] * g3[k][a];
+          vsg += v[k] * S[0][a + 1] * g3[k][b] + v[k] * S[0][b + 1] * g3[k][a];
+        }
+        #pragma unroll
+        for (int k = 0; k < 4; ++k) sl_ += (S[a + 1][k] * H[b + 1][k] + S[b + 1][k] * H[a + 1][k]) / 2.;
+        sls[a][b] = sl_;
+        FDm[a][b] = 0.0;
+        Fsm[a][b] = cg3[a][b] / 2. + cg3[b][a] / 2. + fu[a] * fu[b] * expf / 3.;
+        Fdl[a][b] = S[a + 1][b + 1] * expf;
+        Fdom[a][b] = -(-gugj + sgj - vsg) / 4.;
+        Fdsm[a][b] = -(-gugj - sgj + vsg - 4. * (fu[a] * fu[b]) * sgu / 3.) / 4.
+                   + (4. * fu[a] * fu[b] * (gamma * H[3][3] / t) / 3.) / 4.
+                   - S[a + 1][b + 1] * expf / 3.;
+        #pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          MDm[a][b][k] = gamma * (fu[a] * H[b + 1][k + 1] - fu[a] * H[b + 1][0] * fc[k] / gamma
+                                  + fu[b] * H[a + 1][k + 1] - fu[b] * H[a + 1][0] * fc[k] / gamma);
+          Msm[a][b][k] = -fu[a] * fu[b] * fc[k] / (3. * gamma);
+          Mdl[a][b][k] = -S[a + 1][b + 1] * fc[k] / gamma;
+          Mdom[a][b][k] = (-fu[a] * H[b + 1][k + 1] * gamma - fu[b] * H[a + 1][k + 1] * gamma
+                           + fc[k] * fu[a] * H[b + 1][0] + fc[k] * fu[b] * H[a + 1][0]) / 4.;
+          Mdsm[a][b][k] = (-fu[a] * H[b + 1][k + 1] * gamma - fu[b] * H[a + 1][k + 1] * gamma
+                           + fc[k] * fu[a] * H[b + 1][0] + fc[k] * fu[b] * H[a + 1][0]
+                           + 4. * fu[a] * fu[b] * (H[0][k + 1] - H[0][0] * fc[k] / gamma) / 3.
+                           + 4. * S[a + 1][b + 1] * fc[k] / 3. / gamma) / 4.;
+        }
+        Mdsm[a][b][a] += S[0][b + 1] / 4.;
+        Mdom[a][b][a] += -S[0][b + 1] / 4.;
+        Msm[a][b][a] += -gamma * fu[b] / 2.;
+      }
+      #pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        if (b < a) continue;
+        Msm[a][b][b] += -gamma * fu[a] / 2.;
+        Mdsm[a][b][b] += S[0][a + 1] / 4.;
+        Mdom[a][b][b] += -S[0][a + 1] / 4.;
+      }
+      #pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        Msm[a][a][k] += cmd[a + 1] * fc[k] / (gamma * 3.);
+        Mdsm[a][a][k] += -cmd[a + 1] * (H[0][k + 1] - H[0][0] * fc[k] / gamma) / 3.;
+      }
+      FDm[a][2] += gamma * S[3][a + 1] / t + 0.0 * S[0][a + 1] / t;
+      Fsm[a][a] += -cmd[a + 1] * expf / 3.;
+      Fsm[a][2] += -fu[a] * 0.0 * 0.0 * t / 2. - fu[a] * 0.0 * gamma / t;
+      Fdsm[a][a] += -cmd[a + 1] * sgu / 3. - cmd[a + 1] * (gamma * H[3][3] / t) / 3.;
+    }
+    #pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      #pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        if (b < a) continue;
+        F_es[a][b] = (-S[a + 1][b + 1] - tilde_delta * Fdl[a][b] - tau_pi * FDm[a][b]
+                      + 2. * tau_pi * Fdom[a][b] * use_vort - tau_pipi * Fdsm[a][b]
+                      + (2. * eta_pi + lambda_piPi * bulk) * Fsm[a][b] + phi6 * bulk * S[a + 1][b + 1]
+                      + phi7 * (sls[a][b] + fu[a] * fu[b] * shv_shvcov / 3.)) / (sigma * tau_pi * gamma);
+        #pragma unroll
+        for (int k = 0; k < 2; ++k)
+          M_es[a][b][k] = (-tau_pi * MDm[a][b][k] - tilde_delta * Mdl[a][b][k] + 2. * tau_pi * Mdom[a][b][k] * use_vort
+                           - tau_pipi * Mdsm[a][b][k] + (2. * eta_pi + lambda_piPi * bulk) * Msm[a][b][k])
+                          / (sigma * tau_pi * gamma);
+      }
+      F_es[a][a] += phi7 * (-cmd[a + 1] * shv_shvcov) / (3. * tau_pi * gamma * sigma);
+    }
+  }
+
+  // ---------------- fill_auxiliary_variables ----------------
+  const double zeta_tilde = zeta + (delta_PiPi - tau_Pi) * bulk;
+  double M_eb[2], M_S[2];
+  #pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    M_eb[a] = (zeta_tilde * uc[a] / gamma) / (sigma * gamma * tau_Pi);
+    M_S[a] = (bulk * uc[a] / gamma) / (sigma * gamma * T);
+  }
+  const double R_S[3] = { -muB / T, -muS / T, -muQ / T };
+  double F_eb = -(zeta_tilde * (gamma * divV + gamma / t) + bulk - phi1 * bulk * bulk) / (sigma * gamma * tau_Pi);
+  double F_S = (-bulk * (gamma * divV + gamma / t)) / (sigma * gamma * T);
+  if (SHEAR) {
+    F_S += F_snu / (sigma * gamma * T);
+    F_eb += -(lambda_Pipi * F_snu - phi3 * shv_shvcov) / (sigma * gamma * tau_Pi);
+    #pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      M_eb[a] += -lambda_Pipi * M_snu[a] / (sigma * gamma * tau_Pi);
+      M_S[a] += M_snu[a] / (sigma * gamma * T);
+    }
+  }
+
+  // ---------------- compute_velocity_derivative ----------------
+  // R_N = gamma sigma I, F_N = 0, M_N = 0 (no sources, no diffusion), so the
+  // Schur terms R_u R_N^{-1} M_N and R_u R_N^{-1} F_N vanish.
+  const double gP[2] = { -col(CCAKE_F_GRADP0 + 0), -col(CCAKE_F_GRADP0 + 1) };
+  const double gB[2] = { -col(CCAKE_F_GRADBULK0 + 0), -col(CCAKE_F_GRADBULK0 + 1) };
+  const double rdw = rB * dwdB + rS * dwdS + rQ * dwdQ;
+  double M_w[2];
+  #pragma unroll
+  for (int a = 0; a < 2; ++a) M_w[a] = (s * dwds / gamma + rdw / gamma) * uc[a] / gamma + sigma * dwds * M_S[a];
+  const double F_w = -(s * dwds / gamma + rdw / gamma) * (gamma / t + gamma * divV) + sigma * dwds * F_S;
+  const double F_bulk = sigma * F_eb - bulk * (gamma * divV + gamma / t) / gamma;
+  double Mu[2][2], Fu[2];
+  #pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    #pragma unroll
+    for (int b = 0; b < 2; ++b) Mu[a][b] = u[a] * gamma * (M_eb[b] + M_w[b]) - u[a] * uc[b] * (w + bulk) / gamma;
+    Mu[a][a] += gamma * (w + bulk);
+    Fu[a] = (gP[a] + gB[a]) - gamma * (F_w + F_bulk) * u[a] - (w + bulk) * (gamma * divV + gamma / t) * u[a];
+  }
+  if (SHEAR) {
+    const double ds_[2] = { col(CCAKE_F_DIVSHEAR0 + 0), col(CCAKE_F_DIVSHEAR0 + 1) };
+    const double gs_[2] = { col(CCAKE_F_GRADSHEAR0 + 0), col(CCAKE_F_GRADSHEAR0 + 1) };
+    #pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      Fu[a] += -S[a + 1][0] / t - ds_[a] + gs_[a] - F_0i[a];
+      Mu[a][0] += M_0i[a][0];
+      Mu[a][1] += M_0i[a][1];
+    }
+  }
+  const double det = Mu[0][0] * Mu[1][1] - Mu[0][1] * Mu[1][0];
+  const double du0 = (Mu[1][1] / det) * Fu[0] + (-Mu[0][1] / det) * Fu[1];
+  const double du1 = (-Mu[1][0] / det) * Fu[0] + (Mu[0][0] / det) * Fu[1];
+  du_out[0] = du0; du_out[1] = du1;
+
+  // ---------------- compute_derivatives ----------------
+  dN[0] = 0.0; dN[1] = 0.0; dN[2] = 0.0;
+  *d_s_out = (M_S[0] * du0 + M_S[1] * du1) + F_S + (R_S[0] * 0.0 + R_S[1] * 0.0 + R_S[2] * 0.0);
+  *d_b_out = (M_eb[0] * du0 + M_eb[1] * du1) + F_eb;
+  *snu_out = (M_snu[0] * du0 + M_snu[1] * du1) + F_snu;
+  *theta_out = -1. * (uc[0] * du0 + uc[1] * du1) / gamma + gamma * divV + gamma / t;
+  if (SHEAR) {
+    int q = 0;
+    #pragma unroll
+    for (int a = 0; a < 2; ++a)
+      #pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        if (b < a) continue;
+        d_shv_out[q++] = (M_es[a][b][0] * du0 + M_es[a][b][1] * du1) + F_es[a][b];
+      }
+  }
+  // ---------------- compute_Ez_derivative ----------------
+  const double p = col(CCAKE_F_P), e = col(CCAKE_F_E);
+  *dez_out = ((e + p + bulk) * 0.0 * 0.0 + (p + bulk) / t2 + S[3][3]) * col(CCAKE_F_MASS_S) * t2 / col(CCAKE_F_SIGMA_LAB);
+  (void)e;
+}
+
+// mode 0: store derivatives; mode 1: store derivatives and apply the RK
+// update y = y0 + dt f (src/evolver.cpp:363-484); returns the negative
+// entropy flag through err.
+template <bool SHEAR, bool RK>
+__global__ __launch_bounds__(256) void k_eom(double* __restrict__ F, int npad, int n,
+                                             const int32_t* __restrict__ freeze, double t, double use_vort,
+                                             double dt, int32_t* __restrict__ err)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Cols col{ F, npad, i };
+  double du[2], ds, dN[3], db, dshv[5] = { 0, 0, 0, 0, 0 }, theta, snu, dez;
+  eom_d2<SHEAR>(col, t, use_vort, du, &ds, dN, &db, dshv, &theta, &snu, &dez);
+  col(CCAKE_F_DUDT0 + 0) = du[0]; col(CCAKE_F_DUDT0 + 1) = du[1]; col(CCAKE_F_DUDT0 + 2) = 0.0;
+  col(CCAKE_F_DEXT_S) = ds;
+  col(CCAKE_F_DEXT_B) = dN[0]; col(CCAKE_F_DEXT_SS) = dN[1]; col(CCAKE_F_DEXT_Q) = dN[2];
+  col(CCAKE_F_DEXT_BULK) = db;
+  if (SHEAR) for (int q = 0; q < 5; ++q) col(CCAKE_F_DEXT_SHV0 + q) = dshv[q];
+  col(CCAKE_F_THETA) = theta;
+  col(CCAKE_F_SHV_NABLA_U) = snu;
+  col(CCAKE_F_DEZ) = dez;
+  if (!RK) return;
+  double es = col(P_C_S) + dt * ds;
+  if (es < 0.0 && freeze[i] > 3) es = 1.e-3;
+  else if (es < 0.0) atomicOr(err, DE_NEG_ENTROPY);
+  col(CCAKE_F_EXT_S) = es;
+  col(CCAKE_F_EXT_BULK) = col(P_C_BULK) + dt * db;
+  col(CCAKE_F_EXT_B) = col(P_C_B) + dt * dN[0];
+  col(CCAKE_F_EXT_SS) = col(P_C_SS) + dt * dN[1];
+  col(CCAKE_F_EXT_Q) = col(P_C_Q) + dt * dN[2];
+  col(CCAKE_F_EZ) = col(P_C_EZ) + dt * dez;
+  col(CCAKE_F_X) = col(P_C_X) + dt * col(CCAKE_F_VX);
+  col(CCAKE_F_Y) = col(P_C_Y) + dt * col(CCAKE_F_VY);
+  col(CCAKE_F_UX) = col(P_C_UX) + dt * du[0];
+  col(CCAKE_F_UY) = col(P_C_UY) + dt * du[1];
+  for (int q = 0; q < 5; ++q) col(CCAKE_F_EXT_SHV0 + q) = col(P_C_SHV0 + q) + dt * col(CCAKE_F_DEXT_SHV0 + q);
+  col(CCAKE_F_ETA) = 0.0;
+  col(CCAKE_F_UETA) = 0.0;
+}
+
+// Stand-alone RK update from stored derivatives.
+__global__ __launch_bounds__(256) void k_rk(double* __restrict__ F, int npad, int n,
+                                            const int32_t* __restrict__ freeze, double dt, int32_t* __restrict__ err)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Cols col{ F, npad, i };
+  double es = col(P_C_S) + dt * col(CCAKE_F_DEXT_S);
+  if (es < 0.0 && freeze[i] > 3) es = 1.e-3;
+  else if (es < 0.0) atomicOr(err, DE_NEG_ENTROPY);
+  col(CCAKE_F_EXT_S) = es;
+  col(CCAKE_F_EXT_BULK) = col(P_C_BULK) + dt * col(CCAKE_F_DEXT_BULK);
+  col(CCAKE_F_EXT_B) = col(P_C_B) + dt * col(CCAKE_F_DEXT_B);
+  col(CCAKE_F_EXT_SS) = col(P_C_SS) + dt * col(CCAKE_F_DEXT_SS);
+  col(CCAKE_F_EXT_Q) = col(P_C_Q) + dt * col(CCAKE_F_DEXT_Q);
+  col(CCAKE_F_EZ) = col(P_C_EZ) + dt * col(CCAKE_F_DEZ);
+  col(CCAKE_F_X) = col(P_C_X) + dt * col(CCAKE_F_VX);
+  col(CCAKE_F_Y) = col(P_C_Y) + dt * col(CCAKE_F_VY);
+  col(CCAKE_F_UX) = col(P_C_UX) + dt * col(CCAKE_F_DUDT0 + 0);
+  col(CCAKE_F_UY) = col(P_C_UY) + dt * col(CCAKE_F_DUDT0 + 1);
+  for (int q = 0; q < 5; ++q) col(CCAKE_F_EXT_SHV0 + q) = col(P_C_SHV0 + q) + dt * col(CCAKE_F_DEXT_SHV0 + q);
+  col(CCAKE_F_ETA) = 0.0;
+  col(CCAKE_F_UETA) = 0.0;
+}
+
+// Evolver::set_current_timestep_quantities (src/evolver.cpp:124-168)
+__global__ __launch_bounds__(256) void k_cache(double* __restrict__ F, int npad, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Cols col{ F, npad, i };
+  for (int q = 0; q < 5; ++q) col(P_C_SHV0 + q) = col(CCAKE_F_EXT_SHV0 + q);
+  col(P_C_UX) = col(CCAKE_F_UX); col(P_C_UY) = col(CCAKE_F_UY);
+  col(P_C_X) = col(CCAKE_F_X); col(P_C_Y) = col(CCAKE_F_Y);
+  col(P_C_S) = col(CCAKE_F_EXT_S); col(P_C_BULK) = col(CCAKE_F_EXT_BULK);
+  col(P_C_B) = col(CCAKE_F_EXT_B); col(P_C_SS) = col(CCAKE_F_EXT_SS); col(P_C_Q) = col(CCAKE_F_EXT_Q);
+  col(P_C_EZ) = col(CCAKE_F_EZ);
+}
+
+int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0)
+{
+  (void)t0;
+  const int n = ev->n;
+  const dim3 g((n + 255) / 256), b(256);
+  hipStream_t st = ev->dev->stream;
+  const int32_t* fz = ev->I + (size_t)CCAKE_I_FREEZE * ev->npad;
+  const double uv = ev->p.using_vorticity ? 1.0 : 0.0;
+  if (ev->p.using_shear) {
+    if (fuse_rk) hipLaunchKernelGGL((k_eom<true, true>), g, b, 0, st, ev->F, ev->npad, n, fz, ev->t, uv, dt, ev->err);
+    else hipLaunchKernelGGL((k_eom<true, false>), g, b, 0, st, ev->F, ev->npad, n, fz, ev->t, uv, dt, ev->err);
+  } else {
+    if (fuse_rk) hipLaunchKernelGGL((k_eom<false, true>), g, b, 0, st, ev->F, ev->npad, n, fz, ev->t, uv, dt, ev->err);
+    else hipLaunchKernelGGL((k_eom<false, false>), g, b, 0, st, ev->F, ev->npad, n, fz, ev->t, uv, dt, ev->err);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+}
+
+int launch_rk(ccake_event* ev, double dt, double t0)
+{
+  (void)t0;
+  const int n = ev->n;
+  hipLaunchKernelGGL(k_rk, dim3((n + 255) / 256), dim3(256), 0, ev->dev->stream, ev->F, ev->npad, n,
+                     ev->I + (size_t)CCAKE_I_FREEZE * ev->npad, dt, ev->err);
+  return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+}
+
+int launch_cache(ccake_event* ev)
+{
+  const int n = ev->n;
+  hipLaunchKernelGGL(k_cache, dim3((n + 255) / 256), dim3(256), 0, ev->dev->stream, ev->F, ev->npad, n);
+  return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+}
+
+}  // namespace ccake

@@ -1,0 +1,243 @@
+// Per-particle physics after the smoothing pass, fused in one kernel:
+//   04 EoS inversion        SPHWorkstation::update_all_particle_thermodynamics
+//                           (src/sph_workstation.cpp:1713-1772) -- on the device,
+//                           where the reference runs a serial host loop
+//   05 reset_pi_tensor      EoM_default::reset_pi_tensor (src/eom_default.cpp:56-172)
+//   08 viscosities          SPHWorkstation::update_all_particle_viscosities (1622-1699)
+//   gather of the gradient-pass candidate records in cell-sorted order.
+// Threads walk the cell-sorted order, so the lanes of a wave hold nearby
+// particles with similar (T, mu) and share table cells.
+#include "eos_device.h"
+#include <vector>
+#include <cstring>
+
+namespace ccake {
+
+constexpr int kCandG = 16;   // doubles per gradient candidate record
+
+struct Transport {
+  int eta_mode, shear_relax_mode, zeta_mode, modulate;
+  double eta_over_s, zeta_over_s, zA, zp, gA, gT0, gW;
+  int delta_pipi, tau_pipi, lambda_piPi, phi7, delta_PiPi, lambda_Pipi;
+};
+
+__device__ __forceinline__ double tc_eta(const Transport& P, double T, double w)
+{
+  if (P.eta_mode == 1) return (T <= 0.0) ? 0.0 : P.eta_over_s * (w / T);
+  return 0.20 * (w / T);
+}
+__device__ __forceinline__ double tc_zeta(const Transport& P, double T, double w, double cs2)
+{
+  switch (P.zeta_mode) {
+    case 1: return P.zeta_over_s * (w / T);
+    case 2: {
+      double factor = 1.0;
+      if (P.modulate) {
+        const double Ttr = 150.0 / kHbarcMeVfm, Tsc = 10.0 / kHbarcMeVfm;
+        const double thx = tanh((T - Ttr) / Tsc);
+        const double xp = pow(T / Ttr, P.zp);
+        factor = 0.5 * (1.0 + xp) + 0.5 * (1.0 - xp) * thx;
+      }
+      return P.zA * factor * pow((1.0 / 3.0) - fmin(cs2, 1.0), P.zp) * w / T;
+    }
+    case 3: {
+      const double d = T - P.gT0;
+      return P.gA * exp(-d * d / (2.0 * P.gW * P.gW)) * w / T;
+    }
+    default: return 0.005 * (w / T);
+  }
+}
+
+// transport_coefficients dispatchers (src/transport_coefficients.cpp:284-912)
+__device__ __forceinline__ void transport(const Transport& P, double T, double w, double p, double cs2, double* o)
+{
+  const double eta = tc_eta(P, T, w);
+  const double x = 5.0 * eta / w;
+  const double tau_pi = (P.shear_relax_mode == 2) ? x : fmax(x, P.shear_relax_mode == 1 ? 0.001 : 0.005);
+  const double zeta = tc_zeta(P, T, w, cs2);
+  const double cc = cs2 > 1 ? 1 : cs2;
+  const double pw = pow(1.0 / 3.0 - cc, P.zp);
+  const double tau_Pi = (pw < 1e-10) ? 1e10 : fmax(5.0 * zeta / (pw * w), 0.1);
+  o[0] = eta;
+  o[1] = tau_pi;
+  o[2] = zeta;
+  o[3] = tau_Pi;
+  o[4] = P.delta_PiPi == 1 ? 0.0 : (P.delta_PiPi == 2 ? tau_Pi : 2. * tau_Pi / 3.0);
+  o[5] = P.delta_pipi == 1 ? 0.0 : (P.delta_pipi == 2 ? tau_pi : 4. * tau_pi / 3.0);
+  o[6] = P.lambda_piPi == 1 ? 0.0 : 6. / 5. * tau_pi;
+  o[7] = P.lambda_Pipi == 1 ? 0.0 : 8. * (1. / 3. - cc) * tau_Pi / 5.;
+  o[8] = 0.0;
+  o[9] = 0.0;
+  o[10] = 0.0;
+  o[11] = P.phi7 == 1 ? 0.0 : 9. / (70. * p);
+  o[12] = P.tau_pipi == 1 ? 0.0 : 10. * tau_pi / 7.0;
+}
+
+// MODE 0: s-inversion + reset_pi + transport + gradient gather (sorted order).
+// MODE 1: e-inversion of the inputs (id order), writes input s.
+// MODE 2: s-inversion only (id order).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_eos(EosChain ch, double* __restrict__ F, int npad, int n,
+                                             int32_t* __restrict__ I, const int32_t* __restrict__ perm,
+                                             double t, Transport P, double* __restrict__ cand,
+                                             int32_t* __restrict__ err)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = (MODE == 0) ? perm[k] : k;
+  auto col = [&](int c) -> double& { return F[(size_t)c * npad + i]; };
+  int32_t* eos_id = I + (size_t)CCAKE_I_EOS_ID * npad;
+  int32_t* freeze = I + (size_t)CCAKE_I_FREEZE * npad;
+  double x[4] = { col(CCAKE_F_T), col(CCAKE_F_MUB), col(CCAKE_F_MUQ), col(CCAKE_F_MUS) };
+  int member = eos_id[i];
+  double tgt[4];
+  bool ok;
+  if (MODE == 1) {
+    tgt[0] = col(CCAKE_F_IN_E); tgt[1] = col(CCAKE_F_IN_B); tgt[2] = col(CCAKE_F_IN_SS); tgt[3] = col(CCAKE_F_IN_Q);
+    ok = invert_point(ch, ch.tab, ch.tab4, 1, tgt, x, &member);
+    if (!ok) {
+      // src/sph_workstation.cpp:1825-1848: freeze and retry at zero charge
+      freeze[i] = 4;
+      const double z[4] = { tgt[0], 0.0, 0.0, 0.0 };
+      ok = invert_point(ch, ch.tab, ch.tab4, 1, z, x, &member);
+    }
+  } else {
+    tgt[0] = col(CCAKE_F_S); tgt[1] = col(CCAKE_F_RHOB); tgt[2] = col(CCAKE_F_RHOS); tgt[3] = col(CCAKE_F_RHOQ);
+    ok = invert_point(ch, ch.tab, ch.tab4, 0, tgt, x, &member);
+  }
+  if (!ok) { atomicOr(err, DE_EOS_FAIL); return; }
+  double th[31];
+  set_thermo(ch, ch.tab, member, x, th);
+  for (int q = 0; q < 31; ++q) col(CCAKE_F_T + q) = th[q];
+  eos_id[i] = member;
+  if (MODE == 1) { col(CCAKE_F_IN_S) = th[5]; return; }
+  if (MODE == 2) return;
+
+  // ---- 05 reset_pi_tensor, D = 2 (fixed-size u has u^eta = 0) ----
+  const double gamma = col(CCAKE_F_GAMMA), sigma = col(CCAKE_F_SIGMA);
+  const double t2 = t * t;
+  const double ux = col(CCAKE_F_UX), uy = col(CCAKE_F_UY);
+  const double fc0 = -ux, fc1 = -uy;
+  const double bulk = col(CCAKE_F_EXT_BULK) * sigma;
+  col(CCAKE_F_BULK) = bulk;
+  const double s11 = col(CCAKE_F_EXT_SHV0 + 0) * sigma, s12 = col(CCAKE_F_EXT_SHV0 + 1) * sigma;
+  const double s13 = col(CCAKE_F_EXT_SHV0 + 2) * sigma, s22 = col(CCAKE_F_EXT_SHV0 + 3) * sigma;
+  const double s23 = col(CCAKE_F_EXT_SHV0 + 4) * sigma;
+  const double s00 = (fc0 * fc0 * s11 + fc1 * fc1 * s22 + 2. * fc1 * fc0 * s12) / (gamma * gamma);
+  const double s33 = (s00 - s11 - s22) / t2;
+  const double s01 = -s11 * fc0 / gamma + -s12 * fc1 / gamma;
+  const double s02 = -s12 * fc0 / gamma + -s22 * fc1 / gamma;
+  const double s03 = -s13 * fc0 / gamma + -s23 * fc1 / gamma;
+  const double S[16] = { s00, s01, s02, s03, s01, s11, s12, s13, s02, s12, s22, s23, s03, s13, s23, s33 };
+  for (int q = 0; q < 16; ++q) col(CCAKE_F_SHV0 + q) = S[q];
+
+  // ---- 08 transport coefficients ----
+  double tc[13];
+  transport(P, th[0], th[11], th[9], th[10], tc);
+  for (int q = 0; q < 13; ++q) col(CCAKE_F_ETA_PI + q) = tc[q];
+
+  // ---- gradient candidate record: x, y, m_s, p/sl^2, e, Pi/sl^2, v, pi^{0i}/sl^2, pi^{ij}/sl^2
+  const double sl = col(CCAKE_F_SIGMA_LAB);
+  const double isl2 = 1.0 / (sl * sl);
+  double* o = cand + (size_t)k * kCandG;
+  o[0] = col(CCAKE_F_X); o[1] = col(CCAKE_F_Y); o[2] = col(CCAKE_F_MASS_S);
+  o[3] = th[9] * isl2; o[4] = th[4]; o[5] = bulk * isl2;
+  o[6] = col(CCAKE_F_VX); o[7] = col(CCAKE_F_VY);
+  o[8] = s01 * isl2; o[9] = s02 * isl2;
+  o[10] = s11 * isl2; o[11] = s12 * isl2; o[12] = s22 * isl2;
+  o[13] = sl; o[14] = 0.0; o[15] = 0.0;
+}
+
+static Transport make_transport(const ccake_params& p)
+{
+  Transport T;
+  T.eta_mode = p.eta_mode; T.shear_relax_mode = p.shear_relax_mode; T.zeta_mode = p.zeta_mode;
+  T.modulate = p.modulate_zeta_with_tanh;
+  T.eta_over_s = p.constant_eta_over_s; T.zeta_over_s = p.constant_zeta_over_s;
+  T.zA = p.cs2_dependent_zeta_A; T.zp = p.cs2_dependent_zeta_p;
+  T.gA = p.zeta_gauss_A; T.gT0 = p.zeta_gauss_T0; T.gW = p.zeta_gauss_width;
+  T.delta_pipi = p.delta_pipi_mode; T.tau_pipi = p.tau_pipi_mode; T.lambda_piPi = p.lambda_piPi_mode;
+  T.phi7 = p.phi7_mode; T.delta_PiPi = p.delta_PiPi_mode; T.lambda_Pipi = p.lambda_Pipi_mode;
+  return T;
+}
+
+int launch_eos(ccake_event* ev, int mode, int fuse_post)
+{
+  const int n = ev->n;
+  const dim3 g((n + 255) / 256), b(256);
+  hipStream_t st = ev->dev->stream;
+  const Transport P = make_transport(ev->p);
+  if (mode == 1)
+    hipLaunchKernelGGL(k_eos<1>, g, b, 0, st, ev->eos->chain, ev->F, ev->npad, n, ev->I, ev->perm, ev->t, P,
+                       ev->cand_g, ev->err);
+  else if (fuse_post)
+    hipLaunchKernelGGL(k_eos<0>, g, b, 0, st, ev->eos->chain, ev->F, ev->npad, n, ev->I, ev->perm, ev->t, P,
+                       ev->cand_g, ev->err);
+  else
+    hipLaunchKernelGGL(k_eos<2>, g, b, 0, st, ev->eos->chain, ev->F, ev->npad, n, ev->I, ev->perm, ev->t, P,
+                       ev->cand_g, ev->err);
+  return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+}
+
+// Stand-alone reset_pi_tensor / viscosities / gather (same code paths as the
+// fused kernel, for the phase-by-phase C-ABI).
+__global__ __launch_bounds__(256) void k_post(double* __restrict__ F, int npad, int n,
+                                              const int32_t* __restrict__ perm, double t, Transport P,
+                                              double* __restrict__ cand, int what)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = perm ? perm[k] : k;
+  auto col = [&](int c) -> double& { return F[(size_t)c * npad + i]; };
+  if (what == 0) {
+    const double gamma = col(CCAKE_F_GAMMA), sigma = col(CCAKE_F_SIGMA);
+    const double t2 = t * t;
+    const double fc0 = -col(CCAKE_F_UX), fc1 = -col(CCAKE_F_UY);
+    col(CCAKE_F_BULK) = col(CCAKE_F_EXT_BULK) * sigma;
+    const double s11 = col(CCAKE_F_EXT_SHV0 + 0) * sigma, s12 = col(CCAKE_F_EXT_SHV0 + 1) * sigma;
+    const double s13 = col(CCAKE_F_EXT_SHV0 + 2) * sigma, s22 = col(CCAKE_F_EXT_SHV0 + 3) * sigma;
+    const double s23 = col(CCAKE_F_EXT_SHV0 + 4) * sigma;
+    const double s00 = (fc0 * fc0 * s11 + fc1 * fc1 * s22 + 2. * fc1 * fc0 * s12) / (gamma * gamma);
+    const double s33 = (s00 - s11 - s22) / t2;
+    const double s01 = -s11 * fc0 / gamma + -s12 * fc1 / gamma;
+    const double s02 = -s12 * fc0 / gamma + -s22 * fc1 / gamma;
+    const double s03 = -s13 * fc0 / gamma + -s23 * fc1 / gamma;
+    const double S[16] = { s00, s01, s02, s03, s01, s11, s12, s13, s02, s12, s22, s23, s03, s13, s23, s33 };
+    for (int q = 0; q < 16; ++q) col(CCAKE_F_SHV0 + q) = S[q];
+  } else if (what == 1) {
+    double tc[13];
+    transport(P, col(CCAKE_F_T), col(CCAKE_F_W), col(CCAKE_F_P), col(CCAKE_F_CS2), tc);
+    for (int q = 0; q < 13; ++q) col(CCAKE_F_ETA_PI + q) = tc[q];
+  } else {
+    const double sl = col(CCAKE_F_SIGMA_LAB);
+    const double isl2 = 1.0 / (sl * sl);
+    double* o = cand + (size_t)k * kCandG;
+    o[0] = col(CCAKE_F_X); o[1] = col(CCAKE_F_Y); o[2] = col(CCAKE_F_MASS_S);
+    o[3] = col(CCAKE_F_P) * isl2; o[4] = col(CCAKE_F_E); o[5] = col(CCAKE_F_BULK) * isl2;
+    o[6] = col(CCAKE_F_VX); o[7] = col(CCAKE_F_VY);
+    o[8] = col(CCAKE_F_SHV0 + 1) * isl2; o[9] = col(CCAKE_F_SHV0 + 2) * isl2;
+    o[10] = col(CCAKE_F_SHV0 + 5) * isl2; o[11] = col(CCAKE_F_SHV0 + 6) * isl2;
+    o[12] = col(CCAKE_F_SHV0 + 10) * isl2;
+    o[13] = sl; o[14] = 0.0; o[15] = 0.0;
+  }
+}
+
+static int post(ccake_event* ev, int what, const int32_t* perm)
+{
+  const int n = ev->n;
+  hipLaunchKernelGGL(k_post, dim3((n + 255) / 256), dim3(256), 0, ev->dev->stream, ev->F, ev->npad, n, perm, ev->t,
+                     make_transport(ev->p), ev->cand_g, what);
+  return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+}
+int launch_reset_pi(ccake_event* ev) { return post(ev, 0, nullptr); }
+int launch_viscosities(ccake_event* ev) { return post(ev, 1, nullptr); }
+int launch_gather_grad(ccake_event* ev) { return post(ev, 2, ev->perm); }
+
+int host_eos_full(const ccake_eos* eos, int member, const double* x, double* r17)
+{
+  if (member < 0 || member >= eos->chain.n) return CCAKE_ERR_INVALID;
+  member_fields(eos->chain.m[member], eos->h_tab, x, r17);
+  return 0;
+}
+
+}  // namespace ccake

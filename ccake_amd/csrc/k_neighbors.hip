@@ -1,0 +1,329 @@
+// Linked-cell neighbour build on gfx950: cell keys, a stable LSD radix sort
+// (256 buckets per pass, wave-ballot ranking, device-wide scans), and cell
+// offsets.  Replaces SystemState::reset_neighbour_list
+// (src/system_state.cpp:650-719), which rebuilds a Cabana VerletList every
+// RK stage.  The pair list itself is never materialised on the hot path:
+// pair kernels walk the sorted cells directly.
+#include "ccake_internal.h"
+#include <climits>
+#include <vector>
+#include <algorithm>
+
+namespace ccake {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 4;                       // items per thread per block
+constexpr int kSortBlockItems = kSortThreads * kSortItems;
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+
+// ---------------------------------------------------------------------------
+// Phase 02 fused with the bounding-box partials.
+// EoM_default::update_velocity (src/eom_default.cpp:17-38), D = 2:
+// gamma = sqrt(1 - u_cov.u) with u_cov = -u, v = u/gamma.
+__global__ __launch_bounds__(256) void k_stage_begin(double* __restrict__ F, int npad, int n,
+                                                     double* __restrict__ bbox_part, int do_gamma)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double xmn = INFINITY, xmx = -INFINITY, ymn = INFINITY, ymx = -INFINITY;
+  if (i < n && do_gamma) {
+    const double ux = F[(size_t)CCAKE_F_UX * npad + i], uy = F[(size_t)CCAKE_F_UY * npad + i];
+    const double c = (-ux) * ux + (-uy) * uy;
+    const double g = sqrt(1.0 - c);
+    F[(size_t)CCAKE_F_GAMMA * npad + i] = g;
+    F[(size_t)CCAKE_F_VX * npad + i] = ux / g;
+    F[(size_t)CCAKE_F_VY * npad + i] = uy / g;
+  }
+  if (i < n) {
+    const double x = F[(size_t)CCAKE_F_X * npad + i], y = F[(size_t)CCAKE_F_Y * npad + i];
+    xmn = xmx = x; ymn = ymx = y;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    xmn = fmin(xmn, __shfl_xor(xmn, o)); xmx = fmax(xmx, __shfl_xor(xmx, o));
+    ymn = fmin(ymn, __shfl_xor(ymn, o)); ymx = fmax(ymx, __shfl_xor(ymx, o));
+  }
+  __shared__ double s[4][4];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) { s[w][0] = xmn; s[w][1] = xmx; s[w][2] = ymn; s[w][3] = ymx; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; ++k) {
+      xmn = fmin(xmn, s[k][0]); xmx = fmax(xmx, s[k][1]); ymn = fmin(ymn, s[k][2]); ymx = fmax(ymx, s[k][3]);
+    }
+    double* b = bbox_part + 4 * blockIdx.x;
+    b[0] = xmn; b[1] = xmx; b[2] = ymn; b[3] = ymx;
+  }
+}
+
+// Reduce the partials and lay out the cell grid.  cell = rc / cells_per_rc,
+// doubled until the grid fits the key capacity.
+__global__ __launch_bounds__(256) void k_grid_setup(const double* __restrict__ bbox_part, int nparts,
+                                                    double rc, int cells_per_rc, int cap,
+                                                    Grid* __restrict__ grid, int32_t* __restrict__ err,
+                                                    unsigned long long* __restrict__ pair_count)
+{
+  if (threadIdx.x == 0) *pair_count = 0ull;
+  double xmn = INFINITY, xmx = -INFINITY, ymn = INFINITY, ymx = -INFINITY;
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+    const double* b = bbox_part + 4 * k;
+    xmn = fmin(xmn, b[0]); xmx = fmax(xmx, b[1]); ymn = fmin(ymn, b[2]); ymx = fmax(ymx, b[3]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    xmn = fmin(xmn, __shfl_xor(xmn, o)); xmx = fmax(xmx, __shfl_xor(xmx, o));
+    ymn = fmin(ymn, __shfl_xor(ymn, o)); ymx = fmax(ymx, __shfl_xor(ymx, o));
+  }
+  __shared__ double s[4][4];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) { s[w][0] = xmn; s[w][1] = xmx; s[w][2] = ymn; s[w][3] = ymx; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; ++k) {
+      xmn = fmin(xmn, s[k][0]); xmx = fmax(xmx, s[k][1]); ymn = fmin(ymn, s[k][2]); ymx = fmax(ymx, s[k][3]);
+    }
+    double cell = rc / (double)(cells_per_rc > 0 ? cells_per_rc : 1);
+    int nx = 1, ny = 1;
+    const double wx = xmx - xmn, wy = ymx - ymn;
+    if (!(wx >= 0.0 && wx < 1e300) || !(wy >= 0.0 && wy < 1e300)) {
+      // non-finite positions: one cell, flagged
+      atomicOr(err, DE_GRID_OVERFLOW);
+      xmn = 0.0; ymn = 0.0;
+    } else {
+      for (;;) {
+        const double fx = floor(wx / cell) + 1.0, fy = floor(wy / cell) + 1.0;
+        if (fx * fy <= (double)cap) { nx = (int)fx; ny = (int)fy; break; }
+        cell *= 2.0;
+      }
+    }
+    Grid g;
+    g.lo[0] = xmn; g.lo[1] = ymn; g.cell = cell; g.inv_cell = 1.0 / cell;
+    g.nc[0] = nx; g.nc[1] = ny; g.ncells = nx * ny; g.pad = 0;
+    *grid = g;
+  }
+}
+
+__device__ __forceinline__ int cell_coord(double x, double lo, double inv, int nc)
+{
+  int c = (int)floor((x - lo) * inv);
+  return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
+}
+
+__global__ __launch_bounds__(256) void k_keys(const double* __restrict__ F, int npad, int n,
+                                              const Grid* __restrict__ grid, int32_t* __restrict__ key)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Grid g = *grid;
+  const int cx = cell_coord(F[(size_t)CCAKE_F_X * npad + i], g.lo[0], g.inv_cell, g.nc[0]);
+  const int cy = cell_coord(F[(size_t)CCAKE_F_Y * npad + i], g.lo[1], g.inv_cell, g.nc[1]);
+  key[i] = cy * g.nc[0] + cx;
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort, 8-bit digits.  Block b owns items
+// [b*kSortBlockItems, (b+1)*kSortBlockItems) in input order; hist is laid
+// out digit-major (hist[d*nblk + b]) so that one exclusive scan yields every
+// (digit, block) output base.
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const int32_t* __restrict__ keys, int n,
+                                                             int shift, int nblk, int32_t* __restrict__ hist)
+{
+  __shared__ int32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * kSortBlockItems;
+  for (int r = 0; r < kSortItems; ++r) {
+    const int idx = base + r * kSortThreads + threadIdx.x;
+    if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255], 1);
+  }
+  __syncthreads();
+  hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+// Peers of this lane inside its wave: lanes holding the same 8-bit digit.
+__device__ __forceinline__ uint64_t digit_peers(int digit, bool active)
+{
+  uint64_t m = __ballot(active);
+  #pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const bool bit = (digit >> b) & 1;
+    const uint64_t bb = __ballot(bit);
+    m &= bit ? bb : ~bb;
+  }
+  return m;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const int32_t* __restrict__ keys_in,
+                                                                const int32_t* __restrict__ vals_in,
+                                                                int n, int shift, int nblk,
+                                                                const int32_t* __restrict__ hist_scan,
+                                                                int32_t* __restrict__ keys_out,
+                                                                int32_t* __restrict__ vals_out)
+{
+  __shared__ int32_t base[256];
+  __shared__ int32_t running[256];
+  __shared__ int32_t wcnt[kSortThreads / kWave][256];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  base[threadIdx.x] = hist_scan[threadIdx.x * nblk + blockIdx.x];
+  running[threadIdx.x] = 0;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int blk0 = blockIdx.x * kSortBlockItems;
+  for (int r = 0; r < kSortItems; ++r) {
+    for (int k = 0; k < kSortThreads / kWave; ++k) wcnt[k][threadIdx.x] = 0;
+    __syncthreads();
+    const int idx = blk0 + r * kSortThreads + threadIdx.x;
+    const bool active = idx < n;
+    int key = 0, val = 0, digit = 0;
+    if (active) { key = keys_in[idx]; val = vals_in ? vals_in[idx] : idx; digit = (key >> shift) & 255; }
+    const uint64_t peers = digit_peers(digit, active);
+    const int rank = __popcll(peers & lt);
+    if (active && rank == 0) wcnt[w][digit] = __popcll(peers);
+    __syncthreads();
+    if (active) {
+      int off = running[digit] + rank;
+      for (int k = 0; k < w; ++k) off += wcnt[k][digit];
+      const int dst = base[digit] + off;
+      keys_out[dst] = key;
+      vals_out[dst] = val;
+    }
+    __syncthreads();
+    int add = 0;
+    for (int k = 0; k < kSortThreads / kWave; ++k) add += wcnt[k][threadIdx.x];
+    running[threadIdx.x] += add;
+    __syncthreads();
+  }
+}
+
+// ---- device-wide exclusive scan of int32 (block sums, scan of sums, down-sweep)
+__device__ int32_t block_exclusive_scan(int32_t v, int32_t* total)
+{
+  __shared__ int32_t ws[kScanThreads / kWave];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int32_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) ws[w] = x;
+  __syncthreads();
+  int32_t pre = 0, tot = 0;
+  for (int k = 0; k < kScanThreads / kWave; ++k) { if (k < w) pre += ws[k]; tot += ws[k]; }
+  __syncthreads();
+  *total = tot;
+  return pre + x - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int32_t* __restrict__ in, int n,
+                                                              int32_t* __restrict__ part)
+{
+  const int base = blockIdx.x * kScanThreads * kScanItems;
+  int32_t s = 0;
+  for (int r = 0; r < kScanItems; ++r) {
+    const int idx = base + r * kScanThreads + threadIdx.x;
+    if (idx < n) s += in[idx];
+  }
+  int32_t tot;
+  block_exclusive_scan(s, &tot);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_partials(int32_t* __restrict__ part, int np)
+{
+  int32_t carry = 0;
+  for (int b0 = 0; b0 < np; b0 += kScanThreads) {
+    const int idx = b0 + threadIdx.x;
+    const int32_t v = idx < np ? part[idx] : 0;
+    int32_t tot;
+    const int32_t ex = block_exclusive_scan(v, &tot);
+    if (idx < np) part[idx] = carry + ex;
+    carry += tot;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(const int32_t* __restrict__ in, int n,
+                                                            const int32_t* __restrict__ part,
+                                                            int32_t* __restrict__ out)
+{
+  const int base = blockIdx.x * kScanThreads * kScanItems;
+  // each thread owns kScanItems consecutive items
+  int32_t v[kScanItems];
+  int32_t s = 0;
+  const int t0 = base + threadIdx.x * kScanItems;
+  for (int r = 0; r < kScanItems; ++r) { const int idx = t0 + r; v[r] = idx < n ? in[idx] : 0; s += v[r]; }
+  int32_t tot;
+  int32_t ex = block_exclusive_scan(s, &tot) + part[blockIdx.x];
+  for (int r = 0; r < kScanItems; ++r) { const int idx = t0 + r; if (idx < n) out[idx] = ex; ex += v[r]; }
+}
+
+static int scan_exclusive(ccake_event* ev, const int32_t* in, int32_t* out, int n)
+{
+  const int per = kScanThreads * kScanItems;
+  const int nb = (n + per - 1) / per;
+  hipStream_t st = ev->dev->stream;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanThreads), 0, st, in, n, ev->scan_tmp);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanThreads), 0, st, ev->scan_tmp, nb);
+  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kScanThreads), 0, st, in, n, ev->scan_tmp, out);
+  return 0;
+}
+
+// cell_off[c] = first sorted position whose key >= c, for c in [0, ncells].
+__global__ __launch_bounds__(256) void k_cell_offsets(const int32_t* __restrict__ skey, int n,
+                                                      const Grid* __restrict__ grid,
+                                                      int32_t* __restrict__ cell_off)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = grid->ncells;
+  if (n == 0) {
+    if (k <= nc) cell_off[k] = 0;
+    return;
+  }
+  if (k >= n) return;
+  const int cur = skey[k];
+  const int prev = (k == 0) ? -1 : skey[k - 1];
+  for (int c = prev + 1; c <= cur; ++c) cell_off[c] = k;
+  if (k == n - 1) for (int c = cur + 1; c <= nc; ++c) cell_off[c] = n;
+}
+
+// ---------------------------------------------------------------------------
+static int stage_begin(ccake_event* ev, int do_gamma)
+{
+  const int n = ev->n;
+  const int nb = (n + 255) / 256;
+  hipLaunchKernelGGL(k_stage_begin, dim3(nb), dim3(256), 0, ev->dev->stream, ev->F, ev->npad, n, ev->bbox, do_gamma);
+  return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+}
+
+int launch_stage_begin(ccake_event* ev) { return stage_begin(ev, 1); }
+int launch_bbox(ccake_event* ev) { return stage_begin(ev, 0); }
+int launch_gamma_velocity(ccake_event* ev) { return stage_begin(ev, 1); }
+
+int launch_neighbor_sort(ccake_event* ev)
+{
+  const int n = ev->n;
+  hipStream_t st = ev->dev->stream;
+  const int nb = (n + 255) / 256;
+  const double rc = 2.0 * ev->p.hT;
+  hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(256), 0, st, ev->bbox, nb, rc, ev->p.cells_per_rc,
+                     ev->cap_cells, ev->grid, ev->err, (unsigned long long*)ev->pair_count);
+  hipLaunchKernelGGL(k_keys, dim3(nb), dim3(256), 0, st, ev->F, ev->npad, n, ev->grid, ev->key);
+  const int nblk = ev->radix_blocks;
+  // two 8-bit passes cover the key capacity (cap_cells <= 65536).
+  const int32_t* kin = ev->key;
+  const int32_t* vin = nullptr;
+  int32_t* kout = ev->key_a; int32_t* vout = ev->val_a;
+  for (int pass = 0; pass < 2; ++pass) {
+    const int shift = 8 * pass;
+    hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(kSortThreads), 0, st, kin, n, shift, nblk, ev->hist);
+    scan_exclusive(ev, ev->hist, ev->hist, 256 * nblk);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(kSortThreads), 0, st, kin, vin, n, shift, nblk,
+                       ev->hist, kout, vout);
+    kin = kout; vin = vout;
+    if (kout == ev->key_a) { kout = ev->key_b; vout = ev->val_b; } else { kout = ev->key_a; vout = ev->val_a; }
+  }
+  ev->skey = const_cast<int32_t*>(kin);
+  ev->perm = const_cast<int32_t*>(vin);
+  const int nbc = (n + 255) / 256 + 1;
+  hipLaunchKernelGGL(k_cell_offsets, dim3(nbc), dim3(256), 0, st, ev->skey, n, ev->grid, ev->cell_off);
+  return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+}
+
+}  // namespace ccake

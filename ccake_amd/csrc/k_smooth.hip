@@ -1,0 +1,213 @@
+// Smoothing pair kernel on gfx950 (owner computes, no atomics).
+// Replaces SPHWorkstation::smooth_all_particle_fields
+// (src/sph_workstation.cpp:494-578) and calculate_intial_sigma (582-640);
+// the same walk counts neighbours (btrack, src/system_state.cpp:712-715)
+// and exports the pair set for parity checks.
+#include "pair_common.h"
+#include <vector>
+#include <algorithm>
+
+namespace ccake {
+
+constexpr int kCandS = 8;   // doubles per smoothing candidate record
+
+// Candidate records in sorted order: x, y, m_s, m_s*ext_s, m_B*ext_B,
+// m_S*ext_S, m_Q*ext_Q, 0.
+__global__ __launch_bounds__(256) void k_gather_smooth(const double* __restrict__ F, int npad, int n,
+                                                       const int32_t* __restrict__ perm,
+                                                       double* __restrict__ cand)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = perm[k];
+  auto f = [&](int c) { return F[(size_t)c * npad + i]; };
+  double* o = cand + (size_t)k * kCandS;
+  const double ms = f(CCAKE_F_MASS_S);
+  o[0] = f(CCAKE_F_X);
+  o[1] = f(CCAKE_F_Y);
+  o[2] = ms;
+  o[3] = ms * f(CCAKE_F_EXT_S);
+  o[4] = f(CCAKE_F_MASS_B) * f(CCAKE_F_EXT_B);
+  o[5] = f(CCAKE_F_MASS_SS) * f(CCAKE_F_EXT_SS);
+  o[6] = f(CCAKE_F_MASS_Q) * f(CCAKE_F_EXT_Q);
+  o[7] = 0.0;
+}
+
+struct KernelW {
+  double hinv, norm, norm2;
+  // Cubic spline W(r, h) (src/kernel.cpp:34-48) from r^2.
+  __device__ __forceinline__ double operator()(double r2) const
+  {
+    const double q = sqrt(r2) * hinv;
+    const double qq = q * q;
+    const double a = norm * (1.0 - 1.5 * qq + 0.75 * q * qq);
+    const double t = 2.0 - q;
+    const double b = norm2 * t * t * t;
+    return q >= 2.0 ? 0.0 : (q >= 1.0 ? b : a);
+  }
+};
+
+// MODE 0: fields + LRF + btrack; MODE 1: initial sigma; MODE 2: btrack only;
+// MODE 3: write the pair rows (parity export).
+template <int MODE>
+__global__ __launch_bounds__(64) void k_smooth(const double* __restrict__ cand,
+                                               const int32_t* __restrict__ skey,
+                                               const int32_t* __restrict__ perm,
+                                               const int32_t* __restrict__ cell_off,
+                                               const Grid* __restrict__ gridp, int n,
+                                               double rc2, double rcs, KernelW W, double kern0,
+                                               double t, double* __restrict__ F, int npad,
+                                               int32_t* __restrict__ btrack,
+                                               unsigned long long* __restrict__ pair_total,
+                                               const int64_t* __restrict__ row_off,
+                                               int32_t* __restrict__ out_i, int32_t* __restrict__ out_j)
+{
+  const Grid g = *gridp;
+  const int lane = threadIdx.x;
+  const int k = blockIdx.x * kTile + lane;
+  const bool act = k < n;
+  const int kk = act ? k : 0;
+  const double* me = cand + (size_t)kk * kCandS;
+  const double xi = me[0], yi = me[1];
+  const int myrow = act ? skey[kk] / g.nc[0] : INT_MAX;
+  const int id = act ? perm[kk] : 0;
+
+  // self terms first (src/sph_workstation.cpp:514-523)
+  double sl = me[2] * kern0;
+  double ss = me[3] * kern0, sB = me[4] * kern0, sS = me[5] * kern0, sQ = me[6] * kern0;
+  int cnt = 0;
+  int64_t wpos = 0;
+  if (MODE == 3 && act) wpos = row_off[id];
+
+  for_segments(act, myrow, xi, yi, [&](const Seg& seg, bool in) {
+    for_spans(g, cell_off, seg, rcs, [&](int j) {
+      const double* c = cand + (size_t)j * kCandS;
+      const double dx = xi - c[0], dy = yi - c[1];
+      const double r2 = dist2_nocontract(dx, dy);
+      const bool nb = in && (r2 <= rc2) && (j != k);
+      if (MODE == 2) { cnt += nb; return; }
+      if (MODE == 3) {
+        if (nb) { out_i[wpos] = id; out_j[wpos] = perm[j]; ++wpos; }
+        return;
+      }
+      const double w = nb ? W(r2) : 0.0;
+      cnt += nb;
+      sl += c[2] * w;
+      if (MODE == 0) {
+        ss += c[3] * w; sB += c[4] * w; sS += c[5] * w; sQ += c[6] * w;
+      }
+    });
+  });
+
+  if (!act) return;
+  if (MODE == 0 || MODE == 2) {
+    // pair total for the roofline: one atomic per wave
+    unsigned long long tot = (unsigned long long)cnt;
+    for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
+    if (lane == 0) atomicAdd(pair_total, tot);
+    const int b = btrack[id];
+    btrack[id] = (b == -1) ? -1 : cnt;
+  }
+  if (MODE == 2 || MODE == 3) return;
+  const double gam = F[(size_t)CCAKE_F_GAMMA * npad + id];
+  F[(size_t)CCAKE_F_SIGMA_LAB * npad + id] = sl;
+  // get_LRF = lab/gamma/t (src/eom_default.cpp:198-203)
+  F[(size_t)CCAKE_F_SIGMA * npad + id] = sl / gam / t;
+  if (MODE == 0) {
+    F[(size_t)CCAKE_F_SM_S * npad + id] = ss;
+    F[(size_t)CCAKE_F_SM_B * npad + id] = sB;
+    F[(size_t)CCAKE_F_SM_SS * npad + id] = sS;
+    F[(size_t)CCAKE_F_SM_Q * npad + id] = sQ;
+    F[(size_t)CCAKE_F_S * npad + id] = ss / gam / t;
+    F[(size_t)CCAKE_F_RHOB * npad + id] = sB / gam / t;
+    F[(size_t)CCAKE_F_RHOS * npad + id] = sS / gam / t;
+    F[(size_t)CCAKE_F_RHOQ * npad + id] = sQ / gam / t;
+  }
+}
+
+static KernelW make_w(double hT)
+{
+  KernelW w;
+  w.hinv = 1.0 / hT;
+  w.norm = 10.0 / (7.0 * kPi) / pow(hT, 2);     // src/kernel.cpp:16-17
+  w.norm2 = 5.0 / (14.0 * kPi) / pow(hT, 2);
+  return w;
+}
+
+int launch_gather_smooth(ccake_event* ev)
+{
+  const int n = ev->n;
+  hipLaunchKernelGGL(k_gather_smooth, dim3((n + 255) / 256), dim3(256), 0, ev->dev->stream, ev->F, ev->npad,
+                     n, ev->perm, ev->cand_s);
+  return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+}
+
+int launch_smooth(ccake_event* ev, int mode)
+{
+  const int n = ev->n;
+  const double rc = 2.0 * ev->p.hT;
+  const double rc2 = rc * rc;
+  const double rcs = rc * (1.0 + 1e-12) + 1e-12;   // search radius with a rounding margin
+  const KernelW W = make_w(ev->p.hT);
+  const double kern0 = W.norm;                      // W(0) = norm
+  const dim3 grid((n + kTile - 1) / kTile), blk(kTile);
+  hipStream_t st = ev->dev->stream;
+  int32_t* bt = ev->I + (size_t)CCAKE_I_BTRACK * ev->npad;
+  if (mode == 0)
+    hipLaunchKernelGGL(k_smooth<0>, grid, blk, 0, st, ev->cand_s, ev->skey, ev->perm, ev->cell_off, ev->grid, n,
+                       rc2, rcs, W, kern0, ev->t, ev->F, ev->npad, bt, (unsigned long long*)ev->pair_count,
+                       nullptr, nullptr, nullptr);
+  else if (mode == 1)
+    hipLaunchKernelGGL(k_smooth<1>, grid, blk, 0, st, ev->cand_s, ev->skey, ev->perm, ev->cell_off, ev->grid, n,
+                       rc2, rcs, W, kern0, ev->t, ev->F, ev->npad, bt, (unsigned long long*)ev->pair_count,
+                       nullptr, nullptr, nullptr);
+  else
+    hipLaunchKernelGGL(k_smooth<2>, grid, blk, 0, st, ev->cand_s, ev->skey, ev->perm, ev->cell_off, ev->grid, n,
+                       rc2, rcs, W, kern0, ev->t, ev->F, ev->npad, bt, (unsigned long long*)ev->pair_count,
+                       nullptr, nullptr, nullptr);
+  return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+}
+
+// Parity export: counts are in btrack (sticky -1 rows are recounted here).
+int launch_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, size_t* n_out)
+{
+  const int n = ev->n;
+  hipStream_t st = ev->dev->stream;
+  // fresh counts through btrack, which is saved and restored around the count
+  std::vector<int32_t> saved_bt(n), cnt(n);
+  int32_t* bt = ev->I + (size_t)CCAKE_I_BTRACK * ev->npad;
+  CCAKE_HIP(hipMemcpyAsync(saved_bt.data(), bt, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  CCAKE_HIP(hipMemsetAsync(bt, 0, n * sizeof(int32_t), st));
+  const int st_count = launch_smooth(ev, 2);
+  if (st_count) return st_count;
+  CCAKE_HIP(hipMemcpyAsync(cnt.data(), bt, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  CCAKE_HIP(hipStreamSynchronize(st));
+  CCAKE_HIP(hipMemcpyAsync(bt, saved_bt.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  std::vector<int64_t> off(n + 1, 0);
+  for (int i = 0; i < n; ++i) off[i + 1] = off[i] + cnt[i];
+  const size_t total = (size_t)off[n];
+  *n_out = total;
+  if (total > cap) { CCAKE_HIP(hipStreamSynchronize(st)); return CCAKE_ERR_CAPACITY; }
+  int64_t* d_off = nullptr; int32_t* d_i = nullptr; int32_t* d_j = nullptr;
+  CCAKE_HIP(hipMalloc(&d_off, (n + 1) * sizeof(int64_t)));
+  CCAKE_HIP(hipMalloc(&d_i, (total + 1) * sizeof(int32_t)));
+  CCAKE_HIP(hipMalloc(&d_j, (total + 1) * sizeof(int32_t)));
+  CCAKE_HIP(hipMemcpyAsync(d_off, off.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  const double rc = 2.0 * ev->p.hT;
+  const KernelW W = make_w(ev->p.hT);
+  hipLaunchKernelGGL(k_smooth<3>, dim3((n + kTile - 1) / kTile), dim3(kTile), 0, st, ev->cand_s, ev->skey,
+                     ev->perm, ev->cell_off, ev->grid, n, rc * rc, rc * (1.0 + 1e-12) + 1e-12, W, W.norm, ev->t,
+                     ev->F, ev->npad, bt, (unsigned long long*)ev->pair_count, d_off, d_i, d_j);
+  CCAKE_HIP(hipGetLastError());
+  CCAKE_HIP(hipMemcpyAsync(pi, d_i, total * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  CCAKE_HIP(hipMemcpyAsync(pj, d_j, total * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  CCAKE_HIP(hipStreamSynchronize(st));
+  CCAKE_HIP(hipFree(d_off));
+  CCAKE_HIP(hipFree(d_i));
+  CCAKE_HIP(hipFree(d_j));
+  // canonical order: rows by i, j ascending inside a row
+  for (int i = 0; i < n; ++i) std::sort(pj + off[i], pj + off[i + 1]);
+  return 0;
+}
+
+}  // namespace ccake

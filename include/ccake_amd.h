@@ -1,0 +1,274 @@
+/*
+ * ccake_amd.h -- C-ABI of the MI355X-native SPH time-step library.
+ *
+ * Everything crossing this boundary is plain C: opaque handles, plain
+ * pointers, sizes and int status codes.  No C++ or device types.  The
+ * library owns every device buffer and the EoS tables; the caller owns
+ * host arrays.  Calls on one ccake_dev are stream ordered and asynchronous
+ * except *_download, *_reduce, *_sync and the query calls.
+ *
+ * Each entry point names the reference interface it replaces
+ * (the-nuclear-confectionery/CCAKE, file:line).  Units are the reference's
+ * internal units: lengths in fm, energy/pressure densities in fm^-4,
+ * entropy and charge densities in fm^-3, T and mu in fm^-1.
+ */
+#ifndef CCAKE_AMD_H
+#define CCAKE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(__GNUC__) || defined(__clang__)
+#define CCAKE_API __attribute__((visibility("default")))
+#else
+#define CCAKE_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Status codes.  The library never exits or aborts; the C++ host layer
+ * maps these to the reference behaviour:
+ *   CCAKE_ERR_NEG_ENTROPY     -> exit(EXIT_FAILURE)   src/evolver.cpp:437-450
+ *   CCAKE_ERR_EOS_NO_SOLUTION -> exit(404)            src/sph_workstation.cpp:1752-1760
+ *   CCAKE_ERR_EOS_OUT_OF_RANGE-> exit(8)              src/eos.cpp:154-159          */
+enum ccake_status {
+  CCAKE_OK = 0,
+  CCAKE_ERR_INVALID = 1,          /* bad argument / handle                      */
+  CCAKE_ERR_OOM = 2,              /* HBM budget or hipMalloc failure            */
+  CCAKE_ERR_HIP = 3,              /* HIP runtime error                          */
+  CCAKE_ERR_NEG_ENTROPY = 4,      /* s~ < 0 on a non-frozen particle            */
+  CCAKE_ERR_EOS_NO_SOLUTION = 5,  /* no EoS in the chain inverted the point     */
+  CCAKE_ERR_EOS_OUT_OF_RANGE = 6, /* seed point outside every EoS range         */
+  CCAKE_ERR_CAPACITY = 7,         /* output buffer too small (export_pairs)     */
+  CCAKE_ERR_UNSUPPORTED = 8       /* configuration not built in this library    */
+};
+
+/* ------------------------------------------------------------------ */
+/* Per-particle fields addressable by upload/download.  Vector fields use
+ * three consecutive ids (x, y, eta); D x D matrices nine (row major);
+ * pi^{mu nu} sixteen (row major, index 0 = tau).  The names follow the
+ * reference data model: include/particle.h:54-136,
+ * include/hydrodynamic_info.h:23-310, include/thermodynamic_info.h:18-70. */
+enum ccake_field {
+  CCAKE_F_X = 0, CCAKE_F_Y, CCAKE_F_ETA,
+  CCAKE_F_UX, CCAKE_F_UY, CCAKE_F_UETA,
+  CCAKE_F_GAMMA, CCAKE_F_VX, CCAKE_F_VY, CCAKE_F_VETA,
+  CCAKE_F_MASS_S, CCAKE_F_MASS_B, CCAKE_F_MASS_SS, CCAKE_F_MASS_Q,   /* sph_mass   */
+  CCAKE_F_EXT_S, CCAKE_F_EXT_B, CCAKE_F_EXT_SS, CCAKE_F_EXT_Q,       /* extensive  */
+  CCAKE_F_IN_E, CCAKE_F_IN_S, CCAKE_F_IN_B, CCAKE_F_IN_SS, CCAKE_F_IN_Q, /* input */
+  CCAKE_F_SM_S, CCAKE_F_SM_B, CCAKE_F_SM_SS, CCAKE_F_SM_Q,           /* smoothed   */
+  CCAKE_F_SIGMA_LAB, CCAKE_F_SIGMA,
+  /* thermodynamic_info, in the reference's thermo_info order */
+  CCAKE_F_T, CCAKE_F_MUB, CCAKE_F_MUS, CCAKE_F_MUQ,
+  CCAKE_F_E, CCAKE_F_S, CCAKE_F_RHOB, CCAKE_F_RHOS, CCAKE_F_RHOQ,
+  CCAKE_F_P, CCAKE_F_CS2, CCAKE_F_W,
+  CCAKE_F_DWDS, CCAKE_F_DWDB, CCAKE_F_DWDS_S, CCAKE_F_DWDQ,
+  CCAKE_F_DALPHA0,                       /* 12 dalpha_{B,S,Q}d{s,B,S,Q}      */
+  CCAKE_F_CHIBB = CCAKE_F_DALPHA0 + 12, CCAKE_F_CHIBB0, CCAKE_F_EOS_IS_TABLE,
+  /* viscous state */
+  CCAKE_F_BULK, CCAKE_F_EXT_BULK,
+  CCAKE_F_EXT_SHV0,                      /* 5: (11,12,13,22,23) extensive pi */
+  CCAKE_F_SHV0 = CCAKE_F_EXT_SHV0 + 5,   /* 16: pi^{mu nu}                   */
+  /* transport coefficients (src/sph_workstation.cpp:1631-1697 order) */
+  CCAKE_F_ETA_PI = CCAKE_F_SHV0 + 16, CCAKE_F_TAU_PI, CCAKE_F_ZETA_PI,
+  CCAKE_F_TAU_BULK, CCAKE_F_DELTA_BB, CCAKE_F_DELTA_PIPI, CCAKE_F_LAMBDA_PIB,
+  CCAKE_F_LAMBDA_BPI, CCAKE_F_PHI1, CCAKE_F_PHI3, CCAKE_F_PHI6, CCAKE_F_PHI7,
+  CCAKE_F_TAU_PIPI,
+  /* gradients (src/sph_workstation.cpp:796-986) */
+  CCAKE_F_GRADP0, CCAKE_F_GRADE0 = CCAKE_F_GRADP0 + 3,
+  CCAKE_F_GRADBULK0 = CCAKE_F_GRADE0 + 3, CCAKE_F_GRADSHEAR0 = CCAKE_F_GRADBULK0 + 3,
+  CCAKE_F_DIVSHEAR0 = CCAKE_F_GRADSHEAR0 + 3, CCAKE_F_GRADV0 = CCAKE_F_DIVSHEAR0 + 3,
+  /* time derivatives and EoM outputs (src/eom_default.cpp:211-869) */
+  CCAKE_F_DUDT0 = CCAKE_F_GRADV0 + 9,
+  CCAKE_F_DEXT_S = CCAKE_F_DUDT0 + 3, CCAKE_F_DEXT_B, CCAKE_F_DEXT_SS, CCAKE_F_DEXT_Q,
+  CCAKE_F_DEXT_BULK, CCAKE_F_DEXT_SHV0,
+  CCAKE_F_THETA = CCAKE_F_DEXT_SHV0 + 5, CCAKE_F_SHV_NABLA_U,
+  CCAKE_F_EZ, CCAKE_F_DEZ,               /* contribution_to_total_Ez / dEz   */
+  CCAKE_F_FRZ1_E, CCAKE_F_FRZ2_E,        /* freeze-out energy snapshots      */
+  CCAKE_NUM_FIELDS
+};
+
+/* Integer per-particle fields (upload_int / download_int). */
+enum ccake_ifield {
+  CCAKE_I_FREEZE = 0,   /* Freeze status 0,1,3,4 (include/freeze_out.h:455-542) */
+  CCAKE_I_BTRACK,       /* neighbour count, sticky -1 (src/system_state.cpp:712-715) */
+  CCAKE_I_EOS_ID,       /* index of the EoS in the chain this particle last used */
+  CCAKE_NUM_IFIELDS
+};
+
+/* EoS kinds; a chain is built as in src/eos_initialization.cpp:216-501. */
+enum ccake_eos_kind {
+  CCAKE_EOS_CONFORMAL = 0,   /* chain: conformal -> conformal_diagonal            */
+  CCAKE_EOS_TABLE = 1        /* chain: table -> tanh_conformal -> conformal -> conformal_diagonal */
+};
+
+/* Transport-coefficient modes (src/transport_coefficients.cpp:9-283). */
+enum ccake_eta_mode   { CCAKE_ETA_DEFAULT = 0, CCAKE_ETA_CONSTANT = 1 };
+enum ccake_taupi_mode { CCAKE_TAUPI_DEFAULT = 0, CCAKE_TAUPI_MINVAL = 1, CCAKE_TAUPI_GUBSER = 2 };
+enum ccake_zeta_mode  { CCAKE_ZETA_DEFAULT = 0, CCAKE_ZETA_CONSTANT = 1,
+                        CCAKE_ZETA_CS2_DEPENDENT = 2,
+                        CCAKE_ZETA_GAUSSIAN = 3 /* zeta/s = A exp(-(T-T0)^2/(2 w^2)); not in the reference */ };
+enum ccake_onoff_mode { CCAKE_MODE_DEFAULT = 0, CCAKE_MODE_DISABLED = 1, CCAKE_MODE_IS = 2 };
+
+/* Settings consumed by the time-step path (a POD restatement of the fields
+ * of include/settings.h the path reads). */
+typedef struct ccake_params {
+  int    dim;                 /* spatial dimension D; 2 is built            */
+  int    rk_order;            /* 2 (RK4 is CCAKE_ERR_UNSUPPORTED)            */
+  double hT, hEta;            /* smoothing lengths, fm                       */
+  double t0;                  /* tau_0, fm/c                                 */
+  double dt;                  /* fm/c                                        */
+  double e_freeze;            /* efcheck, fm^-4                              */
+  int    particlization;      /* freeze-out status machine on/off            */
+  int    using_shear;         /* shear evolution (eta/s > 1e-3)              */
+  int    using_vorticity;
+  int    input_initial_shear;
+  int    bulk_from_trace;
+  int    prohibit_unstable_cs2, prohibit_acausal_cs2;
+  /* transport */
+  int    eta_mode;            double constant_eta_over_s;
+  int    shear_relax_mode;
+  int    zeta_mode;           double constant_zeta_over_s;
+  double cs2_dependent_zeta_A, cs2_dependent_zeta_p;
+  int    modulate_zeta_with_tanh;
+  double zeta_gauss_A, zeta_gauss_T0, zeta_gauss_width;   /* fm^-1 */
+  int    delta_pipi_mode, tau_pipi_mode, lambda_piPi_mode, phi6_mode, phi7_mode;
+  int    delta_PiPi_mode, lambda_Pipi_mode, phi1_mode, phi3_mode;
+  /* neighbour search tuning: cells per interaction radius (1 = reference) */
+  int    cells_per_rc;
+} ccake_params;
+
+/* Description of an equation of state.  For CCAKE_EOS_TABLE the table is a
+ * uniform 4D grid over (T, muB, muQ, muS), T slowest, 17 fields per node in
+ * the order p, s, B, S, Q, e, cs2, chiBB, chiQQ, chiSS, chiBQ, chiBS, chiQS,
+ * chiTB, chiTQ, chiTS, chiTT (src/eos_table.cpp:30-34), already converted to
+ * fm units (no 1/T^n normalisation). */
+typedef struct ccake_eos_desc {
+  int    kind;
+  double T_freeze_out;        /* fm^-1; sets the fallback matching scale     */
+  int    dims[4];             /* nodes per axis (T, muB, muQ, muS)            */
+  double mins[4], maxs[4];    /* axis ranges, fm^-1                           */
+  const double* table;        /* dims[0]*dims[1]*dims[2]*dims[3]*17 doubles   */
+} ccake_eos_desc;
+
+typedef struct ccake_dev   ccake_dev;
+typedef struct ccake_eos   ccake_eos;
+typedef struct ccake_event ccake_event;
+
+/* ---------------------------- lifetime ---------------------------- */
+/* Device context: one HIP device, one stream, an HBM budget (0 = all).
+ * Replaces the Kokkos DeviceType/ExecutionSpace typedefs
+ * (include/utilities.h:42-55). */
+CCAKE_API int  ccake_dev_create(int gpu, size_t hbm_budget_bytes, ccake_dev** out);
+CCAKE_API int  ccake_dev_destroy(ccake_dev* dev);
+CCAKE_API int  ccake_dev_sync(ccake_dev* dev);
+/* Bytes of HBM the library holds on this device, and its budget. */
+CCAKE_API int  ccake_dev_memory(ccake_dev* dev, size_t* used, size_t* budget);
+CCAKE_API const char* ccake_last_error(void);
+
+/* EoS chain on the device.  Replaces EquationOfState::init /
+ * set_up_chosen_EOSs (src/eos_initialization.cpp:216-501). */
+CCAKE_API int  ccake_eos_create(ccake_dev* dev, const ccake_eos_desc* desc, ccake_eos** out);
+CCAKE_API int  ccake_eos_destroy(ccake_eos* eos);
+/* e(T_FO, mu=0) of the chain's default EoS: EquationOfState::efreeze
+ * (src/eos.cpp, used at src/sph_workstation.cpp:71). */
+CCAKE_API int  ccake_eos_efreeze(ccake_eos* eos, double T_freeze_out, double* e_out);
+/* Host evaluation of the 17 forward fields of chain member `member` at
+ * (T, muB, muQ, muS): EoS_base::get_full_thermo (include/eos_base.h). */
+CCAKE_API int  ccake_eos_full_thermo(ccake_eos* eos, int member, const double tbqs[4], double out17[17]);
+
+/* One event (one SystemState).  Replaces SystemState::allocate_cabana_particles
+ * (src/system_state.cpp:57-63) and Evolver::allocate_cache (include/evolver.h). */
+CCAKE_API int  ccake_event_create(ccake_dev* dev, const ccake_params* params, ccake_eos* eos,
+                        int n_particles, ccake_event** out);
+CCAKE_API int  ccake_event_destroy(ccake_event* ev);
+CCAKE_API int  ccake_event_size(ccake_event* ev, int* n_particles);
+CCAKE_API int  ccake_event_set_time(ccake_event* ev, double t);        /* SystemState::t */
+CCAKE_API int  ccake_event_get_time(ccake_event* ev, double* t);
+
+/* Host <-> device copies of one field: SystemState::copy_host_to_device /
+ * copy_device_to_host (src/system_state.cpp:71-628), field by field. */
+CCAKE_API int  ccake_event_upload(ccake_event* ev, int field, const double* host, size_t n);
+CCAKE_API int  ccake_event_download(ccake_event* ev, int field, double* host, size_t n);
+CCAKE_API int  ccake_event_upload_int(ccake_event* ev, int ifield, const int32_t* host, size_t n);
+CCAKE_API int  ccake_event_download_int(ccake_event* ev, int ifield, int32_t* host, size_t n);
+
+/* ---------------------------- phases ------------------------------ */
+/* 01: SystemState::reset_neighbour_list (src/system_state.cpp:650-719).
+ * Cell sort of the current positions; btrack = neighbour count. */
+CCAKE_API int  ccake_build_neighbors(ccake_event* ev);
+/* Materialise the pair set (i, j), i != j, |r_i - r_j|^2 <= (2 hT)^2, rows
+ * ordered by i then j ascending (parity checks only; never on the hot path).
+ * Replaces reading Cabana::NeighborList (include/utilities.h:57-62). */
+CCAKE_API int  ccake_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, size_t* n_pairs);
+/* 02: EoM_default::update_velocity (src/eom_default.cpp:17-38). */
+CCAKE_API int  ccake_gamma_velocity(ccake_event* ev);
+/* 03: SPHWorkstation::smooth_all_particle_fields (src/sph_workstation.cpp:494-578). */
+CCAKE_API int  ccake_smooth_fields(ccake_event* ev);
+/* SPHWorkstation::calculate_intial_sigma (src/sph_workstation.cpp:582-640). */
+CCAKE_API int  ccake_initial_sigma(ccake_event* ev);
+/* 04: SPHWorkstation::update_all_particle_thermodynamics (1713-1772):
+ * mode 0 inverts (s, rhoB, rhoS, rhoQ) from the thermo fields;
+ * mode 1 inverts (e, rhoB, rhoS, rhoQ) from the input fields and writes
+ * input.s (locate_phase_diagram_point_eBSQ, 1791-1851). */
+CCAKE_API int  ccake_eos_invert(ccake_event* ev, int mode);
+/* 05: EoM_default::reset_pi_tensor (src/eom_default.cpp:56-172). */
+CCAKE_API int  ccake_reset_pi_tensor(ccake_event* ev);
+/* 08: SPHWorkstation::update_all_particle_viscosities (1622-1699). */
+CCAKE_API int  ccake_viscosities(ccake_event* ev);
+/* 09: SPHWorkstation::smooth_all_particle_gradients (796-986). */
+CCAKE_API int  ccake_smooth_gradients(ccake_event* ev);
+/* 10: EoM_default::evaluate_time_derivatives (src/eom_default.cpp:211-869). */
+CCAKE_API int  ccake_eom(ccake_event* ev);
+/* SPHWorkstation::get_time_derivatives (src/sph_workstation.cpp:1509-1612):
+ * phases 01-10 in order. */
+CCAKE_API int  ccake_get_time_derivatives(ccake_event* ev);
+/* Evolver::set_current_timestep_quantities (src/evolver.cpp:124-168). */
+CCAKE_API int  ccake_cache_state(ccake_event* ev);
+/* Evolver::step_rk update part (src/evolver.cpp:363-484): y = y0 + dt*f(y),
+ * t = t0 + dt.  Flags a negative extensive entropy on a non-frozen
+ * particle; the status is returned by ccake_event_check. */
+CCAKE_API int  ccake_rk_update(ccake_event* ev, double dt, double t0);
+/* SPHWorkstation::advance_timestep (src/sph_workstation.cpp:2042-2073) with
+ * Evolver::advance_timestep_rk2 (src/evolver.cpp:493-517) and, when
+ * particlization is on, freeze_out_particles (1474-1485). */
+CCAKE_API int  ccake_advance_timestep(ccake_event* ev, double dt, int rk_order);
+/* Run `n_steps` timesteps; stops early when continue_evolution is false. */
+CCAKE_API int  ccake_run_steps(ccake_event* ev, double dt, int rk_order, int n_steps, int* steps_done);
+/* Initialisation once per event, after upload of positions, u, input e
+ * (or s) and charges, pi and Pi (BSQHydro::initialize_hydrodynamics,
+ * src/BSQHydro.cpp:568-646): process_initial_conditions,
+ * initialize_entropy_and_charge_densities, initial_smoothing. */
+CCAKE_API int  ccake_initialize(ccake_event* ev);
+/* FreezeOut::check_freeze_out_status + status transitions of bsqsvfreezeout
+ * (include/freeze_out.h:322-542, 549-626). */
+CCAKE_API int  ccake_freeze_out_particles(ccake_event* ev);
+/* SPHWorkstation::continue_evolution (src/sph_workstation.cpp:1992-2024). */
+CCAKE_API int  ccake_continue_evolution(ccake_event* ev, double max_tau, int* keep_going);
+/* Sticky device-side error flags raised by kernels since the last call
+ * (negative entropy, EoS failure); clears them. */
+CCAKE_API int  ccake_event_check(ccake_event* ev);
+
+/* Device reductions: SystemState::conservation_entropy / _BSQ / _energy
+ * (src/system_state.cpp:738-885) and the frozen-out count. */
+enum ccake_reduction { CCAKE_R_ENTROPY = 0, CCAKE_R_BARYON, CCAKE_R_STRANGE, CCAKE_R_CHARGE,
+                       CCAKE_R_ENERGY, CCAKE_R_EZ, CCAKE_R_FROZEN, CCAKE_NUM_REDUCTIONS };
+CCAKE_API int  ccake_reduce(ccake_event* ev, int which, double* out);
+
+/* --------------------------- profiling ----------------------------- */
+/* Per-kernel HIP-event timing on the library's stream (off by default). */
+enum ccake_kernel_id { CCAKE_K_BIN = 0, CCAKE_K_SORT, CCAKE_K_SMOOTH, CCAKE_K_EOS,
+                       CCAKE_K_GRAD, CCAKE_K_EOM, CCAKE_K_RK, CCAKE_K_MISC, CCAKE_NUM_KERNELS };
+CCAKE_API int  ccake_profile_enable(ccake_dev* dev, int enable);
+CCAKE_API int  ccake_profile_read(ccake_dev* dev, int kernel_id, double* total_ms, int64_t* launches);
+CCAKE_API int  ccake_profile_reset(ccake_dev* dev);
+/* Pair count of the last neighbour build (directed pairs, i != j). */
+CCAKE_API int  ccake_last_pair_count(ccake_event* ev, int64_t* n_pairs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CCAKE_AMD_H */

@@ -1,0 +1,77 @@
+/* Internal declarations of the CPU oracle (test infrastructure only). */
+#ifndef ORC_INTERNAL_H
+#define ORC_INTERNAL_H
+#include "ccake_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* include/constants.h:15-16 */
+#define ORC_HBARC_MEVFM 197.3269804
+#define ORC_HBARC_GEVFM 0.1973269804
+#define ORC_PI 3.1415926535897932384626433
+/* include/eos_header.h:16-17, include/rootfinder.h:36-37 */
+#define ORC_TINY 1e-25
+#define ORC_TBQS_INF 1e10
+#define ORC_RF_TOL 1e-10
+
+int orc_nthreads(void);
+
+/* -------- EoS chain members (src/eos_initialization.cpp:216-501) -------- */
+enum { ORC_M_TABLE = 0, ORC_M_TANH, ORC_M_CONF, ORC_M_DIAG };
+
+typedef struct orc_member {
+  int kind;
+  double c, T0, muB0, muS0, muQ0, Tc, Ts;
+  double mins[4], maxs[4];            /* tbqs_minima / tbqs_maxima           */
+  /* table member */
+  int dims[4];
+  double gmin[4], gmax[4], gdel[4];
+  double* tab;                        /* node-major, 17 fields per node      */
+} orc_member;
+
+struct orc_eos {
+  int kind;
+  int n;
+  orc_member m[4];
+};
+
+/* chi indices inside the 17-field thermo vector (src/eos_table.cpp:30-34) */
+enum { X_P = 0, X_S, X_B, X_SS, X_Q, X_E, X_CS2, X_BB, X_QQ, X_SSS, X_BQ, X_BS,
+       X_QS, X_TB, X_TQ, X_TS, X_TT };
+
+void orc_member_full(const orc_member* m, const double x[4], double r[17]);
+
+/* -------- events -------- */
+struct orc_event {
+  orc_params p;
+  orc_eos* eos;
+  int n;
+  double t;
+  double* F[ORC_NUM_FIELDS];
+  int32_t* I[ORC_NUM_IFIELDS];
+  /* RK cache (src/evolver.cpp:124-168) */
+  double* c_u[3];
+  double* c_r[3];
+  double* c_s, *c_B, *c_S, *c_Q, *c_bulk, *c_E0;
+  double* c_shv[5];
+  /* neighbour list (CSR, j ascending) */
+  int64_t* nb_off;
+  int32_t* nb_idx;
+  int64_t nb_cap;
+  int64_t n_pairs;
+  /* freeze-out bookkeeping (include/freeze_out.h:165-222) */
+  int frzc;
+  double* fback_e[4];    /* fback, fback2, fback3, fback4 energy snapshots */
+  double* fback_gradP0[4];
+  double* frz1_gradP0, *frz2_gradP0;
+  int number_part_fo;
+};
+
+#define FLD(ev, f) ((ev)->F[(f)])
+
+#endif
