@@ -141,7 +141,7 @@ int launch_neighbor_sort(ccake_event* ev);          // grid, keys, radix sort, c
 int launch_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, size_t* n);
 // smoothing
 int launch_gather_smooth(ccake_event* ev);
-int launch_smooth(ccake_event* ev, int mode);       // mode 0 fields, 1 initial sigma
+int launch_smooth(ccake_event* ev, int mode, bool count_pairs = true);  // mode 0 fields, 1 initial sigma, 2 counts
 // EoS + per-particle physics
 int launch_eos(ccake_event* ev, int mode, int fuse_post);   // fuse_post: reset_pi + transport + gradient gather
 int launch_reset_pi(ccake_event* ev);

@@ -104,7 +104,7 @@ __global__ __launch_bounds__(64) void k_smooth(const double* __restrict__ cand,
     // pair total for the roofline: one atomic per wave
     unsigned long long tot = (unsigned long long)cnt;
     for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
-    if (lane == 0) atomicAdd(pair_total, tot);
+    if (lane == 0 && pair_total) atomicAdd(pair_total, tot);
     const int b = btrack[id];
     btrack[id] = (b == -1) ? -1 : cnt;
   }
@@ -142,7 +142,7 @@ int launch_gather_smooth(ccake_event* ev)
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
 
-int launch_smooth(ccake_event* ev, int mode)
+int launch_smooth(ccake_event* ev, int mode, bool count_pairs)
 {
   const int n = ev->n;
   const double rc = 2.0 * ev->p.hT;
@@ -153,18 +153,16 @@ int launch_smooth(ccake_event* ev, int mode)
   const dim3 grid((n + kTile - 1) / kTile), blk(kTile);
   hipStream_t st = ev->dev->stream;
   int32_t* bt = ev->I + (size_t)CCAKE_I_BTRACK * ev->npad;
+  unsigned long long* pc = count_pairs ? (unsigned long long*)ev->pair_count : nullptr;
   if (mode == 0)
     hipLaunchKernelGGL(k_smooth<0>, grid, blk, 0, st, ev->cand_s, ev->skey, ev->perm, ev->cell_off, ev->grid, n,
-                       rc2, rcs, W, kern0, ev->t, ev->F, ev->npad, bt, (unsigned long long*)ev->pair_count,
-                       nullptr, nullptr, nullptr);
+                       rc2, rcs, W, kern0, ev->t, ev->F, ev->npad, bt, pc, nullptr, nullptr, nullptr);
   else if (mode == 1)
     hipLaunchKernelGGL(k_smooth<1>, grid, blk, 0, st, ev->cand_s, ev->skey, ev->perm, ev->cell_off, ev->grid, n,
-                       rc2, rcs, W, kern0, ev->t, ev->F, ev->npad, bt, (unsigned long long*)ev->pair_count,
-                       nullptr, nullptr, nullptr);
+                       rc2, rcs, W, kern0, ev->t, ev->F, ev->npad, bt, pc, nullptr, nullptr, nullptr);
   else
     hipLaunchKernelGGL(k_smooth<2>, grid, blk, 0, st, ev->cand_s, ev->skey, ev->perm, ev->cell_off, ev->grid, n,
-                       rc2, rcs, W, kern0, ev->t, ev->F, ev->npad, bt, (unsigned long long*)ev->pair_count,
-                       nullptr, nullptr, nullptr);
+                       rc2, rcs, W, kern0, ev->t, ev->F, ev->npad, bt, pc, nullptr, nullptr, nullptr);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
 
@@ -178,7 +176,7 @@ int launch_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, s
   int32_t* bt = ev->I + (size_t)CCAKE_I_BTRACK * ev->npad;
   CCAKE_HIP(hipMemcpyAsync(saved_bt.data(), bt, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
   CCAKE_HIP(hipMemsetAsync(bt, 0, n * sizeof(int32_t), st));
-  const int st_count = launch_smooth(ev, 2);
+  const int st_count = launch_smooth(ev, 2, false);   // leaves the stage's pair total alone
   if (st_count) return st_count;
   CCAKE_HIP(hipMemcpyAsync(cnt.data(), bt, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
   CCAKE_HIP(hipStreamSynchronize(st));
@@ -197,7 +195,7 @@ int launch_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, s
   const KernelW W = make_w(ev->p.hT);
   hipLaunchKernelGGL(k_smooth<3>, dim3((n + kTile - 1) / kTile), dim3(kTile), 0, st, ev->cand_s, ev->skey,
                      ev->perm, ev->cell_off, ev->grid, n, rc * rc, rc * (1.0 + 1e-12) + 1e-12, W, W.norm, ev->t,
-                     ev->F, ev->npad, bt, (unsigned long long*)ev->pair_count, d_off, d_i, d_j);
+                     ev->F, ev->npad, bt, nullptr, d_off, d_i, d_j);
   CCAKE_HIP(hipGetLastError());
   CCAKE_HIP(hipMemcpyAsync(pi, d_i, total * sizeof(int32_t), hipMemcpyDeviceToHost, st));
   CCAKE_HIP(hipMemcpyAsync(pj, d_j, total * sizeof(int32_t), hipMemcpyDeviceToHost, st));
