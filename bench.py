@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--workload", choices=["gubser", "lumpy40k"], default="gubser")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle sample")
     ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cells-per-rc", type=int, default=3, help="neighbour-grid cells per cutoff radius")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one plain 256 MiB device copy (reference kernel for HBM byte counters)")
     return ap.parse_args()
 
 
@@ -109,6 +112,8 @@ def main():
     from ccake_amd import config
 
     d, cfg, table, wl = workload(args.workload)
+    cfg = dict(cfg, cells_per_rc=args.cells_per_rc)
+    wl["cells_per_rc"] = args.cells_per_rc
     n = len(d["x"])
     T_FO = 150.0 / config.HBARC_MEVFM
     print(f"bench: {wl['workload']}: {n} particles, warmup {args.warmup}, steps {args.steps}",
@@ -191,6 +196,15 @@ def main():
         "stage_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items() if v[1]},
     }
     ev.close(); eos.close(); dev.close()
+    if args.calibrate_copy:
+        # a copy of known size, whose FETCH_SIZE / WRITE_SIZE readings scale the
+        # counters of the path's kernels to bytes
+        import torch
+        src = torch.ones(1 << 25, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = src.numel() * 8
     if not args.no_cpu:
         print("bench: timing the CPU oracle sample", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(d, cfg, table, args.cpu_steps)

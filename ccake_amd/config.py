@@ -36,7 +36,7 @@ def base_params(**kw):
         phi6_mode=MODE_DISABLED, phi7_mode=MODE_DISABLED,
         delta_PiPi_mode=MODE_IS, lambda_Pipi_mode=MODE_DISABLED, phi1_mode=MODE_DISABLED,
         phi3_mode=MODE_DISABLED,
-        cells_per_rc=2,
+        cells_per_rc=3,
     )
     p.update(kw)
     # input.cpp:390-407: eta/s <= 1e-3 disables shear evolution

@@ -20,29 +20,37 @@ constexpr int kCandG = 16;
 
 struct GradW {
   double hinv, ng, ng2;   // kgrad/h^3, kgrad2/h^3 (src/kernel.cpp:95-118)
-  // scalar g with grad W = g * rel
+  // scalar g with grad W = g * rel, for r2 > 0 (coincident particles sit at
+  // q = 0, on the inner branch, which does not divide by r)
   __device__ __forceinline__ double operator()(double r2) const
   {
-    const double r = sqrt(r2);
+    double r, ir;
+    sqrt_rsqrt(r2, r, ir);
     const double q = r * hinv;
     const double t = 2.0 - q;
-    const double far = (ng / r) * t * t;
+    const double far = ng * ir * t * t;
     const double near = ng2 * (1.5 * q - 2.0) * hinv;
-    return q >= 2.0 ? 0.0 : (q >= 1.0 ? far : near);
+    const double v = (q >= 1.0) ? far : near;
+    return (q >= 2.0) ? 0.0 : v;
   }
 };
 
+// One 64-particle tile per workgroup of kSplit waves; each wave walks a
+// slice of the tile's candidates and wave 0 adds the slices in wave order.
 template <bool SHEAR>
-__global__ __launch_bounds__(64) void k_grad(const double* __restrict__ cand,
-                                             const int32_t* __restrict__ skey,
-                                             const int32_t* __restrict__ perm,
-                                             const int32_t* __restrict__ cell_off,
-                                             const Grid* __restrict__ gridp, int n,
-                                             double rc2, double rcs, GradW G,
-                                             double* __restrict__ F, int npad)
+__global__ __launch_bounds__(kSplit * kWave) void k_grad(const double* __restrict__ cand,
+                                                         const int32_t* __restrict__ skey,
+                                                         const int32_t* __restrict__ perm,
+                                                         const int32_t* __restrict__ cell_off,
+                                                         const Grid* __restrict__ gridp, int n,
+                                                         double rc2, double rcs, GradW G,
+                                                         double* __restrict__ F, int npad)
 {
+  constexpr int NACC = SHEAR ? 17 : 12;
+  __shared__ double red[kSplit - 1][NACC][kWave];
   const Grid g = *gridp;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = uni((int)(threadIdx.x / kWave));   // wave-uniform
   const int k = blockIdx.x * kTile + lane;
   const bool act = k < n;
   const int kk = act ? k : 0;
@@ -51,35 +59,62 @@ __global__ __launch_bounds__(64) void k_grad(const double* __restrict__ cand,
   const double vax = me[6], vay = me[7];
   const int myrow = act ? skey[kk] / g.nc[0] : INT_MAX;
 
-  double a1x = 0, a1y = 0, apx = 0, apy = 0, aex = 0, aey = 0, abx = 0, aby = 0;
-  double vxx = 0, vxy = 0, vyx = 0, vyy = 0;
-  double gsx = 0, gsy = 0, acv = 0, dsx = 0, dsy = 0;
+  double acc[NACC];
+  for (int q = 0; q < NACC; ++q) acc[q] = 0.0;
 
   for_segments(act, myrow, xa, ya, [&](const Seg& seg, bool in) {
-    for_spans(g, cell_off, seg, rcs, [&](int j) {
-      const double* c = cand + (size_t)j * kCandG;
-      const double dx = xa - c[0], dy = ya - c[1];
-      const double r2 = dist2_nocontract(dx, dy);
-      const bool nb = in && (r2 <= rc2) && (j != k);
-      const double gg = nb ? G(r2) : 0.0;
-      const double wx = c[2] * (gg * dx), wy = c[2] * (gg * dy);
-      a1x += wx; a1y += wy;
-      apx += c[3] * wx; apy += c[3] * wy;
-      aex += c[4] * wx; aey += c[4] * wy;
-      abx += c[5] * wx; aby += c[5] * wy;
-      vxx += c[6] * wx; vxy += c[7] * wx;
-      vyx += c[6] * wy; vyy += c[7] * wy;
-      if (SHEAR) {
-        const double cv = wx * vax + wy * vay;
-        acv += cv;
-        gsx += cv * c[8]; gsy += cv * c[9];
-        dsx += wx * c[10] + wy * c[11];
-        dsy += wx * c[11] + wy * c[12];
+    for_spans_slice(g, cell_off, seg, rcs, wave, [&](int j0, int j1) {
+      // Candidate records are wave-uniform (scalar loads into SGPRs); two
+      // records are requested together so the second arrives while the
+      // first is being used.
+      constexpr int NR = SHEAR ? 13 : 8;
+      auto one = [&](const double* c, int j) {
+        const double dx = xa - c[0], dy = ya - c[1];
+        const double r2 = dist2_nocontract(dx, dy);
+        const bool nb = in && (r2 <= rc2) && (j != k);
+        if (kSkipEmpty && __ballot(nb) == 0ull) return;
+        const double gv = G(r2);
+        const double gg = nb ? gv : 0.0;
+        const double wx = c[2] * (gg * dx), wy = c[2] * (gg * dy);
+        acc[0] += wx; acc[1] += wy;                    // sum m_b grad W
+        acc[2] += c[3] * wx; acc[3] += c[3] * wy;      // p_b / sl_b^2
+        acc[4] += c[4] * wx; acc[5] += c[4] * wy;      // e_b
+        acc[6] += c[5] * wx; acc[7] += c[5] * wy;      // Pi_b / sl_b^2
+        acc[8] += c[6] * wx; acc[9] += c[7] * wx;      // v_b d_x W
+        acc[10] += c[6] * wy; acc[11] += c[7] * wy;    // v_b d_y W
+        if (SHEAR) {
+          const double cv = wx * vax + wy * vay;
+          acc[12] += cv;
+          acc[13] += cv * c[SHEAR ? 8 : 0]; acc[14] += cv * c[SHEAR ? 9 : 0];
+          acc[15] += wx * c[SHEAR ? 10 : 0] + wy * c[SHEAR ? 11 : 0];
+          acc[16] += wx * c[SHEAR ? 11 : 0] + wy * c[SHEAR ? 12 : 0];
+        }
+      };
+      int j = j0;
+      for (; j + 1 < j1; j += 2) {
+        double ca[NR], cb[NR];
+        for (int q = 0; q < NR; ++q) ca[q] = cand[(size_t)j * kCandG + q];
+        for (int q = 0; q < NR; ++q) cb[q] = cand[(size_t)(j + 1) * kCandG + q];
+        one(ca, j);
+        one(cb, j + 1);
+      }
+      if (j < j1) {
+        double ca[NR];
+        for (int q = 0; q < NR; ++q) ca[q] = cand[(size_t)j * kCandG + q];
+        one(ca, j);
       }
     });
   });
 
-  if (!act) return;
+  if (wave > 0)
+    for (int q = 0; q < NACC; ++q) red[wave - 1][q][lane] = acc[q];
+  __syncthreads();
+  if (wave != 0 || !act) return;
+  for (int w = 0; w < kSplit - 1; ++w)
+    for (int q = 0; q < NACC; ++q) acc[q] += red[w][q][lane];
+
+  const double a1x = acc[0], a1y = acc[1], apx = acc[2], apy = acc[3], aex = acc[4], aey = acc[5];
+  const double abx = acc[6], aby = acc[7], vxx = acc[8], vxy = acc[9], vyx = acc[10], vyy = acc[11];
   const int id = perm[kk];
   const double sla = me[13], isla = 1.0 / sla;
   auto put = [&](int c, double v) { F[(size_t)c * npad + id] = v; };
@@ -100,6 +135,8 @@ __global__ __launch_bounds__(64) void k_grad(const double* __restrict__ cand,
   put(CCAKE_F_GRADV0 + 5, 0.0);
   put(CCAKE_F_GRADV0 + 6, 0.0); put(CCAKE_F_GRADV0 + 7, 0.0); put(CCAKE_F_GRADV0 + 8, 0.0);
   if (SHEAR) {
+    const double acv = acc[SHEAR ? 12 : 0], gsx = acc[SHEAR ? 13 : 0], gsy = acc[SHEAR ? 14 : 0];
+    const double dsx = acc[SHEAR ? 15 : 0], dsy = acc[SHEAR ? 16 : 0];
     put(CCAKE_F_GRADSHEAR0 + 0, sla * (gsx + me[8] * acv));
     put(CCAKE_F_GRADSHEAR0 + 1, sla * (gsy + me[9] * acv));
     put(CCAKE_F_DIVSHEAR0 + 0, sla * (dsx + me[10] * a1x + me[11] * a1y));
@@ -121,7 +158,7 @@ int launch_grad(ccake_event* ev)
   G.hinv = 1.0 / h;
   G.ng = -15.0 / (14.0 * kPi) / pow(h, 3);
   G.ng2 = 15.0 / (7.0 * kPi) / pow(h, 3);
-  const dim3 grid((n + kTile - 1) / kTile), blk(kTile);
+  const dim3 grid((n + kTile - 1) / kTile), blk(kSplit * kWave);
   const double rcs = rc * (1.0 + 1e-12) + 1e-12;
   if (ev->p.using_shear)
     hipLaunchKernelGGL(k_grad<true>, grid, blk, 0, ev->dev->stream, ev->cand_g, ev->skey, ev->perm, ev->cell_off,

@@ -36,14 +36,21 @@ __global__ __launch_bounds__(256) void k_gather_smooth(const double* __restrict_
 struct KernelW {
   double hinv, norm, norm2;
   // Cubic spline W(r, h) (src/kernel.cpp:34-48) from r^2.
-  __device__ __forceinline__ double operator()(double r2) const
+  __device__ __forceinline__ double of_q(double q) const
   {
-    const double q = sqrt(r2) * hinv;
     const double qq = q * q;
     const double a = norm * (1.0 - 1.5 * qq + 0.75 * q * qq);
     const double t = 2.0 - q;
     const double b = norm2 * t * t * t;
-    return q >= 2.0 ? 0.0 : (q >= 1.0 ? b : a);
+    const double v = (q >= 1.0) ? b : a;
+    return (q >= 2.0) ? 0.0 : v;
+  }
+  __device__ __forceinline__ double operator()(double r2) const { return of_q(sqrt(r2) * hinv); }
+  __device__ __forceinline__ double fast(double r2) const
+  {
+    double r, ir;
+    sqrt_rsqrt(r2, r, ir);
+    return of_q(r * hinv);
   }
 };
 
@@ -125,6 +132,113 @@ __global__ __launch_bounds__(64) void k_smooth(const double* __restrict__ cand,
   }
 }
 
+// Hot smoothing pass (MODE 0: fields + LRF + btrack + pair total; MODE 1:
+// initial sigma): one 64-particle tile per workgroup of kSplit waves, each
+// wave walking a slice of the tile's candidates.  Wave 0 starts from the
+// self terms (src/sph_workstation.cpp:514-523) and adds the other waves'
+// partial sums in wave order.
+template <int MODE>
+__global__ __launch_bounds__(kSplit * kWave) void k_smooth_tile(const double* __restrict__ cand,
+                                                                const int32_t* __restrict__ skey,
+                                                                const int32_t* __restrict__ perm,
+                                                                const int32_t* __restrict__ cell_off,
+                                                                const Grid* __restrict__ gridp, int n,
+                                                                double rc2, double rcs, KernelW W, double kern0,
+                                                                double t, double* __restrict__ F, int npad,
+                                                                int32_t* __restrict__ btrack,
+                                                                unsigned long long* __restrict__ pair_total)
+{
+  constexpr int NACC = 6;
+  __shared__ double red[kSplit - 1][NACC][kWave];
+  const Grid g = *gridp;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = uni((int)(threadIdx.x / kWave));   // wave-uniform
+  const int k = blockIdx.x * kTile + lane;
+  const bool act = k < n;
+  const int kk = act ? k : 0;
+  const double* me = cand + (size_t)kk * kCandS;
+  const double xi = me[0], yi = me[1];
+  const int myrow = act ? skey[kk] / g.nc[0] : INT_MAX;
+
+  const double self = (wave == 0) ? kern0 : 0.0;
+  double sl = me[2] * self;
+  double ss = me[3] * self, sB = me[4] * self, sS = me[5] * self, sQ = me[6] * self;
+  int cnt = 0;
+
+  for_segments(act, myrow, xi, yi, [&](const Seg& seg, bool in) {
+    for_spans_slice(g, cell_off, seg, rcs, wave, [&](int j0, int j1) {
+      // wave-uniform candidate records (scalar loads into SGPRs), two
+      // requested together
+      constexpr int NR = (MODE == 0) ? 7 : 3;
+      auto one = [&](const double* c, int j) {
+        const double dx = xi - c[0], dy = yi - c[1];
+        const double r2 = dist2_nocontract(dx, dy);
+        const bool nb = in && (r2 <= rc2) && (j != k);
+        if (kSkipEmpty && __ballot(nb) == 0ull) return;
+        const double wv = W.fast(r2);
+        const double w = nb ? wv : 0.0;
+        cnt += nb;
+        sl += c[2] * w;
+        if (MODE == 0) {
+          ss += c[MODE == 0 ? 3 : 0] * w; sB += c[MODE == 0 ? 4 : 0] * w;
+          sS += c[MODE == 0 ? 5 : 0] * w; sQ += c[MODE == 0 ? 6 : 0] * w;
+        }
+      };
+      int j = j0;
+      for (; j + 1 < j1; j += 2) {
+        double ca[NR], cb[NR];
+        for (int q = 0; q < NR; ++q) ca[q] = cand[(size_t)j * kCandS + q];
+        for (int q = 0; q < NR; ++q) cb[q] = cand[(size_t)(j + 1) * kCandS + q];
+        one(ca, j);
+        one(cb, j + 1);
+      }
+      if (j < j1) {
+        double ca[NR];
+        for (int q = 0; q < NR; ++q) ca[q] = cand[(size_t)j * kCandS + q];
+        one(ca, j);
+      }
+    });
+  });
+
+  if (wave > 0) {
+    double* r = &red[wave - 1][0][0];
+    r[0 * kWave + lane] = sl; r[1 * kWave + lane] = ss; r[2 * kWave + lane] = sB;
+    r[3 * kWave + lane] = sS; r[4 * kWave + lane] = sQ; r[5 * kWave + lane] = (double)cnt;
+  }
+  __syncthreads();
+  if (wave != 0) return;
+  for (int w = 0; w < kSplit - 1; ++w) {
+    sl += red[w][0][lane]; ss += red[w][1][lane]; sB += red[w][2][lane];
+    sS += red[w][3][lane]; sQ += red[w][4][lane]; cnt += (int)red[w][5][lane];
+  }
+  if (MODE == 0) {
+    // pair total for the roofline: one atomic per tile
+    unsigned long long tot = act ? (unsigned long long)cnt : 0ull;
+    for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
+    if (lane == 0 && pair_total) atomicAdd(pair_total, tot);
+  }
+  if (!act) return;
+  const int id = perm[kk];
+  if (MODE == 0) {
+    const int b = btrack[id];
+    btrack[id] = (b == -1) ? -1 : cnt;
+  }
+  const double gam = F[(size_t)CCAKE_F_GAMMA * npad + id];
+  F[(size_t)CCAKE_F_SIGMA_LAB * npad + id] = sl;
+  // get_LRF = lab/gamma/t (src/eom_default.cpp:198-203)
+  F[(size_t)CCAKE_F_SIGMA * npad + id] = sl / gam / t;
+  if (MODE == 0) {
+    F[(size_t)CCAKE_F_SM_S * npad + id] = ss;
+    F[(size_t)CCAKE_F_SM_B * npad + id] = sB;
+    F[(size_t)CCAKE_F_SM_SS * npad + id] = sS;
+    F[(size_t)CCAKE_F_SM_Q * npad + id] = sQ;
+    F[(size_t)CCAKE_F_S * npad + id] = ss / gam / t;
+    F[(size_t)CCAKE_F_RHOB * npad + id] = sB / gam / t;
+    F[(size_t)CCAKE_F_RHOS * npad + id] = sS / gam / t;
+    F[(size_t)CCAKE_F_RHOQ * npad + id] = sQ / gam / t;
+  }
+}
+
 static KernelW make_w(double hT)
 {
   KernelW w;
@@ -154,12 +268,13 @@ int launch_smooth(ccake_event* ev, int mode, bool count_pairs)
   hipStream_t st = ev->dev->stream;
   int32_t* bt = ev->I + (size_t)CCAKE_I_BTRACK * ev->npad;
   unsigned long long* pc = count_pairs ? (unsigned long long*)ev->pair_count : nullptr;
+  const dim3 tblk(kSplit * kWave);
   if (mode == 0)
-    hipLaunchKernelGGL(k_smooth<0>, grid, blk, 0, st, ev->cand_s, ev->skey, ev->perm, ev->cell_off, ev->grid, n,
-                       rc2, rcs, W, kern0, ev->t, ev->F, ev->npad, bt, pc, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_smooth_tile<0>, grid, tblk, 0, st, ev->cand_s, ev->skey, ev->perm, ev->cell_off, ev->grid,
+                       n, rc2, rcs, W, kern0, ev->t, ev->F, ev->npad, bt, pc);
   else if (mode == 1)
-    hipLaunchKernelGGL(k_smooth<1>, grid, blk, 0, st, ev->cand_s, ev->skey, ev->perm, ev->cell_off, ev->grid, n,
-                       rc2, rcs, W, kern0, ev->t, ev->F, ev->npad, bt, pc, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_smooth_tile<1>, grid, tblk, 0, st, ev->cand_s, ev->skey, ev->perm, ev->cell_off, ev->grid,
+                       n, rc2, rcs, W, kern0, ev->t, ev->F, ev->npad, bt, pc);
   else
     hipLaunchKernelGGL(k_smooth<2>, grid, blk, 0, st, ev->cand_s, ev->skey, ev->perm, ev->cell_off, ev->grid, n,
                        rc2, rcs, W, kern0, ev->t, ev->F, ev->npad, bt, pc, nullptr, nullptr, nullptr);

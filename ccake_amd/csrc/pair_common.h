@@ -80,6 +80,74 @@ __device__ __forceinline__ void for_spans(const Grid& g, const int32_t* __restri
   }
 }
 
+// Hot pair kernels give one tile to kSplit waves: every wave of the
+// workgroup computes the same segments and spans (wave-uniform values) and
+// walks a contiguous slice of the flattened candidate list; the slices'
+// partial sums are added in wave order, so results do not depend on
+// scheduling.
+#ifndef CCAKE_PAIR_SPLIT
+#define CCAKE_PAIR_SPLIT 8
+#endif
+constexpr int kSplit = CCAKE_PAIR_SPLIT;   // 8 measured fastest at 40k-250k particles
+// Optionally skip a candidate (wave-uniform branch) when it is out of range
+// for every lane of the tile; only adds of exact zeros are skipped, the sums
+// are unchanged.  Off by default: the branch keeps the compiler from issuing
+// the candidate loads ahead of their use, which costs more than it saves.
+#ifndef CCAKE_PAIR_SKIP
+#define CCAKE_PAIR_SKIP 0
+#endif
+constexpr bool kSkipEmpty = CCAKE_PAIR_SKIP != 0;
+
+template <class Body>
+__device__ __forceinline__ void for_spans_slice(const Grid& g, const int32_t* __restrict__ cell_off,
+                                                const Seg& s, double rcs, int wave, Body&& body)
+{
+  const int ry0 = uni(cell_of(s.ymn - rcs, g.lo[1], g.inv_cell, g.nc[1]));
+  const int ry1 = uni(cell_of(s.ymx + rcs, g.lo[1], g.inv_cell, g.nc[1]));
+  auto span = [&](int ry, int& beg, int& end) {
+    const double ylo = g.lo[1] + ry * g.cell, yhi = ylo + g.cell;
+    const double dy = fmax(0.0, fmax(ylo - s.ymx, s.ymn - yhi));
+    if (dy > rcs) { beg = end = 0; return; }
+    const double hw = sqrt(fmax(rcs * rcs - dy * dy, 0.0));
+    const int cx0 = cell_of(s.xmn - hw, g.lo[0], g.inv_cell, g.nc[0]);
+    const int cx1 = cell_of(s.xmx + hw, g.lo[0], g.inv_cell, g.nc[0]);
+    beg = uni(cell_off[ry * g.nc[0] + cx0]);
+    end = uni(cell_off[ry * g.nc[0] + cx1 + 1]);
+  };
+  int tot = 0;
+  for (int ry = ry0; ry <= ry1; ++ry) { int b, e; span(ry, b, e); tot += e - b; }
+  const int lo = (int)(((long long)tot * wave) / kSplit);
+  const int hi = (int)(((long long)tot * (wave + 1)) / kSplit);
+  int base = 0;
+  for (int ry = ry0; ry <= ry1 && base < hi; ++ry) {
+    int b, e; span(ry, b, e);
+    const int len = e - b;
+    const int t0 = lo > base ? lo - base : 0;
+    const int t1 = hi - base < len ? hi - base : len;
+    if (t0 < t1) body(b + t0, b + t1);
+    base += len;
+  }
+}
+
+// sqrt(x) and 1/sqrt(x) for x >= 0 from the hardware reciprocal square root
+// refined by coupled Newton steps (Goldschmidt's iteration for g = sqrt(x),
+// h = 1/(2 sqrt(x))); accurate to about one ulp, which the pair sums
+// tolerate.  sqrt(0) = 0 (1/sqrt(0) is not finite and must not be used).
+// The pair predicate never uses it.
+__device__ __forceinline__ void sqrt_rsqrt(double x, double& r, double& ir)
+{
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double e = fma(-g, h, 0.5);
+  g = fma(g, e, g);
+  h = fma(h, e, h);
+  const double d = fma(-g, g, x);
+  const double s = fma(d, h, g);
+  const double e2 = fma(-s, h, 0.5);
+  ir = 2.0 * fma(h, e2, h);
+  r = (x > 0.0) ? s : 0.0;
+}
+
 // Iterate over the row segments of the tile owned by this wave.  `fn(seg,
 // in_seg)` runs once per segment with in_seg true on the lanes of the segment.
 template <class Fn>
