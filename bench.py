@@ -102,6 +102,25 @@ def cpu_baseline(d, cfg, table, steps):
                        f"({sec:.1f} s on {oracle.get_threads()} OpenMP threads)")
 
 
+def copy_calibration(nbytes=256 << 20):
+    """One plain device-to-device copy of known size through the HIP runtime
+    (its copy kernel reads and writes `nbytes`); HBM byte counters of the
+    path's kernels are scaled by what the counters read for it."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(a, 0, C.c_size_t(nbytes)) != 0 or hip.hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipMemset failed")
+    if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) != 0 or hip.hipDeviceSynchronize() != 0:   # 3: device to device
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return nbytes
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -197,14 +216,7 @@ def main():
     }
     ev.close(); eos.close(); dev.close()
     if args.calibrate_copy:
-        # a copy of known size, whose FETCH_SIZE / WRITE_SIZE readings scale the
-        # counters of the path's kernels to bytes
-        import torch
-        src = torch.ones(1 << 25, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        out["copy_calibration_bytes"] = src.numel() * 8
+        out["copy_calibration_bytes"] = copy_calibration()
     if not args.no_cpu:
         print("bench: timing the CPU oracle sample", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(d, cfg, table, args.cpu_steps)

@@ -61,7 +61,28 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, OUT)
+    build_host_programs()
     return OUT
+
+
+HOST = os.path.join(HERE, "host")
+BIN = os.path.join(HERE, "bin")
+HOST_PROGRAMS = ["gubser_run"]
+
+
+def build_host_programs():
+    """C++ drivers of the host call surface (ccake_amd/host), linked against
+    the library with an rpath to it."""
+    os.makedirs(BIN, exist_ok=True)
+    for prog in HOST_PROGRAMS:
+        out = os.path.join(BIN, prog)
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", HOST,
+               os.path.join(HOST, prog + ".cpp"), "-o", out + ".tmp", "-L", HERE, "-lccake_amd",
+               "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed on {prog}:\n" + r.stdout + r.stderr)
+        os.replace(out + ".tmp", out)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,419 @@
+// C++ host-side call surface of the per-timestep SPH path, over the C-ABI of
+// libccake_amd.so (include/ccake_amd.h).  The class and method names are the
+// reference's (include/system_state.h:34-201, include/evolver.h:57-116,
+// include/sph_workstation.h:54-159), so that a caller such as
+// BSQHydro::initialize_hydrodynamics / BSQHydro::run (src/BSQHydro.cpp:568-866)
+// keeps its sequence of calls; every call lands on the gfx950 library, which
+// owns the particle state in HBM.  The host keeps a mirror
+// (`SystemState::particles`) that `copy_device_to_host()` fills field by
+// field, as the reference does for its host AoS (src/system_state.cpp:334-555).
+//
+// Errors: a library status other than CCAKE_OK throws ccake::Error.  With
+// Settings::exit_on_error the reference's exit codes are used instead:
+// negative entropy exits with EXIT_FAILURE (src/evolver.cpp:437-450), a
+// failed EoS inversion with 404 (src/sph_workstation.cpp:1752-1760).
+#pragma once
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ccake_amd.h"
+
+namespace ccake {
+
+struct Error : std::runtime_error {
+  int status;
+  Error(int st, const std::string& what)
+    : std::runtime_error(what + ": status " + std::to_string(st) + " (" + ccake_last_error() + ")"), status(st) {}
+};
+
+// The settings the path reads (include/settings.h, include/defaults.h:14-66),
+// under the reference's names.  Transport modes use the C-ABI enums.
+struct Settings {
+  int gpu = 0;
+  size_t hbm_budget = 0;              // 0: no limit
+  std::string eos_type = "conformal"; // "conformal" or "table"
+  // table EoS (eos_type == "table"): 17 fields per node, T slowest
+  std::vector<double> eos_table;
+  int eos_dims[4] = { 0, 0, 0, 0 };
+  double eos_mins[4] = { 0, 0, 0, 0 }, eos_maxs[4] = { 0, 0, 0, 0 };
+  double Freeze_Out_Temperature = 150.0 / 197.3269804;   // fm^-1
+  double t0 = 1.0, dt = 0.005, max_tau = 50.0;
+  double hT = 0.3, hEta = 0.3;
+  int rk_order = 2;
+  bool particlization_enabled = false;
+  bool input_initial_shear = false, bulk_from_trace = false;
+  bool using_shear = false, using_vorticity = false;
+  bool prohibit_unstable_cs2 = true, prohibit_acausal_cs2 = true;
+  int eta_mode = CCAKE_ETA_CONSTANT;           double constant_eta_over_s = 0.0;
+  int shear_relax_mode = CCAKE_TAUPI_DEFAULT;
+  int zeta_mode = CCAKE_ZETA_CONSTANT;         double constant_zeta_over_s = 0.0;
+  double cs2_dependent_zeta_A = 1.67552, cs2_dependent_zeta_p = 2.0;
+  bool modulate_zeta_with_tanh = true;
+  double zeta_gauss_A = 0.0, zeta_gauss_T0 = 0.0, zeta_gauss_width = 1.0;
+  int delta_pipi_mode = CCAKE_MODE_IS, tau_pipi_mode = CCAKE_MODE_DISABLED, lambda_piPi_mode = CCAKE_MODE_DISABLED;
+  int phi6_mode = CCAKE_MODE_DISABLED, phi7_mode = CCAKE_MODE_DISABLED;
+  int delta_PiPi_mode = CCAKE_MODE_IS, lambda_Pipi_mode = CCAKE_MODE_DISABLED;
+  int phi1_mode = CCAKE_MODE_DISABLED, phi3_mode = CCAKE_MODE_DISABLED;
+  int cells_per_rc = 3;
+  bool print_conservation_status = false;
+  bool exit_on_error = false;
+};
+
+inline void check(int st, const char* what, const Settings* s = nullptr)
+{
+  if (st == CCAKE_OK) return;
+  if (s && s->exit_on_error) {
+    std::fprintf(stderr, "%s: %s\n", what, ccake_last_error());
+    if (st == CCAKE_ERR_NEG_ENTROPY) std::exit(EXIT_FAILURE);
+    if (st == CCAKE_ERR_EOS_NO_SOLUTION) std::exit(404);
+    std::exit(EXIT_FAILURE);
+  }
+  throw Error(st, what);
+}
+
+// Host record of one particle: the fields of the reference's Particle that
+// initial conditions set and outputs read (include/particle.h,
+// include/hydrodynamic_info.h, include/thermodynamic_info.h).
+template <unsigned int D>
+struct Particle {
+  int ID = 0;
+  double r[D] = {};
+  double u[D] = {};
+  double input_e = 0, input_rhoB = 0, input_rhoS = 0, input_rhoQ = 0;
+  // filled by copy_device_to_host()
+  double sigma = 0, sigma_lab = 0, gamma = 1, v[D] = {};
+  double s_ext = 0, rhoB_ext = 0, rhoS_ext = 0, rhoQ_ext = 0, Bulk_ext = 0;
+  double T_ = 0, muB = 0, muS = 0, muQ = 0, e_ = 0, s = 0, rhoB = 0, rhoS = 0, rhoQ = 0, p_ = 0, cs2 = 0, w = 0;
+  double Bulk = 0;
+  double shv[4][4] = {};
+  int Freeze = 0, btrack = 0, eos_id = 0;
+  double T() const { return T_; }
+  double e() const { return e_; }
+  double p() const { return p_; }
+};
+
+// EoM_default's static helpers (src/eom_default.cpp:17-38, 180-203): the
+// equations themselves run on the device.
+template <unsigned int D>
+struct EoM_default {
+  static double gamma_calc(const double* u, double /*time_squared*/)
+  {
+    double c = 0.0;
+    for (unsigned a = 0; a < D; ++a) c += u[a] * u[a];
+    return std::sqrt(1.0 + c);
+  }
+  static double get_LRF(double lab, double gamma, double t) { return lab / gamma / t; }
+};
+
+template <unsigned int D>
+class SystemState {
+ public:
+  explicit SystemState(std::shared_ptr<Settings> s) : settingsPtr(s)
+  {
+    static_assert(D == 2, "the gfx950 library is built for 2+1D");
+    check(ccake_dev_create(s->gpu, s->hbm_budget, &dev), "ccake_dev_create", s.get());
+    ccake_eos_desc d{};
+    d.T_freeze_out = s->Freeze_Out_Temperature;
+    if (s->eos_type == "table") {
+      d.kind = CCAKE_EOS_TABLE;
+      for (int a = 0; a < 4; ++a) { d.dims[a] = s->eos_dims[a]; d.mins[a] = s->eos_mins[a]; d.maxs[a] = s->eos_maxs[a]; }
+      d.table = s->eos_table.data();
+    } else {
+      d.kind = CCAKE_EOS_CONFORMAL;
+    }
+    check(ccake_eos_create(dev, &d, &eos), "ccake_eos_create", s.get());
+  }
+  SystemState(const SystemState&) = delete;
+  SystemState& operator=(const SystemState&) = delete;
+  ~SystemState()
+  {
+    if (ev) ccake_event_destroy(ev);
+    if (eos) ccake_eos_destroy(eos);
+    if (dev) ccake_dev_destroy(dev);
+  }
+
+  double t = 0.0;
+  double S = 0.0, S0 = 0.0, Btotal = 0.0, Btotal0 = 0.0, Stotal = 0.0, Stotal0 = 0.0, Qtotal = 0.0, Qtotal0 = 0.0;
+  double E = 0.0, Ez = 0.0, E0 = 0.0, Etot = 0.0;
+  double efcheck = 0.0;
+  int number_part_fo = 0;
+  int n_particles = 0;
+  int number_of_elapsed_timesteps = 0;
+  std::vector<Particle<D>> particles;
+
+  void add_particle(Particle<D> p)
+  {
+    p.ID = (int)particles.size();
+    particles.push_back(p);
+    n_particles = (int)particles.size();
+  }
+  int n() const { return n_particles; }
+
+  // Creates the device event (the reference's Cabana AoSoA) for the
+  // particles added so far.
+  void allocate_cabana_particles()
+  {
+    if (ev) { ccake_event_destroy(ev); ev = nullptr; }
+    const ccake_params p = params();
+    check(ccake_event_create(dev, &p, eos, n_particles, &ev), "ccake_event_create", settingsPtr.get());
+  }
+  void initialize() { allocate_cabana_particles(); }
+
+  // Initial-condition fields host -> device (the path's inputs).
+  void copy_host_to_device()
+  {
+    std::vector<double> a(n_particles);
+    auto up = [&](int f, auto get) {
+      for (int i = 0; i < n_particles; ++i) a[i] = get(particles[i]);
+      check(ccake_event_upload(ev, f, a.data(), a.size()), "ccake_event_upload", settingsPtr.get());
+    };
+    up(CCAKE_F_X, [](const Particle<D>& q) { return q.r[0]; });
+    up(CCAKE_F_Y, [](const Particle<D>& q) { return q.r[1]; });
+    up(CCAKE_F_UX, [](const Particle<D>& q) { return q.u[0]; });
+    up(CCAKE_F_UY, [](const Particle<D>& q) { return q.u[1]; });
+    up(CCAKE_F_IN_E, [](const Particle<D>& q) { return q.input_e; });
+    up(CCAKE_F_IN_B, [](const Particle<D>& q) { return q.input_rhoB; });
+    up(CCAKE_F_IN_SS, [](const Particle<D>& q) { return q.input_rhoS; });
+    up(CCAKE_F_IN_Q, [](const Particle<D>& q) { return q.input_rhoQ; });
+    check(ccake_event_set_time(ev, t), "ccake_event_set_time", settingsPtr.get());
+  }
+
+  // Device -> host mirror, field by field (src/system_state.cpp:334-555).
+  void copy_device_to_host()
+  {
+    std::vector<double> a(n_particles);
+    std::vector<int32_t> b(n_particles);
+    auto dn = [&](int f, auto set) {
+      check(ccake_event_download(ev, f, a.data(), a.size()), "ccake_event_download", settingsPtr.get());
+      for (int i = 0; i < n_particles; ++i) set(particles[i], a[i]);
+    };
+    dn(CCAKE_F_X, [](Particle<D>& q, double v) { q.r[0] = v; });
+    dn(CCAKE_F_Y, [](Particle<D>& q, double v) { q.r[1] = v; });
+    dn(CCAKE_F_UX, [](Particle<D>& q, double v) { q.u[0] = v; });
+    dn(CCAKE_F_UY, [](Particle<D>& q, double v) { q.u[1] = v; });
+    dn(CCAKE_F_VX, [](Particle<D>& q, double v) { q.v[0] = v; });
+    dn(CCAKE_F_VY, [](Particle<D>& q, double v) { q.v[1] = v; });
+    dn(CCAKE_F_GAMMA, [](Particle<D>& q, double v) { q.gamma = v; });
+    dn(CCAKE_F_SIGMA, [](Particle<D>& q, double v) { q.sigma = v; });
+    dn(CCAKE_F_SIGMA_LAB, [](Particle<D>& q, double v) { q.sigma_lab = v; });
+    dn(CCAKE_F_EXT_S, [](Particle<D>& q, double v) { q.s_ext = v; });
+    dn(CCAKE_F_EXT_B, [](Particle<D>& q, double v) { q.rhoB_ext = v; });
+    dn(CCAKE_F_EXT_SS, [](Particle<D>& q, double v) { q.rhoS_ext = v; });
+    dn(CCAKE_F_EXT_Q, [](Particle<D>& q, double v) { q.rhoQ_ext = v; });
+    dn(CCAKE_F_EXT_BULK, [](Particle<D>& q, double v) { q.Bulk_ext = v; });
+    dn(CCAKE_F_T, [](Particle<D>& q, double v) { q.T_ = v; });
+    dn(CCAKE_F_MUB, [](Particle<D>& q, double v) { q.muB = v; });
+    dn(CCAKE_F_MUS, [](Particle<D>& q, double v) { q.muS = v; });
+    dn(CCAKE_F_MUQ, [](Particle<D>& q, double v) { q.muQ = v; });
+    dn(CCAKE_F_E, [](Particle<D>& q, double v) { q.e_ = v; });
+    dn(CCAKE_F_S, [](Particle<D>& q, double v) { q.s = v; });
+    dn(CCAKE_F_RHOB, [](Particle<D>& q, double v) { q.rhoB = v; });
+    dn(CCAKE_F_RHOS, [](Particle<D>& q, double v) { q.rhoS = v; });
+    dn(CCAKE_F_RHOQ, [](Particle<D>& q, double v) { q.rhoQ = v; });
+    dn(CCAKE_F_P, [](Particle<D>& q, double v) { q.p_ = v; });
+    dn(CCAKE_F_CS2, [](Particle<D>& q, double v) { q.cs2 = v; });
+    dn(CCAKE_F_W, [](Particle<D>& q, double v) { q.w = v; });
+    dn(CCAKE_F_BULK, [](Particle<D>& q, double v) { q.Bulk = v; });
+    for (int c = 0; c < 16; ++c) {
+      check(ccake_event_download(ev, CCAKE_F_SHV0 + c, a.data(), a.size()), "ccake_event_download", settingsPtr.get());
+      for (int i = 0; i < n_particles; ++i) particles[i].shv[c / 4][c % 4] = a[i];
+    }
+    auto dni = [&](int f, auto set) {
+      check(ccake_event_download_int(ev, f, b.data(), b.size()), "ccake_event_download_int", settingsPtr.get());
+      for (int i = 0; i < n_particles; ++i) set(particles[i], b[i]);
+    };
+    dni(CCAKE_I_FREEZE, [](Particle<D>& q, int v) { q.Freeze = v; });
+    dni(CCAKE_I_BTRACK, [](Particle<D>& q, int v) { q.btrack = v; });
+    dni(CCAKE_I_EOS_ID, [](Particle<D>& q, int v) { q.eos_id = v; });
+    check(ccake_event_get_time(ev, &t), "ccake_event_get_time", settingsPtr.get());
+  }
+
+  // SystemState::initialize_linklist / reset_neighbour_list
+  // (src/system_state.cpp:650-733): cell sort plus neighbour counts (btrack).
+  void initialize_linklist() { reset_neighbour_list(); }
+  void reset_neighbour_list() { check(ccake_build_neighbors(ev), "ccake_build_neighbors", settingsPtr.get()); }
+
+  // SystemState::conservation_* (src/system_state.cpp:738-885)
+  void conservation_entropy(bool first_iteration = false)
+  {
+    S = reduce(CCAKE_R_ENTROPY);
+    if (first_iteration) S0 = S;
+  }
+  void conservation_BSQ(bool first_iteration = false)
+  {
+    Btotal = reduce(CCAKE_R_BARYON); Stotal = reduce(CCAKE_R_STRANGE); Qtotal = reduce(CCAKE_R_CHARGE);
+    if (first_iteration) { Btotal0 = Btotal; Stotal0 = Stotal; Qtotal0 = Qtotal; }
+  }
+  void conservation_energy(bool first_iteration = false, double /*t*/ = 1.0)
+  {
+    E = reduce(CCAKE_R_ENERGY);
+    Ez = reduce(CCAKE_R_EZ);
+    if (first_iteration) E0 = E;
+    Etot = E + Ez;
+  }
+
+  // handles for the workstation and evolver
+  ccake_event* event() const { return ev; }
+  ccake_eos* equation_of_state() const { return eos; }
+  const Settings& settings() const { return *settingsPtr; }
+
+ private:
+  double reduce(int which)
+  {
+    double v = 0.0;
+    check(ccake_reduce(ev, which, &v), "ccake_reduce", settingsPtr.get());
+    return v;
+  }
+  ccake_params params() const
+  {
+    const Settings& s = *settingsPtr;
+    ccake_params p{};
+    p.dim = (int)D; p.rk_order = s.rk_order; p.hT = s.hT; p.hEta = s.hEta; p.t0 = s.t0; p.dt = s.dt;
+    p.e_freeze = efcheck; p.particlization = s.particlization_enabled; p.using_shear = s.using_shear;
+    p.using_vorticity = s.using_vorticity; p.input_initial_shear = s.input_initial_shear;
+    p.bulk_from_trace = s.bulk_from_trace; p.prohibit_unstable_cs2 = s.prohibit_unstable_cs2;
+    p.prohibit_acausal_cs2 = s.prohibit_acausal_cs2; p.eta_mode = s.eta_mode;
+    p.constant_eta_over_s = s.constant_eta_over_s; p.shear_relax_mode = s.shear_relax_mode;
+    p.zeta_mode = s.zeta_mode; p.constant_zeta_over_s = s.constant_zeta_over_s;
+    p.cs2_dependent_zeta_A = s.cs2_dependent_zeta_A; p.cs2_dependent_zeta_p = s.cs2_dependent_zeta_p;
+    p.modulate_zeta_with_tanh = s.modulate_zeta_with_tanh; p.zeta_gauss_A = s.zeta_gauss_A;
+    p.zeta_gauss_T0 = s.zeta_gauss_T0; p.zeta_gauss_width = s.zeta_gauss_width;
+    p.delta_pipi_mode = s.delta_pipi_mode; p.tau_pipi_mode = s.tau_pipi_mode; p.lambda_piPi_mode = s.lambda_piPi_mode;
+    p.phi6_mode = s.phi6_mode; p.phi7_mode = s.phi7_mode; p.delta_PiPi_mode = s.delta_PiPi_mode;
+    p.lambda_Pipi_mode = s.lambda_Pipi_mode; p.phi1_mode = s.phi1_mode; p.phi3_mode = s.phi3_mode;
+    p.cells_per_rc = s.cells_per_rc;
+    return p;
+  }
+
+  std::shared_ptr<Settings> settingsPtr;
+  ccake_dev* dev = nullptr;
+  ccake_eos* eos = nullptr;
+  ccake_event* ev = nullptr;
+};
+
+// Evolver<D> (src/evolver.cpp:97-517): the RK cache lives on the device.
+template <unsigned int D>
+class Evolver {
+ public:
+  Evolver(std::shared_ptr<Settings> s, std::shared_ptr<SystemState<D>> sys) : settingsPtr(s), systemPtr(sys) {}
+
+  void allocate_cache() {}   // the event owns its cache columns
+  void set_current_timestep_quantities()
+  {
+    check(ccake_cache_state(systemPtr->event()), "ccake_cache_state", settingsPtr.get());
+  }
+  // y = y0 + dt f(y, t), t = t0 + dt, after the caller's derivatives
+  void step_rk(double dt, double t0, std::function<void(void)> time_derivatives_functional)
+  {
+    time_derivatives_functional();
+    check(ccake_rk_update(systemPtr->event(), dt, t0), "ccake_rk_update", settingsPtr.get());
+    systemPtr->t = t0 + dt;
+  }
+  void execute_timestep(double dt, int rk_order, std::function<void(void)> time_derivatives_functional)
+  {
+    if (rk_order != 2) check(CCAKE_ERR_UNSUPPORTED, "Evolver::execute_timestep (rk_order != 2)", settingsPtr.get());
+    const double t0 = systemPtr->t;
+    set_current_timestep_quantities();
+    step_rk(0.5 * dt, t0, time_derivatives_functional);
+    step_rk(dt, t0, time_derivatives_functional);
+  }
+
+ private:
+  std::shared_ptr<Settings> settingsPtr;
+  std::shared_ptr<SystemState<D>> systemPtr;
+};
+
+// SPHWorkstation<D, TEOM> (src/sph_workstation.cpp).
+template <unsigned int D, template <unsigned int> class TEOM>
+class SPHWorkstation {
+ public:
+  SPHWorkstation(std::shared_ptr<Settings> s, std::shared_ptr<SystemState<D>> sys)
+    : settingsPtr(s), systemPtr(sys), evolver(s, sys) {}
+
+  // SPHWorkstation::initialize (src/sph_workstation.cpp:29-110): e_freeze from
+  // the EoS at T_FO, then the device event.
+  void initialize()
+  {
+    double ef = 0.0;
+    check(ccake_eos_efreeze(systemPtr->equation_of_state(), settingsPtr->Freeze_Out_Temperature, &ef),
+          "ccake_eos_efreeze", settingsPtr.get());
+    systemPtr->efcheck = ef;
+    systemPtr->t = settingsPtr->t0;
+    systemPtr->allocate_cabana_particles();
+    systemPtr->copy_host_to_device();
+  }
+  void setup_freeze_out() {}
+
+  // The initialisation sequence of BSQHydro::initialize_hydrodynamics
+  // (src/BSQHydro.cpp:568-640) in one device call: process_initial_conditions,
+  // initialize_linklist, initialize_entropy_and_charge_densities,
+  // calculate_intial_sigma, init_particles, calculate_gamma_and_velocities,
+  // initial_smoothing, calculate_extensive_shv, set_bulk_Pi, reset_pi_tensor,
+  // update_all_particle_thermodynamics and the first freeze-out check.
+  void initialize_hydrodynamics()
+  {
+    check(ccake_initialize(ev()), "ccake_initialize", settingsPtr.get());
+    check(ccake_event_get_time(ev(), &systemPtr->t), "ccake_event_get_time", settingsPtr.get());
+    systemPtr->conservation_entropy(true);
+    systemPtr->conservation_BSQ(true);
+    systemPtr->conservation_energy(true, systemPtr->t);
+  }
+
+  // the phases, one device call each
+  void calculate_gamma_and_velocities() { call(ccake_gamma_velocity(ev()), "calculate_gamma_and_velocities"); }
+  void smooth_all_particle_fields(double /*time_squared*/) { call(ccake_smooth_fields(ev()), "smooth_all_particle_fields"); }
+  void calculate_intial_sigma(double /*time_squared*/) { call(ccake_initial_sigma(ev()), "calculate_intial_sigma"); }
+  void smooth_all_particle_gradients(double /*time_squared*/) { call(ccake_smooth_gradients(ev()), "smooth_all_particle_gradients"); }
+  void update_all_particle_thermodynamics() { call(ccake_eos_invert(ev(), 0), "update_all_particle_thermodynamics"); }
+  void update_all_particle_viscosities() { call(ccake_viscosities(ev()), "update_all_particle_viscosities"); }
+  void reset_pi_tensor(double /*time_squared*/) { call(ccake_reset_pi_tensor(ev()), "reset_pi_tensor"); }
+  // SPHWorkstation::get_time_derivatives (src/sph_workstation.cpp:1509-1612)
+  void get_time_derivatives(double /*dt*/) { call(ccake_get_time_derivatives(ev()), "get_time_derivatives"); }
+  void freeze_out_particles() { call(ccake_freeze_out_particles(ev()), "freeze_out_particles"); }
+
+  // SPHWorkstation::advance_timestep (src/sph_workstation.cpp:2042-2073): the
+  // RK2 step with its freeze-out check, fused on the device.
+  void advance_timestep(double dt, int rk_order)
+  {
+    call(ccake_advance_timestep(ev(), dt, rk_order), "advance_timestep");
+    check(ccake_event_get_time(ev(), &systemPtr->t), "ccake_event_get_time", settingsPtr.get());
+    systemPtr->number_of_elapsed_timesteps++;
+    if (settingsPtr->print_conservation_status) {
+      systemPtr->conservation_entropy();
+      systemPtr->conservation_BSQ();
+      systemPtr->conservation_energy(false, systemPtr->t);
+    }
+  }
+  // the same step through the Evolver and an arbitrary derivative functional
+  void advance_timestep_with_evolver(double dt, int rk_order)
+  {
+    evolver.execute_timestep(dt, rk_order, [this, dt] { this->get_time_derivatives(dt); });
+    if (settingsPtr->particlization_enabled) freeze_out_particles();
+    systemPtr->number_of_elapsed_timesteps++;
+  }
+
+  // SPHWorkstation::continue_evolution (src/sph_workstation.cpp:1992-2024)
+  bool continue_evolution()
+  {
+    int keep = 0;
+    call(ccake_continue_evolution(ev(), settingsPtr->max_tau, &keep), "continue_evolution");
+    return keep != 0;
+  }
+
+  Evolver<D> evolver;
+
+ private:
+  ccake_event* ev() const { return systemPtr->event(); }
+  void call(int st, const char* what) { check(st, what, settingsPtr.get()); }
+  std::shared_ptr<Settings> settingsPtr;
+  std::shared_ptr<SystemState<D>> systemPtr;
+};
+
+}  // namespace ccake

@@ -1,0 +1,125 @@
+// Ideal 2+1D Gubser flow through the C++ call surface (SystemState,
+// SPHWorkstation, Evolver) on the gfx950 library, in the order
+// BSQHydro::run uses (src/BSQHydro.cpp:568-866).  Prints one JSON line:
+// the energy density against the analytic solution, entropy and energy
+// conservation, and (with --evolver-steps K) the largest difference between
+// K steps taken by the fused device step and by Evolver::execute_timestep
+// with the workstation's get_time_derivatives as the functional.
+//
+//   gubser_run [--spacing 0.05] [--steps 120] [--evolver-steps 0]
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+
+#include "ccake.hpp"
+
+namespace {
+
+constexpr double kHbarc = 0.1973269804;   // GeV fm
+
+// Ideal Gubser energy density (fm^-4) and radial four-velocity, q = 1/fm,
+// eps0 = 80 GeV/fm^3 (utilities/ideal_gubser_ic_generator.py:23-38).
+double gubser_e(double tau, double r, double q = 1.0, double eps0 = 80.0)
+{
+  const double den = 1.0 + 2.0 * q * q * (tau * tau + r * r) + std::pow(q, 4) * std::pow(tau * tau - r * r, 2);
+  return (eps0 / kHbarc) * std::pow(2.0 * q, 8.0 / 3.0) / (std::pow(tau, 4.0 / 3.0) * std::pow(den, 4.0 / 3.0));
+}
+double gubser_ur(double tau, double r, double q = 1.0)
+{
+  return std::sinh(std::atanh(2.0 * q * q * tau * r / (1.0 + q * q * tau * tau + q * q * r * r)));
+}
+
+// lattice coordinate -L + a i, rounded to 12 decimals as read from a text file
+double coord(double L, double a, int i)
+{
+  const double v = -L + a * i;
+  return std::nearbyint(v * 1e12) / 1e12 + 0.0;
+}
+
+std::shared_ptr<ccake::SystemState<2>> make_system(std::shared_ptr<ccake::Settings> s, double a)
+{
+  auto sys = std::make_shared<ccake::SystemState<2>>(s);
+  const double L = 5.0;
+  const int n = (int)std::lround(2 * L / a) + 1;
+  for (int iy = 0; iy < n; ++iy)
+    for (int ix = 0; ix < n; ++ix) {
+      ccake::Particle<2> p;
+      p.r[0] = coord(L, a, ix);
+      p.r[1] = coord(L, a, iy);
+      const double r = std::hypot(p.r[0], p.r[1]);
+      p.input_e = gubser_e(s->t0, r);
+      const double ur = gubser_ur(s->t0, r);
+      p.u[0] = r > 0 ? ur * p.r[0] / r : 0.0;
+      p.u[1] = r > 0 ? ur * p.r[1] / r : 0.0;
+      sys->add_particle(p);
+    }
+  return sys;
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+  double a = 0.05;
+  int steps = 120, evolver_steps = 0;
+  for (int k = 1; k + 1 < argc; k += 2) {
+    if (!std::strcmp(argv[k], "--spacing")) a = std::atof(argv[k + 1]);
+    else if (!std::strcmp(argv[k], "--steps")) steps = std::atoi(argv[k + 1]);
+    else if (!std::strcmp(argv[k], "--evolver-steps")) evolver_steps = std::atoi(argv[k + 1]);
+  }
+  auto s = std::make_shared<ccake::Settings>();
+  s->eos_type = "conformal";
+  s->t0 = 1.0; s->dt = 0.005; s->hT = 0.3;
+  s->particlization_enabled = false;
+  s->using_shear = false;
+  s->eta_mode = CCAKE_ETA_CONSTANT; s->constant_eta_over_s = 0.0;
+  s->zeta_mode = CCAKE_ZETA_CONSTANT; s->constant_zeta_over_s = 0.0;
+
+  try {
+    auto sys = make_system(s, a);
+    ccake::SPHWorkstation<2, ccake::EoM_default> ws(s, sys);
+    ws.initialize();
+    ws.initialize_hydrodynamics();
+    for (int k = 0; k < steps; ++k) ws.advance_timestep(s->dt, s->rk_order);
+    sys->conservation_entropy();
+    sys->conservation_energy();
+    sys->copy_device_to_host();
+    double worst = 0.0;
+    for (const auto& p : sys->particles) {
+      const double r = std::hypot(p.r[0], p.r[1]);
+      if (r >= 2.5) continue;
+      const double ea = gubser_e(sys->t, r);
+      worst = std::max(worst, std::fabs(p.e() - ea) / ea);
+    }
+
+    double evolver_diff = -1.0;
+    if (evolver_steps > 0) {
+      auto s1 = make_system(s, a), s2 = make_system(s, a);
+      ccake::SPHWorkstation<2, ccake::EoM_default> w1(s, s1), w2(s, s2);
+      w1.initialize(); w1.initialize_hydrodynamics();
+      w2.initialize(); w2.initialize_hydrodynamics();
+      for (int k = 0; k < evolver_steps; ++k) {
+        w1.advance_timestep(s->dt, 2);
+        w2.advance_timestep_with_evolver(s->dt, 2);
+      }
+      s1->copy_device_to_host();
+      s2->copy_device_to_host();
+      evolver_diff = std::fabs(s1->t - s2->t);
+      for (int i = 0; i < s1->n(); ++i) {
+        const auto &p = s1->particles[i], &q = s2->particles[i];
+        evolver_diff = std::max({ evolver_diff, std::fabs(p.e() - q.e()), std::fabs(p.u[0] - q.u[0]),
+                                  std::fabs(p.u[1] - q.u[1]), std::fabs(p.r[0] - q.r[0]), std::fabs(p.s_ext - q.s_ext) });
+      }
+    }
+    std::printf("{\"particles\": %d, \"steps\": %d, \"t\": %.15g, \"max_rel_err_e_r_lt_2.5\": %.6g, "
+                "\"entropy_rel_change\": %.6g, \"energy_rel_change\": %.6g, \"evolver_vs_fused_max_abs_diff\": %.6g}\n",
+                sys->n(), steps, sys->t, worst, sys->S / sys->S0 - 1.0, (sys->Etot) / (sys->E0) - 1.0, evolver_diff);
+  } catch (const ccake::Error& e) {
+    std::fprintf(stderr, "gubser_run: %s\n", e.what());
+    return 2;
+  }
+  return 0;
+}
