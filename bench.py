@@ -102,6 +102,62 @@ def cpu_baseline(d, cfg, table, steps):
                        f"({sec:.1f} s on {oracle.get_threads()} OpenMP threads)")
 
 
+# Compulsory HBM bytes per particle of the fused per-particle kernel k_eos<0>
+# (EoS inversion + reset_pi_tensor + transport + gradient-record gather):
+# reads 24 doubles (seed T, mu; s, rho; gamma, sigma, u; extensive Pi and 5
+# shear; sigma_lab, x, y, m_s, v) + 2 int32 (perm, EoS member); writes 77
+# doubles (31 thermo, Pi, 16 pi^{mu nu}, 13 transport, 16-double record)
+# + 1 int32.
+BYTES_PER_PARTICLE_EOS = 24 * 8 + 2 * 4 + 77 * 8 + 4
+
+KERNEL_OF_GROUP = {"SMOOTH": "k_smooth_tile", "GRAD": "k_grad", "EOS": "k_eos"}
+
+
+def traffic_of(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the same
+    counter's reading for a plain copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    d = json.load(open(path))
+    k = d.get("kernels", {}).get(kernel)
+    if not k:
+        return None
+    c = d["copy"]
+    f_scale = c["bytes"] / (c["fetch_kib"] * 1024.0)
+    w_scale = c["bytes"] / (c["write_kib"] * 1024.0)
+    return round(k["fetch_kib"] * 1024.0 * f_scale + k["write_kib"] * 1024.0 * w_scale)
+
+
+def make_roofline(prof, pairs, n, cfg, workload):
+    """Roofline of the dominant kernel group (by measured time) among the
+    three with a work model: the smoothing and gradient pair kernels are
+    FP64-VALU bound (work = flop per directed pair x pairs per launch), the
+    fused EoS kernel is HBM bound (work = compulsory bytes per particle x N)."""
+    cand = {g: prof[g] for g in KERNEL_OF_GROUP if prof[g][1]}
+    dom = max(cand, key=lambda g: cand[g][0])
+    ms = cand[dom][0] / cand[dom][1]
+    sec = ms * 1e-3
+    if dom == "EOS":
+        work = BYTES_PER_PARTICLE_EOS * n
+        achieved = work / sec / 1e9
+        r = dict(bound="hbm", achieved=round(achieved, 2), peak=PEAK_HBM_GBS, unit="GB/s",
+                 frac=round(achieved / PEAK_HBM_GBS, 4),
+                 work=f"{BYTES_PER_PARTICLE_EOS} B/particle x {n} particles per launch")
+    else:
+        fpp = FLOP_PER_PAIR_SMOOTH if dom == "SMOOTH" else (
+            FLOP_PER_PAIR_GRAD_SHEAR if cfg["using_shear"] else FLOP_PER_PAIR_GRAD_IDEAL)
+        achieved = fpp * pairs / sec / 1e12
+        r = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
+                 frac=round(achieved / PEAK_FP64_TFLOPS, 4), engine="FP64 VALU (no matrix work on this path)",
+                 work=f"{fpp} flop/pair x {pairs:.0f} pairs per launch")
+    r["traffic"] = traffic_of(KERNEL_OF_GROUP[dom], workload)
+    r["kernel"] = KERNEL_OF_GROUP[dom]
+    r["avg_launch_ms"] = round(ms, 4)
+    return r
+
+
 def copy_calibration(nbytes=256 << 20):
     """One plain device-to-device copy of known size through the HIP runtime
     (its copy kernel reads and writes `nbytes`); HBM byte counters of the
@@ -155,6 +211,7 @@ def main():
 
     dev.profile(True)
     dev.profile_reset()
+    ev.pair_total(reset=True)
     dev.sync()
     t0 = time.perf_counter()
     done = ev.run_steps(cfg["dt"], args.steps)
@@ -172,31 +229,14 @@ def main():
             continue
         ms, cnt = dev.profile_read(k)
         prof[k[len("CCAKE_K_"):]] = (ms, cnt)
-    pairs = ev.last_pair_count()
     stages = 2 * steps
+    pairs_per_launch = ev.pair_total() / max(prof["SMOOTH"][1], 1)
     print("bench: per-stage ms: " + " ".join(
         f"{k}={v[0] / max(v[1], 1):.3f}x{v[1]}" for k, v in prof.items() if v[1]), file=sys.stderr, flush=True)
-    print(f"bench: {pairs} directed pairs per stage, {sec / steps * 1e3:.3f} ms per step", file=sys.stderr,
-          flush=True)
+    print(f"bench: {pairs_per_launch:.0f} directed pairs per stage, {sec / steps * 1e3:.3f} ms per step",
+          file=sys.stderr, flush=True)
 
-    # dominant kernel group by measured time, with its algorithmic work
-    dom = max(prof, key=lambda k: prof[k][0])
-    gms, gcnt = prof["GRAD"]
-    flop_grad = FLOP_PER_PAIR_GRAD_SHEAR if cfg["using_shear"] else FLOP_PER_PAIR_GRAD_IDEAL
-    if dom == "SMOOTH":
-        sms, scnt = prof["SMOOTH"]
-        achieved = FLOP_PER_PAIR_SMOOTH * pairs / (sms / scnt * 1e-3) / 1e12
-        kern, per = "smoothing (k_gather_smooth + k_smooth)", f"{FLOP_PER_PAIR_SMOOTH} flop/pair"
-    else:
-        # the gradient pair kernel (k_grad): the path's FP64 VALU-bound kernel
-        achieved = flop_grad * pairs / (gms / gcnt * 1e-3) / 1e12
-        kern, per = "gradients (k_grad)", f"{flop_grad} flop/pair"
-    roofline = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
-                    frac=round(achieved / PEAK_FP64_TFLOPS, 4), traffic=None,
-                    kernel=kern, work=f"{per} x {pairs} pairs per launch", engine="FP64 VALU",
-                    dominant_group=dom, avg_launch_ms=round((gms / gcnt) if dom != "SMOOTH" else
-                                                            prof["SMOOTH"][0] / prof["SMOOTH"][1], 4))
-
+    roofline = make_roofline(prof, pairs_per_launch, n, cfg, args.workload)
     out = {
         "metric": "particle-steps/s",
         "value": n * steps / sec,

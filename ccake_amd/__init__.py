@@ -131,6 +131,7 @@ ABI = {
     "ccake_profile_read": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_int64)]),
     "ccake_profile_reset": (C.c_int, [_vp]),
     "ccake_last_pair_count": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
+    "ccake_pair_total": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int64)]),
 }
 
 _lib = None
@@ -339,6 +340,11 @@ class Event:
         _ok(lib().ccake_export_pairs(self.h, pi.ctypes.data_as(_ip), pj.ctypes.data_as(_ip), n.value,
                                      C.byref(n)), "export_pairs")
         return pi, pj
+
+    def pair_total(self, reset=False):
+        v = C.c_int64()
+        _ok(lib().ccake_pair_total(self.h, int(reset), C.byref(v)), "pair_total")
+        return v.value
 
     def last_pair_count(self):
         v = C.c_int64()

@@ -76,9 +76,9 @@ static int time_derivatives(ccake_event* ev, int fuse_rk, double dt_stage, doubl
   prof_end(d, CCAKE_K_BIN);
   prof_begin(d, CCAKE_K_SORT);
   if ((rc = launch_neighbor_sort(ev))) return rc;               // 01 cell sort
+  if ((rc = launch_gather_smooth(ev))) return rc;               // candidate records in cell order
   prof_end(d, CCAKE_K_SORT);
   prof_begin(d, CCAKE_K_SMOOTH);
-  if ((rc = launch_gather_smooth(ev))) return rc;
   if ((rc = launch_smooth(ev, 0))) return rc;                   // 03 (+ btrack)
   prof_end(d, CCAKE_K_SMOOTH);
   prof_begin(d, CCAKE_K_EOS);
@@ -309,7 +309,7 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
   rc |= dalloc(ev, &ev->hist, (size_t)nscan);
   rc |= dalloc(ev, &ev->scan_tmp, (size_t)nscan_blocks);
   rc |= dalloc(ev, &ev->cell_off, (size_t)ev->cap_cells + 1);
-  rc |= dalloc(ev, &ev->pair_count, 1);
+  rc |= dalloc(ev, &ev->pair_count, 2);
   rc |= dalloc(ev, &ev->cand_s, 8 * np);
   rc |= dalloc(ev, &ev->cand_g, 16 * np);
   rc |= dalloc(ev, &ev->red, 1024);
@@ -319,7 +319,7 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
   CCAKE_HIP(hipMemsetAsync(ev->F, 0, (size_t)NUM_COLUMNS * np * sizeof(double), st));
   CCAKE_HIP(hipMemsetAsync(ev->I, 0, (size_t)CCAKE_NUM_IFIELDS * np * sizeof(int32_t), st));
   CCAKE_HIP(hipMemsetAsync(ev->err, 0, sizeof(int32_t), st));
-  CCAKE_HIP(hipMemsetAsync(ev->pair_count, 0, sizeof(int64_t), st));
+  CCAKE_HIP(hipMemsetAsync(ev->pair_count, 0, 2 * sizeof(int64_t), st));
   // sph_mass = 1 and gamma = 1 until the caller overrides them
   std::vector<double> ones(n, 1.0);
   for (int f : { (int)CCAKE_F_MASS_S, (int)CCAKE_F_MASS_B, (int)CCAKE_F_MASS_SS, (int)CCAKE_F_MASS_Q, (int)CCAKE_F_GAMMA })
@@ -571,6 +571,20 @@ int ccake_last_pair_count(ccake_event* ev, int64_t* n_pairs)
   CCAKE_HIP(hipMemcpyAsync(&v, ev->pair_count, sizeof(int64_t), hipMemcpyDeviceToHost, ev->dev->stream));
   CCAKE_HIP(hipStreamSynchronize(ev->dev->stream));
   *n_pairs = v;
+  return 0;
+}
+
+int ccake_pair_total(ccake_event* ev, int reset, int64_t* n_pairs)
+{
+  if (!ev || !n_pairs) return CCAKE_ERR_INVALID;
+  int64_t v = 0;
+  CCAKE_HIP(hipMemcpyAsync(&v, ev->pair_count + 1, sizeof(int64_t), hipMemcpyDeviceToHost, ev->dev->stream));
+  CCAKE_HIP(hipStreamSynchronize(ev->dev->stream));
+  *n_pairs = v;
+  if (reset) {
+    CCAKE_HIP(hipMemsetAsync(ev->pair_count + 1, 0, sizeof(int64_t), ev->dev->stream));
+    CCAKE_HIP(hipStreamSynchronize(ev->dev->stream));
+  }
   return 0;
 }
 

@@ -107,7 +107,7 @@ struct ccake_event {
   int32_t* perm = nullptr;      // sorted position -> particle id (alias of the final val buffer)
   int32_t* skey = nullptr;      // sorted keys (alias)
   int32_t* rank = nullptr;      // particle id -> sorted position
-  int64_t* pair_count = nullptr;  // device accumulators
+  int64_t* pair_count = nullptr;  // device accumulators: [0] last stage, [1] since the last reset
   int radix_blocks = 0;
   // gathered candidates (sorted order)
   double* cand_s = nullptr;     // 8 doubles per particle

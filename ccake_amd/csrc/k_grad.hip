@@ -51,7 +51,7 @@ __global__ __launch_bounds__(kSplit * kWave) void k_grad(const double* __restric
   const Grid g = *gridp;
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = uni((int)(threadIdx.x / kWave));   // wave-uniform
-  const int k = blockIdx.x * kTile + lane;
+  const int k = xcd_tile(blockIdx.x, gridDim.x) * kTile + lane;
   const bool act = k < n;
   const int kk = act ? k : 0;
   const double* me = cand + (size_t)kk * kCandG;

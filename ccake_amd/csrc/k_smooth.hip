@@ -54,21 +54,22 @@ struct KernelW {
   }
 };
 
-// MODE 0: fields + LRF + btrack; MODE 1: initial sigma; MODE 2: btrack only;
-// MODE 3: write the pair rows (parity export).
+// Neighbour walk without kernel sums, one wave per tile.  MODE 2: neighbour
+// counts (btrack) and the pair total; MODE 3: write the pair rows (parity
+// export) at the row offsets row_off.
 template <int MODE>
-__global__ __launch_bounds__(64) void k_smooth(const double* __restrict__ cand,
-                                               const int32_t* __restrict__ skey,
-                                               const int32_t* __restrict__ perm,
-                                               const int32_t* __restrict__ cell_off,
-                                               const Grid* __restrict__ gridp, int n,
-                                               double rc2, double rcs, KernelW W, double kern0,
-                                               double t, double* __restrict__ F, int npad,
-                                               int32_t* __restrict__ btrack,
-                                               unsigned long long* __restrict__ pair_total,
-                                               const int64_t* __restrict__ row_off,
-                                               int32_t* __restrict__ out_i, int32_t* __restrict__ out_j)
+__global__ __launch_bounds__(64) void k_walk(const double* __restrict__ cand,
+                                             const int32_t* __restrict__ skey,
+                                             const int32_t* __restrict__ perm,
+                                             const int32_t* __restrict__ cell_off,
+                                             const Grid* __restrict__ gridp, int n,
+                                             double rc2, double rcs,
+                                             int32_t* __restrict__ btrack,
+                                             unsigned long long* __restrict__ pair_total,
+                                             const int64_t* __restrict__ row_off,
+                                             int32_t* __restrict__ out_i, int32_t* __restrict__ out_j)
 {
+  static_assert(MODE == 2 || MODE == 3, "walk modes");
   const Grid g = *gridp;
   const int lane = threadIdx.x;
   const int k = blockIdx.x * kTile + lane;
@@ -78,10 +79,6 @@ __global__ __launch_bounds__(64) void k_smooth(const double* __restrict__ cand,
   const double xi = me[0], yi = me[1];
   const int myrow = act ? skey[kk] / g.nc[0] : INT_MAX;
   const int id = act ? perm[kk] : 0;
-
-  // self terms first (src/sph_workstation.cpp:514-523)
-  double sl = me[2] * kern0;
-  double ss = me[3] * kern0, sB = me[4] * kern0, sS = me[5] * kern0, sQ = me[6] * kern0;
   int cnt = 0;
   int64_t wpos = 0;
   if (MODE == 3 && act) wpos = row_off[id];
@@ -92,44 +89,17 @@ __global__ __launch_bounds__(64) void k_smooth(const double* __restrict__ cand,
       const double dx = xi - c[0], dy = yi - c[1];
       const double r2 = dist2_nocontract(dx, dy);
       const bool nb = in && (r2 <= rc2) && (j != k);
-      if (MODE == 2) { cnt += nb; return; }
-      if (MODE == 3) {
-        if (nb) { out_i[wpos] = id; out_j[wpos] = perm[j]; ++wpos; }
-        return;
-      }
-      const double w = nb ? W(r2) : 0.0;
-      cnt += nb;
-      sl += c[2] * w;
-      if (MODE == 0) {
-        ss += c[3] * w; sB += c[4] * w; sS += c[5] * w; sQ += c[6] * w;
-      }
+      if (MODE == 2) cnt += nb;
+      if (MODE == 3 && nb) { out_i[wpos] = id; out_j[wpos] = perm[j]; ++wpos; }
     });
   });
 
-  if (!act) return;
-  if (MODE == 0 || MODE == 2) {
-    // pair total for the roofline: one atomic per wave
-    unsigned long long tot = (unsigned long long)cnt;
-    for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
-    if (lane == 0 && pair_total) atomicAdd(pair_total, tot);
-    const int b = btrack[id];
-    btrack[id] = (b == -1) ? -1 : cnt;
-  }
-  if (MODE == 2 || MODE == 3) return;
-  const double gam = F[(size_t)CCAKE_F_GAMMA * npad + id];
-  F[(size_t)CCAKE_F_SIGMA_LAB * npad + id] = sl;
-  // get_LRF = lab/gamma/t (src/eom_default.cpp:198-203)
-  F[(size_t)CCAKE_F_SIGMA * npad + id] = sl / gam / t;
-  if (MODE == 0) {
-    F[(size_t)CCAKE_F_SM_S * npad + id] = ss;
-    F[(size_t)CCAKE_F_SM_B * npad + id] = sB;
-    F[(size_t)CCAKE_F_SM_SS * npad + id] = sS;
-    F[(size_t)CCAKE_F_SM_Q * npad + id] = sQ;
-    F[(size_t)CCAKE_F_S * npad + id] = ss / gam / t;
-    F[(size_t)CCAKE_F_RHOB * npad + id] = sB / gam / t;
-    F[(size_t)CCAKE_F_RHOS * npad + id] = sS / gam / t;
-    F[(size_t)CCAKE_F_RHOQ * npad + id] = sQ / gam / t;
-  }
+  if (!act || MODE == 3) return;
+  unsigned long long tot = (unsigned long long)cnt;
+  for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
+  if (lane == 0 && pair_total) atomicAdd(pair_total, tot);
+  const int b = btrack[id];
+  btrack[id] = (b == -1) ? -1 : cnt;
 }
 
 // Hot smoothing pass (MODE 0: fields + LRF + btrack + pair total; MODE 1:
@@ -153,7 +123,7 @@ __global__ __launch_bounds__(kSplit * kWave) void k_smooth_tile(const double* __
   const Grid g = *gridp;
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = uni((int)(threadIdx.x / kWave));   // wave-uniform
-  const int k = blockIdx.x * kTile + lane;
+  const int k = xcd_tile(blockIdx.x, gridDim.x) * kTile + lane;
   const bool act = k < n;
   const int kk = act ? k : 0;
   const double* me = cand + (size_t)kk * kCandS;
@@ -215,7 +185,7 @@ __global__ __launch_bounds__(kSplit * kWave) void k_smooth_tile(const double* __
     // pair total for the roofline: one atomic per tile
     unsigned long long tot = act ? (unsigned long long)cnt : 0ull;
     for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
-    if (lane == 0 && pair_total) atomicAdd(pair_total, tot);
+    if (lane == 0 && pair_total) { atomicAdd(pair_total, tot); atomicAdd(pair_total + 1, tot); }
   }
   if (!act) return;
   const int id = perm[kk];
@@ -276,8 +246,8 @@ int launch_smooth(ccake_event* ev, int mode, bool count_pairs)
     hipLaunchKernelGGL(k_smooth_tile<1>, grid, tblk, 0, st, ev->cand_s, ev->skey, ev->perm, ev->cell_off, ev->grid,
                        n, rc2, rcs, W, kern0, ev->t, ev->F, ev->npad, bt, pc);
   else
-    hipLaunchKernelGGL(k_smooth<2>, grid, blk, 0, st, ev->cand_s, ev->skey, ev->perm, ev->cell_off, ev->grid, n,
-                       rc2, rcs, W, kern0, ev->t, ev->F, ev->npad, bt, pc, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_walk<2>, grid, blk, 0, st, ev->cand_s, ev->skey, ev->perm, ev->cell_off, ev->grid, n,
+                       rc2, rcs, bt, pc, nullptr, nullptr, nullptr);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
 
@@ -307,10 +277,9 @@ int launch_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, s
   CCAKE_HIP(hipMalloc(&d_j, (total + 1) * sizeof(int32_t)));
   CCAKE_HIP(hipMemcpyAsync(d_off, off.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
   const double rc = 2.0 * ev->p.hT;
-  const KernelW W = make_w(ev->p.hT);
-  hipLaunchKernelGGL(k_smooth<3>, dim3((n + kTile - 1) / kTile), dim3(kTile), 0, st, ev->cand_s, ev->skey,
-                     ev->perm, ev->cell_off, ev->grid, n, rc * rc, rc * (1.0 + 1e-12) + 1e-12, W, W.norm, ev->t,
-                     ev->F, ev->npad, bt, nullptr, d_off, d_i, d_j);
+  hipLaunchKernelGGL(k_walk<3>, dim3((n + kTile - 1) / kTile), dim3(kTile), 0, st, ev->cand_s, ev->skey,
+                     ev->perm, ev->cell_off, ev->grid, n, rc * rc, rc * (1.0 + 1e-12) + 1e-12, bt, nullptr, d_off,
+                     d_i, d_j);
   CCAKE_HIP(hipGetLastError());
   CCAKE_HIP(hipMemcpyAsync(pi, d_i, total * sizeof(int32_t), hipMemcpyDeviceToHost, st));
   CCAKE_HIP(hipMemcpyAsync(pj, d_j, total * sizeof(int32_t), hipMemcpyDeviceToHost, st));

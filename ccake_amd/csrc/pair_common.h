@@ -80,6 +80,17 @@ __device__ __forceinline__ void for_spans(const Grid& g, const int32_t* __restri
   }
 }
 
+// Workgroups are dispatched round-robin over the chip's 8 XCDs, each with
+// its own L2.  Remap the workgroup index so that every XCD walks one
+// contiguous range of tiles (neighbouring tiles share most candidates).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int bid, int nblk)
+{
+  const int x = bid % kXcds, r = bid / kXcds;
+  const int per = nblk / kXcds, rem = nblk % kXcds;
+  return x * per + (x < rem ? x : rem) + r;
+}
+
 // Hot pair kernels give one tile to kSplit waves: every wave of the
 // workgroup computes the same segments and spans (wave-uniform values) and
 // walks a contiguous slice of the flattened candidate list; the slices'

@@ -267,6 +267,10 @@ CCAKE_API int  ccake_profile_read(ccake_dev* dev, int kernel_id, double* total_m
 CCAKE_API int  ccake_profile_reset(ccake_dev* dev);
 /* Pair count of the last neighbour build (directed pairs, i != j). */
 CCAKE_API int  ccake_last_pair_count(ccake_event* ev, int64_t* n_pairs);
+/* Directed pairs summed over every smoothing pass since the event was
+ * created or since the last call with reset != 0 (the work count behind the
+ * pair kernels' throughput).  Diagnostic; no counterpart in the reference. */
+CCAKE_API int  ccake_pair_total(ccake_event* ev, int reset, int64_t* n_pairs);
 
 #ifdef __cplusplus
 }

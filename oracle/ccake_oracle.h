@@ -10,6 +10,14 @@
  * (the-nuclear-confectionery/CCAKE); each function cites the file:line it
  * follows.  Field numbering mirrors the product's field ids so that a test
  * can compare field k of both sides directly.
+ *
+ * Pinned by tests/test_oracle.py against the reference's known answers,
+ * brute force, closed forms, finite differences and the analytic ideal
+ * Gubser solution.  Parity unpinned:
+ *   - which root the reference's GSL hybrids solver returns where several
+ *     exist (the inversion here is Newton with the same absolute stop);
+ *   - the d alpha / d(s, rho) fields (not read by the path without diffusion);
+ *   - the shear and bulk terms of the equations of motion.
  */
 #ifndef CCAKE_ORACLE_H
 #define CCAKE_ORACLE_H
