@@ -13,11 +13,14 @@
  *
  * Pinned by tests/test_oracle.py against the reference's known answers,
  * brute force, closed forms, finite differences and the analytic ideal
- * Gubser solution.  Parity unpinned:
+ * Gubser solution, and by tests/test_gubser_viscous.py against the
+ * reference's semi-analytic Israel-Stewart Gubser profiles (shear terms,
+ * Gubser-mode tau_pi).  Parity unpinned:
  *   - which root the reference's GSL hybrids solver returns where several
  *     exist (the inversion here is Newton with the same absolute stop);
  *   - the d alpha / d(s, rho) fields (not read by the path without diffusion);
- *   - the shear and bulk terms of the equations of motion.
+ *   - the bulk-pressure terms of the equations of motion (zero in conformal
+ *     flows, so no analytic solution exercises them).
  */
 #ifndef CCAKE_ORACLE_H
 #define CCAKE_ORACLE_H
