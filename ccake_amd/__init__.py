@@ -10,6 +10,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import re
+import weakref
 
 import numpy as np
 
@@ -176,9 +177,17 @@ class Device:
         h = C.c_void_p()
         _ok(lib().ccake_dev_create(gpu, hbm_budget, C.byref(h)), "ccake_dev_create")
         self.h = h
+        self._children = weakref.WeakSet()   # Eos and Event handles on this device
 
     def close(self):
+        """Destroys the device after any events and EoS objects still open on it."""
         if getattr(self, "h", None):
+            kids = list(self._children)
+            for c in kids:
+                if isinstance(c, Event):
+                    c.close()
+            for c in kids:
+                c.close()
             lib().ccake_dev_destroy(self.h)
             self.h = None
 
@@ -221,9 +230,13 @@ class Eos:
         _ok(lib().ccake_eos_create(dev.h, C.byref(d), C.byref(h)), "ccake_eos_create")
         self.h = h
         self.dev = dev
+        self._events = weakref.WeakSet()
+        dev._children.add(self)
 
     def close(self):
         if getattr(self, "h", None):
+            for e in list(self._events):
+                e.close()
             lib().ccake_eos_destroy(self.h)
             self.h = None
 
@@ -251,6 +264,8 @@ class Event:
         h = C.c_void_p()
         _ok(lib().ccake_event_create(dev.h, C.byref(self._p), eos.h, self.n, C.byref(h)), "ccake_event_create")
         self.h = h
+        dev._children.add(self)
+        eos._events.add(self)
 
     def close(self):
         if getattr(self, "h", None):

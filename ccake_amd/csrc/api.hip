@@ -2,6 +2,7 @@
 // the RK2 step and the initialisation sequence.  See include/ccake_amd.h.
 #include "ccake_internal.h"
 #include "eos_device.h"
+#include <cmath>
 #include <cstring>
 #include <cstdio>
 #include <mutex>
@@ -152,6 +153,10 @@ int ccake_dev_create(int gpu, size_t budget, ccake_dev** out)
 int ccake_dev_destroy(ccake_dev* d)
 {
   if (!d) return CCAKE_ERR_INVALID;
+  if (d->live_eos || d->live_events) {
+    set_error("ccake_dev_destroy: destroy the device's events and EoS objects first");
+    return CCAKE_ERR_INVALID;
+  }
   if (hipStreamSynchronize(d->stream) != hipSuccess) { /* fall through: still release */ }
   prof_drain(d);
   hipError_t e = hipStreamDestroy(d->stream);
@@ -198,6 +203,7 @@ int ccake_eos_create(ccake_dev* dev, const ccake_eos_desc* desc, ccake_eos** out
     ch.n = 2;
     init_analytic(ch.m[0], M_CONF, c0, 1, 1, 1, 1, 0, 0);
     init_analytic(ch.m[1], M_DIAG, c0, 1, 1, 1, 1, 0, 0);
+    dev->live_eos++;
     *out = e;
     return 0;
   }
@@ -251,6 +257,7 @@ int ccake_eos_create(ccake_dev* dev, const ccake_eos_desc* desc, ccake_eos** out
   init_analytic(ch.m[1], M_TANH, c, 1.0, mu0[1], mu0[2], mu0[3], Tc, Ts);
   init_analytic(ch.m[2], M_CONF, c, 1.0, mu0[1], mu0[2], mu0[3], 0, 0);
   init_analytic(ch.m[3], M_DIAG, c, 1.0, mud[1], mud[2], mud[3], 0, 0);
+  dev->live_eos++;
   *out = e;
   return 0;
 }
@@ -258,6 +265,11 @@ int ccake_eos_create(ccake_dev* dev, const ccake_eos_desc* desc, ccake_eos** out
 int ccake_eos_destroy(ccake_eos* e)
 {
   if (!e) return CCAKE_ERR_INVALID;
+  if (e->live_events) {
+    set_error("ccake_eos_destroy: destroy the events that use this EoS first");
+    return CCAKE_ERR_INVALID;
+  }
+  if (e->dev) e->dev->live_eos--;
   if (e->d_tab) (void)hipFree(e->d_tab);
   if (e->d_tab4) (void)hipFree(e->d_tab4);
   if (e->dev) e->dev->used -= (e->dev->used >= e->bytes ? e->bytes : e->dev->used);
@@ -327,6 +339,9 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
   CCAKE_HIP(hipStreamSynchronize(st));
   ev->perm = ev->val_a;
   ev->skey = ev->key_a;
+  dev->live_events++;
+  eos->live_events++;
+  ev->counted = true;
   *out = ev;
   return 0;
 }
@@ -339,6 +354,7 @@ int ccake_event_destroy(ccake_event* ev)
                    ev->scan_tmp, ev->cell_off, ev->pair_count, ev->cand_s, ev->cand_g, ev->red, ev->err };
   for (void* b : bufs) if (b) (void)hipFree(b);
   ev->dev->used -= (ev->dev->used >= ev->bytes ? ev->bytes : ev->dev->used);
+  if (ev->counted) { ev->dev->live_events--; ev->eos->live_events--; }
   delete ev;
   return 0;
 }
@@ -350,6 +366,9 @@ int ccake_event_get_time(ccake_event* ev, double* t) { if (!ev || !t) return CCA
 int ccake_event_upload(ccake_event* ev, int field, const double* host, size_t n)
 {
   if (!ev || !host || field < 0 || field >= CCAKE_NUM_FIELDS || n != (size_t)ev->n) return CCAKE_ERR_INVALID;
+  if (field == CCAKE_F_X || field == CCAKE_F_Y || field == CCAKE_F_ETA)
+    for (size_t i = 0; i < n; ++i)
+      if (!std::isfinite(host[i])) { set_error("non-finite particle position"); return CCAKE_ERR_INVALID; }
   CCAKE_HIP(hipMemcpyAsync(col(ev, field), host, n * sizeof(double), hipMemcpyHostToDevice, ev->dev->stream));
   CCAKE_HIP(hipStreamSynchronize(ev->dev->stream));
   return 0;

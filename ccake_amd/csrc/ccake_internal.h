@@ -70,6 +70,8 @@ struct ccake_dev {
   hipStream_t stream = nullptr;
   size_t budget = 0;
   size_t used = 0;
+  int live_eos = 0;          // handles created on this device and not yet destroyed
+  int live_events = 0;
   ccake::Profile prof;
 };
 
@@ -79,6 +81,7 @@ struct ccake_eos {
   double* d_tab = nullptr;
   double* d_tab4 = nullptr;
   size_t bytes = 0;
+  int live_events = 0;       // events created on this EoS and not yet destroyed
   // host copy for host-side evaluation (efreeze / full_thermo)
   double* h_tab = nullptr;
 };
@@ -86,6 +89,7 @@ struct ccake_eos {
 struct ccake_event {
   ccake_dev* dev = nullptr;
   ccake_eos* eos = nullptr;
+  bool counted = false;      // included in the device's and EoS's live counts
   ccake_params p{};
   int n = 0;
   int npad = 0;

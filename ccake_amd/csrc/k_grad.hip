@@ -20,8 +20,8 @@ constexpr int kCandG = 16;
 
 struct GradW {
   double hinv, ng, ng2;   // kgrad/h^3, kgrad2/h^3 (src/kernel.cpp:95-118)
-  // scalar g with grad W = g * rel, for r2 > 0 (coincident particles sit at
-  // q = 0, on the inner branch, which does not divide by r)
+  // scalar g with grad W = g * rel for a neighbour (coincident particles sit
+  // at q = 0, on the inner branch, which does not divide by r)
   __device__ __forceinline__ double operator()(double r2) const
   {
     double r, ir;
@@ -30,8 +30,8 @@ struct GradW {
     const double t = 2.0 - q;
     const double far = ng * ir * t * t;
     const double near = ng2 * (1.5 * q - 2.0) * hinv;
-    const double v = (q >= 1.0) ? far : near;
-    return (q >= 2.0) ? 0.0 : v;
+    // only neighbours (r <= 2h) use the value: at q >= 2, t^2 = 0 to rounding
+    return (q >= 1.0) ? far : near;
   }
 };
 

@@ -45,12 +45,22 @@ struct KernelW {
     const double v = (q >= 1.0) ? b : a;
     return (q >= 2.0) ? 0.0 : v;
   }
+  // For neighbours only (r <= 2h): the q >= 2 case is then (2 - q)^3 = 0 up
+  // to rounding, so the outer branch needs no cut.
+  __device__ __forceinline__ double of_q_inside(double q) const
+  {
+    const double qq = q * q;
+    const double a = norm * (1.0 - 1.5 * qq + 0.75 * q * qq);
+    const double t = 2.0 - q;
+    const double b = norm2 * t * t * t;
+    return (q >= 1.0) ? b : a;
+  }
   __device__ __forceinline__ double operator()(double r2) const { return of_q(sqrt(r2) * hinv); }
   __device__ __forceinline__ double fast(double r2) const
   {
     double r, ir;
     sqrt_rsqrt(r2, r, ir);
-    return of_q(r * hinv);
+    return of_q_inside(r * hinv);
   }
 };
 

@@ -140,23 +140,19 @@ __device__ __forceinline__ void for_spans_slice(const Grid& g, const int32_t* __
   }
 }
 
-// sqrt(x) and 1/sqrt(x) for x >= 0 from the hardware reciprocal square root
-// refined by coupled Newton steps (Goldschmidt's iteration for g = sqrt(x),
-// h = 1/(2 sqrt(x))); accurate to about one ulp, which the pair sums
-// tolerate.  sqrt(0) = 0 (1/sqrt(0) is not finite and must not be used).
-// The pair predicate never uses it.
+// sqrt(x) and 1/sqrt(x) for x >= 0: the hardware reciprocal square root
+// (v_rsq_f64) refined by one Newton step y <- y (3 - x y^2) / 2, relative
+// error about 1e-14, which the pair sums tolerate.  x is floored at 1e-300
+// so that coincident particles get r = 1e-150 (q = 0 to rounding) instead of
+// 0 * inf.  The pair predicate never uses it.
 __device__ __forceinline__ void sqrt_rsqrt(double x, double& r, double& ir)
 {
-  const double y = __builtin_amdgcn_rsq(x);
-  double g = x * y, h = 0.5 * y;
-  const double e = fma(-g, h, 0.5);
-  g = fma(g, e, g);
-  h = fma(h, e, h);
-  const double d = fma(-g, g, x);
-  const double s = fma(d, h, g);
-  const double e2 = fma(-s, h, 0.5);
-  ir = 2.0 * fma(h, e2, h);
-  r = (x > 0.0) ? s : 0.0;
+  const double xc = fmax(x, 1e-300);
+  const double y = __builtin_amdgcn_rsq(xc);
+  const double h = 0.5 * xc;
+  const double e = fma(-(h * y), y, 1.5);
+  ir = y * e;
+  r = xc * ir;
 }
 
 // Iterate over the row segments of the tile owned by this wave.  `fn(seg,

@@ -163,6 +163,9 @@ typedef struct ccake_event ccake_event;
  * Replaces the Kokkos DeviceType/ExecutionSpace typedefs
  * (include/utilities.h:42-55). */
 CCAKE_API int  ccake_dev_create(int gpu, size_t hbm_budget_bytes, ccake_dev** out);
+/* Handles are destroyed children first: events, then EoS objects, then the
+ * device; destroying a parent that still has live children is refused
+ * (CCAKE_ERR_INVALID) and leaves everything valid. */
 CCAKE_API int  ccake_dev_destroy(ccake_dev* dev);
 CCAKE_API int  ccake_dev_sync(ccake_dev* dev);
 /* Bytes of HBM the library holds on this device, and its budget. */

@@ -396,3 +396,108 @@ def test_full_size_table_event_properties(dev, orc, table):
     assert np.array_equal(p.g.download("CCAKE_F_EXT_B"), B0)
     g2.close()
     p.close()
+
+
+# ------------------------------------------------------------------ edge cases
+def test_invalid_inputs_rejected(dev):
+    """Empty events and non-finite positions are refused on the host."""
+    eos = ccake_amd.Eos(dev, 0)
+    with pytest.raises(ccake_amd.CcakeError):
+        ccake_amd.Event(dev, eos, config.base_params(), 0)
+    ev = ccake_amd.Event(dev, eos, config.base_params(), 10)
+    x = np.zeros(10); x[3] = np.nan
+    with pytest.raises(ccake_amd.CcakeError):
+        ev.upload("CCAKE_F_X", x)
+    ev.close(); eos.close()
+
+
+def test_pairs_bit_exact_on_coarsened_grid(dev, orc):
+    """A domain too wide for the 65,536-cell cap (cells are doubled until it
+    fits): clusters far apart, pairs still bit-exact."""
+    rng = np.random.default_rng(21)
+    c = rng.uniform(-400, 400, size=(40, 2))
+    pts = np.concatenate([cc + rng.uniform(-1.5, 1.5, size=(60, 2)) for cc in c])
+    x, y = pts[:, 0].copy(), pts[:, 1].copy()
+    n = len(x)
+    cfg = config.base_params(hT=0.3)
+    eos = ccake_amd.Eos(dev, 0)
+    ev = ccake_amd.Event(dev, eos, cfg, n)
+    ev.upload("CCAKE_F_X", x); ev.upload("CCAKE_F_Y", y)
+    ev.build_neighbors()
+    gi, gj = ev.pairs()
+    oeos = orc.Eos(0)
+    o = orc.Event(cfg, oeos, n)
+    o.set("ORC_F_X", x); o.set("ORC_F_Y", y)
+    o.build_neighbors(True)
+    oi, oj = o.pairs()
+    assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
+    ev.close(); eos.close()
+
+
+def test_sparse_event_with_isolated_particles(dev, orc):
+    """A sparse viscous event where some particles have no neighbours: the
+    freeze-out machine marks them (Freeze 4, btrack -1) identically, and the
+    fields agree after 3 steps."""
+    rng = np.random.default_rng(5)
+    n = 600
+    x = rng.uniform(-6, 6, n); y = rng.uniform(-6, 6, n)
+    e = (5.0 + 20.0 * np.exp(-(x * x + y * y) / 8.0)) / config.HBARC_GEVFM
+    z = np.zeros(n)
+    d = dict(x=x, y=y, e=e, ux=z, uy=z.copy(), rhoB=z.copy(), rhoS=z.copy(), rhoQ=z.copy())
+    cfg = config.base_params(t0=0.6, dt=0.01, particlization=1, eta_mode=config.ETA_CONSTANT,
+                             constant_eta_over_s=0.08)
+    p = Pair(dev, orc, cfg, d)
+    p.initialize()
+    ints_equal(p.g, p.o, "BTRACK")
+    ints_equal(p.g, p.o, "FREEZE")
+    assert np.sum(p.o.ifield("ORC_I_BTRACK") == 0) + np.sum(p.o.ifield("ORC_I_BTRACK") == -1) > 0
+    p.step(3)
+    ints_equal(p.g, p.o, "FREEZE")
+    ints_equal(p.g, p.o, "BTRACK")
+    assert np.any(p.o.ifield("ORC_I_FREEZE") == 4)
+    compare(p.g, p.o, STATE + DERIVS, what="sparse 3 steps")
+    p.close()
+
+
+def test_reductions_match_oracle(dev, orc):
+    """Conservation sums and the frozen count (SystemState::conservation_*)
+    on the viscous lumpy event after 3 steps."""
+    d = lumpy_small(0.1)
+    p = Pair(dev, orc, config.lumpy_params(), d)
+    p.initialize()
+    p.step(3)
+    for name in ("ENTROPY", "BARYON", "STRANGE", "CHARGE", "ENERGY", "EZ", "FROZEN"):
+        a = p.g.reduce("CCAKE_R_" + name)
+        b = p.o.reduce(ccake_amd.REDUCTIONS["CCAKE_R_" + name])
+        assert abs(a - b) <= 1e-12 * max(abs(b), 1e-300) or a == b, (name, a, b)
+    p.close()
+
+
+def test_freeze_out_run_to_completion_on_gpu(dev):
+    """The 10,201-particle lumpy event on the table EoS evolved until every
+    particle has frozen out (SPHWorkstation::continue_evolution): the run
+    ends before max_tau, every particle ends with Freeze >= 4, and two runs
+    agree to the bit."""
+    tab, dims, mins, maxs = eos_tables.tanh_conformal_table()
+    eos = ccake_amd.Eos(dev, 1, T_FO, dims, mins, maxs, tab)
+    cfg = dict(config.lumpy_params(), e_freeze=eos.efreeze(T_FO))
+    d = lumpy_small(0.1)
+    outs = []
+    for _ in range(2):
+        ev = ccake_amd.Event(dev, eos, cfg, len(d["x"]))
+        ccake_amd.load_event(ev, d)
+        ev.initialize()
+        steps = 0
+        while ev.continue_evolution(max_tau=30.0):
+            ev.advance_timestep(cfg["dt"], 2)
+            steps += 1
+            assert steps < 3000
+        fz = ev.download_int("CCAKE_I_FREEZE")
+        outs.append((steps, ev.t, fz, ev.download("CCAKE_F_T"), ev.download("CCAKE_F_UX")))
+        ev.close()
+    (s1, t1, f1, T1, u1), (s2, t2, f2, T2, u2) = outs
+    print("steps", s1, "t", t1)
+    assert t1 < 30.0 and np.all(f1 >= 4)
+    assert s1 == s2 and t1 == t2 and np.array_equal(f1, f2)
+    assert np.array_equal(T1, T2) and np.array_equal(u1, u2)
+    eos.close()

@@ -119,7 +119,10 @@ def test_gpu_viscous_gubser_parity(dev, orc, sol):
 @pytest.mark.gpu
 def test_gpu_viscous_gubser_physics(dev, sol):
     """100 RK2 steps (tau 1.0 -> 1.5) on the 0.05 fm lattice (40,401
-    particles) against the semi-analytic profile."""
+    particles) against the semi-analytic profile.  The oracle's errors for
+    this run are T 1.1 %, u 1.8 %, pi^xx 3.0 %, pi^xy 6.0 %; with eta/s
+    halved they are T 2.0 %, u 3.9 %, pi^xx 35 %, pi^xy 43 %.  The bounds
+    sit between the two."""
     import ccake_amd
     ev, eos, cfg = gpu_event(dev, sol, 0.05)
     ev.initialize()
@@ -131,6 +134,6 @@ def test_gpu_viscous_gubser_physics(dev, sol):
                           ev.download(s0 + 6) * gv.HBARC)
     print(e)
     assert abs(ev.t - 1.5) < 1e-9
-    assert e["T"] < 0.03 and e["ux"] < 0.013
-    assert e["pixx"] < 0.06 and e["piyy"] < 0.06 and e["pixy"] < 0.12
+    assert e["T"] < 0.016 and e["ux"] < 0.028
+    assert e["pixx"] < 0.10 and e["piyy"] < 0.10 and e["pixy"] < 0.15
     ev.close(); eos.close()
