@@ -49,11 +49,14 @@ def gubser_ideal(spacing=0.05, half_width=5.0, tau0=1.0, q=1.0, eps0_gev=80.0):
 
 
 def lumpy(seed=0xCCA4E001, spacing=0.02, half_width=5.0, n_blobs=50, sigma=0.4,
-          center_sd=3.0, amp_median_gev=20.0, amp_logsd=0.5, charge_frac=0.05):
+          center_sd=3.0, amp_median_gev=20.0, amp_logsd=0.5, charge_frac=0.05, e_cutoff_gev=0.0):
     """Lumpy event: e = sum of n_blobs 2D Gaussians (width sigma) with centres
     ~ N(0, center_sd) per axis and amplitudes ~ LogNormal(ln amp_median, amp_logsd)
     GeV/fm^3; rho_X = sum_k f_{X,k} blob_k with f ~ U(-charge_frac, charge_frac)
-    (in fm^-3 per GeV/fm^3).  Draw order per blob: c_x, c_y, A, f_B, f_S, f_Q."""
+    (in fm^-3 per GeV/fm^3).  Draw order per blob: c_x, c_y, A, f_B, f_S, f_Q.
+    Lattice points with e <= e_cutoff_gev (GeV/fm^3) are dropped, as the
+    reference's initial-condition readers do (src/BSQHydro.cpp:129, 402-405;
+    default 0.15 GeV/fm^3 in include/defaults.h:31)."""
     rng = np.random.Generator(np.random.MT19937(seed))
     x, y = lattice(spacing, half_width)
     e = np.zeros_like(x)
@@ -68,5 +71,7 @@ def lumpy(seed=0xCCA4E001, spacing=0.02, half_width=5.0, n_blobs=50, sigma=0.4,
         rb += fB * blob
         rs += fS * blob
         rq += fQ * blob
+    keep = e > e_cutoff_gev
+    x, y, e, rb, rs, rq = x[keep], y[keep], e[keep], rb[keep], rs[keep], rq[keep]
     z = np.zeros_like(x)
     return dict(x=x, y=y, e=e / HBARC_GEVFM, ux=z, uy=z.copy(), rhoB=rb, rhoS=rs, rhoQ=rq)

@@ -473,31 +473,52 @@ def test_reductions_match_oracle(dev, orc):
     p.close()
 
 
-def test_freeze_out_run_to_completion_on_gpu(dev):
-    """The 10,201-particle lumpy event on the table EoS evolved until every
-    particle has frozen out (SPHWorkstation::continue_evolution): the run
-    ends before max_tau, every particle ends with Freeze >= 4, and two runs
-    agree to the bit."""
+def test_freeze_out_run_to_completion_on_gpu(dev, orc):
+    """The lumpy event on the table EoS (0.1 fm lattice, particles below the
+    reference's default e_cutoff of 0.15 GeV/fm^3 dropped: 8,077 particles)
+    evolved until every particle has frozen out
+    (SPHWorkstation::continue_evolution): the run ends before max_tau with
+    every particle at Freeze >= 4, two GPU runs agree to the bit, and the
+    oracle needs the same number of steps (within 3) with the same frozen
+    counts along the way (within 0.5 %)."""
     tab, dims, mins, maxs = eos_tables.tanh_conformal_table()
     eos = ccake_amd.Eos(dev, 1, T_FO, dims, mins, maxs, tab)
     cfg = dict(config.lumpy_params(), e_freeze=eos.efreeze(T_FO))
-    d = lumpy_small(0.1)
+    d = ic.lumpy(spacing=0.1, e_cutoff_gev=0.15)
+    n = len(d["x"])
     outs = []
     for _ in range(2):
-        ev = ccake_amd.Event(dev, eos, cfg, len(d["x"]))
+        ev = ccake_amd.Event(dev, eos, cfg, n)
         ccake_amd.load_event(ev, d)
         ev.initialize()
-        steps = 0
+        steps, frozen = 0, []
         while ev.continue_evolution(max_tau=30.0):
             ev.advance_timestep(cfg["dt"], 2)
             steps += 1
+            if steps % 50 == 0:
+                frozen.append(ev.reduce("CCAKE_R_FROZEN"))
             assert steps < 3000
         fz = ev.download_int("CCAKE_I_FREEZE")
-        outs.append((steps, ev.t, fz, ev.download("CCAKE_F_T"), ev.download("CCAKE_F_UX")))
+        outs.append((steps, ev.t, fz, ev.download("CCAKE_F_T"), ev.download("CCAKE_F_UX"), frozen))
         ev.close()
-    (s1, t1, f1, T1, u1), (s2, t2, f2, T2, u2) = outs
-    print("steps", s1, "t", t1)
+    (s1, t1, f1, T1, u1, fr1), (s2, t2, f2, T2, u2, fr2) = outs
+    print("steps", s1, "t", t1, "frozen every 50 steps", fr1)
     assert t1 < 30.0 and np.all(f1 >= 4)
-    assert s1 == s2 and t1 == t2 and np.array_equal(f1, f2)
+    assert s1 == s2 and t1 == t2 and np.array_equal(f1, f2) and fr1 == fr2
     assert np.array_equal(T1, T2) and np.array_equal(u1, u2)
     eos.close()
+
+    oeos = orc.Eos(1, T_FO, dims, mins, maxs, tab)
+    o = orc.Event(cfg, oeos, n)
+    oracle_load(o, d)
+    assert o.initialize() == 0
+    steps, frozen = 0, []
+    while not np.all(o.ifield("ORC_I_FREEZE") >= 4) and o.t <= 30.0:
+        assert o.advance_timestep(cfg["dt"], 2) == 0
+        steps += 1
+        if steps % 50 == 0:
+            frozen.append(o.reduce(ccake_amd.REDUCTIONS["CCAKE_R_FROZEN"]))
+    print("oracle steps", steps, "frozen every 50 steps", frozen)
+    assert abs(steps - s1) <= 3
+    for a, b in zip(fr1, frozen):
+        assert abs(a - b) <= 0.005 * n
