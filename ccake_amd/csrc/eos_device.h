@@ -74,55 +74,26 @@ CK_HD void diag_fields(const EosMember& m, const double* x, double* r)
   r[X_TT] = 12.0 * cp * T * T / (T0 * T0 * T0 * T0);
 }
 
-// 4x4 linear solve with partial pivoting (first largest |pivot|); false if
-// singular.  Rows are swapped with selects over compile-time indices so the
-// matrix stays in registers; the arithmetic is that of the textbook loop.
+// 4x4 linear solve with partial pivoting; false if singular.
 CK_HD bool solve_4x4(const double* J, const double* b, double* x)
 {
   double A[4][5];
-  #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    #pragma unroll
-    for (int j = 0; j < 4; ++j) A[i][j] = J[4 * i + j];
-    A[i][4] = b[i];
-  }
-  bool ok = true;
-  #pragma unroll
+  for (int i = 0; i < 4; ++i) { for (int j = 0; j < 4; ++j) A[i][j] = J[4 * i + j]; A[i][4] = b[i]; }
   for (int c = 0; c < 4; ++c) {
     int pv = c;
-    double best = fabs(A[c][c]);
-    #pragma unroll
-    for (int i = c + 1; i < 4; ++i) {
-      const double v = fabs(A[i][c]);
-      if (v > best) { best = v; pv = i; }
-    }
-    #pragma unroll
-    for (int i = c + 1; i < 4; ++i) {
-      const bool sw = (pv == i);
-      #pragma unroll
-      for (int j = 0; j < 5; ++j) {
-        const double a = A[c][j], t = A[i][j];
-        A[c][j] = sw ? t : a;
-        A[i][j] = sw ? a : t;
-      }
-    }
-    if (A[c][c] == 0.0 || !__builtin_isfinite(A[c][c])) ok = false;
-    #pragma unroll
+    for (int i = c + 1; i < 4; ++i) if (fabs(A[i][c]) > fabs(A[pv][c])) pv = i;
+    if (A[pv][c] == 0.0 || !__builtin_isfinite(A[pv][c])) return false;
+    if (pv != c) for (int j = 0; j < 5; ++j) { const double t = A[c][j]; A[c][j] = A[pv][j]; A[pv][j] = t; }
     for (int i = c + 1; i < 4; ++i) {
       const double f = A[i][c] / A[c][c];
-      #pragma unroll
       for (int j = c; j < 5; ++j) A[i][j] -= f * A[c][j];
     }
   }
-  if (!ok) return false;
-  #pragma unroll
   for (int i = 3; i >= 0; --i) {
     double v = A[i][4];
-    #pragma unroll
     for (int j = i + 1; j < 4; ++j) v -= A[i][j] * x[j];
     x[i] = v / A[i][i];
   }
-  #pragma unroll
   for (int i = 0; i < 4; ++i) if (!__builtin_isfinite(x[i])) return false;
   return true;
 }
@@ -380,29 +351,12 @@ CK_HD double schur3(double lead, double a0, double a1, double a2,
   if (fabs(m11) < EPS) m11 = (m11 >= 0.0 ? EPS : -EPS);
   if (fabs(m22) < EPS) m22 = (m22 >= 0.0 ? EPS : -EPS);
   double A[3][4] = { { m00, m01, m02, b0 }, { m10, m11, m12, b1 }, { m20, m21, m22, b2 } };
-  #pragma unroll
   for (int c = 0; c < 3; ++c) {
     int pv = c;
-    double best = fabs(A[c][c]);
-    #pragma unroll
-    for (int i = c + 1; i < 3; ++i) {
-      const double v = fabs(A[i][c]);
-      if (v > best) { best = v; pv = i; }
-    }
-    #pragma unroll
-    for (int i = c + 1; i < 3; ++i) {
-      const bool sw = (pv == i);
-      #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const double a = A[c][j], t = A[i][j];
-        A[c][j] = sw ? t : a;
-        A[i][j] = sw ? a : t;
-      }
-    }
-    #pragma unroll
+    for (int i = c + 1; i < 3; ++i) if (fabs(A[i][c]) > fabs(A[pv][c])) pv = i;
+    if (pv != c) for (int j = 0; j < 4; ++j) { const double t = A[c][j]; A[c][j] = A[pv][j]; A[pv][j] = t; }
     for (int i = c + 1; i < 3; ++i) {
       const double f = A[i][c] / A[c][c];
-      #pragma unroll
       for (int j = c; j < 4; ++j) A[i][j] -= f * A[c][j];
     }
   }
