@@ -110,7 +110,7 @@ def cpu_baseline(d, cfg, table, steps):
 # + 1 int32.
 BYTES_PER_PARTICLE_EOS = 24 * 8 + 2 * 4 + 77 * 8 + 4
 
-KERNEL_OF_GROUP = {"SMOOTH": "k_smooth_tile", "GRAD": "k_grad", "EOS": "k_eos"}
+KERNEL_OF_GROUP = {"SMOOTH": "k_smooth_slice", "GRAD": "k_grad_slice", "EOS": "k_eos"}
 
 
 def traffic_of(kernel, workload):

@@ -2,6 +2,7 @@
 // the RK2 step and the initialisation sequence.  See include/ccake_amd.h.
 #include "ccake_internal.h"
 #include "eos_device.h"
+#include "pair_common.h"
 #include <cmath>
 #include <cstring>
 #include <cstdio>
@@ -80,14 +81,20 @@ static int time_derivatives(ccake_event* ev, int fuse_rk, double dt_stage, doubl
   if ((rc = launch_gather_smooth(ev))) return rc;               // candidate records in cell order
   prof_end(d, CCAKE_K_SORT);
   prof_begin(d, CCAKE_K_SMOOTH);
-  if ((rc = launch_smooth(ev, 0))) return rc;                   // 03 (+ btrack)
+  if ((rc = launch_smooth_pairs(ev, 0, true))) return rc;       // 03 pair sums per slice
   prof_end(d, CCAKE_K_SMOOTH);
+  prof_begin(d, CCAKE_K_SMOOTH_SUM);
+  if ((rc = launch_smooth_sum(ev, 0, true))) return rc;         // 03 fields (+ btrack)
+  prof_end(d, CCAKE_K_SMOOTH_SUM);
   prof_begin(d, CCAKE_K_EOS);
   if ((rc = launch_eos(ev, 0, 1))) return rc;                   // 04 + 05 + 08 + gather
   prof_end(d, CCAKE_K_EOS);
   prof_begin(d, CCAKE_K_GRAD);
-  if ((rc = launch_grad(ev))) return rc;                        // 09
+  if ((rc = launch_grad_pairs(ev))) return rc;                  // 09 pair sums per slice
   prof_end(d, CCAKE_K_GRAD);
+  prof_begin(d, CCAKE_K_GRAD_SUM);
+  if ((rc = launch_grad_sum(ev))) return rc;                    // 09 gradients
+  prof_end(d, CCAKE_K_GRAD_SUM);
   prof_begin(d, CCAKE_K_EOM);
   if ((rc = launch_eom(ev, fuse_rk, dt_stage, t0))) return rc;  // 10 (+ RK update)
   prof_end(d, CCAKE_K_EOM);
@@ -144,6 +151,9 @@ int ccake_dev_create(int gpu, size_t budget, ccake_dev** out)
   ccake_dev* d = new ccake_dev();
   d->gpu = gpu;
   d->budget = budget;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, gpu) == hipSuccess && prop.multiProcessorCount > 0)
+    d->simds = 4 * prop.multiProcessorCount;
   hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete d; return hip_fail(e, "hipStreamCreate"); }
   *out = d;
@@ -322,6 +332,8 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
   rc |= dalloc(ev, &ev->scan_tmp, (size_t)nscan_blocks);
   rc |= dalloc(ev, &ev->cell_off, (size_t)ev->cap_cells + 1);
   rc |= dalloc(ev, &ev->pair_count, 2);
+  ev->nslices = choose_slices((n + kTile - 1) / kTile, dev->simds);
+  rc |= dalloc(ev, &ev->part, (size_t)ev->nslices * kMaxAcc * np);
   rc |= dalloc(ev, &ev->cand_s, 8 * np);
   rc |= dalloc(ev, &ev->cand_g, 16 * np);
   rc |= dalloc(ev, &ev->red, 1024);
@@ -351,7 +363,8 @@ int ccake_event_destroy(ccake_event* ev)
   if (!ev) return CCAKE_ERR_INVALID;
   (void)hipStreamSynchronize(ev->dev->stream);
   void* bufs[] = { ev->F, ev->I, ev->grid, ev->bbox, ev->key, ev->key_a, ev->key_b, ev->val_a, ev->val_b, ev->hist,
-                   ev->scan_tmp, ev->cell_off, ev->pair_count, ev->cand_s, ev->cand_g, ev->red, ev->err };
+                   ev->scan_tmp, ev->cell_off, ev->pair_count, ev->part, ev->cand_s, ev->cand_g, ev->red,
+                   ev->err };
   for (void* b : bufs) if (b) (void)hipFree(b);
   ev->dev->used -= (ev->dev->used >= ev->bytes ? ev->bytes : ev->dev->used);
   if (ev->counted) { ev->dev->live_events--; ev->eos->live_events--; }

@@ -67,6 +67,7 @@ struct Profile {
 
 struct ccake_dev {
   int gpu = 0;
+  int simds = 1024;          // SIMDs of the device (4 per CU)
   hipStream_t stream = nullptr;
   size_t budget = 0;
   size_t used = 0;
@@ -113,6 +114,9 @@ struct ccake_event {
   int32_t* rank = nullptr;      // particle id -> sorted position
   int64_t* pair_count = nullptr;  // device accumulators: [0] last stage, [1] since the last reset
   int radix_blocks = 0;
+  // pair-kernel partial sums: nslices x (up to 17) x npad, sorted order
+  int nslices = 1;
+  double* part = nullptr;
   // gathered candidates (sorted order)
   double* cand_s = nullptr;     // 8 doubles per particle
   double* cand_g = nullptr;     // 16 doubles per particle
@@ -146,6 +150,8 @@ int launch_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, s
 // smoothing
 int launch_gather_smooth(ccake_event* ev);
 int launch_smooth(ccake_event* ev, int mode, bool count_pairs = true);  // mode 0 fields, 1 initial sigma, 2 counts
+int launch_smooth_pairs(ccake_event* ev, int mode, bool count_pairs);   // pair part (mode 2: whole count walk)
+int launch_smooth_sum(ccake_event* ev, int mode, bool count_pairs);     // slice sums + fields (modes 0, 1)
 // EoS + per-particle physics
 int launch_eos(ccake_event* ev, int mode, int fuse_post);   // fuse_post: reset_pi + transport + gradient gather
 int launch_reset_pi(ccake_event* ev);
@@ -154,6 +160,8 @@ int launch_gather_grad(ccake_event* ev);
 int host_eos_full(const ccake_eos* eos, int member, const double* x, double* r17);
 // gradients
 int launch_grad(ccake_event* ev);
+int launch_grad_pairs(ccake_event* ev);
+int launch_grad_sum(ccake_event* ev);
 // EoM + RK
 int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0);
 int launch_cache(ccake_event* ev);

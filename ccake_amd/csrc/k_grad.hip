@@ -1,4 +1,4 @@
-// Gradient pair kernel on gfx950 (owner computes, no atomics).
+// Gradient pair kernels on gfx950 (owner computes, no atomics).
 // Replaces SPHWorkstation::smooth_all_particle_gradients
 // (src/sph_workstation.cpp:796-986), D = 2, diffusion terms off.
 //
@@ -18,40 +18,27 @@ namespace ccake {
 
 constexpr int kCandG = 16;
 
-struct GradW {
-  double hinv, ng, ng2;   // kgrad/h^3, kgrad2/h^3 (src/kernel.cpp:95-118)
-  // scalar g with grad W = g * rel for a neighbour (coincident particles sit
-  // at q = 0, on the inner branch, which does not divide by r)
-  __device__ __forceinline__ double operator()(double r2) const
-  {
-    double r, ir;
-    sqrt_rsqrt(r2, r, ir);
-    const double q = r * hinv;
-    const double t = 2.0 - q;
-    const double far = ng * ir * t * t;
-    const double near = ng2 * (1.5 * q - 2.0) * hinv;
-    // only neighbours (r <= 2h) use the value: at q >= 2, t^2 = 0 to rounding
-    return (q >= 1.0) ? far : near;
-  }
-};
-
-// One 64-particle tile per workgroup of kSplit waves; each wave walks a
-// slice of the tile's candidates and wave 0 adds the slices in wave order.
+// Pair part: one single-wave workgroup per (tile, slice) unit walks its
+// slice of the tile's candidates and stores the slice's neighbour-side sums;
+// k_grad_sum adds the slices in slice order and applies the owner terms.
 template <bool SHEAR>
-__global__ __launch_bounds__(kSplit * kWave) void k_grad(const double* __restrict__ cand,
-                                                         const int32_t* __restrict__ skey,
-                                                         const int32_t* __restrict__ perm,
-                                                         const int32_t* __restrict__ cell_off,
-                                                         const Grid* __restrict__ gridp, int n,
-                                                         double rc2, double rcs, GradW G,
-                                                         double* __restrict__ F, int npad)
+constexpr int grad_nacc() { return SHEAR ? 17 : 12; }
+
+template <bool SHEAR>
+__global__ __launch_bounds__(kWave) void k_grad_slice(const double* __restrict__ cand,
+                                                      const int32_t* __restrict__ skey,
+                                                      const int32_t* __restrict__ cell_off,
+                                                      const Grid* __restrict__ gridp, int n,
+                                                      double rcs, double hinv2, int nslices,
+                                                      double* __restrict__ part, int npad)
 {
-  constexpr int NACC = SHEAR ? 17 : 12;
-  __shared__ double red[kSplit - 1][NACC][kWave];
+  constexpr int NACC = grad_nacc<SHEAR>();
   const Grid g = *gridp;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wave = uni((int)(threadIdx.x / kWave));   // wave-uniform
-  const int k = xcd_tile(blockIdx.x, gridDim.x) * kTile + lane;
+  const int lane = threadIdx.x;
+  const int unit = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = unit / nslices;
+  const int slice = unit - tile * nslices;
+  const int k = tile * kTile + lane;
   const bool act = k < n;
   const int kk = act ? k : 0;
   const double* me = cand + (size_t)kk * kCandG;
@@ -63,20 +50,19 @@ __global__ __launch_bounds__(kSplit * kWave) void k_grad(const double* __restric
   for (int q = 0; q < NACC; ++q) acc[q] = 0.0;
 
   for_segments(act, myrow, xa, ya, [&](const Seg& seg, bool in) {
-    for_spans_slice(g, cell_off, seg, rcs, wave, [&](int j0, int j1) {
-      // Candidate records are wave-uniform (scalar loads into SGPRs); two
-      // records are requested together so the second arrives while the
-      // first is being used.
+    if (!in) return;   // lanes of other cell rows sit this walk out
+    for_spans_slice(g, cell_off, seg, rcs, slice, nslices, [&](int j0, int j1) {
+      // Candidate records are wave-uniform (scalar loads into SGPRs); the
+      // shear record is too large to hold two steps of two in SGPRs.
       constexpr int NR = SHEAR ? 13 : 8;
-      auto one = [&](const double* c, int j) {
-        const double dx = xa - c[0], dy = ya - c[1];
+      walk_records<NR, kCandG, SHEAR ? 1 : 2>(cand, xa, ya, j0, j1,
+                                              [&](const double* c, double dx, double dy, int j, bool valid) {
         const double r2 = dist2_nocontract(dx, dy);
-        const bool nb = in && (r2 <= rc2) && (j != k);
-        if (kSkipEmpty && __ballot(nb) == 0ull) return;
-        const double gv = G(r2);
-        const double gg = nb ? gv : 0.0;
-        const double wx = c[2] * (gg * dx), wy = c[2] * (gg * dy);
-        acc[0] += wx; acc[1] += wy;                    // sum m_b grad W
+        double q, iq;
+        q_and_inverse(r2, hinv2, q, iq);
+        const double gm = valid ? c[2] * spline_g(q, iq) : 0.0;
+        const double wx = gm * dx, wy = gm * dy;
+        acc[0] += wx; acc[1] += wy;                    // sum m_b grad W / C
         acc[2] += c[3] * wx; acc[3] += c[3] * wy;      // p_b / sl_b^2
         acc[4] += c[4] * wx; acc[5] += c[4] * wy;      // e_b
         acc[6] += c[5] * wx; acc[7] += c[5] * wy;      // Pi_b / sl_b^2
@@ -84,38 +70,46 @@ __global__ __launch_bounds__(kSplit * kWave) void k_grad(const double* __restric
         acc[10] += c[6] * wy; acc[11] += c[7] * wy;    // v_b d_y W
         if (SHEAR) {
           const double cv = wx * vax + wy * vay;
-          acc[12] += cv;
-          acc[13] += cv * c[SHEAR ? 8 : 0]; acc[14] += cv * c[SHEAR ? 9 : 0];
-          acc[15] += wx * c[SHEAR ? 10 : 0] + wy * c[SHEAR ? 11 : 0];
-          acc[16] += wx * c[SHEAR ? 11 : 0] + wy * c[SHEAR ? 12 : 0];
+          acc[SHEAR ? 12 : 0] += cv;
+          acc[SHEAR ? 13 : 0] += cv * c[SHEAR ? 8 : 0]; acc[SHEAR ? 14 : 0] += cv * c[SHEAR ? 9 : 0];
+          acc[SHEAR ? 15 : 0] += wx * c[SHEAR ? 10 : 0] + wy * c[SHEAR ? 11 : 0];
+          acc[SHEAR ? 16 : 0] += wx * c[SHEAR ? 11 : 0] + wy * c[SHEAR ? 12 : 0];
         }
-      };
-      int j = j0;
-      for (; j + 1 < j1; j += 2) {
-        double ca[NR], cb[NR];
-        for (int q = 0; q < NR; ++q) ca[q] = cand[(size_t)j * kCandG + q];
-        for (int q = 0; q < NR; ++q) cb[q] = cand[(size_t)(j + 1) * kCandG + q];
-        one(ca, j);
-        one(cb, j + 1);
-      }
-      if (j < j1) {
-        double ca[NR];
-        for (int q = 0; q < NR; ++q) ca[q] = cand[(size_t)j * kCandG + q];
-        one(ca, j);
-      }
+      });
     });
   });
 
-  if (wave > 0)
-    for (int q = 0; q < NACC; ++q) red[wave - 1][q][lane] = acc[q];
-  __syncthreads();
-  if (wave != 0 || !act) return;
-  for (int w = 0; w < kSplit - 1; ++w)
-    for (int q = 0; q < NACC; ++q) acc[q] += red[w][q][lane];
+  double* p = part + (size_t)slice * NACC * npad + k;
+  for (int q = 0; q < NACC; ++q) p[(size_t)q * npad] = acc[q];
+}
+
+// Sum pass: the slices' sums in slice order, scaled by C = kgrad / h^4,
+// then the owner terms, written by particle id.  S is the slice count, so
+// every load of a particle is in flight at once.
+template <bool SHEAR, int S>
+__global__ __launch_bounds__(256) void k_grad_sum(const double* __restrict__ cand,
+                                                  const int32_t* __restrict__ perm, int n, double C,
+                                                  const double* __restrict__ part, int npad,
+                                                  double* __restrict__ F)
+{
+  constexpr int NACC = grad_nacc<SHEAR>();
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double acc[NACC];
+  #pragma unroll
+  for (int q = 0; q < NACC; ++q) acc[q] = 0.0;
+  #pragma unroll
+  for (int sli = 0; sli < S; ++sli)
+    #pragma unroll
+    for (int q = 0; q < NACC; ++q) acc[q] += part[((size_t)sli * NACC + q) * npad + k];
+  #pragma unroll
+  for (int q = 0; q < NACC; ++q) acc[q] *= C;
+  const double* me = cand + (size_t)k * kCandG;
+  const double vax = me[6], vay = me[7];
 
   const double a1x = acc[0], a1y = acc[1], apx = acc[2], apy = acc[3], aex = acc[4], aey = acc[5];
   const double abx = acc[6], aby = acc[7], vxx = acc[8], vxy = acc[9], vyx = acc[10], vyy = acc[11];
-  const int id = perm[kk];
+  const int id = perm[k];
   const double sla = me[13], isla = 1.0 / sla;
   auto put = [&](int c, double v) { F[(size_t)c * npad + id] = v; };
   put(CCAKE_F_GRADP0 + 0, sla * (apx + me[3] * a1x));
@@ -149,24 +143,52 @@ __global__ __launch_bounds__(kSplit * kWave) void k_grad(const double* __restric
   put(CCAKE_F_DIVSHEAR0 + 2, 0.0);
 }
 
-int launch_grad(ccake_event* ev)
+int launch_grad_pairs(ccake_event* ev)
 {
   const int n = ev->n;
   const double h = ev->p.hT;
   const double rc = 2.0 * h;
-  GradW G;
-  G.hinv = 1.0 / h;
-  G.ng = -15.0 / (14.0 * kPi) / pow(h, 3);
-  G.ng2 = 15.0 / (7.0 * kPi) / pow(h, 3);
-  const dim3 grid((n + kTile - 1) / kTile), blk(kSplit * kWave);
+  const double hinv2 = 1.0 / (h * h);
+  const int ntiles = (n + kTile - 1) / kTile;
+  const dim3 grid(ntiles * ev->nslices), blk(kWave);
   const double rcs = rc * (1.0 + 1e-12) + 1e-12;
   if (ev->p.using_shear)
-    hipLaunchKernelGGL(k_grad<true>, grid, blk, 0, ev->dev->stream, ev->cand_g, ev->skey, ev->perm, ev->cell_off,
-                       ev->grid, n, rc * rc, rcs, G, ev->F, ev->npad);
+    hipLaunchKernelGGL(k_grad_slice<true>, grid, blk, 0, ev->dev->stream, ev->cand_g, ev->skey, ev->cell_off,
+                       ev->grid, n, rcs, hinv2, ev->nslices, ev->part, ev->npad);
   else
-    hipLaunchKernelGGL(k_grad<false>, grid, blk, 0, ev->dev->stream, ev->cand_g, ev->skey, ev->perm, ev->cell_off,
-                       ev->grid, n, rc * rc, rcs, G, ev->F, ev->npad);
+    hipLaunchKernelGGL(k_grad_slice<false>, grid, blk, 0, ev->dev->stream, ev->cand_g, ev->skey, ev->cell_off,
+                       ev->grid, n, rcs, hinv2, ev->nslices, ev->part, ev->npad);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+}
+
+template <bool SHEAR, int S = 1>
+static int grad_sum_dispatch(ccake_event* ev, double C)
+{
+  if constexpr (S > kMaxSlices) {
+    set_error("slice count out of range");
+    return CCAKE_ERR_INVALID;
+  } else {
+    if (ev->nslices != S) return grad_sum_dispatch<SHEAR, S + 1>(ev, C);
+    const int n = ev->n;
+    hipLaunchKernelGGL((k_grad_sum<SHEAR, S>), dim3((n + 255) / 256), dim3(256), 0, ev->dev->stream, ev->cand_g,
+                       ev->perm, n, C, ev->part, ev->npad, ev->F);
+    return 0;
+  }
+}
+
+int launch_grad_sum(ccake_event* ev)
+{
+  const double h = ev->p.hT;
+  const double C = -0.5 * (15.0 / (7.0 * kPi)) / pow(h, 4);   // kgrad / h^4 = -kgrad2 / (2 h^4)
+  const int rc = ev->p.using_shear ? grad_sum_dispatch<true>(ev, C) : grad_sum_dispatch<false>(ev, C);
+  if (rc) return rc;
+  return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+}
+
+int launch_grad(ccake_event* ev)
+{
+  int rc = launch_grad_pairs(ev);
+  return rc ? rc : launch_grad_sum(ev);
 }
 
 }  // namespace ccake

@@ -34,34 +34,7 @@ __global__ __launch_bounds__(256) void k_gather_smooth(const double* __restrict_
 }
 
 struct KernelW {
-  double hinv, norm, norm2;
-  // Cubic spline W(r, h) (src/kernel.cpp:34-48) from r^2.
-  __device__ __forceinline__ double of_q(double q) const
-  {
-    const double qq = q * q;
-    const double a = norm * (1.0 - 1.5 * qq + 0.75 * q * qq);
-    const double t = 2.0 - q;
-    const double b = norm2 * t * t * t;
-    const double v = (q >= 1.0) ? b : a;
-    return (q >= 2.0) ? 0.0 : v;
-  }
-  // For neighbours only (r <= 2h): the q >= 2 case is then (2 - q)^3 = 0 up
-  // to rounding, so the outer branch needs no cut.
-  __device__ __forceinline__ double of_q_inside(double q) const
-  {
-    const double qq = q * q;
-    const double a = norm * (1.0 - 1.5 * qq + 0.75 * q * qq);
-    const double t = 2.0 - q;
-    const double b = norm2 * t * t * t;
-    return (q >= 1.0) ? b : a;
-  }
-  __device__ __forceinline__ double operator()(double r2) const { return of_q(sqrt(r2) * hinv); }
-  __device__ __forceinline__ double fast(double r2) const
-  {
-    double r, ir;
-    sqrt_rsqrt(r2, r, ir);
-    return of_q_inside(r * hinv);
-  }
+  double norm, norm2;   // src/kernel.cpp:16-17 (D = 2): W(0) = norm, norm / norm2 = 4
 };
 
 // Neighbour walk without kernel sums, one wave per tile.  MODE 2: neighbour
@@ -112,102 +85,122 @@ __global__ __launch_bounds__(64) void k_walk(const double* __restrict__ cand,
   btrack[id] = (b == -1) ? -1 : cnt;
 }
 
-// Hot smoothing pass (MODE 0: fields + LRF + btrack + pair total; MODE 1:
-// initial sigma): one 64-particle tile per workgroup of kSplit waves, each
-// wave walking a slice of the tile's candidates.  Wave 0 starts from the
-// self terms (src/sph_workstation.cpp:514-523) and adds the other waves'
-// partial sums in wave order.
+// Hot smoothing pass, pair part (MODE 0: fields + LRF + btrack + pair
+// total; MODE 1: initial sigma).  One single-wave workgroup per (tile,
+// slice) unit walks its slice of the tile's candidates and stores the
+// slice's partial sums of m_b w(q) (the self pair included, W = norm2 w);
+// k_smooth_sum adds them in slice order.
 template <int MODE>
-__global__ __launch_bounds__(kSplit * kWave) void k_smooth_tile(const double* __restrict__ cand,
-                                                                const int32_t* __restrict__ skey,
-                                                                const int32_t* __restrict__ perm,
-                                                                const int32_t* __restrict__ cell_off,
-                                                                const Grid* __restrict__ gridp, int n,
-                                                                double rc2, double rcs, KernelW W, double kern0,
-                                                                double t, double* __restrict__ F, int npad,
-                                                                int32_t* __restrict__ btrack,
-                                                                unsigned long long* __restrict__ pair_total)
+constexpr int smooth_nacc() { return MODE == 0 ? 6 : 2; }
+
+template <int MODE>
+__global__ __launch_bounds__(kWave) void k_smooth_slice(const double* __restrict__ cand,
+                                                        const int32_t* __restrict__ skey,
+                                                        const int32_t* __restrict__ cell_off,
+                                                        const Grid* __restrict__ gridp, int n,
+                                                        double rc2, double rcs, double hinv2, int nslices,
+                                                        double* __restrict__ part, int npad)
 {
-  constexpr int NACC = 6;
-  __shared__ double red[kSplit - 1][NACC][kWave];
+  constexpr int NACC = smooth_nacc<MODE>();
   const Grid g = *gridp;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wave = uni((int)(threadIdx.x / kWave));   // wave-uniform
-  const int k = xcd_tile(blockIdx.x, gridDim.x) * kTile + lane;
+  const int lane = threadIdx.x;
+  const int unit = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = unit / nslices;
+  const int slice = unit - tile * nslices;
+  const int k = tile * kTile + lane;
   const bool act = k < n;
   const int kk = act ? k : 0;
   const double* me = cand + (size_t)kk * kCandS;
   const double xi = me[0], yi = me[1];
   const int myrow = act ? skey[kk] / g.nc[0] : INT_MAX;
 
-  const double self = (wave == 0) ? kern0 : 0.0;
-  double sl = me[2] * self;
-  double ss = me[3] * self, sB = me[4] * self, sS = me[5] * self, sQ = me[6] * self;
-  int cnt = 0;
+  double sl = 0.0, ss = 0.0, sB = 0.0, sS = 0.0, sQ = 0.0;
+  int cnt = 0;   // r2 <= rc2, the self pair included
 
   for_segments(act, myrow, xi, yi, [&](const Seg& seg, bool in) {
-    for_spans_slice(g, cell_off, seg, rcs, wave, [&](int j0, int j1) {
-      // wave-uniform candidate records (scalar loads into SGPRs), two
-      // requested together
+    if (!in) return;   // lanes of other cell rows sit this walk out
+    for_spans_slice(g, cell_off, seg, rcs, slice, nslices, [&](int j0, int j1) {
+      // wave-uniform candidate records (scalar loads into SGPRs)
       constexpr int NR = (MODE == 0) ? 7 : 3;
-      auto one = [&](const double* c, int j) {
-        const double dx = xi - c[0], dy = yi - c[1];
+      walk_records<NR, kCandS, 2>(cand, xi, yi, j0, j1,
+                                  [&](const double* c, double dx, double dy, int j, bool valid) {
         const double r2 = dist2_nocontract(dx, dy);
-        const bool nb = in && (r2 <= rc2) && (j != k);
-        if (kSkipEmpty && __ballot(nb) == 0ull) return;
-        const double wv = W.fast(r2);
-        const double w = nb ? wv : 0.0;
-        cnt += nb;
+        cnt += (valid && r2 <= rc2) ? 1 : 0;
+        double q, iq;
+        q_and_inverse(r2, hinv2, q, iq);
+        const double w = valid ? spline_w(q) : 0.0;
         sl += c[2] * w;
         if (MODE == 0) {
           ss += c[MODE == 0 ? 3 : 0] * w; sB += c[MODE == 0 ? 4 : 0] * w;
           sS += c[MODE == 0 ? 5 : 0] * w; sQ += c[MODE == 0 ? 6 : 0] * w;
         }
-      };
-      int j = j0;
-      for (; j + 1 < j1; j += 2) {
-        double ca[NR], cb[NR];
-        for (int q = 0; q < NR; ++q) ca[q] = cand[(size_t)j * kCandS + q];
-        for (int q = 0; q < NR; ++q) cb[q] = cand[(size_t)(j + 1) * kCandS + q];
-        one(ca, j);
-        one(cb, j + 1);
-      }
-      if (j < j1) {
-        double ca[NR];
-        for (int q = 0; q < NR; ++q) ca[q] = cand[(size_t)j * kCandS + q];
-        one(ca, j);
-      }
+      });
     });
   });
 
-  if (wave > 0) {
-    double* r = &red[wave - 1][0][0];
-    r[0 * kWave + lane] = sl; r[1 * kWave + lane] = ss; r[2 * kWave + lane] = sB;
-    r[3 * kWave + lane] = sS; r[4 * kWave + lane] = sQ; r[5 * kWave + lane] = (double)cnt;
-  }
-  __syncthreads();
-  if (wave != 0) return;
-  for (int w = 0; w < kSplit - 1; ++w) {
-    sl += red[w][0][lane]; ss += red[w][1][lane]; sB += red[w][2][lane];
-    sS += red[w][3][lane]; sQ += red[w][4][lane]; cnt += (int)red[w][5][lane];
-  }
+  double* p = part + (size_t)slice * NACC * npad + k;
+  p[0] = sl;
+  p[(size_t)npad] = (double)cnt;
   if (MODE == 0) {
-    // pair total for the roofline: one atomic per tile
-    unsigned long long tot = act ? (unsigned long long)cnt : 0ull;
+    p[(size_t)2 * npad] = ss; p[(size_t)3 * npad] = sB;
+    p[(size_t)4 * npad] = sS; p[(size_t)5 * npad] = sQ;
+  }
+}
+
+// Sum pass: the slices' partial sums in slice order (the self terms of
+// src/sph_workstation.cpp:514-523 are among them), scaled by norm2, then the
+// lab and LRF fields by particle id.  S is the slice count, so every load
+// of a particle is in flight at once.
+constexpr int kSumThreads = 256;   // small workgroups spread the pass over every CU
+template <int MODE, int S>
+__global__ __launch_bounds__(kSumThreads) void k_smooth_sum(const int32_t* __restrict__ perm, int n, double norm2,
+                                                            double t, const double* __restrict__ part, int npad,
+                                                            double* __restrict__ F,
+                                                            int32_t* __restrict__ btrack,
+                                                            unsigned long long* __restrict__ pair_total)
+{
+  constexpr int NACC = smooth_nacc<MODE>();
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = k < n;
+  double acc[NACC];
+  #pragma unroll
+  for (int f = 0; f < NACC; ++f) acc[f] = 0.0;
+  if (act) {
+    #pragma unroll
+    for (int sli = 0; sli < S; ++sli)
+      #pragma unroll
+      for (int f = 0; f < NACC; ++f) acc[f] += part[((size_t)sli * NACC + f) * npad + k];
+  }
+  const int cnt = act ? (int)acc[1] - 1 : 0;    // neighbours, self excluded
+  if (MODE == 0 && pair_total) {
+    // pair total for the roofline: one pair of atomics per workgroup (a
+    // burst of same-address atomics from every wave costs more than the
+    // whole sum pass)
+    __shared__ unsigned long long wsum[kSumThreads / kWave];
+    unsigned long long tot = (unsigned long long)cnt;
     for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
-    if (lane == 0 && pair_total) { atomicAdd(pair_total, tot); atomicAdd(pair_total + 1, tot); }
+    if ((threadIdx.x & (kWave - 1)) == 0) wsum[threadIdx.x / kWave] = tot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long b = 0;
+      for (int w = 0; w < kSumThreads / kWave; ++w) b += wsum[w];
+      if (b) { atomicAdd(pair_total, b); atomicAdd(pair_total + 1, b); }
+    }
   }
   if (!act) return;
-  const int id = perm[kk];
+  const int id = perm[k];
   if (MODE == 0) {
     const int b = btrack[id];
     btrack[id] = (b == -1) ? -1 : cnt;
   }
+  const double sl = acc[0] * norm2;
   const double gam = F[(size_t)CCAKE_F_GAMMA * npad + id];
   F[(size_t)CCAKE_F_SIGMA_LAB * npad + id] = sl;
   // get_LRF = lab/gamma/t (src/eom_default.cpp:198-203)
   F[(size_t)CCAKE_F_SIGMA * npad + id] = sl / gam / t;
   if (MODE == 0) {
+    const double ss = acc[MODE == 0 ? 2 : 0] * norm2, sB = acc[MODE == 0 ? 3 : 0] * norm2;
+    const double sS = acc[MODE == 0 ? 4 : 0] * norm2, sQ = acc[MODE == 0 ? 5 : 0] * norm2;
     F[(size_t)CCAKE_F_SM_S * npad + id] = ss;
     F[(size_t)CCAKE_F_SM_B * npad + id] = sB;
     F[(size_t)CCAKE_F_SM_SS * npad + id] = sS;
@@ -222,7 +215,6 @@ __global__ __launch_bounds__(kSplit * kWave) void k_smooth_tile(const double* __
 static KernelW make_w(double hT)
 {
   KernelW w;
-  w.hinv = 1.0 / hT;
   w.norm = 10.0 / (7.0 * kPi) / pow(hT, 2);     // src/kernel.cpp:16-17
   w.norm2 = 5.0 / (14.0 * kPi) / pow(hT, 2);
   return w;
@@ -236,29 +228,71 @@ int launch_gather_smooth(ccake_event* ev)
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
 
-int launch_smooth(ccake_event* ev, int mode, bool count_pairs)
+// Pair part of a smoothing pass (modes 0, 1) or the neighbour count walk
+// (mode 2, which is complete by itself).
+int launch_smooth_pairs(ccake_event* ev, int mode, bool count_pairs)
 {
   const int n = ev->n;
   const double rc = 2.0 * ev->p.hT;
   const double rc2 = rc * rc;
   const double rcs = rc * (1.0 + 1e-12) + 1e-12;   // search radius with a rounding margin
-  const KernelW W = make_w(ev->p.hT);
-  const double kern0 = W.norm;                      // W(0) = norm
-  const dim3 grid((n + kTile - 1) / kTile), blk(kTile);
+  const int ntiles = (n + kTile - 1) / kTile;
   hipStream_t st = ev->dev->stream;
+  if (mode == 2) {
+    int32_t* bt = ev->I + (size_t)CCAKE_I_BTRACK * ev->npad;
+    unsigned long long* pc = count_pairs ? (unsigned long long*)ev->pair_count : nullptr;
+    hipLaunchKernelGGL(k_walk<2>, dim3(ntiles), dim3(kTile), 0, st, ev->cand_s, ev->skey, ev->perm, ev->cell_off,
+                       ev->grid, n, rc2, rcs, bt, pc, nullptr, nullptr, nullptr);
+    return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+  }
+  const dim3 grid(ntiles * ev->nslices), blk(kWave);
+  const double hinv2 = 1.0 / (ev->p.hT * ev->p.hT);
+  if (mode == 0)
+    hipLaunchKernelGGL(k_smooth_slice<0>, grid, blk, 0, st, ev->cand_s, ev->skey, ev->cell_off, ev->grid, n, rc2,
+                       rcs, hinv2, ev->nslices, ev->part, ev->npad);
+  else
+    hipLaunchKernelGGL(k_smooth_slice<1>, grid, blk, 0, st, ev->cand_s, ev->skey, ev->cell_off, ev->grid, n, rc2,
+                       rcs, hinv2, ev->nslices, ev->part, ev->npad);
+  return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+}
+
+template <int MODE, int S>
+static void smooth_sum_s(ccake_event* ev, double norm2, int32_t* bt, unsigned long long* pc)
+{
+  const int n = ev->n;
+  const dim3 grid((n + kSumThreads - 1) / kSumThreads), blk(kSumThreads);
+  hipLaunchKernelGGL((k_smooth_sum<MODE, S>), grid, blk, 0, ev->dev->stream, ev->perm, n, norm2, ev->t, ev->part,
+                     ev->npad, ev->F, bt, pc);
+}
+
+template <int MODE, int S = 1>
+static int smooth_sum_dispatch(ccake_event* ev, double norm2, int32_t* bt, unsigned long long* pc)
+{
+  if constexpr (S > kMaxSlices) {
+    set_error("slice count out of range");
+    return CCAKE_ERR_INVALID;
+  } else {
+    if (ev->nslices == S) { smooth_sum_s<MODE, S>(ev, norm2, bt, pc); return 0; }
+    return smooth_sum_dispatch<MODE, S + 1>(ev, norm2, bt, pc);
+  }
+}
+
+// Sum part of a smoothing pass (modes 0, 1).
+int launch_smooth_sum(ccake_event* ev, int mode, bool count_pairs)
+{
+  const KernelW W = make_w(ev->p.hT);
   int32_t* bt = ev->I + (size_t)CCAKE_I_BTRACK * ev->npad;
   unsigned long long* pc = count_pairs ? (unsigned long long*)ev->pair_count : nullptr;
-  const dim3 tblk(kSplit * kWave);
-  if (mode == 0)
-    hipLaunchKernelGGL(k_smooth_tile<0>, grid, tblk, 0, st, ev->cand_s, ev->skey, ev->perm, ev->cell_off, ev->grid,
-                       n, rc2, rcs, W, kern0, ev->t, ev->F, ev->npad, bt, pc);
-  else if (mode == 1)
-    hipLaunchKernelGGL(k_smooth_tile<1>, grid, tblk, 0, st, ev->cand_s, ev->skey, ev->perm, ev->cell_off, ev->grid,
-                       n, rc2, rcs, W, kern0, ev->t, ev->F, ev->npad, bt, pc);
-  else
-    hipLaunchKernelGGL(k_walk<2>, grid, blk, 0, st, ev->cand_s, ev->skey, ev->perm, ev->cell_off, ev->grid, n,
-                       rc2, rcs, bt, pc, nullptr, nullptr, nullptr);
+  const int rc = (mode == 0) ? smooth_sum_dispatch<0>(ev, W.norm2, bt, pc) : smooth_sum_dispatch<1>(ev, W.norm2, bt, pc);
+  if (rc) return rc;
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+}
+
+int launch_smooth(ccake_event* ev, int mode, bool count_pairs)
+{
+  int rc = launch_smooth_pairs(ev, mode, count_pairs);
+  if (rc || mode == 2) return rc;
+  return launch_smooth_sum(ev, mode, count_pairs);
 }
 
 // Parity export: counts are in btrack (sticky -1 rows are recounted here).

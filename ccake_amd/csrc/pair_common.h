@@ -91,27 +91,15 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblk)
   return x * per + (x < rem ? x : rem) + r;
 }
 
-// Hot pair kernels give one tile to kSplit waves: every wave of the
-// workgroup computes the same segments and spans (wave-uniform values) and
-// walks a contiguous slice of the flattened candidate list; the slices'
-// partial sums are added in wave order, so results do not depend on
-// scheduling.
-#ifndef CCAKE_PAIR_SPLIT
-#define CCAKE_PAIR_SPLIT 8
-#endif
-constexpr int kSplit = CCAKE_PAIR_SPLIT;   // 8 measured fastest at 40k-250k particles
-// Optionally skip a candidate (wave-uniform branch) when it is out of range
-// for every lane of the tile; only adds of exact zeros are skipped, the sums
-// are unchanged.  Off by default: the branch keeps the compiler from issuing
-// the candidate loads ahead of their use, which costs more than it saves.
-#ifndef CCAKE_PAIR_SKIP
-#define CCAKE_PAIR_SKIP 0
-#endif
-constexpr bool kSkipEmpty = CCAKE_PAIR_SKIP != 0;
+// Hot pair kernels split a tile's flattened candidate list into `nslices`
+// contiguous slices; slice `slice` is walked by one wave (one workgroup), so
+// the work units are small enough to spread evenly over the chip's SIMDs
+// even when there are few tiles.  Every wave of a tile computes the same
+// segments and spans (wave-uniform values).
 
 template <class Body>
 __device__ __forceinline__ void for_spans_slice(const Grid& g, const int32_t* __restrict__ cell_off,
-                                                const Seg& s, double rcs, int wave, Body&& body)
+                                                const Seg& s, double rcs, int slice, int nslices, Body&& body)
 {
   const int ry0 = uni(cell_of(s.ymn - rcs, g.lo[1], g.inv_cell, g.nc[1]));
   const int ry1 = uni(cell_of(s.ymx + rcs, g.lo[1], g.inv_cell, g.nc[1]));
@@ -127,8 +115,8 @@ __device__ __forceinline__ void for_spans_slice(const Grid& g, const int32_t* __
   };
   int tot = 0;
   for (int ry = ry0; ry <= ry1; ++ry) { int b, e; span(ry, b, e); tot += e - b; }
-  const int lo = (int)(((long long)tot * wave) / kSplit);
-  const int hi = (int)(((long long)tot * (wave + 1)) / kSplit);
+  const int lo = (int)(((long long)tot * slice) / nslices);
+  const int hi = (int)(((long long)tot * (slice + 1)) / nslices);
   int base = 0;
   for (int ry = ry0; ry <= ry1 && base < hi; ++ry) {
     int b, e; span(ry, b, e);
@@ -140,19 +128,105 @@ __device__ __forceinline__ void for_spans_slice(const Grid& g, const int32_t* __
   }
 }
 
-// sqrt(x) and 1/sqrt(x) for x >= 0: the hardware reciprocal square root
-// (v_rsq_f64) refined by one Newton step y <- y (3 - x y^2) / 2, relative
-// error about 1e-14, which the pair sums tolerate.  x is floored at 1e-300
-// so that coincident particles get r = 1e-150 (q = 0 to rounding) instead of
-// 0 * inf.  The pair predicate never uses it.
-__device__ __forceinline__ void sqrt_rsqrt(double x, double& r, double& ir)
+// Number of slices per tile for `ntiles` tiles on `simds` SIMDs: the
+// smallest that gives near-whole rounds of single-wave work units and at
+// least about four waves per SIMD (to hide load latency), at most 16.
+inline int choose_slices(int ntiles, int simds)
 {
-  const double xc = fmax(x, 1e-300);
-  const double y = __builtin_amdgcn_rsq(xc);
-  const double h = 0.5 * xc;
-  const double e = fma(-(h * y), y, 1.5);
-  ir = y * e;
-  r = xc * ir;
+  int best = 1;
+  double best_score = -1.0;
+  for (int s = 1; s <= 16; ++s) {
+    const double w = (double)ntiles * s / (simds > 0 ? simds : 1);
+    const double rounds = w <= 1.0 ? 1.0 : __builtin_ceil(w);
+    const double score = (w / rounds) * (w < 4.0 ? w / 4.0 : 1.0);
+    if (score > best_score + 1e-9) { best_score = score; best = s; }
+  }
+  return best;
+}
+
+// Per-slice partial sums: part[(slice * nacc + f) * npad + k] for sorted
+// position k (every tile lane, k < ntiles * 64 <= npad, has a slot).
+constexpr int kMaxAcc = 17;
+constexpr int kMaxSlices = 16;
+
+// Walk candidates j0 .. j1-1 (wave-uniform) U at a time with the next U
+// records requested while the current ones are evaluated.  Scalar loads may
+// return out of order, so a wait always covers every load in flight; the
+// first use of the current records (their offsets dx, dy) is therefore
+// pinned before the next request, and the wait for the next records falls
+// after the current arithmetic.  Indices past j1 - 1 are clamped (those
+// loads are in bounds and their results unused).  body(c, dx, dy, j, valid)
+// evaluates one candidate; `valid` is false for the padding slot of a short
+// last step.
+template <int NR, int STRIDE, int U, class Body>
+__device__ __forceinline__ void walk_records(const double* __restrict__ cand, double xi, double yi, int j0, int j1,
+                                             Body&& body)
+{
+  static_assert(U == 1 || U == 2, "one or two records per step");
+  if (j0 >= j1) return;
+  double cur[U][NR];
+  #pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int ju = j0 + u < j1 ? j0 + u : j1 - 1;
+    #pragma unroll
+    for (int q = 0; q < NR; ++q) cur[u][q] = cand[(size_t)ju * STRIDE + q];
+  }
+  for (int j = j0; j < j1; j += U) {
+    double dx[U], dy[U];
+    #pragma unroll
+    for (int u = 0; u < U; ++u) { dx[u] = xi - cur[u][0]; dy[u] = yi - cur[u][1]; }
+    __builtin_amdgcn_sched_barrier(0);
+    double nxt[U][NR];
+    #pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int ju = j + U + u < j1 ? j + U + u : j1 - 1;
+      #pragma unroll
+      for (int q = 0; q < NR; ++q) nxt[u][q] = cand[(size_t)ju * STRIDE + q];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    #pragma unroll
+    for (int u = 0; u < U; ++u) body(cur[u], dx[u], dy[u], j + u, j + u < j1);
+    #pragma unroll
+    for (int u = 0; u < U; ++u)
+      #pragma unroll
+      for (int q = 0; q < NR; ++q) cur[u][q] = nxt[u][q];
+  }
+}
+
+// q = r/h and 1/q from r^2: the hardware reciprocal square root
+// (v_rsq_f64) of q^2 refined by one Newton step y <- y (3 - x y^2) / 2,
+// relative error about 1e-14, which the pair sums tolerate.  1e-300 is added
+// to q^2 so that coincident particles get q = 1e-150 (0 to rounding)
+// instead of 0 * inf.  The pair predicate never uses it.
+__device__ __forceinline__ void q_and_inverse(double r2, double hinv2, double& q, double& iq)
+{
+  const double x = fma(r2, hinv2, 1e-300);
+  const double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  const double e = fma(-(hx * y), y, 1.5);
+  iq = y * e;
+  q = x * iq;
+}
+
+// Cubic spline in branch-free form (src/kernel.cpp:34-48 and 73-118, D = 2),
+// with t = max(2 - q, 0) and u = max(1 - q, 0):
+//   W      = norm2 * (t^3 - 4 u^3)       [4 - 6q^2 + 3q^3 for q < 1, since
+//                                         norm / norm2 = 4; (2 - q)^3 for
+//                                         1 <= q < 2; 0 beyond]
+//   grad W = C * rel * (t^2 - 4 u^2) / q [4 - 3q for q < 1; (2 - q)^2 / q
+//                                         for 1 <= q < 2; 0 beyond], with
+//   C = kgrad / h^4 = -kgrad2 / (2 h^4), kgrad = -15/(14 pi), kgrad2 = 15/(7 pi).
+// Non-neighbours weigh exactly 0 and the self pair (q = 0 to rounding)
+// weighs W(0) and has zero gradient, so the pair loops need no selects.
+__device__ __forceinline__ double spline_w(double q)
+{
+  const double t = fmax(2.0 - q, 0.0), u = fmax(1.0 - q, 0.0);
+  return fma(-4.0, u * u * u, t * t * t);
+}
+__device__ __forceinline__ double spline_g(double q, double iq)
+{
+  const double t = fmax(2.0 - q, 0.0), u = fmax(1.0 - q, 0.0);
+  return fma(-4.0, u * u, t * t) * iq;
 }
 
 // Iterate over the row segments of the tile owned by this wave.  `fn(seg,

@@ -262,9 +262,12 @@ enum ccake_reduction { CCAKE_R_ENTROPY = 0, CCAKE_R_BARYON, CCAKE_R_STRANGE, CCA
 CCAKE_API int  ccake_reduce(ccake_event* ev, int which, double* out);
 
 /* --------------------------- profiling ----------------------------- */
-/* Per-kernel HIP-event timing on the library's stream (off by default). */
+/* Per-kernel HIP-event timing on the library's stream (off by default).
+ * SMOOTH and GRAD time the pair kernels alone; SMOOTH_SUM and GRAD_SUM the
+ * passes that add their per-slice partial sums and write the fields. */
 enum ccake_kernel_id { CCAKE_K_BIN = 0, CCAKE_K_SORT, CCAKE_K_SMOOTH, CCAKE_K_EOS,
-                       CCAKE_K_GRAD, CCAKE_K_EOM, CCAKE_K_RK, CCAKE_K_MISC, CCAKE_NUM_KERNELS };
+                       CCAKE_K_GRAD, CCAKE_K_EOM, CCAKE_K_RK, CCAKE_K_MISC,
+                       CCAKE_K_SMOOTH_SUM, CCAKE_K_GRAD_SUM, CCAKE_NUM_KERNELS };
 CCAKE_API int  ccake_profile_enable(ccake_dev* dev, int enable);
 CCAKE_API int  ccake_profile_read(ccake_dev* dev, int kernel_id, double* total_ms, int64_t* launches);
 CCAKE_API int  ccake_profile_reset(ccake_dev* dev);
