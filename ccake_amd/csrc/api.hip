@@ -77,8 +77,7 @@ static int time_derivatives(ccake_event* ev, int fuse_rk, double dt_stage, doubl
   if ((rc = launch_stage_begin(ev))) return rc;                 // 02 gamma, v (+ bbox)
   prof_end(d, CCAKE_K_BIN);
   prof_begin(d, CCAKE_K_SORT);
-  if ((rc = launch_neighbor_sort(ev))) return rc;               // 01 cell sort
-  if ((rc = launch_gather_smooth(ev))) return rc;               // candidate records in cell order
+  if ((rc = launch_neighbor_sort(ev))) return rc;               // 01 cell sort + candidate records
   prof_end(d, CCAKE_K_SORT);
   prof_begin(d, CCAKE_K_SMOOTH);
   if ((rc = launch_smooth_pairs(ev, 0, true))) return rc;       // 03 pair sums per slice

@@ -145,7 +145,7 @@ inline int32_t* icol(ccake_event* ev, int f) { return ev->I + (size_t)f * ev->np
 // neighbours
 int launch_stage_begin(ccake_event* ev);            // gamma/v, bbox
 int launch_bbox(ccake_event* ev);                   // bbox only
-int launch_neighbor_sort(ccake_event* ev);          // grid, keys, radix sort, cell offsets, btrack reset
+int launch_neighbor_sort(ccake_event* ev);          // grid, keys, radix sort, cell offsets, smoothing records
 int launch_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, size_t* n);
 // smoothing
 int launch_gather_smooth(ccake_event* ev);
@@ -171,5 +171,22 @@ int launch_gamma_velocity(ccake_event* ev);
 int launch_init_particles(ccake_event* ev, int phase);
 int launch_freeze(ccake_event* ev, int phase);
 int launch_reduce(ccake_event* ev, int which, double* out);
+
+// Smoothing candidate record of particle i (8 doubles): x, y, m_s,
+// m_s*ext_s, m_B*ext_B, m_S*ext_S, m_Q*ext_Q, 0.
+constexpr int kCandS = 8;
+__device__ __forceinline__ void smooth_record(const double* __restrict__ F, int npad, int i, double* __restrict__ o)
+{
+  auto f = [&](int c) { return F[(size_t)c * npad + i]; };
+  const double ms = f(CCAKE_F_MASS_S);
+  o[0] = f(CCAKE_F_X);
+  o[1] = f(CCAKE_F_Y);
+  o[2] = ms;
+  o[3] = ms * f(CCAKE_F_EXT_S);
+  o[4] = f(CCAKE_F_MASS_B) * f(CCAKE_F_EXT_B);
+  o[5] = f(CCAKE_F_MASS_SS) * f(CCAKE_F_EXT_SS);
+  o[6] = f(CCAKE_F_MASS_Q) * f(CCAKE_F_EXT_Q);
+  o[7] = 0.0;
+}
 
 }  // namespace ccake

@@ -50,8 +50,8 @@ __global__ __launch_bounds__(kWave) void k_grad_slice(const double* __restrict__
   for (int q = 0; q < NACC; ++q) acc[q] = 0.0;
 
   for_segments(act, myrow, xa, ya, [&](const Seg& seg, bool in) {
-    if (!in) return;   // lanes of other cell rows sit this walk out
-    for_spans_slice(g, cell_off, seg, rcs, slice, nslices, [&](int j0, int j1) {
+    // lanes of other cell rows sit this segment's walk out
+    for_spans_slice(g, cell_off, seg, rcs, slice, nslices, in, [&](int j0, int j1) {
       // Candidate records are wave-uniform (scalar loads into SGPRs); the
       // shear record is too large to hold two steps of two in SGPRs.
       constexpr int NR = SHEAR ? 13 : 8;

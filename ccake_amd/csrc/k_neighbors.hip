@@ -107,24 +107,19 @@ __device__ __forceinline__ int cell_coord(double x, double lo, double inv, int n
   return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
 }
 
-__global__ __launch_bounds__(256) void k_keys(const double* __restrict__ F, int npad, int n,
-                                              const Grid* __restrict__ grid, int32_t* __restrict__ key)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const Grid g = *grid;
-  const int cx = cell_coord(F[(size_t)CCAKE_F_X * npad + i], g.lo[0], g.inv_cell, g.nc[0]);
-  const int cy = cell_coord(F[(size_t)CCAKE_F_Y * npad + i], g.lo[1], g.inv_cell, g.nc[1]);
-  key[i] = cy * g.nc[0] + cx;
-}
-
 // ---------------------------------------------------------------------------
 // Stable LSD radix sort, 8-bit digits.  Block b owns items
 // [b*kSortBlockItems, (b+1)*kSortBlockItems) in input order; hist is laid
 // out digit-major (hist[d*nblk + b]) so that one exclusive scan yields every
 // (digit, block) output base.
-__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const int32_t* __restrict__ keys, int n,
-                                                             int shift, int nblk, int32_t* __restrict__ hist)
+// KEYS: the first pass also computes the cell key of every particle from
+// its position and the grid, and stores it (id order) for the scatter.
+template <bool KEYS>
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const int32_t* __restrict__ keys_in, int n,
+                                                             int shift, int nblk, int32_t* __restrict__ hist,
+                                                             const double* __restrict__ F, int npad,
+                                                             const Grid* __restrict__ grid,
+                                                             int32_t* __restrict__ keys_out)
 {
   __shared__ int32_t h[256];
   h[threadIdx.x] = 0;
@@ -132,7 +127,18 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const int32_t* __re
   const int base = blockIdx.x * kSortBlockItems;
   for (int r = 0; r < kSortItems; ++r) {
     const int idx = base + r * kSortThreads + threadIdx.x;
-    if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255], 1);
+    if (idx >= n) continue;
+    int key;
+    if (KEYS) {
+      const Grid g = *grid;
+      const int cx = cell_coord(F[(size_t)CCAKE_F_X * npad + idx], g.lo[0], g.inv_cell, g.nc[0]);
+      const int cy = cell_coord(F[(size_t)CCAKE_F_Y * npad + idx], g.lo[1], g.inv_cell, g.nc[1]);
+      key = cy * g.nc[0] + cx;
+      keys_out[idx] = key;
+    } else {
+      key = keys_in[idx];
+    }
+    atomicAdd(&h[(key >> shift) & 255], 1);
   }
   __syncthreads();
   hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
@@ -254,6 +260,35 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const int32_t* __res
   for (int r = 0; r < kScanItems; ++r) { const int idx = t0 + r; if (idx < n) out[idx] = ex; ex += v[r]; }
 }
 
+// In-place exclusive scan of a[0, n) by one workgroup: thread t owns the
+// contiguous run [t*per, min((t+1)*per, n)) with per = ceil(n / 1024), so
+// every load of a thread is issued at once; one block scan of the run sums
+// gives each run's offset.
+constexpr int kScanOneThreads = 1024;
+constexpr int kScanOneMax = 65536;   // histograms of up to 256 radix blocks
+__global__ __launch_bounds__(kScanOneThreads) void k_scan_one(int32_t* __restrict__ a, int n)
+{
+  __shared__ int32_t wtot[kScanOneThreads / kWave];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int per = (n + kScanOneThreads - 1) / kScanOneThreads;
+  const int b = threadIdx.x * per;
+  const int e = min(b + per, n);                    // e <= b (empty run) past the end
+  int32_t s = 0;
+  #pragma unroll 16
+  for (int i = b; i < e; ++i) s += a[i];
+  int32_t x = s;                                    // inclusive scan of run sums within the wave
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == kWave - 1) wtot[w] = x;
+  __syncthreads();
+  int32_t run = x - s;
+  for (int q = 0; q < w; ++q) run += wtot[q];
+  #pragma unroll 16
+  for (int i = b; i < e; ++i) { const int32_t v = a[i]; a[i] = run; run += v; }
+}
+
 static int scan_exclusive(ccake_event* ev, const int32_t* in, int32_t* out, int n)
 {
   const int per = kScanThreads * kScanItems;
@@ -265,10 +300,14 @@ static int scan_exclusive(ccake_event* ev, const int32_t* in, int32_t* out, int 
   return 0;
 }
 
-// cell_off[c] = first sorted position whose key >= c, for c in [0, ncells].
-__global__ __launch_bounds__(256) void k_cell_offsets(const int32_t* __restrict__ skey, int n,
-                                                      const Grid* __restrict__ grid,
-                                                      int32_t* __restrict__ cell_off)
+// cell_off[c] = first sorted position whose key >= c, for c in [0, ncells],
+// and the smoothing candidate records in sorted order.
+__global__ __launch_bounds__(256) void k_cells_records(const int32_t* __restrict__ skey,
+                                                       const int32_t* __restrict__ perm, int n,
+                                                       const Grid* __restrict__ grid,
+                                                       int32_t* __restrict__ cell_off,
+                                                       const double* __restrict__ F, int npad,
+                                                       double* __restrict__ cand)
 {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int nc = grid->ncells;
@@ -281,6 +320,7 @@ __global__ __launch_bounds__(256) void k_cell_offsets(const int32_t* __restrict_
   const int prev = (k == 0) ? -1 : skey[k - 1];
   for (int c = prev + 1; c <= cur; ++c) cell_off[c] = k;
   if (k == n - 1) for (int c = cur + 1; c <= nc; ++c) cell_off[c] = n;
+  smooth_record(F, npad, perm[k], cand + (size_t)k * kCandS);
 }
 
 // ---------------------------------------------------------------------------
@@ -296,6 +336,16 @@ int launch_stage_begin(ccake_event* ev) { return stage_begin(ev, 1); }
 int launch_bbox(ccake_event* ev) { return stage_begin(ev, 0); }
 int launch_gamma_velocity(ccake_event* ev) { return stage_begin(ev, 1); }
 
+static void scan_hist(ccake_event* ev, int nscan)
+{
+  if (nscan <= kScanOneMax)
+    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(kScanOneThreads), 0, ev->dev->stream, ev->hist, nscan);
+  else
+    scan_exclusive(ev, ev->hist, ev->hist, nscan);
+}
+
+// Grid, keys, stable two-pass radix sort (cap_cells <= 65536 keys fit 16
+// bits), cell offsets and the smoothing candidate records.
 int launch_neighbor_sort(ccake_event* ev)
 {
   const int n = ev->n;
@@ -304,25 +354,23 @@ int launch_neighbor_sort(ccake_event* ev)
   const double rc = 2.0 * ev->p.hT;
   hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(256), 0, st, ev->bbox, nb, rc, ev->p.cells_per_rc,
                      ev->cap_cells, ev->grid, ev->err, (unsigned long long*)ev->pair_count);
-  hipLaunchKernelGGL(k_keys, dim3(nb), dim3(256), 0, st, ev->F, ev->npad, n, ev->grid, ev->key);
   const int nblk = ev->radix_blocks;
-  // two 8-bit passes cover the key capacity (cap_cells <= 65536).
-  const int32_t* kin = ev->key;
-  const int32_t* vin = nullptr;
-  int32_t* kout = ev->key_a; int32_t* vout = ev->val_a;
-  for (int pass = 0; pass < 2; ++pass) {
-    const int shift = 8 * pass;
-    hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(kSortThreads), 0, st, kin, n, shift, nblk, ev->hist);
-    scan_exclusive(ev, ev->hist, ev->hist, 256 * nblk);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(kSortThreads), 0, st, kin, vin, n, shift, nblk,
-                       ev->hist, kout, vout);
-    kin = kout; vin = vout;
-    if (kout == ev->key_a) { kout = ev->key_b; vout = ev->val_b; } else { kout = ev->key_a; vout = ev->val_a; }
-  }
-  ev->skey = const_cast<int32_t*>(kin);
-  ev->perm = const_cast<int32_t*>(vin);
+  const int nscan = 256 * nblk;
+  hipLaunchKernelGGL(k_radix_hist<true>, dim3(nblk), dim3(kSortThreads), 0, st, nullptr, n, 0, nblk, ev->hist,
+                     ev->F, ev->npad, ev->grid, ev->key);
+  scan_hist(ev, nscan);
+  hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(kSortThreads), 0, st, ev->key, nullptr, n, 0, nblk,
+                     ev->hist, ev->key_a, ev->val_a);
+  hipLaunchKernelGGL(k_radix_hist<false>, dim3(nblk), dim3(kSortThreads), 0, st, ev->key_a, n, 8, nblk, ev->hist,
+                     nullptr, 0, nullptr, nullptr);
+  scan_hist(ev, nscan);
+  hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(kSortThreads), 0, st, ev->key_a, ev->val_a, n, 8, nblk,
+                     ev->hist, ev->key_b, ev->val_b);
+  ev->skey = ev->key_b;
+  ev->perm = ev->val_b;
   const int nbc = (n + 255) / 256 + 1;
-  hipLaunchKernelGGL(k_cell_offsets, dim3(nbc), dim3(256), 0, st, ev->skey, n, ev->grid, ev->cell_off);
+  hipLaunchKernelGGL(k_cells_records, dim3(nbc), dim3(256), 0, st, ev->skey, ev->perm, n, ev->grid, ev->cell_off,
+                     ev->F, ev->npad, ev->cand_s);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
 

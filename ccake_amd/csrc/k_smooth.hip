@@ -9,28 +9,14 @@
 
 namespace ccake {
 
-constexpr int kCandS = 8;   // doubles per smoothing candidate record
-
-// Candidate records in sorted order: x, y, m_s, m_s*ext_s, m_B*ext_B,
-// m_S*ext_S, m_Q*ext_Q, 0.
+// Candidate records in sorted order (smooth_record).
 __global__ __launch_bounds__(256) void k_gather_smooth(const double* __restrict__ F, int npad, int n,
                                                        const int32_t* __restrict__ perm,
                                                        double* __restrict__ cand)
 {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  const int i = perm[k];
-  auto f = [&](int c) { return F[(size_t)c * npad + i]; };
-  double* o = cand + (size_t)k * kCandS;
-  const double ms = f(CCAKE_F_MASS_S);
-  o[0] = f(CCAKE_F_X);
-  o[1] = f(CCAKE_F_Y);
-  o[2] = ms;
-  o[3] = ms * f(CCAKE_F_EXT_S);
-  o[4] = f(CCAKE_F_MASS_B) * f(CCAKE_F_EXT_B);
-  o[5] = f(CCAKE_F_MASS_SS) * f(CCAKE_F_EXT_SS);
-  o[6] = f(CCAKE_F_MASS_Q) * f(CCAKE_F_EXT_Q);
-  o[7] = 0.0;
+  smooth_record(F, npad, perm[k], cand + (size_t)k * kCandS);
 }
 
 struct KernelW {
@@ -118,8 +104,8 @@ __global__ __launch_bounds__(kWave) void k_smooth_slice(const double* __restrict
   int cnt = 0;   // r2 <= rc2, the self pair included
 
   for_segments(act, myrow, xi, yi, [&](const Seg& seg, bool in) {
-    if (!in) return;   // lanes of other cell rows sit this walk out
-    for_spans_slice(g, cell_off, seg, rcs, slice, nslices, [&](int j0, int j1) {
+    // lanes of other cell rows sit this segment's walk out
+    for_spans_slice(g, cell_off, seg, rcs, slice, nslices, in, [&](int j0, int j1) {
       // wave-uniform candidate records (scalar loads into SGPRs)
       constexpr int NR = (MODE == 0) ? 7 : 3;
       walk_records<NR, kCandS, 2>(cand, xi, yi, j0, j1,

@@ -97,34 +97,83 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblk)
 // even when there are few tiles.  Every wave of a tile computes the same
 // segments and spans (wave-uniform values).
 
+__device__ __forceinline__ int wave_sum_i(int v)
+{
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ int wave_excl_scan_i(int v)
+{
+  const int lane = threadIdx.x & (kWave - 1);
+  int x = v;
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  return x - v;
+}
+
+// Candidate run of cell row ry0 + r for a segment: [b, e) in sorted order
+// (empty when r >= nrow or the row is out of reach).  Evaluated per lane.
+__device__ __forceinline__ void row_run(const Grid& g, const int32_t* __restrict__ cell_off, const Seg& s,
+                                        double rcs, int ry0, int nrow, int r, int& b, int& e)
+{
+  b = e = 0;
+  if (r >= nrow) return;
+  const int ry = ry0 + r;
+  const double ylo = g.lo[1] + ry * g.cell, yhi = ylo + g.cell;
+  const double dy = fmax(0.0, fmax(ylo - s.ymx, s.ymn - yhi));
+  if (dy > rcs) return;
+  const double hw = sqrt(fmax(rcs * rcs - dy * dy, 0.0));
+  const int cx0 = cell_of(s.xmn - hw, g.lo[0], g.inv_cell, g.nc[0]);
+  const int cx1 = cell_of(s.xmx + hw, g.lo[0], g.inv_cell, g.nc[0]);
+  b = cell_off[ry * g.nc[0] + cx0];
+  e = cell_off[ry * g.nc[0] + cx1 + 1];
+}
+
+// Walk slice `slice` of `nslices` of a segment's flattened candidate list
+// (rows ascending, cells ascending within a row).  The rows are resolved
+// lane-parallel, lane r holding cell row ry0 + r (64 rows at a time), so
+// the cell offsets of every row arrive with one vector load; the walk reads
+// a row's run back with readlane.  Called with every lane of the wave
+// active; body(j0, j1) runs on the lanes with `in` set, with wave-uniform
+// sorted indices j0 < j1.
 template <class Body>
 __device__ __forceinline__ void for_spans_slice(const Grid& g, const int32_t* __restrict__ cell_off,
-                                                const Seg& s, double rcs, int slice, int nslices, Body&& body)
+                                                const Seg& s, double rcs, int slice, int nslices, bool in,
+                                                Body&& body)
 {
+  const int lane = threadIdx.x & (kWave - 1);
   const int ry0 = uni(cell_of(s.ymn - rcs, g.lo[1], g.inv_cell, g.nc[1]));
   const int ry1 = uni(cell_of(s.ymx + rcs, g.lo[1], g.inv_cell, g.nc[1]));
-  auto span = [&](int ry, int& beg, int& end) {
-    const double ylo = g.lo[1] + ry * g.cell, yhi = ylo + g.cell;
-    const double dy = fmax(0.0, fmax(ylo - s.ymx, s.ymn - yhi));
-    if (dy > rcs) { beg = end = 0; return; }
-    const double hw = sqrt(fmax(rcs * rcs - dy * dy, 0.0));
-    const int cx0 = cell_of(s.xmn - hw, g.lo[0], g.inv_cell, g.nc[0]);
-    const int cx1 = cell_of(s.xmx + hw, g.lo[0], g.inv_cell, g.nc[0]);
-    beg = uni(cell_off[ry * g.nc[0] + cx0]);
-    end = uni(cell_off[ry * g.nc[0] + cx1 + 1]);
-  };
-  int tot = 0;
-  for (int ry = ry0; ry <= ry1; ++ry) { int b, e; span(ry, b, e); tot += e - b; }
+  const int nrow = ry1 - ry0 + 1;
+  int b, e;
+  row_run(g, cell_off, s, rcs, ry0, nrow, lane, b, e);
+  int tot = uni(wave_sum_i(e - b));
+  for (int r0 = kWave; r0 < nrow; r0 += kWave) {     // more than 64 rows: rare
+    int b2, e2;
+    row_run(g, cell_off, s, rcs, ry0, nrow, r0 + lane, b2, e2);
+    tot += uni(wave_sum_i(e2 - b2));
+  }
   const int lo = (int)(((long long)tot * slice) / nslices);
   const int hi = (int)(((long long)tot * (slice + 1)) / nslices);
   int base = 0;
-  for (int ry = ry0; ry <= ry1 && base < hi; ++ry) {
-    int b, e; span(ry, b, e);
+  for (int r0 = 0; r0 < nrow && base < hi; r0 += kWave) {
+    if (r0 > 0) row_run(g, cell_off, s, rcs, ry0, nrow, r0 + lane, b, e);
     const int len = e - b;
-    const int t0 = lo > base ? lo - base : 0;
-    const int t1 = hi - base < len ? hi - base : len;
-    if (t0 < t1) body(b + t0, b + t1);
-    base += len;
+    const int pre = wave_excl_scan_i(len);
+    const int ctot = uni(wave_sum_i(len));
+    const int nr = nrow - r0 < kWave ? nrow - r0 : kWave;
+    for (int r = 0; r < nr; ++r) {
+      const int rbase = base + __builtin_amdgcn_readlane(pre, r);
+      if (rbase >= hi) break;
+      const int rb = __builtin_amdgcn_readlane(b, r);
+      const int rlen = __builtin_amdgcn_readlane(len, r);
+      const int t0 = lo > rbase ? lo - rbase : 0;
+      const int t1 = hi - rbase < rlen ? hi - rbase : rlen;
+      if (t0 < t1 && in) body(rb + t0, rb + t1);
+    }
+    base += ctot;
   }
 }
 
