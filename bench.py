@@ -16,9 +16,11 @@ gradients, equations of motion, RK update, freeze-out check).  Inputs are
 resident in HBM before the timed region.
 
 Prints one JSON line with the throughput in particle-steps/s, the roofline
-of the dominant kernel (measured with HIP events on the library's stream
-over the timed region) and the CPU oracle timed on a bounded sample on the
-host cores.
+of the dominant kernel (measured with HIP events on the event's stream
+over the timed region), the CPU oracle timed on a bounded sample on the
+host cores, and a served multi-event measurement ("serving"): a batch of
+ideal Gubser events of uneven size run one at a time (the plain path) and
+several at a time on separate streams, as events/s with p50/p99 latency.
 """
 from __future__ import annotations
 
@@ -51,6 +53,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle sample")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--cells-per-rc", type=int, default=3, help="neighbour-grid cells per cutoff radius")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served multi-event measurement")
+    ap.add_argument("--serve-events", type=int, default=24)
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain 256 MiB device copy (reference kernel for HBM byte counters)")
     return ap.parse_args()
@@ -158,6 +163,55 @@ def make_roofline(prof, pairs, n, cfg, workload):
     return r
 
 
+def serving_requests(k, steps, seed=0x5E4F):
+    """k ideal Gubser events on the 0.05 fm lattice with lognormal particle
+    counts (median 25,000, clipped to [10,000, 60,000]) set by the lattice
+    half-width; `steps` RK2 steps each; all arriving at once."""
+    import numpy as np
+    from ccake_amd import config, ic, serve
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for rid in range(k):
+        n_want = float(np.clip(np.exp(rng.normal(np.log(25000.0), 0.5)), 10000, 60000))
+        half = 0.05 * round((np.sqrt(n_want) - 1.0) / 2.0)
+        reqs.append(serve.Request(rid=rid, ic=ic.gubser_ideal(spacing=0.05, half_width=half),
+                                  params=config.gubser_ideal_params(), max_steps=steps,
+                                  fields=("CCAKE_F_E", "CCAKE_F_UX", "CCAKE_F_UY")))
+    return reqs
+
+
+def serving_run(dev, events, slots, steps=40):
+    """Events/s and latency of the same request batch run one event at a
+    time (the plain path) and served `slots` events at a time, each on its
+    own stream.  The served results must equal the sequential ones."""
+    import numpy as np
+    import ccake_amd
+    from ccake_amd import serve
+    reqs = serving_requests(events, steps)
+    eos = ccake_amd.Eos(dev, 0)
+    serve.Server(dev, eos, slots=slots).run(reqs[:min(len(reqs), slots)])   # warm-up (code objects, pools)
+    out = {}
+    runs = {}
+    for name, sl in (("sequential", 1), ("served", slots)):
+        dev.sync()
+        t0 = time.perf_counter()
+        res = serve.Server(dev, eos, slots=sl).run(reqs)
+        wall = time.perf_counter() - t0
+        runs[name] = res
+        out[name] = dict(slots=sl, **{k: round(v, 4) if isinstance(v, float) else v
+                                      for k, v in serve.summarize(res, wall).items()})
+    eos.close()
+    same = all(np.array_equal(a.fields[f], b.fields[f]) for a, b in zip(runs["sequential"], runs["served"])
+               for f in a.fields)
+    n = [len(r.ic["x"]) for r in reqs]
+    out.update(events=events, steps_per_event=steps, particles_min=min(n), particles_max=max(n),
+               particles_median=int(np.median(n)), results_equal=bool(same),
+               speedup=round(out["served"]["events_per_s"] / out["sequential"]["events_per_s"], 3),
+               workload="ideal Gubser 2+1D events, 0.05 fm lattice, lognormal particle counts, "
+                        "all arriving at once")
+    return out
+
+
 def copy_calibration(nbytes=256 << 20):
     """One plain device-to-device copy of known size through the HIP runtime
     (its copy kernel reads and writes `nbytes`); HBM byte counters of the
@@ -254,7 +308,12 @@ def main():
         "roofline": roofline,
         "stage_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items() if v[1]},
     }
-    ev.close(); eos.close(); dev.close()
+    ev.close(); eos.close()
+    if not args.no_serve:
+        print(f"bench: serving {args.serve_events} events, {args.serve_slots} slots", file=sys.stderr, flush=True)
+        out["serving"] = serving_run(dev, args.serve_events, args.serve_slots)
+        print(f"bench: serving {json.dumps(out['serving'])}", file=sys.stderr, flush=True)
+    dev.close()
     if args.calibrate_copy:
         out["copy_calibration_bytes"] = copy_calibration()
     if not args.no_cpu:

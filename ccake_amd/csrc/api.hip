@@ -4,6 +4,7 @@
 #include "eos_device.h"
 #include "pair_common.h"
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <cstdio>
 #include <mutex>
@@ -34,17 +35,17 @@ static hipEvent_t get_event()
   if (hipEventCreate(&e) != hipSuccess) return nullptr;
   return e;
 }
-void prof_begin(ccake_dev* d, int k)
+void prof_begin(ccake_event* ev, int k)
 {
-  if (!d->prof.enabled) return;
+  if (!ev->dev->prof.enabled) return;
   hipEvent_t e = get_event();
-  if (e && hipEventRecord(e, d->stream) == hipSuccess) t_open[k] = e;
+  if (e && hipEventRecord(e, ev->stream) == hipSuccess) t_open[k] = e;
 }
-void prof_end(ccake_dev* d, int k)
+void prof_end(ccake_event* ev, int k)
 {
-  if (!d->prof.enabled || !t_open[k]) return;
+  if (!ev->dev->prof.enabled || !t_open[k]) return;
   hipEvent_t e = get_event();
-  if (e && hipEventRecord(e, d->stream) == hipSuccess) {
+  if (e && hipEventRecord(e, ev->stream) == hipSuccess) {
     std::lock_guard<std::mutex> lk(g_prof_mu);
     g_pending.push_back({ k, t_open[k], e });
   }
@@ -52,7 +53,7 @@ void prof_end(ccake_dev* d, int k)
 }
 static void prof_drain(ccake_dev* d)
 {
-  if (hipStreamSynchronize(d->stream) != hipSuccess) return;
+  if (hipSetDevice(d->gpu) != hipSuccess || hipDeviceSynchronize() != hipSuccess) return;
   std::lock_guard<std::mutex> lk(g_prof_mu);
   for (auto& p : g_pending) {
     float ms = 0.f;
@@ -71,50 +72,49 @@ static int check_launch(int rc) { return rc; }
 // ---- one RK stage: SPHWorkstation::get_time_derivatives ------------------
 static int time_derivatives(ccake_event* ev, int fuse_rk, double dt_stage, double t0)
 {
-  ccake_dev* d = ev->dev;
   int rc;
-  prof_begin(d, CCAKE_K_BIN);
+  prof_begin(ev, CCAKE_K_BIN);
   if ((rc = launch_stage_begin(ev))) return rc;                 // 02 gamma, v (+ bbox)
-  prof_end(d, CCAKE_K_BIN);
-  prof_begin(d, CCAKE_K_SORT);
+  prof_end(ev, CCAKE_K_BIN);
+  prof_begin(ev, CCAKE_K_SORT);
   if ((rc = launch_neighbor_sort(ev))) return rc;               // 01 cell sort + candidate records
-  prof_end(d, CCAKE_K_SORT);
-  prof_begin(d, CCAKE_K_SMOOTH);
+  prof_end(ev, CCAKE_K_SORT);
+  prof_begin(ev, CCAKE_K_SMOOTH);
   if ((rc = launch_smooth_pairs(ev, 0, true))) return rc;       // 03 pair sums per slice
-  prof_end(d, CCAKE_K_SMOOTH);
-  prof_begin(d, CCAKE_K_SMOOTH_SUM);
+  prof_end(ev, CCAKE_K_SMOOTH);
+  prof_begin(ev, CCAKE_K_SMOOTH_SUM);
   if ((rc = launch_smooth_sum(ev, 0, true))) return rc;         // 03 fields (+ btrack)
-  prof_end(d, CCAKE_K_SMOOTH_SUM);
-  prof_begin(d, CCAKE_K_EOS);
+  prof_end(ev, CCAKE_K_SMOOTH_SUM);
+  prof_begin(ev, CCAKE_K_EOS);
   if ((rc = launch_eos(ev, 0, 1))) return rc;                   // 04 + 05 + 08 + gather
-  prof_end(d, CCAKE_K_EOS);
-  prof_begin(d, CCAKE_K_GRAD);
+  prof_end(ev, CCAKE_K_EOS);
+  prof_begin(ev, CCAKE_K_GRAD);
   if ((rc = launch_grad_pairs(ev))) return rc;                  // 09 pair sums per slice
-  prof_end(d, CCAKE_K_GRAD);
-  prof_begin(d, CCAKE_K_GRAD_SUM);
+  prof_end(ev, CCAKE_K_GRAD);
+  prof_begin(ev, CCAKE_K_GRAD_SUM);
   if ((rc = launch_grad_sum(ev))) return rc;                    // 09 gradients
-  prof_end(d, CCAKE_K_GRAD_SUM);
-  prof_begin(d, CCAKE_K_EOM);
+  prof_end(ev, CCAKE_K_GRAD_SUM);
+  prof_begin(ev, CCAKE_K_EOM);
   if ((rc = launch_eom(ev, fuse_rk, dt_stage, t0))) return rc;  // 10 (+ RK update)
-  prof_end(d, CCAKE_K_EOM);
+  prof_end(ev, CCAKE_K_EOM);
   return 0;
 }
 
 static int rk2_step(ccake_event* ev, double dt)
 {
   int rc;
-  prof_begin(ev->dev, CCAKE_K_RK);
+  prof_begin(ev, CCAKE_K_RK);
   if ((rc = launch_cache(ev))) return rc;
-  prof_end(ev->dev, CCAKE_K_RK);
+  prof_end(ev, CCAKE_K_RK);
   const double t0 = ev->t;
   if ((rc = time_derivatives(ev, 1, 0.5 * dt, t0))) return rc;
   ev->t = t0 + 0.5 * dt;
   if ((rc = time_derivatives(ev, 1, dt, t0))) return rc;
   ev->t = t0 + dt;
   if (ev->p.particlization) {
-    prof_begin(ev->dev, CCAKE_K_MISC);
+    prof_begin(ev, CCAKE_K_MISC);
     if ((rc = launch_freeze(ev, ev->frzc))) return rc;
-    prof_end(ev->dev, CCAKE_K_MISC);
+    prof_end(ev, CCAKE_K_MISC);
     if (ev->frzc < 2) ev->frzc++;
   }
   return 0;
@@ -125,9 +125,17 @@ static int dalloc(ccake_event* ev, T** p, size_t count)
 {
   const size_t bytes = count * sizeof(T) + 256;
   ccake_dev* d = ev->dev;
-  if (d->budget && d->used + bytes > d->budget) { set_error("HBM budget exceeded"); return CCAKE_ERR_OOM; }
-  if (hipMalloc((void**)p, bytes) != hipSuccess) { set_error("hipMalloc failed"); return CCAKE_ERR_OOM; }
-  d->used += bytes;
+  {
+    std::lock_guard<std::mutex> lk(d->mu);
+    if (d->budget && d->used + bytes > d->budget) { set_error("HBM budget exceeded"); return CCAKE_ERR_OOM; }
+    d->used += bytes;                 // reserved before the allocation, so concurrent creators see it
+  }
+  if (hipMalloc((void**)p, bytes) != hipSuccess) {
+    std::lock_guard<std::mutex> lk(d->mu);
+    d->used -= bytes;
+    set_error("hipMalloc failed");
+    return CCAKE_ERR_OOM;
+  }
   ev->bytes += bytes;
   return 0;
 }
@@ -153,8 +161,6 @@ int ccake_dev_create(int gpu, size_t budget, ccake_dev** out)
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, gpu) == hipSuccess && prop.multiProcessorCount > 0)
     d->simds = 4 * prop.multiProcessorCount;
-  hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) { delete d; return hip_fail(e, "hipStreamCreate"); }
   *out = d;
   return 0;
 }
@@ -162,27 +168,30 @@ int ccake_dev_create(int gpu, size_t budget, ccake_dev** out)
 int ccake_dev_destroy(ccake_dev* d)
 {
   if (!d) return CCAKE_ERR_INVALID;
-  if (d->live_eos || d->live_events) {
-    set_error("ccake_dev_destroy: destroy the device's events and EoS objects first");
-    return CCAKE_ERR_INVALID;
+  {
+    std::lock_guard<std::mutex> lk(d->mu);
+    if (d->live_eos || d->live_events) {
+      set_error("ccake_dev_destroy: destroy the device's events and EoS objects first");
+      return CCAKE_ERR_INVALID;
+    }
   }
-  if (hipStreamSynchronize(d->stream) != hipSuccess) { /* fall through: still release */ }
   prof_drain(d);
-  hipError_t e = hipStreamDestroy(d->stream);
   delete d;
-  return e == hipSuccess ? 0 : hip_fail(e, "hipStreamDestroy");
+  return 0;
 }
 
 int ccake_dev_sync(ccake_dev* d)
 {
   if (!d) return CCAKE_ERR_INVALID;
-  CCAKE_HIP(hipStreamSynchronize(d->stream));
+  CCAKE_HIP(hipSetDevice(d->gpu));
+  CCAKE_HIP(hipDeviceSynchronize());
   return 0;
 }
 
 int ccake_dev_memory(ccake_dev* d, size_t* used, size_t* budget)
 {
   if (!d) return CCAKE_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(d->mu);
   if (used) *used = d->used;
   if (budget) *budget = d->budget;
   return 0;
@@ -212,6 +221,7 @@ int ccake_eos_create(ccake_dev* dev, const ccake_eos_desc* desc, ccake_eos** out
     ch.n = 2;
     init_analytic(ch.m[0], M_CONF, c0, 1, 1, 1, 1, 0, 0);
     init_analytic(ch.m[1], M_DIAG, c0, 1, 1, 1, 1, 0, 0);
+    std::lock_guard<std::mutex> lk(dev->mu);
     dev->live_eos++;
     *out = e;
     return 0;
@@ -238,10 +248,15 @@ int ccake_eos_create(ccake_dev* dev, const ccake_eos_desc* desc, ccake_eos** out
     t5[k * 5 + 4] = c[X_E];
   }
   e->bytes = nn * 22 * sizeof(double);
-  if (dev->budget && dev->used + e->bytes > dev->budget) { delete[] e->h_tab; delete e; set_error("HBM budget exceeded"); return CCAKE_ERR_OOM; }
+  {
+    std::lock_guard<std::mutex> lk(dev->mu);
+    if (dev->budget && dev->used + e->bytes > dev->budget) {
+      delete[] e->h_tab; delete e; set_error("HBM budget exceeded"); return CCAKE_ERR_OOM;
+    }
+    dev->used += e->bytes;
+  }
   CCAKE_HIP(hipMalloc((void**)&e->d_tab, nn * 17 * sizeof(double)));
   CCAKE_HIP(hipMalloc((void**)&e->d_tab4, nn * 5 * sizeof(double)));
-  dev->used += e->bytes;
   CCAKE_HIP(hipMemcpy(e->d_tab, desc->table, nn * 17 * sizeof(double), hipMemcpyHostToDevice));
   CCAKE_HIP(hipMemcpy(e->d_tab4, t5.data(), nn * 5 * sizeof(double), hipMemcpyHostToDevice));
   ch.tab = e->d_tab;
@@ -266,6 +281,7 @@ int ccake_eos_create(ccake_dev* dev, const ccake_eos_desc* desc, ccake_eos** out
   init_analytic(ch.m[1], M_TANH, c, 1.0, mu0[1], mu0[2], mu0[3], Tc, Ts);
   init_analytic(ch.m[2], M_CONF, c, 1.0, mu0[1], mu0[2], mu0[3], 0, 0);
   init_analytic(ch.m[3], M_DIAG, c, 1.0, mud[1], mud[2], mud[3], 0, 0);
+  std::lock_guard<std::mutex> lk(dev->mu);
   dev->live_eos++;
   *out = e;
   return 0;
@@ -274,14 +290,17 @@ int ccake_eos_create(ccake_dev* dev, const ccake_eos_desc* desc, ccake_eos** out
 int ccake_eos_destroy(ccake_eos* e)
 {
   if (!e) return CCAKE_ERR_INVALID;
-  if (e->live_events) {
-    set_error("ccake_eos_destroy: destroy the events that use this EoS first");
-    return CCAKE_ERR_INVALID;
+  {
+    std::lock_guard<std::mutex> lk(e->dev->mu);
+    if (e->live_events) {
+      set_error("ccake_eos_destroy: destroy the events that use this EoS first");
+      return CCAKE_ERR_INVALID;
+    }
+    e->dev->live_eos--;
+    e->dev->used -= (e->dev->used >= e->bytes ? e->bytes : e->dev->used);
   }
-  if (e->dev) e->dev->live_eos--;
   if (e->d_tab) (void)hipFree(e->d_tab);
   if (e->d_tab4) (void)hipFree(e->d_tab4);
-  if (e->dev) e->dev->used -= (e->dev->used >= e->bytes ? e->bytes : e->dev->used);
   delete[] e->h_tab;
   delete e;
   return 0;
@@ -314,6 +333,10 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
   ev->t = p->t0;
   ev->cap_cells = 65536;
   ev->radix_blocks = (n + 1023) / 1024;
+  {
+    const hipError_t se = hipStreamCreateWithFlags(&ev->stream, hipStreamNonBlocking);
+    if (se != hipSuccess) { ev->stream = nullptr; delete ev; return hip_fail(se, "hipStreamCreateWithFlags"); }
+  }
   int rc = 0;
   const size_t np = (size_t)ev->npad;
   const int nscan = 256 * ev->radix_blocks;
@@ -332,13 +355,17 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
   rc |= dalloc(ev, &ev->cell_off, (size_t)ev->cap_cells + 1);
   rc |= dalloc(ev, &ev->pair_count, 2);
   ev->nslices = choose_slices((n + kTile - 1) / kTile, dev->simds);
+  if (const char* ov = std::getenv("CCAKE_PAIR_SLICES")) {     // tuning override, 1..16
+    const int v = std::atoi(ov);
+    if (v >= 1 && v <= kMaxSlices) ev->nslices = v;
+  }
   rc |= dalloc(ev, &ev->part, (size_t)ev->nslices * kMaxAcc * np);
   rc |= dalloc(ev, &ev->cand_s, 8 * np);
   rc |= dalloc(ev, &ev->cand_g, 16 * np);
   rc |= dalloc(ev, &ev->red, 1024);
   rc |= dalloc(ev, &ev->err, 1);
   if (rc) { ccake_event_destroy(ev); return CCAKE_ERR_OOM; }
-  hipStream_t st = dev->stream;
+  hipStream_t st = ev->stream;
   CCAKE_HIP(hipMemsetAsync(ev->F, 0, (size_t)NUM_COLUMNS * np * sizeof(double), st));
   CCAKE_HIP(hipMemsetAsync(ev->I, 0, (size_t)CCAKE_NUM_IFIELDS * np * sizeof(int32_t), st));
   CCAKE_HIP(hipMemsetAsync(ev->err, 0, sizeof(int32_t), st));
@@ -350,6 +377,7 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
   CCAKE_HIP(hipStreamSynchronize(st));
   ev->perm = ev->val_a;
   ev->skey = ev->key_a;
+  std::lock_guard<std::mutex> lk(dev->mu);
   dev->live_events++;
   eos->live_events++;
   ev->counted = true;
@@ -360,13 +388,17 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
 int ccake_event_destroy(ccake_event* ev)
 {
   if (!ev) return CCAKE_ERR_INVALID;
-  (void)hipStreamSynchronize(ev->dev->stream);
+  if (ev->stream) (void)hipStreamSynchronize(ev->stream);
   void* bufs[] = { ev->F, ev->I, ev->grid, ev->bbox, ev->key, ev->key_a, ev->key_b, ev->val_a, ev->val_b, ev->hist,
                    ev->scan_tmp, ev->cell_off, ev->pair_count, ev->part, ev->cand_s, ev->cand_g, ev->red,
                    ev->err };
   for (void* b : bufs) if (b) (void)hipFree(b);
-  ev->dev->used -= (ev->dev->used >= ev->bytes ? ev->bytes : ev->dev->used);
-  if (ev->counted) { ev->dev->live_events--; ev->eos->live_events--; }
+  if (ev->stream) (void)hipStreamDestroy(ev->stream);
+  {
+    std::lock_guard<std::mutex> lk(ev->dev->mu);
+    ev->dev->used -= (ev->dev->used >= ev->bytes ? ev->bytes : ev->dev->used);
+    if (ev->counted) { ev->dev->live_events--; ev->eos->live_events--; }
+  }
   delete ev;
   return 0;
 }
@@ -381,32 +413,32 @@ int ccake_event_upload(ccake_event* ev, int field, const double* host, size_t n)
   if (field == CCAKE_F_X || field == CCAKE_F_Y || field == CCAKE_F_ETA)
     for (size_t i = 0; i < n; ++i)
       if (!std::isfinite(host[i])) { set_error("non-finite particle position"); return CCAKE_ERR_INVALID; }
-  CCAKE_HIP(hipMemcpyAsync(col(ev, field), host, n * sizeof(double), hipMemcpyHostToDevice, ev->dev->stream));
-  CCAKE_HIP(hipStreamSynchronize(ev->dev->stream));
+  CCAKE_HIP(hipMemcpyAsync(col(ev, field), host, n * sizeof(double), hipMemcpyHostToDevice, ev->stream));
+  CCAKE_HIP(hipStreamSynchronize(ev->stream));
   return 0;
 }
 
 int ccake_event_download(ccake_event* ev, int field, double* host, size_t n)
 {
   if (!ev || !host || field < 0 || field >= CCAKE_NUM_FIELDS || n != (size_t)ev->n) return CCAKE_ERR_INVALID;
-  CCAKE_HIP(hipMemcpyAsync(host, col(ev, field), n * sizeof(double), hipMemcpyDeviceToHost, ev->dev->stream));
-  CCAKE_HIP(hipStreamSynchronize(ev->dev->stream));
+  CCAKE_HIP(hipMemcpyAsync(host, col(ev, field), n * sizeof(double), hipMemcpyDeviceToHost, ev->stream));
+  CCAKE_HIP(hipStreamSynchronize(ev->stream));
   return 0;
 }
 
 int ccake_event_upload_int(ccake_event* ev, int f, const int32_t* host, size_t n)
 {
   if (!ev || !host || f < 0 || f >= CCAKE_NUM_IFIELDS || n != (size_t)ev->n) return CCAKE_ERR_INVALID;
-  CCAKE_HIP(hipMemcpyAsync(icol(ev, f), host, n * sizeof(int32_t), hipMemcpyHostToDevice, ev->dev->stream));
-  CCAKE_HIP(hipStreamSynchronize(ev->dev->stream));
+  CCAKE_HIP(hipMemcpyAsync(icol(ev, f), host, n * sizeof(int32_t), hipMemcpyHostToDevice, ev->stream));
+  CCAKE_HIP(hipStreamSynchronize(ev->stream));
   return 0;
 }
 
 int ccake_event_download_int(ccake_event* ev, int f, int32_t* host, size_t n)
 {
   if (!ev || !host || f < 0 || f >= CCAKE_NUM_IFIELDS || n != (size_t)ev->n) return CCAKE_ERR_INVALID;
-  CCAKE_HIP(hipMemcpyAsync(host, icol(ev, f), n * sizeof(int32_t), hipMemcpyDeviceToHost, ev->dev->stream));
-  CCAKE_HIP(hipStreamSynchronize(ev->dev->stream));
+  CCAKE_HIP(hipMemcpyAsync(host, icol(ev, f), n * sizeof(int32_t), hipMemcpyDeviceToHost, ev->stream));
+  CCAKE_HIP(hipStreamSynchronize(ev->stream));
   return 0;
 }
 
@@ -555,7 +587,7 @@ int ccake_event_check(ccake_event* ev)
 {
   if (!ev) return CCAKE_ERR_INVALID;
   int32_t e = 0;
-  hipStream_t st = ev->dev->stream;
+  hipStream_t st = ev->stream;
   CCAKE_HIP(hipMemcpyAsync(&e, ev->err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
   CCAKE_HIP(hipStreamSynchronize(st));
   if (e) CCAKE_HIP(hipMemsetAsync(ev->err, 0, sizeof(int32_t), st));
@@ -599,8 +631,8 @@ int ccake_last_pair_count(ccake_event* ev, int64_t* n_pairs)
 {
   if (!ev || !n_pairs) return CCAKE_ERR_INVALID;
   int64_t v = 0;
-  CCAKE_HIP(hipMemcpyAsync(&v, ev->pair_count, sizeof(int64_t), hipMemcpyDeviceToHost, ev->dev->stream));
-  CCAKE_HIP(hipStreamSynchronize(ev->dev->stream));
+  CCAKE_HIP(hipMemcpyAsync(&v, ev->pair_count, sizeof(int64_t), hipMemcpyDeviceToHost, ev->stream));
+  CCAKE_HIP(hipStreamSynchronize(ev->stream));
   *n_pairs = v;
   return 0;
 }
@@ -609,12 +641,12 @@ int ccake_pair_total(ccake_event* ev, int reset, int64_t* n_pairs)
 {
   if (!ev || !n_pairs) return CCAKE_ERR_INVALID;
   int64_t v = 0;
-  CCAKE_HIP(hipMemcpyAsync(&v, ev->pair_count + 1, sizeof(int64_t), hipMemcpyDeviceToHost, ev->dev->stream));
-  CCAKE_HIP(hipStreamSynchronize(ev->dev->stream));
+  CCAKE_HIP(hipMemcpyAsync(&v, ev->pair_count + 1, sizeof(int64_t), hipMemcpyDeviceToHost, ev->stream));
+  CCAKE_HIP(hipStreamSynchronize(ev->stream));
   *n_pairs = v;
   if (reset) {
-    CCAKE_HIP(hipMemsetAsync(ev->pair_count + 1, 0, sizeof(int64_t), ev->dev->stream));
-    CCAKE_HIP(hipStreamSynchronize(ev->dev->stream));
+    CCAKE_HIP(hipMemsetAsync(ev->pair_count + 1, 0, sizeof(int64_t), ev->stream));
+    CCAKE_HIP(hipStreamSynchronize(ev->stream));
   }
   return 0;
 }

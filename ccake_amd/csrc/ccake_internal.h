@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include "../../include/ccake_amd.h"
 
@@ -68,7 +69,7 @@ struct Profile {
 struct ccake_dev {
   int gpu = 0;
   int simds = 1024;          // SIMDs of the device (4 per CU)
-  hipStream_t stream = nullptr;
+  std::mutex mu;             // guards the memory accounting and live counts below
   size_t budget = 0;
   size_t used = 0;
   int live_eos = 0;          // handles created on this device and not yet destroyed
@@ -90,6 +91,7 @@ struct ccake_eos {
 struct ccake_event {
   ccake_dev* dev = nullptr;
   ccake_eos* eos = nullptr;
+  hipStream_t stream = nullptr;  // every launch and copy of the event (default priority)
   bool counted = false;      // included in the device's and EoS's live counts
   ccake_params p{};
   int n = 0;
@@ -134,9 +136,9 @@ void set_error(const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 #define CCAKE_HIP(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return ccake::hip_fail(e_, #x); } while (0)
 
-// profiling helpers
-void prof_begin(ccake_dev* d, int k);
-void prof_end(ccake_dev* d, int k);
+// profiling helpers (HIP events on the event's stream)
+void prof_begin(ccake_event* ev, int k);
+void prof_end(ccake_event* ev, int k);
 
 inline double* col(ccake_event* ev, int f) { return ev->F + (size_t)f * ev->npad; }
 inline int32_t* icol(ccake_event* ev, int f) { return ev->I + (size_t)f * ev->npad; }

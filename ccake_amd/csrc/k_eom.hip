@@ -401,7 +401,7 @@ int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0)
   (void)t0;
   const int n = ev->n;
   const dim3 g((n + 255) / 256), b(256);
-  hipStream_t st = ev->dev->stream;
+  hipStream_t st = ev->stream;
   const int32_t* fz = ev->I + (size_t)CCAKE_I_FREEZE * ev->npad;
   const double uv = ev->p.using_vorticity ? 1.0 : 0.0;
   if (ev->p.using_shear) {
@@ -418,7 +418,7 @@ int launch_rk(ccake_event* ev, double dt, double t0)
 {
   (void)t0;
   const int n = ev->n;
-  hipLaunchKernelGGL(k_rk, dim3((n + 255) / 256), dim3(256), 0, ev->dev->stream, ev->F, ev->npad, n,
+  hipLaunchKernelGGL(k_rk, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n,
                      ev->I + (size_t)CCAKE_I_FREEZE * ev->npad, dt, ev->err);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
@@ -426,7 +426,7 @@ int launch_rk(ccake_event* ev, double dt, double t0)
 int launch_cache(ccake_event* ev)
 {
   const int n = ev->n;
-  hipLaunchKernelGGL(k_cache, dim3((n + 255) / 256), dim3(256), 0, ev->dev->stream, ev->F, ev->npad, n);
+  hipLaunchKernelGGL(k_cache, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
 

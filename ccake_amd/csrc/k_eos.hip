@@ -165,7 +165,7 @@ int launch_eos(ccake_event* ev, int mode, int fuse_post)
 {
   const int n = ev->n;
   const dim3 g((n + 255) / 256), b(256);
-  hipStream_t st = ev->dev->stream;
+  hipStream_t st = ev->stream;
   const Transport P = make_transport(ev->p);
   if (mode == 1)
     hipLaunchKernelGGL(k_eos<1>, g, b, 0, st, ev->eos->chain, ev->F, ev->npad, n, ev->I, ev->perm, ev->t, P,
@@ -225,7 +225,7 @@ __global__ __launch_bounds__(256) void k_post(double* __restrict__ F, int npad, 
 static int post(ccake_event* ev, int what, const int32_t* perm)
 {
   const int n = ev->n;
-  hipLaunchKernelGGL(k_post, dim3((n + 255) / 256), dim3(256), 0, ev->dev->stream, ev->F, ev->npad, n, perm, ev->t,
+  hipLaunchKernelGGL(k_post, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n, perm, ev->t,
                      make_transport(ev->p), ev->cand_g, what);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }

@@ -153,10 +153,10 @@ int launch_grad_pairs(ccake_event* ev)
   const dim3 grid(ntiles * ev->nslices), blk(kWave);
   const double rcs = rc * (1.0 + 1e-12) + 1e-12;
   if (ev->p.using_shear)
-    hipLaunchKernelGGL(k_grad_slice<true>, grid, blk, 0, ev->dev->stream, ev->cand_g, ev->skey, ev->cell_off,
+    hipLaunchKernelGGL(k_grad_slice<true>, grid, blk, 0, ev->stream, ev->cand_g, ev->skey, ev->cell_off,
                        ev->grid, n, rcs, hinv2, ev->nslices, ev->part, ev->npad);
   else
-    hipLaunchKernelGGL(k_grad_slice<false>, grid, blk, 0, ev->dev->stream, ev->cand_g, ev->skey, ev->cell_off,
+    hipLaunchKernelGGL(k_grad_slice<false>, grid, blk, 0, ev->stream, ev->cand_g, ev->skey, ev->cell_off,
                        ev->grid, n, rcs, hinv2, ev->nslices, ev->part, ev->npad);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
@@ -170,7 +170,7 @@ static int grad_sum_dispatch(ccake_event* ev, double C)
   } else {
     if (ev->nslices != S) return grad_sum_dispatch<SHEAR, S + 1>(ev, C);
     const int n = ev->n;
-    hipLaunchKernelGGL((k_grad_sum<SHEAR, S>), dim3((n + 255) / 256), dim3(256), 0, ev->dev->stream, ev->cand_g,
+    hipLaunchKernelGGL((k_grad_sum<SHEAR, S>), dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->cand_g,
                        ev->perm, n, C, ev->part, ev->npad, ev->F);
     return 0;
   }

@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void k_init(double* __restrict__ F, int npad, 
 int launch_init_particles(ccake_event* ev, int phase)
 {
   const int n = ev->n;
-  hipLaunchKernelGGL(k_init, dim3((n + 255) / 256), dim3(256), 0, ev->dev->stream, ev->F, ev->npad, n, ev->I, phase,
+  hipLaunchKernelGGL(k_init, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n, ev->I, phase,
                      ev->p.t0, ev->p.e_freeze, ev->p.input_initial_shear, ev->p.bulk_from_trace);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(256) void k_freeze(double* __restrict__ F, int npad
 int launch_freeze(ccake_event* ev, int phase)
 {
   const int n = ev->n;
-  hipLaunchKernelGGL(k_freeze, dim3((n + 255) / 256), dim3(256), 0, ev->dev->stream, ev->F, ev->npad, n, ev->I,
+  hipLaunchKernelGGL(k_freeze, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n, ev->I,
                      ev->p.e_freeze, phase);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
@@ -175,7 +175,7 @@ constexpr int kRedBlocks = 512;
 
 int launch_reduce(ccake_event* ev, int which, double* out)
 {
-  hipStream_t st = ev->dev->stream;
+  hipStream_t st = ev->stream;
   hipLaunchKernelGGL(k_reduce, dim3(kRedBlocks), dim3(256), 0, st, ev->F, ev->npad, ev->n, ev->I, which, ev->t, ev->red);
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, ev->red, kRedBlocks);
   CCAKE_HIP(hipGetLastError());

@@ -293,7 +293,7 @@ static int scan_exclusive(ccake_event* ev, const int32_t* in, int32_t* out, int 
 {
   const int per = kScanThreads * kScanItems;
   const int nb = (n + per - 1) / per;
-  hipStream_t st = ev->dev->stream;
+  hipStream_t st = ev->stream;
   hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanThreads), 0, st, in, n, ev->scan_tmp);
   hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanThreads), 0, st, ev->scan_tmp, nb);
   hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kScanThreads), 0, st, in, n, ev->scan_tmp, out);
@@ -328,7 +328,7 @@ static int stage_begin(ccake_event* ev, int do_gamma)
 {
   const int n = ev->n;
   const int nb = (n + 255) / 256;
-  hipLaunchKernelGGL(k_stage_begin, dim3(nb), dim3(256), 0, ev->dev->stream, ev->F, ev->npad, n, ev->bbox, do_gamma);
+  hipLaunchKernelGGL(k_stage_begin, dim3(nb), dim3(256), 0, ev->stream, ev->F, ev->npad, n, ev->bbox, do_gamma);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
 
@@ -339,7 +339,7 @@ int launch_gamma_velocity(ccake_event* ev) { return stage_begin(ev, 1); }
 static void scan_hist(ccake_event* ev, int nscan)
 {
   if (nscan <= kScanOneMax)
-    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(kScanOneThreads), 0, ev->dev->stream, ev->hist, nscan);
+    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(kScanOneThreads), 0, ev->stream, ev->hist, nscan);
   else
     scan_exclusive(ev, ev->hist, ev->hist, nscan);
 }
@@ -349,7 +349,7 @@ static void scan_hist(ccake_event* ev, int nscan)
 int launch_neighbor_sort(ccake_event* ev)
 {
   const int n = ev->n;
-  hipStream_t st = ev->dev->stream;
+  hipStream_t st = ev->stream;
   const int nb = (n + 255) / 256;
   const double rc = 2.0 * ev->p.hT;
   hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(256), 0, st, ev->bbox, nb, rc, ev->p.cells_per_rc,

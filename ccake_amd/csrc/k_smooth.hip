@@ -209,7 +209,7 @@ static KernelW make_w(double hT)
 int launch_gather_smooth(ccake_event* ev)
 {
   const int n = ev->n;
-  hipLaunchKernelGGL(k_gather_smooth, dim3((n + 255) / 256), dim3(256), 0, ev->dev->stream, ev->F, ev->npad,
+  hipLaunchKernelGGL(k_gather_smooth, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad,
                      n, ev->perm, ev->cand_s);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
@@ -223,7 +223,7 @@ int launch_smooth_pairs(ccake_event* ev, int mode, bool count_pairs)
   const double rc2 = rc * rc;
   const double rcs = rc * (1.0 + 1e-12) + 1e-12;   // search radius with a rounding margin
   const int ntiles = (n + kTile - 1) / kTile;
-  hipStream_t st = ev->dev->stream;
+  hipStream_t st = ev->stream;
   if (mode == 2) {
     int32_t* bt = ev->I + (size_t)CCAKE_I_BTRACK * ev->npad;
     unsigned long long* pc = count_pairs ? (unsigned long long*)ev->pair_count : nullptr;
@@ -247,7 +247,7 @@ static void smooth_sum_s(ccake_event* ev, double norm2, int32_t* bt, unsigned lo
 {
   const int n = ev->n;
   const dim3 grid((n + kSumThreads - 1) / kSumThreads), blk(kSumThreads);
-  hipLaunchKernelGGL((k_smooth_sum<MODE, S>), grid, blk, 0, ev->dev->stream, ev->perm, n, norm2, ev->t, ev->part,
+  hipLaunchKernelGGL((k_smooth_sum<MODE, S>), grid, blk, 0, ev->stream, ev->perm, n, norm2, ev->t, ev->part,
                      ev->npad, ev->F, bt, pc);
 }
 
@@ -285,7 +285,7 @@ int launch_smooth(ccake_event* ev, int mode, bool count_pairs)
 int launch_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, size_t* n_out)
 {
   const int n = ev->n;
-  hipStream_t st = ev->dev->stream;
+  hipStream_t st = ev->stream;
   // fresh counts through btrack, which is saved and restored around the count
   std::vector<int32_t> saved_bt(n), cnt(n);
   int32_t* bt = ev->I + (size_t)CCAKE_I_BTRACK * ev->npad;

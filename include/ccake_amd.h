@@ -4,8 +4,11 @@
  * Everything crossing this boundary is plain C: opaque handles, plain
  * pointers, sizes and int status codes.  No C++ or device types.  The
  * library owns every device buffer and the EoS tables; the caller owns
- * host arrays.  Calls on one ccake_dev are stream ordered and asynchronous
- * except *_download, *_reduce, *_sync and the query calls.
+ * host arrays.  Every event owns a HIP stream (default priority): the calls
+ * on one event are ordered on it and asynchronous except *_upload,
+ * *_download, *_reduce, *_check, *_sync and the query calls.  Different
+ * events may be driven from different host threads at once (their work
+ * overlaps on the device); one event is driven by one thread at a time.
  *
  * Each entry point names the reference interface it replaces
  * (the-nuclear-confectionery/CCAKE, file:line).  Units are the reference's
@@ -159,7 +162,8 @@ typedef struct ccake_eos   ccake_eos;
 typedef struct ccake_event ccake_event;
 
 /* ---------------------------- lifetime ---------------------------- */
-/* Device context: one HIP device, one stream, an HBM budget (0 = all).
+/* Device context: one HIP device and an HBM budget (0 = all) that event and
+ * EoS creation is admitted against (CCAKE_ERR_OOM beyond it).
  * Replaces the Kokkos DeviceType/ExecutionSpace typedefs
  * (include/utilities.h:42-55). */
 CCAKE_API int  ccake_dev_create(int gpu, size_t hbm_budget_bytes, ccake_dev** out);
@@ -167,6 +171,7 @@ CCAKE_API int  ccake_dev_create(int gpu, size_t hbm_budget_bytes, ccake_dev** ou
  * device; destroying a parent that still has live children is refused
  * (CCAKE_ERR_INVALID) and leaves everything valid. */
 CCAKE_API int  ccake_dev_destroy(ccake_dev* dev);
+/* Waits for the work of every event on the device. */
 CCAKE_API int  ccake_dev_sync(ccake_dev* dev);
 /* Bytes of HBM the library holds on this device, and its budget. */
 CCAKE_API int  ccake_dev_memory(ccake_dev* dev, size_t* used, size_t* budget);
@@ -262,7 +267,7 @@ enum ccake_reduction { CCAKE_R_ENTROPY = 0, CCAKE_R_BARYON, CCAKE_R_STRANGE, CCA
 CCAKE_API int  ccake_reduce(ccake_event* ev, int which, double* out);
 
 /* --------------------------- profiling ----------------------------- */
-/* Per-kernel HIP-event timing on the library's stream (off by default).
+/* Per-kernel HIP-event timing on the events' streams (off by default).
  * SMOOTH and GRAD time the pair kernels alone; SMOOTH_SUM and GRAD_SUM the
  * passes that add their per-slice partial sums and write the fields. */
 enum ccake_kernel_id { CCAKE_K_BIN = 0, CCAKE_K_SORT, CCAKE_K_SMOOTH, CCAKE_K_EOS,
