@@ -29,7 +29,7 @@ __global__ __launch_bounds__(kWave) void k_grad_slice(const double* __restrict__
                                                       const int32_t* __restrict__ skey,
                                                       const int32_t* __restrict__ cell_off,
                                                       const Grid* __restrict__ gridp, int n,
-                                                      double rcs, double hinv2, int nslices,
+                                                      double rc2, double rcs, double hinv2, int nslices,
                                                       double* __restrict__ part, int npad)
 {
   constexpr int NACC = grad_nacc<SHEAR>();
@@ -58,6 +58,7 @@ __global__ __launch_bounds__(kWave) void k_grad_slice(const double* __restrict__
       walk_records<NR, kCandG, SHEAR ? 1 : 2>(cand, xa, ya, j0, j1,
                                               [&](const double* c, double dx, double dy, int j, bool valid) {
         const double r2 = dist2_nocontract(dx, dy);
+        if (kSkipEmpty && __ballot(valid && r2 <= rc2) == 0ull) return;
         double q, iq;
         q_and_inverse(r2, hinv2, q, iq);
         const double gm = valid ? c[2] * spline_g(q, iq) : 0.0;
@@ -154,10 +155,10 @@ int launch_grad_pairs(ccake_event* ev)
   const double rcs = rc * (1.0 + 1e-12) + 1e-12;
   if (ev->p.using_shear)
     hipLaunchKernelGGL(k_grad_slice<true>, grid, blk, 0, ev->stream, ev->cand_g, ev->skey, ev->cell_off,
-                       ev->grid, n, rcs, hinv2, ev->nslices, ev->part, ev->npad);
+                       ev->grid, n, rc * rc, rcs, hinv2, ev->nslices, ev->part, ev->npad);
   else
     hipLaunchKernelGGL(k_grad_slice<false>, grid, blk, 0, ev->stream, ev->cand_g, ev->skey, ev->cell_off,
-                       ev->grid, n, rcs, hinv2, ev->nslices, ev->part, ev->npad);
+                       ev->grid, n, rc * rc, rcs, hinv2, ev->nslices, ev->part, ev->npad);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
 

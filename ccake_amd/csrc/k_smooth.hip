@@ -111,7 +111,9 @@ __global__ __launch_bounds__(kWave) void k_smooth_slice(const double* __restrict
       walk_records<NR, kCandS, 2>(cand, xi, yi, j0, j1,
                                   [&](const double* c, double dx, double dy, int j, bool valid) {
         const double r2 = dist2_nocontract(dx, dy);
-        cnt += (valid && r2 <= rc2) ? 1 : 0;
+        const bool nb = valid && r2 <= rc2;
+        cnt += nb ? 1 : 0;
+        if (kSkipEmpty && __ballot(nb) == 0ull) return;
         double q, iq;
         q_and_inverse(r2, hinv2, q, iq);
         const double w = valid ? spline_w(q) : 0.0;
