@@ -69,16 +69,53 @@ static void prof_drain(ccake_dev* d)
 
 static int check_launch(int rc) { return rc; }
 
-// ---- one RK stage: SPHWorkstation::get_time_derivatives ------------------
-static int time_derivatives(ccake_event* ev, int fuse_rk, double dt_stage, double t0)
+// ---- reuse of the cell sort ------------------------------------------------
+// Each RK update moves a particle by less than c * dt (|v| = |u| / gamma < 1).
+// A sort taken at positions x_s therefore stays usable for the pair walks
+// while a host-side bound B >= |x - x_s| is added to their search radius;
+// the exact predicate still decides every pair, so the pair set is the one a
+// fresh sort gives.  B is 0 right after a sort, grows by dt/2 into stage 2
+// and by dt per completed step; a stage sorts again when B would exceed
+// skin_frac * rc (default 0.04; CCAKE_SORT_SKIN overrides it for tuning).
+
+static int fresh_sort(ccake_event* ev)
 {
   int rc;
+  if ((rc = launch_bbox(ev))) return rc;
+  if ((rc = launch_neighbor_sort(ev))) return rc;
+  ev->sort_valid = true;
+  ev->sort_disp = 0.0;
+  return 0;
+}
+
+// Walking entry points of the phase-by-phase C-ABI sort first when the
+// positions changed by an unknown amount since the last sort.
+static int ensure_sorted(ccake_event* ev)
+{
+  return ev->sort_valid ? 0 : fresh_sort(ev);
+}
+
+// ---- one RK stage: SPHWorkstation::get_time_derivatives ------------------
+// may_reuse: the stage may keep the last cell sort (ev->sort_disp bounds the
+// displacement since); *sorted reports whether it sorted.
+static int time_derivatives(ccake_event* ev, int fuse_rk, double dt_stage, double t0, bool may_reuse,
+                            bool* sorted = nullptr)
+{
+  int rc;
+  const bool reuse = may_reuse && ev->sort_valid && ev->sort_disp <= ev->skin_frac * 2.0 * ev->p.hT;
   prof_begin(ev, CCAKE_K_BIN);
   if ((rc = launch_stage_begin(ev))) return rc;                 // 02 gamma, v (+ bbox)
   prof_end(ev, CCAKE_K_BIN);
   prof_begin(ev, CCAKE_K_SORT);
-  if ((rc = launch_neighbor_sort(ev))) return rc;               // 01 cell sort + candidate records
+  if (reuse) {
+    if ((rc = launch_gather_smooth(ev))) return rc;             // candidate records at current positions
+  } else {
+    if ((rc = launch_neighbor_sort(ev))) return rc;             // 01 cell sort + candidate records
+    ev->sort_valid = true;
+    ev->sort_disp = 0.0;
+  }
   prof_end(ev, CCAKE_K_SORT);
+  if (sorted) *sorted = !reuse;
   prof_begin(ev, CCAKE_K_SMOOTH);
   if ((rc = launch_smooth_pairs(ev, 0, true))) return rc;       // 03 pair sums per slice
   prof_end(ev, CCAKE_K_SMOOTH);
@@ -107,9 +144,15 @@ static int rk2_step(ccake_event* ev, double dt)
   if ((rc = launch_cache(ev))) return rc;
   prof_end(ev, CCAKE_K_RK);
   const double t0 = ev->t;
-  if ((rc = time_derivatives(ev, 1, 0.5 * dt, t0))) return rc;
+  const double adt = fabs(dt);
+  bool sorted = false;
+  if ((rc = time_derivatives(ev, 1, 0.5 * dt, t0, true, &sorted))) return rc;
+  const double d0 = ev->sort_disp;                 // bound at the step's start positions x0
+  ev->sort_disp = d0 + 0.5 * adt;                  // x0 + dt/2 v
   ev->t = t0 + 0.5 * dt;
-  if ((rc = time_derivatives(ev, 1, dt, t0))) return rc;
+  if ((rc = time_derivatives(ev, 1, dt, t0, true, &sorted))) return rc;
+  // end of step: x0 + dt v_mid, within dt of x0 and 1.5 dt of x0 + dt/2 v
+  ev->sort_disp = sorted ? 1.5 * adt : d0 + adt;
   ev->t = t0 + dt;
   if (ev->p.particlization) {
     prof_begin(ev, CCAKE_K_MISC);
@@ -359,6 +402,10 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
     const int v = std::atoi(ov);
     if (v >= 1 && v <= kMaxSlices) ev->nslices = v;
   }
+  if (const char* ov = std::getenv("CCAKE_SORT_SKIN")) {       // tuning override, fraction of rc
+    const double v = std::atof(ov);
+    if (v >= 0.0 && v <= 1.0) ev->skin_frac = v;
+  }
   rc |= dalloc(ev, &ev->part, (size_t)ev->nslices * kMaxAcc * np);
   rc |= dalloc(ev, &ev->cand_s, 8 * np);
   rc |= dalloc(ev, &ev->cand_g, 16 * np);
@@ -410,9 +457,11 @@ int ccake_event_get_time(ccake_event* ev, double* t) { if (!ev || !t) return CCA
 int ccake_event_upload(ccake_event* ev, int field, const double* host, size_t n)
 {
   if (!ev || !host || field < 0 || field >= CCAKE_NUM_FIELDS || n != (size_t)ev->n) return CCAKE_ERR_INVALID;
-  if (field == CCAKE_F_X || field == CCAKE_F_Y || field == CCAKE_F_ETA)
+  if (field == CCAKE_F_X || field == CCAKE_F_Y || field == CCAKE_F_ETA) {
     for (size_t i = 0; i < n; ++i)
       if (!std::isfinite(host[i])) { set_error("non-finite particle position"); return CCAKE_ERR_INVALID; }
+    ev->sort_valid = false;             // the cell sort no longer matches
+  }
   CCAKE_HIP(hipMemcpyAsync(col(ev, field), host, n * sizeof(double), hipMemcpyHostToDevice, ev->stream));
   CCAKE_HIP(hipStreamSynchronize(ev->stream));
   return 0;
@@ -447,16 +496,15 @@ int ccake_build_neighbors(ccake_event* ev)
 {
   if (!ev) return CCAKE_ERR_INVALID;
   int rc;
-  if ((rc = launch_bbox(ev))) return rc;
-  if ((rc = launch_neighbor_sort(ev))) return rc;
-  if ((rc = launch_gather_smooth(ev))) return rc;
+  if ((rc = fresh_sort(ev))) return rc;
   return launch_smooth(ev, 2);
 }
 
 int ccake_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, size_t* n_pairs)
 {
   if (!ev || !n_pairs) return CCAKE_ERR_INVALID;
-  int rc = launch_gather_smooth(ev);
+  int rc = ensure_sorted(ev);
+  if (!rc) rc = launch_gather_smooth(ev);
   if (rc) return rc;
   return launch_export_pairs(ev, pi, pj, cap, n_pairs);
 }
@@ -466,14 +514,16 @@ int ccake_gamma_velocity(ccake_event* ev) { return ev ? launch_stage_begin(ev) :
 int ccake_smooth_fields(ccake_event* ev)
 {
   if (!ev) return CCAKE_ERR_INVALID;
-  int rc = launch_gather_smooth(ev);
+  int rc = ensure_sorted(ev);
+  if (!rc) rc = launch_gather_smooth(ev);
   return rc ? rc : launch_smooth(ev, 0);
 }
 
 int ccake_initial_sigma(ccake_event* ev)
 {
   if (!ev) return CCAKE_ERR_INVALID;
-  int rc = launch_gather_smooth(ev);
+  int rc = ensure_sorted(ev);
+  if (!rc) rc = launch_gather_smooth(ev);
   return rc ? rc : launch_smooth(ev, 1);
 }
 
@@ -491,7 +541,8 @@ int ccake_viscosities(ccake_event* ev) { return ev ? launch_viscosities(ev) : CC
 int ccake_smooth_gradients(ccake_event* ev)
 {
   if (!ev) return CCAKE_ERR_INVALID;
-  int rc = launch_gather_grad(ev);
+  int rc = ensure_sorted(ev);
+  if (!rc) rc = launch_gather_grad(ev);
   return rc ? rc : launch_grad(ev);
 }
 
@@ -500,7 +551,7 @@ int ccake_eom(ccake_event* ev) { return ev ? launch_eom(ev, 0, 0.0, ev->t) : CCA
 int ccake_get_time_derivatives(ccake_event* ev)
 {
   if (!ev) return CCAKE_ERR_INVALID;
-  int rc = time_derivatives(ev, 0, 0.0, ev->t);
+  int rc = time_derivatives(ev, 0, 0.0, ev->t, false);
   if (rc) return rc;
   return ccake_event_check(ev);
 }
@@ -512,6 +563,7 @@ int ccake_rk_update(ccake_event* ev, double dt, double t0)
   if (!ev) return CCAKE_ERR_INVALID;
   int rc = launch_rk(ev, dt, t0);
   if (rc) return rc;
+  ev->sort_valid = false;               // positions moved from outside a step
   ev->t = t0 + dt;
   return 0;
 }

@@ -116,6 +116,10 @@ struct ccake_event {
   int32_t* rank = nullptr;      // particle id -> sorted position
   int64_t* pair_count = nullptr;  // device accumulators: [0] last stage, [1] since the last reset
   int radix_blocks = 0;
+  // reuse of the cell sort: bound on |x - x_sorted| of every particle
+  bool sort_valid = false;
+  double sort_disp = 0.0;
+  double skin_frac = 0.04;      // re-sort when the bound would exceed skin_frac * rc (measured best at 40k)
   // pair-kernel partial sums: nslices x (up to 17) x npad, sorted order
   int nslices = 1;
   double* part = nullptr;
@@ -141,6 +145,17 @@ void prof_begin(ccake_event* ev, int k);
 void prof_end(ccake_event* ev, int k);
 
 inline double* col(ccake_event* ev, int f) { return ev->F + (size_t)f * ev->npad; }
+
+// Search radius of the pair walks: rc with a rounding margin, widened by the
+// bound on how far particles have moved since the cell sort (a candidate's
+// sorted cell then lies within the radius of every query it can reach).
+inline double search_radius(const ccake_event* ev)
+{
+  const double rc = 2.0 * ev->p.hT;
+  double r = rc * (1.0 + 1e-12) + 1e-12;
+  if (ev->sort_disp > 0.0) r += ev->sort_disp * (1.0 + 1e-12) + 1e-12;
+  return r;
+}
 inline int32_t* icol(ccake_event* ev, int f) { return ev->I + (size_t)f * ev->npad; }
 
 // ---- launchers (one per .hip unit) ----

@@ -152,7 +152,7 @@ int launch_grad_pairs(ccake_event* ev)
   const double hinv2 = 1.0 / (h * h);
   const int ntiles = (n + kTile - 1) / kTile;
   const dim3 grid(ntiles * ev->nslices), blk(kWave);
-  const double rcs = rc * (1.0 + 1e-12) + 1e-12;
+  const double rcs = search_radius(ev);
   if (ev->p.using_shear)
     hipLaunchKernelGGL(k_grad_slice<true>, grid, blk, 0, ev->stream, ev->cand_g, ev->skey, ev->cell_off,
                        ev->grid, n, rc * rc, rcs, hinv2, ev->nslices, ev->part, ev->npad);

@@ -223,7 +223,7 @@ int launch_smooth_pairs(ccake_event* ev, int mode, bool count_pairs)
   const int n = ev->n;
   const double rc = 2.0 * ev->p.hT;
   const double rc2 = rc * rc;
-  const double rcs = rc * (1.0 + 1e-12) + 1e-12;   // search radius with a rounding margin
+  const double rcs = search_radius(ev);
   const int ntiles = (n + kTile - 1) / kTile;
   hipStream_t st = ev->stream;
   if (mode == 2) {
@@ -310,7 +310,7 @@ int launch_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, s
   CCAKE_HIP(hipMemcpyAsync(d_off, off.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
   const double rc = 2.0 * ev->p.hT;
   hipLaunchKernelGGL(k_walk<3>, dim3((n + kTile - 1) / kTile), dim3(kTile), 0, st, ev->cand_s, ev->skey,
-                     ev->perm, ev->cell_off, ev->grid, n, rc * rc, rc * (1.0 + 1e-12) + 1e-12, bt, nullptr, d_off,
+                     ev->perm, ev->cell_off, ev->grid, n, rc * rc, search_radius(ev), bt, nullptr, d_off,
                      d_i, d_j);
   CCAKE_HIP(hipGetLastError());
   CCAKE_HIP(hipMemcpyAsync(pi, d_i, total * sizeof(int32_t), hipMemcpyDeviceToHost, st));
