@@ -274,6 +274,35 @@ def test_pairs_full_size_lattice(dev, orc):
     ev.close(); eos.close()
 
 
+@pytest.mark.parametrize("skin", ["0.04", "0.5"])
+def test_pairs_bit_exact_on_a_kept_sort(dev, orc, monkeypatch, skin):
+    """Time steps keep the last cell sort while the displacement bound
+    allows, widening the search radius by it.  After 8 steps of Gubser flow
+    (0.1 fm lattice; at skin 0.5 rc the sort from initialize is never
+    renewed) the exported pair set equals a brute-force build on the current
+    positions, bit for bit.  (btrack holds the counts of the last stage's
+    positions, as in the reference, so it is not compared here.)"""
+    monkeypatch.setenv("CCAKE_SORT_SKIN", skin)
+    cfg = config.gubser_ideal_params()
+    d = ic.gubser_ideal(spacing=0.1)
+    n = len(d["x"])
+    eos = ccake_amd.Eos(dev, 0)
+    ev = ccake_amd.Event(dev, eos, cfg, n)
+    ccake_amd.load_event(ev, d)
+    ev.initialize()
+    assert ev.run_steps(cfg["dt"], 8) == 8
+    x, y = ev.download("CCAKE_F_X"), ev.download("CCAKE_F_Y")
+    assert np.max(np.hypot(x - d["x"], y - d["y"])) > 0.02      # particles left their sorted cells' positions
+    gi, gj = ev.pairs()
+    oeos = orc.Eos(0)
+    o = orc.Event(cfg, oeos, n)
+    o.set("ORC_F_X", x); o.set("ORC_F_Y", y)
+    o.build_neighbors(True)                       # brute force
+    oi, oj = o.pairs()
+    assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
+    ev.close(); eos.close()
+
+
 # ------------------------------------------------------------------ ideal Gubser
 @pytest.fixture(scope="module")
 def gubser(dev, orc):
