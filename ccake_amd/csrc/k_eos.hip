@@ -161,10 +161,18 @@ static Transport make_transport(const ccake_params& p)
   return T;
 }
 
+// Workgroup size of the EoS kernel.  It runs at one wave per SIMD (its
+// register footprint), so small workgroups spread the waves over every CU
+// (64 measured best at 32k-40k particles, 256 and 128 slower).
+#ifndef CCAKE_EOS_THREADS
+#define CCAKE_EOS_THREADS 64
+#endif
+constexpr int kEosThreads = CCAKE_EOS_THREADS;
+
 int launch_eos(ccake_event* ev, int mode, int fuse_post)
 {
   const int n = ev->n;
-  const dim3 g((n + 255) / 256), b(256);
+  const dim3 g((n + kEosThreads - 1) / kEosThreads), b(kEosThreads);
   hipStream_t st = ev->stream;
   const Transport P = make_transport(ev->p);
   if (mode == 1)
