@@ -55,6 +55,7 @@ IFIELDS = _parse_enum("ccake_ifield")
 KERNELS = _parse_enum("ccake_kernel_id")
 REDUCTIONS = _parse_enum("ccake_reduction")
 STATUS = {v: k for k, v in _parse_enum("ccake_status").items()}
+FO_ROW = 42       # CCAKE_FO_ROW: id, step, then the 40 freeze_out.dat columns
 
 PARAM_FIELDS = [
     ("dim", C.c_int), ("rk_order", C.c_int),
@@ -126,6 +127,7 @@ ABI = {
     "ccake_initialize": (C.c_int, [_vp]),
     "ccake_freeze_out_particles": (C.c_int, [_vp]),
     "ccake_continue_evolution": (C.c_int, [_vp, C.c_double, C.POINTER(C.c_int)]),
+    "ccake_freeze_out_surface": (C.c_int, [_vp, _dp, C.c_size_t, C.POINTER(C.c_size_t)]),
     "ccake_event_check": (C.c_int, [_vp]),
     "ccake_reduce": (C.c_int, [_vp, C.c_int, _dp]),
     "ccake_profile_enable": (C.c_int, [_vp, C.c_int]),
@@ -355,6 +357,19 @@ class Event:
         _ok(lib().ccake_export_pairs(self.h, pi.ctypes.data_as(_ip), pj.ctypes.data_as(_ip), n.value,
                                      C.byref(n)), "export_pairs")
         return pi, pj
+
+    def freeze_out_surface(self):
+        """The freeze-out hypersurface so far: one row per frozen-out particle,
+        ordered by (step, particle id); columns: id, step, then the 40 columns
+        of freeze_out.dat (divEener, divE[2], gsub, u[2], swsub, bulk, shear
+        00/11/22/33/12, tau, r[2], s, e, T, muB, muS, muQ, w, cs2, rhoB, rhoS,
+        rhoQ, diffusion[12], eos flag)."""
+        n = C.c_size_t()
+        _ok(lib().ccake_freeze_out_surface(self.h, None, 0, C.byref(n)), "freeze_out_surface")
+        rows = np.zeros((n.value, FO_ROW))
+        _ok(lib().ccake_freeze_out_surface(self.h, rows.ctypes.data_as(_dp), n.value, C.byref(n)),
+            "freeze_out_surface")
+        return rows[: n.value]
 
     def pair_total(self, reset=False):
         v = C.c_int64()

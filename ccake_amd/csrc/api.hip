@@ -3,6 +3,7 @@
 #include "ccake_internal.h"
 #include "eos_device.h"
 #include "pair_common.h"
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -137,6 +138,24 @@ static int time_derivatives(ccake_event* ev, int fuse_rk, double dt_stage, doubl
   return 0;
 }
 
+// SPHWorkstation::freeze_out_particles (src/sph_workstation.cpp:1475-1485):
+// status check and bsqsvfreezeout with its snapshot times
+// (include/freeze_out.h:227-281, 549-636): taupp is set at the first step,
+// taup at the second, and from the third on they move by one step after the
+// interpolation.
+static int freeze_step(ccake_event* ev)
+{
+  const int phase = ev->frzc;
+  ev->fo_step++;
+  if (phase == 0) ev->taupp = ev->t;
+  else if (phase == 1) ev->taup = ev->t;
+  const int rc = launch_freeze(ev, phase);
+  if (rc) return rc;
+  if (phase >= 2) { ev->taupp = ev->taup; ev->taup = ev->t; }
+  if (ev->frzc < 2) ev->frzc++;
+  return 0;
+}
+
 static int rk2_step(ccake_event* ev, double dt)
 {
   int rc;
@@ -156,9 +175,8 @@ static int rk2_step(ccake_event* ev, double dt)
   ev->t = t0 + dt;
   if (ev->p.particlization) {
     prof_begin(ev, CCAKE_K_MISC);
-    if ((rc = launch_freeze(ev, ev->frzc))) return rc;
+    if ((rc = freeze_step(ev))) return rc;
     prof_end(ev, CCAKE_K_MISC);
-    if (ev->frzc < 2) ev->frzc++;
   }
   return 0;
 }
@@ -411,12 +429,21 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
   rc |= dalloc(ev, &ev->cand_g, 16 * np);
   rc |= dalloc(ev, &ev->red, 1024);
   rc |= dalloc(ev, &ev->err, 1);
+  if (p->particlization) {
+    rc |= dalloc(ev, &ev->snaps, (size_t)6 * 33 * np);
+    rc |= dalloc(ev, &ev->fo_rows, (size_t)n * CCAKE_FO_ROW);
+    rc |= dalloc(ev, &ev->fo_count, 1);
+  }
   if (rc) { ccake_event_destroy(ev); return CCAKE_ERR_OOM; }
   hipStream_t st = ev->stream;
   CCAKE_HIP(hipMemsetAsync(ev->F, 0, (size_t)NUM_COLUMNS * np * sizeof(double), st));
   CCAKE_HIP(hipMemsetAsync(ev->I, 0, (size_t)CCAKE_NUM_IFIELDS * np * sizeof(int32_t), st));
   CCAKE_HIP(hipMemsetAsync(ev->err, 0, sizeof(int32_t), st));
   CCAKE_HIP(hipMemsetAsync(ev->pair_count, 0, 2 * sizeof(int64_t), st));
+  if (ev->snaps) {
+    CCAKE_HIP(hipMemsetAsync(ev->snaps, 0, (size_t)6 * 33 * np * sizeof(double), st));
+    CCAKE_HIP(hipMemsetAsync(ev->fo_count, 0, sizeof(int), st));
+  }
   // sph_mass = 1 and gamma = 1 until the caller overrides them
   std::vector<double> ones(n, 1.0);
   for (int f : { (int)CCAKE_F_MASS_S, (int)CCAKE_F_MASS_B, (int)CCAKE_F_MASS_SS, (int)CCAKE_F_MASS_Q, (int)CCAKE_F_GAMMA })
@@ -438,7 +465,7 @@ int ccake_event_destroy(ccake_event* ev)
   if (ev->stream) (void)hipStreamSynchronize(ev->stream);
   void* bufs[] = { ev->F, ev->I, ev->grid, ev->bbox, ev->key, ev->key_a, ev->key_b, ev->val_a, ev->val_b, ev->hist,
                    ev->scan_tmp, ev->cell_off, ev->pair_count, ev->part, ev->cand_s, ev->cand_g, ev->red,
-                   ev->err };
+                   ev->err, ev->snaps, ev->fo_rows, ev->fo_count };
   for (void* b : bufs) if (b) (void)hipFree(b);
   if (ev->stream) (void)hipStreamDestroy(ev->stream);
   {
@@ -597,6 +624,12 @@ int ccake_initialize(ccake_event* ev)
   int rc;
   ev->t = ev->p.t0;
   ev->frzc = 0;
+  ev->fo_step = 0;
+  ev->taup = ev->taupp = 0.0;
+  if (ev->snaps) {                                          // a new surface
+    CCAKE_HIP(hipMemsetAsync(ev->snaps, 0, (size_t)6 * 33 * ev->npad * sizeof(double), ev->stream));
+    CCAKE_HIP(hipMemsetAsync(ev->fo_count, 0, sizeof(int), ev->stream));
+  }
   if ((rc = launch_init_particles(ev, 0))) return rc;       // process_initial_conditions
   if ((rc = ccake_build_neighbors(ev))) return rc;          // initialize_linklist
   if ((rc = launch_eos(ev, 1, 0))) return rc;               // e -> s inversion
@@ -615,10 +648,7 @@ int ccake_initialize(ccake_event* ev)
 int ccake_freeze_out_particles(ccake_event* ev)
 {
   if (!ev) return CCAKE_ERR_INVALID;
-  int rc = launch_freeze(ev, ev->frzc);
-  if (rc) return rc;
-  if (ev->frzc < 2) ev->frzc++;
-  return 0;
+  return freeze_step(ev);
 }
 
 int ccake_continue_evolution(ccake_event* ev, double max_tau, int* keep_going)
@@ -653,6 +683,36 @@ int ccake_reduce(ccake_event* ev, int which, double* out)
 {
   if (!ev || !out || which < 0 || which >= CCAKE_NUM_REDUCTIONS) return CCAKE_ERR_INVALID;
   return launch_reduce(ev, which, out);
+}
+
+int ccake_freeze_out_surface(ccake_event* ev, double* rows, size_t cap_rows, size_t* n_rows)
+{
+  if (!ev || !n_rows) return CCAKE_ERR_INVALID;
+  if (!ev->fo_rows) { set_error("freeze-out surface needs particlization at event creation"); return CCAKE_ERR_INVALID; }
+  int cnt = 0;
+  CCAKE_HIP(hipMemcpyAsync(&cnt, ev->fo_count, sizeof(int), hipMemcpyDeviceToHost, ev->stream));
+  CCAKE_HIP(hipStreamSynchronize(ev->stream));
+  const size_t n = (size_t)(cnt < ev->n ? cnt : ev->n);    // each particle appends at most once
+  *n_rows = n;
+  if (!rows) return 0;
+  if (cap_rows < n) { set_error("row buffer too small"); return CCAKE_ERR_CAPACITY; }
+  std::vector<double> tmp(n * CCAKE_FO_ROW);
+  if (n) {
+    CCAKE_HIP(hipMemcpyAsync(tmp.data(), ev->fo_rows, n * CCAKE_FO_ROW * sizeof(double), hipMemcpyDeviceToHost,
+                             ev->stream));
+    CCAKE_HIP(hipStreamSynchronize(ev->stream));
+  }
+  // the device appends in arrival order; order by (step, particle id)
+  std::vector<size_t> idx(n);
+  for (size_t k = 0; k < n; ++k) idx[k] = k;
+  std::sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+    const double* ra = &tmp[a * CCAKE_FO_ROW];
+    const double* rb = &tmp[b * CCAKE_FO_ROW];
+    return ra[1] != rb[1] ? ra[1] < rb[1] : ra[0] < rb[0];
+  });
+  for (size_t k = 0; k < n; ++k)
+    std::memcpy(rows + k * CCAKE_FO_ROW, &tmp[idx[k] * CCAKE_FO_ROW], CCAKE_FO_ROW * sizeof(double));
+  return 0;
 }
 
 int ccake_profile_enable(ccake_dev* d, int enable)

@@ -120,6 +120,13 @@ struct ccake_event {
   bool sort_valid = false;
   double sort_disp = 0.0;
   double skin_frac = 0.04;      // re-sort when the bound would exceed skin_frac * rc (measured best at 40k)
+  // freeze-out hypersurface (particlization on): 6 snapshot sets x 33 columns
+  // x npad, the appended rows (n x CCAKE_FO_ROW) and their count
+  double* snaps = nullptr;
+  double* fo_rows = nullptr;
+  int* fo_count = nullptr;
+  double taup = 0.0, taupp = 0.0;   // times of the last two freeze-out snapshots
+  int fo_step = 0;                  // steps since initialize
   // pair-kernel partial sums: nslices x (up to 17) x npad, sorted order
   int nslices = 1;
   double* part = nullptr;

@@ -266,6 +266,24 @@ enum ccake_reduction { CCAKE_R_ENTROPY = 0, CCAKE_R_BARYON, CCAKE_R_STRANGE, CCA
                        CCAKE_R_ENERGY, CCAKE_R_EZ, CCAKE_R_FROZEN, CCAKE_NUM_REDUCTIONS };
 CCAKE_API int  ccake_reduce(ccake_event* ev, int which, double* out);
 
+/* ------------------------ freeze-out surface ------------------------ */
+/* The freeze-out hypersurface accumulated since ccake_initialize, for events
+ * created with particlization on: one row of CCAKE_FO_ROW doubles for every
+ * particle that froze out through the interpolation (status 3 -> 4): the
+ * particle id, the step number (1 = the first step after initialize), then
+ * the 40 columns of freeze_out.dat for D = 2 in the order of
+ * Output<2, EoM_default>::print_freeze_out (src/output.cpp:502-560):
+ * divEener, divE[2], gsub, uout[2], swsub, bulksub, shear 00, 11, 22, 33, 12,
+ * tlist, rsub[2], sFO, E, T (GeV), muB, muS, muQ, w, cs2, rhoB, rhoS, rhoQ,
+ * the 12 diffusion components (zero) and the table-EoS flag.  Rows are
+ * ordered by step, then particle id, the order the reference writes them.
+ * Replaces FreezeOut::bsqsvinterpolate's results with
+ * Output::print_freeze_out / flush_freeze_out (include/freeze_out.h:638-930,
+ * src/output.cpp:412-498).  rows == NULL or cap_rows too small: *n_rows is
+ * set and CCAKE_ERR_CAPACITY returned (rows NULL: 0 returned). */
+#define CCAKE_FO_ROW 42
+CCAKE_API int  ccake_freeze_out_surface(ccake_event* ev, double* rows, size_t cap_rows, size_t* n_rows);
+
 /* --------------------------- profiling ----------------------------- */
 /* Per-kernel HIP-event timing on the events' streams (off by default).
  * SMOOTH and GRAD time the pair kernels alone; SMOOTH_SUM and GRAD_SUM the

@@ -72,6 +72,7 @@ def _parse_enum(header, prefix):
 
 FIELDS = _parse_enum(os.path.join(_HERE, "ccake_oracle.h"), "orc_field")
 IFIELDS = _parse_enum(os.path.join(_HERE, "ccake_oracle.h"), "orc_ifield")
+FO_ROW = 42       # hypersurface row: id, step, then the 40 freeze_out.dat columns
 
 
 def build(force=False):
@@ -122,6 +123,7 @@ def lib():
             "orc_advance_timestep": (C.c_int, [vp, C.c_double, C.c_int]),
             "orc_initialize": (C.c_int, [vp]),
             "orc_freeze_out_particles": (None, [vp]),
+            "orc_freeze_out_surface": (C.c_int, [vp, dp, C.c_int]),
             "orc_continue_evolution": (C.c_int, [vp, C.c_double]),
             "orc_reduce": (C.c_double, [vp, C.c_int]),
             "orc_set_threads": (None, [C.c_int]),
@@ -226,6 +228,13 @@ class Event:
                                     pj.ctypes.data_as(C.POINTER(C.c_int32)), n)
         assert st == 0
         return pi, pj
+
+    def freeze_out_surface(self):
+        """Hypersurface rows (id, step, 40 freeze_out.dat columns)."""
+        k = lib().orc_freeze_out_surface(self.h, None, 0)
+        rows = np.zeros((k, FO_ROW))
+        lib().orc_freeze_out_surface(self.h, rows.ctypes.data_as(C.POINTER(C.c_double)), k)
+        return rows
 
     def __getattr__(self, name):
         fn = getattr(lib(), "orc_" + name)

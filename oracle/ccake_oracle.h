@@ -150,6 +150,13 @@ int      orc_rk_update(orc_event* ev, double dt, double t0);
 int      orc_advance_timestep(orc_event* ev, double dt, int rk_order);
 int      orc_initialize(orc_event* ev);
 void     orc_freeze_out_particles(orc_event* ev);
+/* Freeze-out hypersurface rows accumulated since orc_initialize (events with
+ * particlization): ORC_FO_ROW doubles each -- particle id, step number, then
+ * the 40 columns of Output<2,EoM_default>::print_freeze_out
+ * (src/output.cpp:502-560) -- in the order the reference writes them (step,
+ * then particle index).  Returns the row count; copies min(count, cap). */
+#define ORC_FO_ROW 42
+int      orc_freeze_out_surface(orc_event* ev, double* rows, int cap);
 int      orc_continue_evolution(orc_event* ev, double max_tau);
 double   orc_reduce(orc_event* ev, int which);
 /* thread count used by the OpenMP loops (0 = runtime default) */

@@ -4,6 +4,7 @@
  * status and reductions (test infrastructure only; see ccake_oracle.h).
  */
 #include "orc_internal.h"
+#include <string.h>
 
 static int g_threads = 0;
 void orc_set_threads(int n) { g_threads = n; }
@@ -53,6 +54,75 @@ static double distance(int D, const double* a, const double* b)
 }
 
 /* ------------------------------------------------------------------ */
+/* freeze_array storage (include/freeze_out.h:29-35), one member per array */
+#define SNAP_MEMBERS(X) \
+  X(shear[0][0]) X(shear[0][1]) X(shear[0][2]) X(shear[1][0]) X(shear[1][1]) X(shear[1][2]) \
+  X(shear[2][0]) X(shear[2][1]) X(shear[2][2]) X(r[0]) X(r[1]) X(u[0]) X(u[1]) \
+  X(gradP[0]) X(gradP[1]) X(gradE[0]) X(gradE[1]) X(s) X(e) X(rhoB) X(rhoS) X(rhoQ) \
+  X(T) X(muB) X(muS) X(muQ) X(theta) X(bulk) X(sigma_lab) X(shear33) X(shv_nabla_u) X(wfz) X(cs2fz)
+
+static orc_snap* snap_alloc(int n)
+{
+  orc_snap* f = (orc_snap*)calloc(1, sizeof(orc_snap));
+#define ALLOC_(m) f->m = (double*)calloc((size_t)n + 1, sizeof(double));
+  SNAP_MEMBERS(ALLOC_)
+#undef ALLOC_
+  return f;
+}
+
+static void snap_free(orc_snap* f)
+{
+  if (!f) return;
+#define FREE_(m) free(f->m);
+  SNAP_MEMBERS(FREE_)
+#undef FREE_
+  free(f);
+}
+
+static void snap_zero(orc_snap* f, int n)
+{
+#define ZERO_(m) memset(f->m, 0, (size_t)n * sizeof(double));
+  SNAP_MEMBERS(ZERO_)
+#undef ZERO_
+}
+
+/* FreezeOut::shift_snapshot (include/freeze_out.h:383-419): copy the masked
+ * particles of src into dst. */
+static void snap_shift(orc_snap* dst, const orc_snap* src, const unsigned char* mask, int n)
+{
+  for (int i = 0; i < n; ++i) {
+    if (!mask[i]) continue;
+#define SHIFT_(m) dst->m[i] = src->m[i];
+    SNAP_MEMBERS(SHIFT_)
+#undef SHIFT_
+  }
+}
+
+/* The particle's current state into a snapshot (freezeout_first_time_step,
+ * freezeout_second_time_step, setup_next_step; include/freeze_out.h:227-318,
+ * 594-631), D = 2. */
+static void snap_take(orc_snap* f, orc_event* ev)
+{
+  for (int i = 0; i < ev->n; ++i) {
+    for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) f->shear[a][b][i] = FLD(ev, ORC_F_SHV0 + 4*a + b)[i];
+    for (int d = 0; d < 2; ++d) {
+      f->r[d][i] = FLD(ev, ORC_F_X + d)[i];
+      f->u[d][i] = FLD(ev, ORC_F_UX + d)[i];
+      f->gradP[d][i] = FLD(ev, ORC_F_GRADP0 + d)[i];
+      f->gradE[d][i] = FLD(ev, ORC_F_GRADE0 + d)[i];
+    }
+    f->s[i] = FLD(ev, ORC_F_S)[i];       f->e[i] = FLD(ev, ORC_F_E)[i];
+    f->rhoB[i] = FLD(ev, ORC_F_RHOB)[i]; f->rhoS[i] = FLD(ev, ORC_F_RHOS)[i]; f->rhoQ[i] = FLD(ev, ORC_F_RHOQ)[i];
+    f->T[i] = FLD(ev, ORC_F_T)[i];       f->muB[i] = FLD(ev, ORC_F_MUB)[i];
+    f->muS[i] = FLD(ev, ORC_F_MUS)[i];   f->muQ[i] = FLD(ev, ORC_F_MUQ)[i];
+    f->theta[i] = FLD(ev, ORC_F_THETA)[i]; f->bulk[i] = FLD(ev, ORC_F_BULK)[i];
+    f->sigma_lab[i] = FLD(ev, ORC_F_SIGMA_LAB)[i];
+    f->shear33[i] = FLD(ev, ORC_F_SHV0 + 15)[i];
+    f->shv_nabla_u[i] = FLD(ev, ORC_F_SHV_NABLA_U)[i];
+    f->wfz[i] = FLD(ev, ORC_F_W)[i];     f->cs2fz[i] = FLD(ev, ORC_F_CS2)[i];
+  }
+}
+
 orc_event* orc_event_create(const orc_params* p, orc_eos* eos, int n)
 {
   orc_event* ev = (orc_event*)calloc(1, sizeof(orc_event));
@@ -69,6 +139,10 @@ orc_event* orc_event_create(const orc_params* p, orc_eos* eos, int n)
   for (int k = 0; k < 4; ++k) { ev->fback_e[k] = calloc((size_t)n+1, 8); ev->fback_gradP0[k] = calloc((size_t)n+1, 8); }
   ev->frz1_gradP0 = calloc((size_t)n+1, 8); ev->frz2_gradP0 = calloc((size_t)n+1, 8);
   ev->nb_off = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
+  if (p->particlization) {
+    for (int k = 0; k < 6; ++k) ev->snap[k] = snap_alloc(n);
+    ev->fo_rows = (double*)calloc((size_t)n * ORC_FO_ROW + 1, sizeof(double));
+  }
   for (int i = 0; i < n; ++i) {
     FLD(ev, ORC_F_MASS_S)[i] = FLD(ev, ORC_F_MASS_B)[i] = 1.0;
     FLD(ev, ORC_F_MASS_SS)[i] = FLD(ev, ORC_F_MASS_Q)[i] = 1.0;
@@ -88,6 +162,8 @@ void orc_event_destroy(orc_event* ev)
   for (int k = 0; k < 4; ++k) { free(ev->fback_e[k]); free(ev->fback_gradP0[k]); }
   free(ev->frz1_gradP0); free(ev->frz2_gradP0);
   free(ev->nb_off); free(ev->nb_idx);
+  for (int k = 0; k < 6; ++k) snap_free(ev->snap[k]);
+  free(ev->fo_rows);
   free(ev);
 }
 
@@ -508,25 +584,132 @@ static void snap(double* de, double* dg, const double* se, const double* sg, con
   for (int i = 0; i < n; ++i) if (mask[i]) { de[i] = se[i]; dg[i] = sg[i]; }
 }
 
+/* FreezeOut::bsqsvinterpolate (include/freeze_out.h:638-930) for the
+ * particles at status 3, D = 2, hyperbolic coordinates: one row each,
+ * appended in particle order with the column order of
+ * Output<2,EoM_default>::print_freeze_out (src/output.cpp:502-560). */
+static void interpolate(orc_event* ev)
+{
+  const double efc = ev->p.e_freeze;
+  const double hbarc_GeVfm = 0.1973269804;            /* include/constants.h:15 */
+  int32_t* fz = ev->I[ORC_I_FREEZE];
+  int32_t* bt = ev->I[ORC_I_BTRACK];
+  const orc_snap* f1 = ev->snap[0];
+  const orc_snap* f2 = ev->snap[1];
+  for (int i = 0; i < ev->n; ++i) {
+    if (fz[i] != 3) continue;
+    /* the snapshot closer to the freeze-out energy */
+    const int swit = (fabs(f1->e[i] - efc) < fabs(f2->e[i] - efc)) ? 1 : 2;
+    const orc_snap* f = (swit == 1) ? f1 : f2;
+    const double tau_s = (swit == 1) ? ev->taup : ev->taupp;
+    const double tlist = (bt[i] != -1) ? tau_s : tau_s - ev->p.dt;
+    double uout[2], gradPsub[2], gradEsub[2], uout_cov[2], gradPsub_contra[2], gradEsub_contra[2];
+    for (int d = 0; d < 2; ++d) {
+      uout[d] = f->u[d][i]; gradPsub[d] = f->gradP[d][i]; gradEsub[d] = f->gradE[d][i];
+      /* milne::Vector<double,2>::make_covariant / make_contravariant
+       * (include/milne.hpp:353-364): a sign flip in 2+1D */
+      uout_cov[d] = -f->u[d][i];
+      gradPsub_contra[d] = f->gradP[d][i] / -1.0;
+      gradEsub_contra[d] = f->gradE[d][i] / -1.0;
+    }
+    double norm2 = 0.0;
+    for (int d = 0; d < 2; ++d) norm2 += uout[d]*uout_cov[d];
+    const double gsub = sqrt(-norm2 + 1);
+    double sigsub = f->sigma_lab[i];
+    sigsub /= gsub*tlist;                                  /* "hyperbolic" */
+    const double swsub = FLD(ev, ORC_F_MASS_S)[i]/sigsub;
+    double sFO = f->s[i];
+    double divT[2], divP[2], divE[2], divT_contra[2];
+    for (int d = 0; d < 2; ++d) {
+      divT[d] = (1.0/sFO)*gradPsub[d];
+      divP[d] = gradPsub[d];
+      divE[d] = gradEsub[d];
+      divT_contra[d] = (1.0/sFO)*gradPsub_contra[d];
+    }
+    (void)divP;
+    const double cs2 = f->cs2fz[i], w = f->wfz[i], bulk = f->bulk[i], theta = f->theta[i];
+    const double shv_nabla_u = f->shv_nabla_u[i];
+    double inner = 0.0;
+    for (int d = 0; d < 2; ++d) inner += uout[d]*gradPsub[d];
+    const double divPpress = -(1.0/gsub)*(cs2*(w + bulk)*theta - cs2*shv_nabla_u + inner);   /* arXiv:1406.3333 (C3) */
+    double divTtemp = divPpress/sFO;
+    inner = 0.0;
+    for (int d = 0; d < 2; ++d) inner += uout[d]*gradEsub[d];
+    double divEener = -(1.0/gsub)*((w + bulk)*theta - shv_nabla_u + inner);                /* (C2) */
+    norm2 = 0.0;
+    for (int d = 0; d < 2; ++d) norm2 += divT[d]*divT_contra[d];
+    const double insub = divTtemp*divTtemp + norm2;
+    double norm = -sqrt(fabs(insub));
+    norm = fabs(norm) > 1.E-14 ? norm : 1.E-14;
+    divTtemp /= norm;
+    for (int d = 0; d < 2; ++d) divT[d] /= norm;
+    norm2 = 0.0;
+    for (int d = 0; d < 2; ++d) norm2 += divE[d]*gradEsub_contra[d];
+    const double insubE = divEener*divEener + norm2;
+    double normE = -sqrt(fabs(insubE));
+    normE = fabs(normE) > 1.E-14 ? normE : 1.E-14;
+    divEener /= normE;
+    for (int d = 0; d < 2; ++d) divE[d] /= normE;
+    double Tfluc = f->T[i];
+    sFO *= pow(Tfluc*hbarc_GeVfm, 3);
+    Tfluc *= hbarc_GeVfm;
+    double* row = ev->fo_rows + (size_t)ev->fo_count*ORC_FO_ROW;
+    double* c = row + 2;
+    row[0] = i; row[1] = ev->fo_step;
+    c[0] = divEener; c[1] = divE[0]; c[2] = divE[1];
+    c[3] = gsub; c[4] = uout[0]; c[5] = uout[1];
+    c[6] = swsub; c[7] = bulk;
+    c[8] = f->shear[0][0][i]; c[9] = f->shear[1][1][i]; c[10] = f->shear[2][2][i];
+    c[11] = f->shear33[i]; c[12] = f->shear[1][2][i];
+    c[13] = tlist; c[14] = f->r[0][i]; c[15] = f->r[1][i];
+    c[16] = sFO; c[17] = f->e[i]; c[18] = Tfluc;
+    c[19] = f->muB[i]; c[20] = f->muS[i]; c[21] = f->muQ[i];
+    c[22] = w; c[23] = cs2;
+    c[24] = f->rhoB[i]; c[25] = f->rhoS[i]; c[26] = f->rhoQ[i];
+    for (int q = 27; q < 39; ++q) c[q] = 0.0;            /* diffout: diffusion off */
+    c[39] = FLD(ev, ORC_F_EOS_IS_TABLE)[i];
+    ev->fo_count++;
+    fz[i] = 4;
+  }
+}
+
+int orc_freeze_out_surface(orc_event* ev, double* rows, int cap)
+{
+  if (!ev->fo_rows) return 0;
+  const int k = ev->fo_count < cap ? ev->fo_count : cap;
+  if (rows && k > 0) memcpy(rows, ev->fo_rows, (size_t)k*ORC_FO_ROW*sizeof(double));
+  return ev->fo_count;
+}
+
 void orc_freeze_out_particles(orc_event* ev)
 {
   const int n = ev->n;
   const int curfrz = check_status(ev);
+  const int full = ev->fo_rows != NULL;
+  orc_snap** S = ev->snap;        /* 0 frz1, 1 frz2, 2 fback, 3 fback2, 4 fback3, 5 fback4 */
   int32_t* fz = ev->I[ORC_I_FREEZE];
   int32_t* bt = ev->I[ORC_I_BTRACK];
   double* e1 = FLD(ev, ORC_F_FRZ1_E); double* e2 = FLD(ev, ORC_F_FRZ2_E);
   double* g1 = ev->frz1_gradP0; double* g2 = ev->frz2_gradP0;
   double** fe = ev->fback_e; double** fg = ev->fback_gradP0;
-  if (ev->frzc == 0) {
+  ev->fo_step++;
+  if (ev->frzc == 0) {                                  /* freezeout_first_time_step */
+    ev->taupp = ev->t;
     for (int i = 0; i < n; ++i) { e2[i] = FLD(ev, ORC_F_E)[i]; g2[i] = FLD(ev, ORC_F_GRADP0)[i]; }
+    if (full) snap_take(S[1], ev);
     ev->frzc = 1;
     return;
   }
   unsigned char* m = (unsigned char*)calloc((size_t)n, 1);
-  if (ev->frzc == 1) {
+  if (ev->frzc == 1) {                                  /* freezeout_second_time_step */
+    ev->taup = ev->t;
     for (int i = 0; i < n; ++i) { e1[i] = FLD(ev, ORC_F_E)[i]; g1[i] = FLD(ev, ORC_F_GRADP0)[i]; }
+    if (full) snap_take(S[0], ev);
     ev->frzc = 2;
-    if (curfrz > 0) for (int i = 0; i < n; ++i) if (fz[i] == 3) fz[i] = 4;
+    if (curfrz > 0) {
+      if (full) interpolate(ev);
+      else for (int i = 0; i < n; ++i) if (fz[i] == 3) fz[i] = 4;
+    }
     free(m);
     return;
   }
@@ -536,17 +719,29 @@ void orc_freeze_out_particles(orc_event* ev)
   snap(fe[2], fg[2], fe[0], fg[0], m, n);     /* fback3 <- fback  */
   snap(fe[1], fg[1], e2, g2, m, n);           /* fback2 <- frz2   */
   snap(fe[0], fg[0], e1, g1, m, n);           /* fback  <- frz1   */
+  if (full) { snap_shift(S[5], S[3], m, n); snap_shift(S[4], S[2], m, n);
+              snap_shift(S[3], S[1], m, n); snap_shift(S[2], S[0], m, n); }
   for (int i = 0; i < n; ++i) m[i] = (fz[i] < 4 && bt[i] == 0 && fg[0][i] != 0);
   snap(e2, g2, fe[1], fg[1], m, n);
   snap(e1, g1, fe[0], fg[0], m, n);
+  if (full) { snap_shift(S[1], S[3], m, n); snap_shift(S[0], S[2], m, n); }
   for (int i = 0; i < n; ++i) m[i] = (fz[i] < 4 && bt[i] == 0 && fg[0][i] == 0);
   snap(e2, g2, fe[3], fg[3], m, n);
   snap(e1, g1, fe[2], fg[2], m, n);
+  if (full) { snap_shift(S[1], S[5], m, n); snap_shift(S[0], S[4], m, n); }
   for (int i = 0; i < n; ++i) if (fz[i] < 4 && bt[i] == 0) { fz[i] = 4; bt[i] = -1; }
-  if (curfrz > 0) for (int i = 0; i < n; ++i) if (fz[i] == 3) fz[i] = 4;
+  const double tau = ev->t;
+  if (curfrz > 0) {
+    if (full) interpolate(ev);
+    else for (int i = 0; i < n; ++i) if (fz[i] == 3) fz[i] = 4;
+  }
   for (int i = 0; i < n; ++i) m[i] = (fz[i] < 4);
   snap(e2, g2, e1, g1, m, n);
+  if (full) snap_shift(S[1], S[0], m, n);
   for (int i = 0; i < n; ++i) { e1[i] = FLD(ev, ORC_F_E)[i]; g1[i] = FLD(ev, ORC_F_GRADP0)[i]; }
+  if (full) snap_take(S[0], ev);
+  ev->taupp = ev->taup;
+  ev->taup = tau;
   free(m);
 }
 
@@ -625,6 +820,8 @@ int orc_initialize(orc_event* ev)
     if (FLD(ev, ORC_F_IN_S)[i] < 0.0) ev->I[ORC_I_FREEZE][i] = 4;
   }
   ev->frzc = 0;                                          /* setup_freeze_out */
+  ev->fo_step = 0; ev->fo_count = 0; ev->taup = ev->taupp = 0.0;
+  for (int k = 0; k < 6; ++k) if (ev->snap[k]) snap_zero(ev->snap[k], n);
   orc_gamma_velocity(ev);                               /* calculate_gamma_and_velocities */
   /* initial_smoothing (394-457) */
   orc_gamma_velocity(ev);

@@ -64,6 +64,12 @@ struct orc_event {
   int32_t* nb_idx;
   int64_t nb_cap;
   int64_t n_pairs;
+  /* freeze-out hypersurface (include/freeze_out.h:29-35, 165-222): the six
+   * snapshot sets and the appended rows (particlization on) */
+  struct orc_snap* snap[6];      /* frz1, frz2, fback, fback2, fback3, fback4 */
+  double* fo_rows;               /* n x ORC_FO_ROW */
+  int fo_count, fo_step;
+  double taup, taupp;
   /* freeze-out bookkeeping (include/freeze_out.h:165-222) */
   int frzc;
   double* fback_e[4];    /* fback, fback2, fback3, fback4 energy snapshots */
@@ -73,5 +79,16 @@ struct orc_event {
 };
 
 #define FLD(ev, f) ((ev)->F[(f)])
+
+/* One freeze_array for D = 2 (include/freeze_out.h:29-35): the members the
+ * hypersurface reads; diffusion is off and stays zero. */
+typedef struct orc_snap {
+  double* shear[3][3];
+  double* r[2]; double* u[2]; double* gradP[2]; double* gradE[2];
+  double* s; double* e; double* rhoB; double* rhoS; double* rhoQ;
+  double* T; double* muB; double* muS; double* muQ;
+  double* theta; double* bulk; double* sigma_lab; double* shear33; double* shv_nabla_u;
+  double* wfz; double* cs2fz;
+} orc_snap;
 
 #endif
