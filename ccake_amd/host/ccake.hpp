@@ -17,10 +17,13 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
 #include <functional>
 #include <memory>
+#include <sstream>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "ccake_amd.h"
@@ -330,12 +333,86 @@ class Evolver {
   std::shared_ptr<SystemState<D>> systemPtr;
 };
 
+// FreezeOut<D> (include/freeze_out.h:136-930): the snapshots, the status
+// machine and the interpolation run on the device inside every step; this
+// handle reads the hypersurface they have produced.
+template <unsigned int D>
+class FreezeOut {
+ public:
+  explicit FreezeOut(std::shared_ptr<SystemState<D>> sys) : systemPtr(std::move(sys)) {}
+  // Rows of CCAKE_FO_ROW doubles (particle id, step, then the columns of
+  // freeze_out.dat), ordered by (step, particle id).
+  std::vector<double> surface(size_t* n_rows) const
+  {
+    ccake_event* ev = systemPtr->event();
+    size_t n = 0;
+    check(ccake_freeze_out_surface(ev, nullptr, 0, &n), "ccake_freeze_out_surface", &systemPtr->settings());
+    std::vector<double> rows(n * CCAKE_FO_ROW);
+    if (n) check(ccake_freeze_out_surface(ev, rows.data(), n, &n), "ccake_freeze_out_surface", &systemPtr->settings());
+    *n_rows = n;
+    return rows;
+  }
+
+ private:
+  std::shared_ptr<SystemState<D>> systemPtr;
+};
+
+// Output<D, TEOM>'s freeze-out writer (src/output.cpp:412-498): rows are
+// buffered as text at every print_freeze_out and written to
+// output_directory/freeze_out.dat by flush_freeze_out, after a header line
+// "# B S Q" with the totals passed at the first call.  Numbers use the
+// default stream format (6 significant digits), columns separated by one
+// space, as the reference writes them.
+template <unsigned int D, template <unsigned int> class TEOM>
+class Output {
+ public:
+  explicit Output(std::string dir) : output_directory(std::move(dir)) {}
+
+  void print_freeze_out(std::shared_ptr<FreezeOut<D>> freeze_out, double B, double Q, double S)
+  {
+    if (fo_header.empty()) {
+      std::ostringstream hdr;
+      hdr << "# " << B << " " << S << " " << Q << "\n";
+      fo_header = hdr.str();
+    }
+    size_t n = 0;
+    const std::vector<double> rows = freeze_out->surface(&n);
+    int count = 0;
+    for (size_t k = rows_buffered; k < n; ++k) {      // rows of the steps since the last call
+      const double* c = &rows[k * CCAKE_FO_ROW + 2];
+      for (int q = 0; q < CCAKE_FO_ROW - 3; ++q) fo_buffer << c[q] << " ";
+      fo_buffer << c[CCAKE_FO_ROW - 3] << "\n";
+      ++count;
+    }
+    rows_buffered = n;
+    fo_particles_buffered += count;
+  }
+
+  void flush_freeze_out()
+  {
+    const std::string name = output_directory + "/freeze_out.dat";
+    std::ofstream FO(name, std::ios::out | std::ios::trunc);
+    if (!FO) throw std::runtime_error("Could not open " + name);
+    FO << fo_header;
+    FO << fo_buffer.str();
+  }
+
+  int particles_buffered() const { return fo_particles_buffered; }
+  std::string output_directory;
+
+ private:
+  std::string fo_header;
+  std::ostringstream fo_buffer;
+  size_t rows_buffered = 0;
+  int fo_particles_buffered = 0;
+};
+
 // SPHWorkstation<D, TEOM> (src/sph_workstation.cpp).
 template <unsigned int D, template <unsigned int> class TEOM>
 class SPHWorkstation {
  public:
   SPHWorkstation(std::shared_ptr<Settings> s, std::shared_ptr<SystemState<D>> sys)
-    : settingsPtr(s), systemPtr(sys), evolver(s, sys) {}
+    : evolver(s, sys), freezePtr(std::make_shared<FreezeOut<D>>(sys)), settingsPtr(s), systemPtr(sys) {}
 
   // SPHWorkstation::initialize (src/sph_workstation.cpp:29-110): e_freeze from
   // the EoS at T_FO, then the device event.
@@ -408,6 +485,7 @@ class SPHWorkstation {
   }
 
   Evolver<D> evolver;
+  std::shared_ptr<FreezeOut<D>> freezePtr;
 
  private:
   ccake_event* ev() const { return systemPtr->event(); }

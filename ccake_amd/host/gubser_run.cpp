@@ -4,15 +4,20 @@
 // the energy density against the analytic solution, entropy and energy
 // conservation, and (with --evolver-steps K) the largest difference between
 // K steps taken by the fused device step and by Evolver::execute_timestep
-// with the workstation's get_time_derivatives as the functional.
+// with the workstation's get_time_derivatives as the functional.  With
+// --freeze-out-T (MeV) particlization is on, the hypersurface is buffered
+// after every step (Output::print_freeze_out, as BSQHydro::run does,
+// src/BSQHydro.cpp:797) and written to OUTPUT/freeze_out.dat at the end (859-860).
 //
-//   gubser_run [--spacing 0.05] [--steps 120] [--evolver-steps 0]
+//   gubser_run [--spacing 0.05] [--steps 120] [--dt 0.005] [--evolver-steps 0]
+//              [--freeze-out-T MeV --output DIR]
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <string>
 
 #include "ccake.hpp"
 
@@ -65,15 +70,21 @@ int main(int argc, char** argv)
 {
   double a = 0.05;
   int steps = 120, evolver_steps = 0;
+  double dt = 0.005, t_fo_mev = 0.0;
+  std::string out_dir = ".";
   for (int k = 1; k + 1 < argc; k += 2) {
     if (!std::strcmp(argv[k], "--spacing")) a = std::atof(argv[k + 1]);
     else if (!std::strcmp(argv[k], "--steps")) steps = std::atoi(argv[k + 1]);
+    else if (!std::strcmp(argv[k], "--dt")) dt = std::atof(argv[k + 1]);
     else if (!std::strcmp(argv[k], "--evolver-steps")) evolver_steps = std::atoi(argv[k + 1]);
+    else if (!std::strcmp(argv[k], "--freeze-out-T")) t_fo_mev = std::atof(argv[k + 1]);
+    else if (!std::strcmp(argv[k], "--output")) out_dir = argv[k + 1];
   }
   auto s = std::make_shared<ccake::Settings>();
   s->eos_type = "conformal";
-  s->t0 = 1.0; s->dt = 0.005; s->hT = 0.3;
-  s->particlization_enabled = false;
+  s->t0 = 1.0; s->dt = dt; s->hT = 0.3;
+  s->particlization_enabled = t_fo_mev > 0.0;
+  if (s->particlization_enabled) s->Freeze_Out_Temperature = t_fo_mev / 197.3269804;
   s->using_shear = false;
   s->eta_mode = CCAKE_ETA_CONSTANT; s->constant_eta_over_s = 0.0;
   s->zeta_mode = CCAKE_ZETA_CONSTANT; s->constant_zeta_over_s = 0.0;
@@ -83,7 +94,13 @@ int main(int argc, char** argv)
     ccake::SPHWorkstation<2, ccake::EoM_default> ws(s, sys);
     ws.initialize();
     ws.initialize_hydrodynamics();
-    for (int k = 0; k < steps; ++k) ws.advance_timestep(s->dt, s->rk_order);
+    ccake::Output<2, ccake::EoM_default> out(out_dir);
+    for (int k = 0; k < steps; ++k) {
+      ws.advance_timestep(s->dt, s->rk_order);
+      if (s->particlization_enabled)
+        out.print_freeze_out(ws.freezePtr, sys->Btotal0, sys->Qtotal0, sys->Stotal0);
+    }
+    if (s->particlization_enabled) out.flush_freeze_out();
     sys->conservation_entropy();
     sys->conservation_energy();
     sys->copy_device_to_host();
@@ -115,8 +132,10 @@ int main(int argc, char** argv)
       }
     }
     std::printf("{\"particles\": %d, \"steps\": %d, \"t\": %.15g, \"max_rel_err_e_r_lt_2.5\": %.6g, "
-                "\"entropy_rel_change\": %.6g, \"energy_rel_change\": %.6g, \"evolver_vs_fused_max_abs_diff\": %.6g}\n",
-                sys->n(), steps, sys->t, worst, sys->S / sys->S0 - 1.0, (sys->Etot) / (sys->E0) - 1.0, evolver_diff);
+                "\"entropy_rel_change\": %.6g, \"energy_rel_change\": %.6g, \"evolver_vs_fused_max_abs_diff\": %.6g, "
+                "\"freeze_out_rows\": %d}\n",
+                sys->n(), steps, sys->t, worst, sys->S / sys->S0 - 1.0, (sys->Etot) / (sys->E0) - 1.0, evolver_diff,
+                out.particles_buffered());
   } catch (const ccake::Error& e) {
     std::fprintf(stderr, "gubser_run: %s\n", e.what());
     return 2;

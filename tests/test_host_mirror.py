@@ -36,3 +36,43 @@ def test_gubser_through_cpp_call_surface():
     assert abs(d["entropy_rel_change"]) < 1e-14
     assert abs(d["energy_rel_change"]) < 0.01
     assert 0.0 <= d["evolver_vs_fused_max_abs_diff"] < 1e-12
+
+
+@pytest.mark.gpu
+def test_freeze_out_dat_through_cpp_call_surface(tmp_path, orc):
+    """Ideal Gubser flow (0.1 fm lattice, 10,201 particles) with
+    particlization at T_FO = 200 MeV, 20 steps of 0.02 fm/c through
+    SPHWorkstation::advance_timestep and Output::print_freeze_out after each
+    step: freeze_out.dat holds the "# B S Q" header and one line of 40
+    columns per frozen-out particle, equal to the oracle's hypersurface to
+    the 6 significant digits the default stream format keeps."""
+    import numpy as np
+    from ccake_amd import config, ic
+    t_fo = 200.0
+    r = subprocess.run([PROG, "--spacing", "0.1", "--steps", "20", "--dt", "0.02", "--freeze-out-T", str(t_fo),
+                        "--output", str(tmp_path)], capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    lines = open(tmp_path / "freeze_out.dat").read().splitlines()
+    assert lines[0].startswith("# ") and len(lines[0][2:].split()) == 3
+    assert [float(v) for v in lines[0][2:].split()] == [0.0, 0.0, 0.0]   # no charges in Gubser flow
+    got = np.array([[float(v) for v in ln.split()] for ln in lines[1:]])
+    assert got.shape[1] == 40 and len(got) == d["freeze_out_rows"]
+
+    T = t_fo / config.HBARC_MEVFM
+    oeos = orc.Eos(0, T)
+    cfg = config.gubser_ideal_params(particlization=1, e_freeze=oeos.efreeze(T), dt=0.02)
+    dd = ic.gubser_ideal(spacing=0.1)
+    o = orc.Event(cfg, oeos, len(dd["x"]))
+    for k, f in [("x", "X"), ("y", "Y"), ("e", "IN_E"), ("ux", "UX"), ("uy", "UY"),
+                 ("rhoB", "IN_B"), ("rhoS", "IN_SS"), ("rhoQ", "IN_Q")]:
+        o.set("ORC_F_" + f, dd[k])
+    assert o.initialize() == 0
+    for _ in range(20):
+        assert o.advance_timestep(cfg["dt"], 2) == 0
+    want = o.freeze_out_surface()[:, 2:]
+    print("rows", len(got), "oracle", len(want), flush=True)
+    assert got.shape == want.shape and len(got) > 300
+    colmax = np.max(np.abs(want), axis=0)
+    assert np.all(np.abs(got - want) <= 6e-6 * np.abs(want) + 1e-12 * colmax)
