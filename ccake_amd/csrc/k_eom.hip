@@ -22,10 +22,55 @@ struct Cols {
   __device__ __forceinline__ double& operator()(int c) const { return F[(size_t)c * npad + i]; }
 };
 
+// EoM_default::compute_hydro_numbers (src/eom_default.cpp:1831-1995) for
+// D = 2 without diffusion: shear and bulk Knudsen numbers and inverse
+// Reynolds numbers.  s11, s12 are sigma^{ij} at (i, j) = (1, 1), (1, 2) of
+// the MRF (M_sigma du/dt + F_sigma).  The reference stores sigma^{ij} at
+// (i+1, j+1) of its 4x4 and reads it back into (i, j); the time components
+// it then derives from u overwrite (0, 0..2), so only (1, 1) and (1, 2)
+// survive the read, and (2, 2) stays zero.  Both contractions lower with
+// Matrix<double,4,4>::make_covariant (include/milne.hpp:550-590): slots 1, 2
+// by -1, slot 3 by -t^2, and contract() runs over columns, then rows.
+__device__ __forceinline__ void hydro_numbers(const double S[4][4], double s11, double s12, double ux, double uy,
+                                              double gamma, double t2, double tau_pi, double tau_Pi,
+                                              double theta, double bulk, double pressure, double* hn)
+{
+  double sg[4][4];
+  #pragma unroll
+  for (int a = 0; a < 4; ++a)
+    #pragma unroll
+    for (int b = 0; b < 4; ++b) sg[a][b] = 0.0;
+  sg[1][1] = s11; sg[1][2] = s12; sg[2][1] = s12;
+  const double uc[2] = { -ux, -uy };
+  sg[0][1] = -1. * ((0.0 + uc[0] * sg[1][1]) + uc[1] * sg[1][2]) / gamma;
+  sg[0][2] = -1. * ((0.0 + uc[0] * sg[2][1]) + uc[1] * sg[2][2]) / gamma;
+  sg[1][0] = sg[0][1]; sg[2][0] = sg[0][2];
+  const double fc0 = -ux, fc1 = -uy, fc2 = 0.0 * -t2;
+  sg[0][0] = (fc0 * fc0 * sg[1][1] + fc1 * fc1 * sg[2][2] + 2. * fc1 * fc0 * sg[2][1] + 2. * fc2 * fc0 * sg[3][1]
+              + 2. * fc2 * fc1 * sg[3][2] - fc2 * fc2 * (sg[1][1] + sg[2][2]) / t2)
+           / (gamma * gamma - fc2 * fc2 / t2);
+  sg[3][3] = (sg[0][0] - sg[1][1] - sg[2][2]) / t2;
+  const double g[4] = { 1.0, -1.0, -1.0, -t2 };
+  double sn = 0.0, pn = 0.0;
+  #pragma unroll
+  for (int c = 0; c < 4; ++c)
+    #pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double sc = (c == 0 ? sg[r][c] : sg[r][c] * g[c]);
+      const double pc = (c == 0 ? S[r][c] : S[r][c] * g[c]);
+      sn += sg[r][c] * (r == 0 ? sc : sc * g[r]);
+      pn += S[r][c] * (r == 0 ? pc : pc * g[r]);
+    }
+  hn[0] = tau_pi * sqrt(fabs(sn));
+  hn[1] = tau_Pi * fabs(theta);
+  hn[2] = sqrt(fabs(pn)) / pressure;
+  hn[3] = fabs(bulk) / pressure;
+}
+
 template <bool SHEAR>
 __device__ __forceinline__ void eom_d2(const Cols& col, double t, double use_vort, double* du_out,
                                        double* d_s_out, double* dN, double* d_b_out, double* d_shv_out,
-                                       double* theta_out, double* snu_out, double* dez_out)
+                                       double* theta_out, double* snu_out, double* dez_out, double* hn_out)
 {
   const double t2 = t * t;
   const double cmd3 = -1.0 / t / t;                      // contra metric (1,-1,-1,-1/t^2)
@@ -71,6 +116,7 @@ __device__ __forceinline__ void eom_d2(const Cols& col, double t, double use_vor
   double F_0i[2] = { 0.0, 0.0 }, M_0i[2][2] = { { 0.0, 0.0 }, { 0.0, 0.0 } };
   double F_es[2][3] = { { 0, 0, 0 }, { 0, 0, 0 } };
   double M_es[2][3][2] = { { { 0, 0 }, { 0, 0 }, { 0, 0 } }, { { 0, 0 }, { 0, 0 }, { 0, 0 } } };
+  double sgF[2] = { 0.0, 0.0 }, sgM[2][2] = { { 0.0, 0.0 }, { 0.0, 0.0 } };   // sigma^{11}, sigma^{12} MRFs
   if (SHEAR) {
     // ---------------- calculate_MRF_shear ----------------
     double gu[2][2];
@@ -221,6 +267,9 @@ __device__ __forceinline__ void eom_d2(const Cols& col, double t, double use_vor
       Fsm[a][2] += -fu[a] * 0.0 * 0.0 * t / 2. - fu[a] * 0.0 * gamma / t;
       Fdsm[a][a] += -cmd[a + 1] * sgu / 3. - cmd[a + 1] * (gamma * H[3][3] / t) / 3.;
     }
+    sgF[0] = Fsm[1][1]; sgF[1] = Fsm[1][2];
+    sgM[0][0] = Msm[1][1][0]; sgM[0][1] = Msm[1][1][1];
+    sgM[1][0] = Msm[1][2][0]; sgM[1][1] = Msm[1][2][1];
     #pragma unroll
     for (int a = 0; a < 2; ++a) {
       #pragma unroll
@@ -313,6 +362,11 @@ __device__ __forceinline__ void eom_d2(const Cols& col, double t, double use_vor
   }
   // ---------------- compute_Ez_derivative ----------------
   const double p = col(CCAKE_F_P), e = col(CCAKE_F_E);
+  if (SHEAR) {
+    const double s11 = ((0.0 + sgM[0][0] * du0) + sgM[0][1] * du1) + sgF[0];
+    const double s12 = ((0.0 + sgM[1][0] * du0) + sgM[1][1] * du1) + sgF[1];
+    hydro_numbers(S, s11, s12, u[0], u[1], gamma, t2, tau_pi, tau_Pi, *theta_out, bulk, p, hn_out);
+  }
   *dez_out = ((e + p + bulk) * 0.0 * 0.0 + (p + bulk) / t2 + S[3][3]) * col(CCAKE_F_MASS_S) * t2 / col(CCAKE_F_SIGMA_LAB);
   (void)e;
 }
@@ -328,8 +382,8 @@ __global__ __launch_bounds__(256) void k_eom(double* __restrict__ F, int npad, i
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const Cols col{ F, npad, i };
-  double du[2], ds, dN[3], db, dshv[5] = { 0, 0, 0, 0, 0 }, theta, snu, dez;
-  eom_d2<SHEAR>(col, t, use_vort, du, &ds, dN, &db, dshv, &theta, &snu, &dez);
+  double du[2], ds, dN[3], db, dshv[5] = { 0, 0, 0, 0, 0 }, theta, snu, dez, hn[4];
+  eom_d2<SHEAR>(col, t, use_vort, du, &ds, dN, &db, dshv, &theta, &snu, &dez, hn);
   col(CCAKE_F_DUDT0 + 0) = du[0]; col(CCAKE_F_DUDT0 + 1) = du[1]; col(CCAKE_F_DUDT0 + 2) = 0.0;
   col(CCAKE_F_DEXT_S) = ds;
   col(CCAKE_F_DEXT_B) = dN[0]; col(CCAKE_F_DEXT_SS) = dN[1]; col(CCAKE_F_DEXT_Q) = dN[2];
@@ -338,6 +392,10 @@ __global__ __launch_bounds__(256) void k_eom(double* __restrict__ F, int npad, i
   col(CCAKE_F_THETA) = theta;
   col(CCAKE_F_SHV_NABLA_U) = snu;
   col(CCAKE_F_DEZ) = dez;
+  if (SHEAR) {
+    col(CCAKE_F_SHEAR_KNUDSEN) = hn[0]; col(CCAKE_F_BULK_KNUDSEN) = hn[1];
+    col(CCAKE_F_INV_REYNOLDS_SHEAR) = hn[2]; col(CCAKE_F_INV_REYNOLDS_BULK) = hn[3];
+  }
   if (!RK) return;
   double es = col(P_C_S) + dt * ds;
   if (es < 0.0 && freeze[i] > 3) es = 1.e-3;

@@ -91,6 +91,9 @@ enum ccake_field {
   CCAKE_F_THETA = CCAKE_F_DEXT_SHV0 + 5, CCAKE_F_SHV_NABLA_U,
   CCAKE_F_EZ, CCAKE_F_DEZ,               /* contribution_to_total_Ez / dEz   */
   CCAKE_F_FRZ1_E, CCAKE_F_FRZ2_E,        /* freeze-out energy snapshots      */
+  /* Knudsen and inverse Reynolds numbers, written by the EoM when shear is on
+     (EoM_default::compute_hydro_numbers, src/eom_default.cpp:1831-1995) */
+  CCAKE_F_SHEAR_KNUDSEN, CCAKE_F_BULK_KNUDSEN, CCAKE_F_INV_REYNOLDS_SHEAR, CCAKE_F_INV_REYNOLDS_BULK,
   CCAKE_NUM_FIELDS
 };
 

@@ -62,6 +62,7 @@ enum orc_field {
   ORC_F_THETA = ORC_F_DEXT_SHV0 + 5, ORC_F_SHV_NABLA_U,
   ORC_F_EZ, ORC_F_DEZ,
   ORC_F_FRZ1_E, ORC_F_FRZ2_E,
+  ORC_F_SHEAR_KNUDSEN, ORC_F_BULK_KNUDSEN, ORC_F_INV_REYNOLDS_SHEAR, ORC_F_INV_REYNOLDS_BULK,
   ORC_NUM_FIELDS
 };
 

@@ -243,6 +243,7 @@ static void eom_particle(orc_event* ev, int i)
   double M_shv_nabla_u[3] = {0,0,0}, F_shv_nabla_u = 0.0;
   double F_0i[3] = {0,0,0}, M_0i[3][3] = {{0}};
   double F_es[2][3] = {{0}}, M_es[2][3][3] = {{{0}}};
+  double sigF[2][3] = {{0}}, sigM[2][3][3] = {{{0}}};   /* sigma^{ij} MRFs (compute_derivatives 784-808) */
   if (using_shear) {
     double grad_uj[3][3] = {{0}}, grad_u0[3] = {0,0,0};
     for (int a = 0; a < D; ++a) for (int b = 0; b < D; ++b) {
@@ -411,6 +412,8 @@ static void eom_particle(orc_event* ev, int i)
       }
       F_es[a][a] += phi7*(-cmd[a+1]*shv_shvcov)/(3.*tau_pi*gamma*sigma);
     }
+    memcpy(sigF, Fsm, sizeof(sigF));
+    memcpy(sigM, Msm, sizeof(sigM));
   }
 
   /* ---------------- fill_auxiliary_variables (230-362) ---------------- */
@@ -531,6 +534,47 @@ static void eom_particle(orc_event* ev, int i)
 
   /* ---------------- compute_Ez_derivative (828-862) ---------------- */
   const double p = FLD(ev, ORC_F_P)[i], e = FLD(ev, ORC_F_E)[i];
+
+  /* ---------------- compute_hydro_numbers (1831-1995), with shear on ---------------- */
+  if (using_shear && D == 2) {
+    /* the stored sigma tensor: (a+1, b+1) = M_sigma du/dt + F_sigma (784-808) */
+    double stored[4][4] = {{0}};
+    for (int a = 0; a < 2; ++a) for (int b = a; b < 3; ++b) {
+      double m = 0.0;
+      for (int k = 0; k < D; ++k) m += sigM[a][b][k]*du_dt[k];
+      stored[a+1][b+1] = m + sigF[a][b];
+    }
+    /* read back into (a, b) (1893-1898) */
+    double sg[4][4] = {{0}};
+    for (int a = 0; a < 2; ++a) for (int b = a; b < 3; ++b) sg[a][b] = stored[a+1][b+1];
+    for (int a = 1; a < 4; ++a) for (int b = a+1; b < 4; ++b) sg[b][a] = sg[a][b];
+    double uc2[2] = { u[0], u[1] };
+    vec_cov(D, uc2, t2);
+    for (int a = 1; a < D+1; ++a) {
+      double row[2];
+      for (int b = 1; b < D+1; ++b) row[b-1] = sg[a][b];
+      double c = 0.0;
+      for (int k = 0; k < D; ++k) c += uc2[k]*row[k];
+      sg[0][a] = -1.*c/gamma;
+    }
+    for (int a = 1; a < 4; ++a) sg[a][0] = sg[0][a];
+    sg[0][0] = (fc[0]*fc[0]*sg[1][1] + fc[1]*fc[1]*sg[2][2] + 2.*fc[1]*fc[0]*sg[2][1]
+                + 2.*fc[2]*fc[0]*sg[3][1] + 2.*fc[2]*fc[1]*sg[3][2]
+                - fc[2]*fc[2]*(sg[1][1] + sg[2][2])/t2)/(gamma*gamma - fc[2]*fc[2]/t2);
+    sg[3][3] = (sg[0][0] - sg[1][1] - sg[2][2])/t2;
+    double sh[4][4], scov[4][4];
+    memcpy(sh, sg, sizeof(sg)); mat4_cov(sh, 1, t2);
+    memcpy(scov, sh, sizeof(sg)); mat4_cov(scov, 0, t2);
+    double sigma_norm = 0.0, shv_norm = 0.0;      /* milne::contract: columns outer, rows inner */
+    for (int c = 0; c < 4; ++c) for (int r = 0; r < 4; ++r) {
+      sigma_norm += sg[r][c]*scov[r][c];
+      shv_norm += S[r][c]*C[r][c];
+    }
+    FLD(ev, ORC_F_SHEAR_KNUDSEN)[i] = tau_pi*sqrt(fabs(sigma_norm));
+    FLD(ev, ORC_F_BULK_KNUDSEN)[i] = tau_Pi*fabs(theta);
+    FLD(ev, ORC_F_INV_REYNOLDS_SHEAR)[i] = sqrt(fabs(shv_norm))/p;
+    FLD(ev, ORC_F_INV_REYNOLDS_BULK)[i] = fabs(bulk)/p;
+  }
   const double u3 = (D == 3) ? u[2] : 0.0;
   FLD(ev, ORC_F_DEZ)[i] = ((e + p + bulk)*u3*u3 + (p + bulk)/t2 + S[3][3])
                           * FLD(ev, ORC_F_MASS_S)[i]*t2/FLD(ev, ORC_F_SIGMA_LAB)[i];

@@ -140,6 +140,7 @@ DERIVS = ([f"GRADP0+{c}" for c in range(2)] + [f"GRADE0+{c}" for c in range(2)]
           + [f"DIVSHEAR0+{c}" for c in range(2)] + ["GRADV0+0", "GRADV0+1", "GRADV0+3", "GRADV0+4"]
           + [f"DUDT0+{c}" for c in range(2)]
           + ["DEXT_S", "DEXT_B", "DEXT_SS", "DEXT_Q", "DEXT_BULK", "THETA", "SHV_NABLA_U", "DEZ"]
+          + ["SHEAR_KNUDSEN", "BULK_KNUDSEN", "INV_REYNOLDS_SHEAR", "INV_REYNOLDS_BULK"]
           + [f"DEXT_SHV0+{c}" for c in range(5)])
 
 

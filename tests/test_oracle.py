@@ -261,6 +261,53 @@ def test_transport_closed_forms(orc):
     assert np.allclose(ev.field("ORC_F_DELTA_BB"), ev.field("ORC_F_TAU_BULK"))
 
 
+def test_hydro_numbers_closed_forms_at_rest(orc):
+    """EoM_default::compute_hydro_numbers (src/eom_default.cpp:1831-1995) on
+    fluid cells at rest (u = 0, gamma = 1) with prescribed velocity gradients,
+    pi^{ij} and Pi, at tau = 1.3: the MRF shear rate reduces to the textbook
+    sigma^{ij} = -(d_i v^j + d_j v^i)/2 + delta_ij (1/tau + div v)/3, of which
+    the reference's read-back keeps sigma^{yy} (stored at (2,2), read into
+    (1,1); its time components then follow from u = 0, and tau^2 sigma^{etaeta}
+    from tracelessness), so Kn_shear = tau_pi sqrt(2) |sigma^{yy}|;
+    Kn_bulk = tau_Pi |theta| with theta = 1/tau + div v;
+    Re^-1_shear = sqrt(pi:pi)/p with pi:pi = pi_xx^2 + 2 pi_xy^2 + pi_yy^2 + tau^4 (pi^{etaeta})^2;
+    Re^-1_bulk = |Pi|/p."""
+    rng = np.random.default_rng(11)
+    n = 5
+    cfg = config.base_params(t0=1.3, constant_eta_over_s=0.08)
+    ev = _event(orc, cfg, n)
+    ev.t = 1.3
+    t = 1.3
+    ev.set("ORC_F_GAMMA", np.ones(n))
+    for f in ("UX", "UY", "VX", "VY"):
+        ev.set("ORC_F_" + f, np.zeros(n))
+    for f in ("T", "W", "S", "P", "SIGMA", "SIGMA_LAB", "MASS_S", "DWDS", "TAU_PI", "TAU_BULK", "ETA_PI",
+              "ZETA_PI", "DELTA_PIPI", "DELTA_BB"):
+        ev.set("ORC_F_" + f, rng.uniform(0.5, 2.0, n))
+    g = rng.normal(size=(4, n))                     # d_x v^x, d_x v^y, d_y v^x, d_y v^y
+    fid = orc.FIELDS
+    for k, c in enumerate((0, 1, 3, 4)):
+        ev.field(fid["ORC_F_GRADV0"] + c)[:] = g[k]
+    pxx, pxy, pyy, pee = rng.normal(scale=0.1, size=(4, n))
+    shv = {5: pxx, 6: pxy, 9: pxy, 10: pyy, 15: pee}
+    for c in range(16):
+        ev.field(fid["ORC_F_SHV0"] + c)[:] = shv.get(c, np.zeros(n))
+    Pi = rng.normal(scale=0.1, size=n)
+    ev.set("ORC_F_BULK", Pi)
+    ev.eom()
+    divv = g[0] + g[3]
+    theta = 1.0 / t + divv
+    assert np.allclose(ev.field("ORC_F_THETA"), theta, rtol=1e-13)
+    s_yy = -g[3] + theta / 3.0
+    p = ev.field("ORC_F_P")
+    assert np.allclose(ev.field("ORC_F_SHEAR_KNUDSEN"), ev.field("ORC_F_TAU_PI") * np.sqrt(2.0) * np.abs(s_yy),
+                       rtol=1e-12)
+    assert np.allclose(ev.field("ORC_F_BULK_KNUDSEN"), ev.field("ORC_F_TAU_BULK") * np.abs(theta), rtol=1e-13)
+    pipi = pxx ** 2 + 2 * pxy ** 2 + pyy ** 2 + t ** 4 * pee ** 2
+    assert np.allclose(ev.field("ORC_F_INV_REYNOLDS_SHEAR"), np.sqrt(pipi) / p, rtol=1e-13)
+    assert np.allclose(ev.field("ORC_F_INV_REYNOLDS_BULK"), np.abs(Pi) / p, rtol=1e-15)
+
+
 # ---------------------------------------------------------------- evolution
 def _gubser_event(orc, spacing, **kw):
     cfg = config.gubser_ideal_params(**kw)
