@@ -386,7 +386,9 @@ STATUS_CAPACITY = 7
 
 
 def load_event(ev, ic):
-    """Upload an initial condition dict (x, y, e, ux, uy, rhoB, rhoS, rhoQ)."""
+    """Upload an initial condition dict (x, y, e, ux, uy, rhoB, rhoS, rhoQ;
+    optionally bulk and shv, the initial Pi and pi^{mu nu} that initialize
+    reads with bulk_from_trace / input_initial_shear)."""
     ev.upload("CCAKE_F_X", ic["x"])
     ev.upload("CCAKE_F_Y", ic["y"])
     ev.upload("CCAKE_F_IN_E", ic["e"])
@@ -395,3 +397,8 @@ def load_event(ev, ic):
     ev.upload("CCAKE_F_IN_B", ic["rhoB"])
     ev.upload("CCAKE_F_IN_SS", ic["rhoS"])
     ev.upload("CCAKE_F_IN_Q", ic["rhoQ"])
+    if "bulk" in ic:
+        ev.upload("CCAKE_F_BULK", ic["bulk"])
+    if "shv" in ic:
+        for c in range(16):
+            ev.upload(FIELDS["CCAKE_F_SHV0"] + c, ic["shv"][c])

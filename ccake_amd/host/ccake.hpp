@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <fstream>
 #include <functional>
+#include <iomanip>
 #include <memory>
 #include <sstream>
 #include <stdexcept>
@@ -67,6 +68,17 @@ struct Settings {
   int cells_per_rc = 3;
   bool print_conservation_status = false;
   bool exit_on_error = false;
+  // initial conditions and outputs (include/settings.h, include/defaults.h:25-72)
+  std::string IC_type = "ccake";      // "ccake" or "ICCING"
+  std::string IC_file;
+  double e_cutoff = 0.15;             // GeV/fm^3; points at or below it are dropped
+  bool input_as_entropy = false;
+  bool input_initial_diffusion = false;
+  bool baryon_charge_enabled = true, strange_charge_enabled = true, electric_charge_enabled = true;
+  double stepx = 0.0, stepy = 0.0, stepEta = 0.0, xmin = 0.0, ymin = 0.0, etamin = 0.0;
+  std::vector<std::string> headers;
+  std::string results_directory = ".";
+  bool txt_evolution = false, hdf_evolution = false;
 };
 
 inline void check(int st, const char* what, const Settings* s = nullptr)
@@ -95,11 +107,19 @@ struct Particle {
   double s_ext = 0, rhoB_ext = 0, rhoS_ext = 0, rhoQ_ext = 0, Bulk_ext = 0;
   double T_ = 0, muB = 0, muS = 0, muQ = 0, e_ = 0, s = 0, rhoB = 0, rhoS = 0, rhoQ = 0, p_ = 0, cs2 = 0, w = 0;
   double Bulk = 0;
-  double shv[4][4] = {};
+  double shv[4][4] = {};              // also the initial pi^{mu nu} an IC reader sets
+  double diffusion[3][4] = {};        // initial diffusion currents (read, not evolved)
+  // transport coefficients and diagnostics (include/hydrodynamic_info.h:29-62)
+  double eta_pi = 0, zeta_Pi = 0, tau_Pi = 0, tau_pi = 0, theta = 0;
+  double shear_knudsen = 0, bulk_knudsen = 0, inverse_reynolds_shear = 0, inverse_reynolds_bulk = 0;
+  double knudsen_diffusion[3] = {}, inverse_reynolds_diffusion[3] = {};   // zero: diffusion is off
+  int causality = 0;                  // check_causality is not on this path
+  std::string eos_name = "default";   // the host name the reference prints (never set from the device)
   int Freeze = 0, btrack = 0, eos_id = 0;
   double T() const { return T_; }
   double e() const { return e_; }
   double p() const { return p_; }
+  std::string get_current_eos_name() const { return eos_name; }
 };
 
 // EoM_default's static helpers (src/eom_default.cpp:17-38, 180-203): the
@@ -185,6 +205,11 @@ class SystemState {
     up(CCAKE_F_IN_B, [](const Particle<D>& q) { return q.input_rhoB; });
     up(CCAKE_F_IN_SS, [](const Particle<D>& q) { return q.input_rhoS; });
     up(CCAKE_F_IN_Q, [](const Particle<D>& q) { return q.input_rhoQ; });
+    // initial Pi and pi^{mu nu}: read by initialize when bulk_from_trace /
+    // input_initial_shear are set, zeroed there otherwise
+    up(CCAKE_F_BULK, [](const Particle<D>& q) { return q.Bulk; });
+    for (int c = 0; c < 16; ++c)
+      up(CCAKE_F_SHV0 + c, [c](const Particle<D>& q) { return q.shv[c / 4][c % 4]; });
     check(ccake_event_set_time(ev, t), "ccake_event_set_time", settingsPtr.get());
   }
 
@@ -224,6 +249,15 @@ class SystemState {
     dn(CCAKE_F_CS2, [](Particle<D>& q, double v) { q.cs2 = v; });
     dn(CCAKE_F_W, [](Particle<D>& q, double v) { q.w = v; });
     dn(CCAKE_F_BULK, [](Particle<D>& q, double v) { q.Bulk = v; });
+    dn(CCAKE_F_ETA_PI, [](Particle<D>& q, double v) { q.eta_pi = v; });
+    dn(CCAKE_F_ZETA_PI, [](Particle<D>& q, double v) { q.zeta_Pi = v; });
+    dn(CCAKE_F_TAU_BULK, [](Particle<D>& q, double v) { q.tau_Pi = v; });
+    dn(CCAKE_F_TAU_PI, [](Particle<D>& q, double v) { q.tau_pi = v; });
+    dn(CCAKE_F_THETA, [](Particle<D>& q, double v) { q.theta = v; });
+    dn(CCAKE_F_SHEAR_KNUDSEN, [](Particle<D>& q, double v) { q.shear_knudsen = v; });
+    dn(CCAKE_F_BULK_KNUDSEN, [](Particle<D>& q, double v) { q.bulk_knudsen = v; });
+    dn(CCAKE_F_INV_REYNOLDS_SHEAR, [](Particle<D>& q, double v) { q.inverse_reynolds_shear = v; });
+    dn(CCAKE_F_INV_REYNOLDS_BULK, [](Particle<D>& q, double v) { q.inverse_reynolds_bulk = v; });
     for (int c = 0; c < 16; ++c) {
       check(ccake_event_download(ev, CCAKE_F_SHV0 + c, a.data(), a.size()), "ccake_event_download", settingsPtr.get());
       for (int i = 0; i < n_particles; ++i) particles[i].shv[c / 4][c % 4] = a[i];
@@ -357,16 +391,67 @@ class FreezeOut {
   std::shared_ptr<SystemState<D>> systemPtr;
 };
 
-// Output<D, TEOM>'s freeze-out writer (src/output.cpp:412-498): rows are
-// buffered as text at every print_freeze_out and written to
-// output_directory/freeze_out.dat by flush_freeze_out, after a header line
-// "# B S Q" with the totals passed at the first call.  Numbers use the
-// default stream format (6 significant digits), columns separated by one
-// space, as the reference writes them.
+// Output<D, TEOM> (src/output.cpp): the text outputs of the run.
+//  - print_system_state (63-83) downloads the particles and writes
+//    system_state_<k>.dat (print_system_state_to_txt, 102-172) when
+//    txt_evolution is set; k counts the calls.  HDF5 output (hdf_evolution)
+//    needs an HDF5 library, which this build does not link: asking for it
+//    fails at construction.
+//  - freeze-out rows are buffered as text at every print_freeze_out and
+//    written to output_directory/freeze_out.dat by flush_freeze_out
+//    (412-498), after a header line "# B S Q" with the totals passed at the
+//    first call.
+// Numbers are formatted as the reference formats them: the default stream
+// format (6 significant digits) for freeze_out.dat and the first two columns
+// of the first system-state line, std::scientific with 6 digits after it.
 template <unsigned int D, template <unsigned int> class TEOM>
 class Output {
  public:
   explicit Output(std::string dir) : output_directory(std::move(dir)) {}
+  Output(std::shared_ptr<Settings> s, std::shared_ptr<SystemState<D>> sys)
+    : output_directory(s->results_directory), settingsPtr(std::move(s)), systemPtr(std::move(sys))
+  {
+    if (settingsPtr->hdf_evolution)
+      check(CCAKE_ERR_UNSUPPORTED, "Output (hdf_evolution: HDF5 output is not built)", settingsPtr.get());
+  }
+
+  void print_system_state()
+  {
+    systemPtr->copy_device_to_host();
+    if (settingsPtr->txt_evolution) print_system_state_to_txt();
+    n_timesteps_output++;
+  }
+
+  void print_system_state_to_txt()
+  {
+    constexpr double hc = 0.1973269804;      // include/constants.h:15
+    const std::string name = output_directory + "/system_state_" + std::to_string(n_timesteps_output) + ".dat";
+    std::ofstream out(name);
+    if (!out) throw std::runtime_error("Could not open " + name);
+    const SystemState<D>& sys = *systemPtr;
+    out << sys.t << "\n";
+    int id = 0;
+    for (const Particle<D>& p : sys.particles) {
+      out << id++ << " " << sys.t << " " << std::setprecision(6) << std::scientific;
+      for (unsigned a = 0; a < D; ++a) out << p.r[a] << " ";
+      for (unsigned a = D; a < 3; ++a) out << 0.0 << " ";
+      out << p.p() * hc << " " << p.T() * hc << " " << p.muB * hc << " " << p.muS * hc << " " << p.muQ * hc << " "
+          << p.e() * hc << " " << p.rhoB << " " << p.rhoS << " " << p.rhoQ << " " << p.s << " "
+          << p.eta_pi << " " << p.zeta_Pi << " " << p.tau_Pi << " " << p.tau_pi << " " << p.theta << " "
+          << p.inverse_reynolds_shear << " " << p.inverse_reynolds_bulk << " "
+          << p.shear_knudsen << " " << p.bulk_knudsen << " "
+          << p.shv[0][0] << " " << p.shv[1][1] << " " << p.shv[2][2] << " " << p.shv[0][1] << " "
+          << p.shv[0][2] << " " << p.shv[1][2] << " " << p.shv[1][3] << " " << p.shv[2][3] << " "
+          << p.shv[3][3] << " ";
+      for (unsigned a = 0; a < D; ++a) out << p.u[a] << " ";
+      for (unsigned a = D; a < 3; ++a) out << 0.0 << " ";
+      out << p.gamma << " " << p.Freeze << " " << p.get_current_eos_name() << " " << p.causality << " ";
+      for (int c = 0; c < 3; ++c) out << p.knudsen_diffusion[c] << " ";
+      for (int c = 0; c < 3; ++c) out << p.inverse_reynolds_diffusion[c] << " ";
+      out << "\n";
+    }
+    out << std::flush;
+  }
 
   void print_freeze_out(std::shared_ptr<FreezeOut<D>> freeze_out, double B, double Q, double S)
   {
@@ -399,8 +484,11 @@ class Output {
 
   int particles_buffered() const { return fo_particles_buffered; }
   std::string output_directory;
+  int n_timesteps_output = 0;
 
  private:
+  std::shared_ptr<Settings> settingsPtr;
+  std::shared_ptr<SystemState<D>> systemPtr;
   std::string fo_header;
   std::ostringstream fo_buffer;
   size_t rows_buffered = 0;

@@ -8,9 +8,14 @@
 // --freeze-out-T (MeV) particlization is on, the hypersurface is buffered
 // after every step (Output::print_freeze_out, as BSQHydro::run does,
 // src/BSQHydro.cpp:797) and written to OUTPUT/freeze_out.dat at the end (859-860).
+// With --ic FILE the particles come from a ccake-format initial-condition
+// file (BSQHydro::read_ccake) instead of the built-in lattice; with
+// --txt-evolution K the system state is written to OUTPUT/system_state_<k>.dat
+// after initialisation and every K steps (src/BSQHydro.cpp:689-691, 791-796).
 //
 //   gubser_run [--spacing 0.05] [--steps 120] [--dt 0.005] [--evolver-steps 0]
-//              [--freeze-out-T MeV --output DIR]
+//              [--freeze-out-T MeV] [--ic FILE --e-cutoff GeV/fm^3]
+//              [--txt-evolution K] [--output DIR]
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -20,6 +25,7 @@
 #include <string>
 
 #include "ccake.hpp"
+#include "ccake_io.hpp"
 
 namespace {
 
@@ -70,8 +76,9 @@ int main(int argc, char** argv)
 {
   double a = 0.05;
   int steps = 120, evolver_steps = 0;
-  double dt = 0.005, t_fo_mev = 0.0;
-  std::string out_dir = ".";
+  double dt = 0.005, t_fo_mev = 0.0, e_cutoff = 0.15;
+  int txt_stride = 0;
+  std::string out_dir = ".", ic_file;
   for (int k = 1; k + 1 < argc; k += 2) {
     if (!std::strcmp(argv[k], "--spacing")) a = std::atof(argv[k + 1]);
     else if (!std::strcmp(argv[k], "--steps")) steps = std::atoi(argv[k + 1]);
@@ -79,6 +86,9 @@ int main(int argc, char** argv)
     else if (!std::strcmp(argv[k], "--evolver-steps")) evolver_steps = std::atoi(argv[k + 1]);
     else if (!std::strcmp(argv[k], "--freeze-out-T")) t_fo_mev = std::atof(argv[k + 1]);
     else if (!std::strcmp(argv[k], "--output")) out_dir = argv[k + 1];
+    else if (!std::strcmp(argv[k], "--ic")) ic_file = argv[k + 1];
+    else if (!std::strcmp(argv[k], "--e-cutoff")) e_cutoff = std::atof(argv[k + 1]);
+    else if (!std::strcmp(argv[k], "--txt-evolution")) txt_stride = std::atoi(argv[k + 1]);
   }
   auto s = std::make_shared<ccake::Settings>();
   s->eos_type = "conformal";
@@ -88,15 +98,27 @@ int main(int argc, char** argv)
   s->using_shear = false;
   s->eta_mode = CCAKE_ETA_CONSTANT; s->constant_eta_over_s = 0.0;
   s->zeta_mode = CCAKE_ZETA_CONSTANT; s->constant_zeta_over_s = 0.0;
+  s->results_directory = out_dir;
+  s->txt_evolution = txt_stride > 0;
+  s->IC_file = ic_file;
+  s->e_cutoff = e_cutoff;
 
   try {
-    auto sys = make_system(s, a);
+    std::shared_ptr<ccake::SystemState<2>> sys;
+    if (ic_file.empty()) {
+      sys = make_system(s, a);
+    } else {
+      sys = std::make_shared<ccake::SystemState<2>>(s);
+      ccake::read_in_initial_conditions(*s, *sys);
+    }
     ccake::SPHWorkstation<2, ccake::EoM_default> ws(s, sys);
     ws.initialize();
     ws.initialize_hydrodynamics();
-    ccake::Output<2, ccake::EoM_default> out(out_dir);
+    ccake::Output<2, ccake::EoM_default> out(s, sys);
+    if (s->txt_evolution) out.print_system_state();
     for (int k = 0; k < steps; ++k) {
       ws.advance_timestep(s->dt, s->rk_order);
+      if (s->txt_evolution && (k + 1) % txt_stride == 0) out.print_system_state();
       if (s->particlization_enabled)
         out.print_freeze_out(ws.freezePtr, sys->Btotal0, sys->Qtotal0, sys->Stotal0);
     }

@@ -8,6 +8,8 @@ Gubser profile is the closed form of utilities/ideal_gubser_ic_generator.py
 """
 from __future__ import annotations
 
+import re
+
 import numpy as np
 
 from .config import HBARC_GEVFM
@@ -75,3 +77,184 @@ def lumpy(seed=0xCCA4E001, spacing=0.02, half_width=5.0, n_blobs=50, sigma=0.4,
     x, y, e, rb, rs, rq = x[keep], y[keep], e[keep], rb[keep], rs[keep], rq[keep]
     z = np.zeros_like(x)
     return dict(x=x, y=y, e=e / HBARC_GEVFM, ux=z, uy=z.copy(), rhoB=rb, rhoS=rs, rhoQ=rq)
+
+
+# ---------------------------------------------------------------- file formats
+# Readers of the reference's initial-condition files, the Python twins of
+# read_ccake / read_ICCING in ccake_amd/host/ccake_io.hpp: the same fields,
+# units, cutoff and charge switches, and the same treatment of short or
+# malformed lines, because numbers are taken the way std::istringstream >>
+# takes them (libstdc++ num_get: optional sign, digits, point, digits,
+# exponent; an incomplete or empty number reads as 0 and fails the line; a
+# read at the end of the line fails and leaves the variable as it was).
+
+_WS = " \t\n\v\f\r"
+_DIGITS = "0123456789"
+_FULL = re.compile(r"[+-]?(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?\Z")
+
+
+class _LineStream:
+    """std::istringstream over one line, read with operator>>(double&)."""
+
+    def __init__(self, text, pos=0, eof=False):
+        self.s, self.pos, self.failed = text, pos, eof
+
+    def read(self, current):
+        if self.failed:
+            return current
+        s, p, n = self.s, self.pos, len(self.s)
+        while p < n and s[p] in _WS:
+            p += 1
+        if p >= n:                          # sentry fails: value unchanged
+            self.failed, self.pos = True, p
+            return current
+        q = p
+        if q < n and s[q] in "+-":
+            q += 1
+        while q < n and s[q] in _DIGITS:
+            q += 1
+        if q < n and s[q] == ".":
+            q += 1
+            while q < n and s[q] in _DIGITS:
+                q += 1
+        if q < n and s[q] in "eE":
+            q += 1
+            if q < n and s[q] in "+-":
+                q += 1
+            while q < n and s[q] in _DIGITS:
+                q += 1
+        tok = s[p:q]
+        self.pos = q
+        if _FULL.match(tok):
+            return float(tok)
+        self.failed = True                  # nothing or an incomplete number: 0
+        return 0.0
+
+
+def _ignore_to_hash(line):
+    """istream::ignore(256, '#') on the header line: the stream after it."""
+    k = line.find("#", 0, 256)
+    if k >= 0:
+        return _LineStream(line, k + 1)
+    if len(line) > 256:
+        return _LineStream(line, 256)
+    return _LineStream(line, len(line), eof=True)
+
+
+def read_ccake(path, e_cutoff_gev=0.15, baryon=True, strange=True, electric=True, initial_diffusion=False,
+               input_as_entropy=False):
+    """BSQHydro::read_ccake (src/BSQHydro.cpp:160-425) for 2+1D: header
+    "#<ignored> dx dy deta <ignored> xmin ymin etamin", then per line x y eta
+    e[GeV/fm^3] rhoB rhoS rhoQ ux uy ueta Pi pi^xx pi^xy pi^xeta pi^yy
+    pi^yeta pi^etaeta (GeV/fm^3 for e, Pi and pi), and with
+    initial_diffusion 12 more columns qB0..3 qS0..3 qQ0..3.  Blank and '#'
+    lines after the header are skipped; points with e <= e_cutoff_gev are
+    dropped; disabled charges read as 0.  Returns the load_event dict (e, Pi
+    and pi in fm^-4; shv as 16 rows, pi^{mu nu} row-major) with 'diffusion'
+    (n x 3 x 4) and 'grid' (dx, dy, deta, xmin, ymin, etamin)."""
+    if input_as_entropy:
+        raise NotImplementedError("input_as_entropy initial conditions are not supported on this path")
+    hc = HBARC_GEVFM
+    names = ["x", "y", "eta", "col4", "rB", "rS", "rQ", "ux", "uy", "ueta", "bulk",
+             "pxx", "pxy", "pxe", "pyy", "pye", "pee"]
+    v = dict.fromkeys(names, 0.0)
+    q = [0.0] * 12
+    rows, grid, headers = [], (0.0,) * 6, []
+    with open(path) as f:
+        lines = f.read().split("\n")
+    if lines and lines[-1] == "":
+        lines.pop()
+    for k, line in enumerate(lines):
+        if k == 0:
+            headers.append(line)
+            st = _ignore_to_hash(line)
+            g = [0.0] * 8
+            for j in range(8):
+                g[j] = st.read(g[j])
+            grid = (g[1], g[2], g[3], g[5], g[6], g[7])
+            continue
+        stripped = line.lstrip(" \t\r\n")
+        if not stripped or stripped[0] == "#":
+            continue
+        st = _LineStream(line)
+        for nme in names:
+            v[nme] = st.read(v[nme])
+        if initial_diffusion:
+            for j in range(12):
+                q[j] = st.read(q[j])
+        e = v["col4"]
+        v["col4"] = 0.0
+        e /= hc
+        for nme in ("pxx", "pxy", "pxe", "pyy", "pye", "pee", "bulk"):
+            v[nme] /= hc
+        if e > e_cutoff_gev / hc:
+            diff = [qq if on else 0.0 for qq, on in zip(q, [baryon] * 4 + [strange] * 4 + [electric] * 4)] \
+                if initial_diffusion else [0.0] * 12
+            rows.append((v["x"], v["y"], e, v["rB"] if baryon else 0.0, v["rS"] if strange else 0.0,
+                         v["rQ"] if electric else 0.0, v["ux"], v["uy"], v["bulk"],
+                         v["pxx"], v["pxy"], v["pxe"], v["pyy"], v["pye"], v["pee"], diff))
+    n = len(rows)
+    out = {k: np.array([r[j] for r in rows], dtype=np.float64)
+           for j, k in enumerate(["x", "y", "e", "rhoB", "rhoS", "rhoQ", "ux", "uy", "bulk"])}
+    shv = np.zeros((16, n))
+    for j, (a, b) in enumerate([(1, 1), (1, 2), (1, 3), (2, 2), (2, 3), (3, 3)]):
+        col = np.array([r[9 + j] for r in rows], dtype=np.float64)
+        shv[4 * a + b] = col
+        shv[4 * b + a] = col
+    out["shv"] = shv
+    out["diffusion"] = np.array([r[15] for r in rows], dtype=np.float64).reshape(n, 3, 4)
+    out["grid"] = grid
+    out["headers"] = headers
+    return out
+
+
+def read_iccing(path, e_cutoff_gev=0.15, baryon=True, strange=True, electric=True):
+    """BSQHydro<2, EoM_default>::read_ICCING (src/BSQHydro.cpp:83-139): header
+    "<ignored> dx dy <ignored> xmin ymin", then x y e[GeV/fm^3] rhoB rhoS
+    rhoQ per line, fluid at rest; every line after the header is a point."""
+    hc = HBARC_GEVFM
+    x = y = e = rB = rS = rQ = 0.0
+    rows, grid, headers = [], (0.0,) * 6, []
+    with open(path) as f:
+        lines = f.read().split("\n")
+    if lines and lines[-1] == "":
+        lines.pop()
+    for k, line in enumerate(lines):
+        st = _LineStream(line)
+        if k == 0:
+            headers.append(line)
+            g = [0.0] * 6
+            for j in range(6):
+                g[j] = st.read(g[j])
+            grid = (g[1], g[2], 0.0, g[4], g[5], 0.0)
+            continue
+        x, y, e, rB, rS, rQ = (st.read(c) for c in (x, y, e, rB, rS, rQ))
+        e /= hc
+        if e > e_cutoff_gev / hc:
+            rows.append((x, y, e, rB if baryon else 0.0, rS if strange else 0.0, rQ if electric else 0.0))
+    n = len(rows)
+    out = {k: np.array([r[j] for r in rows], dtype=np.float64)
+           for j, k in enumerate(["x", "y", "e", "rhoB", "rhoS", "rhoQ"])}
+    out["ux"], out["uy"], out["bulk"] = np.zeros(n), np.zeros(n), np.zeros(n)
+    out["shv"] = np.zeros((16, n))
+    out["diffusion"] = np.zeros((n, 3, 4))
+    out["grid"] = grid
+    out["headers"] = headers
+    return out
+
+
+def write_ccake(path, d, grid=(0.0, 0.0, 0.0, 0.0, 0.0, 0.0)):
+    """Writes an initial condition in the ccake format read above (e, Pi and
+    pi converted back to GeV/fm^3; 17 significant digits)."""
+    hc = HBARC_GEVFM
+    n = len(d["x"])
+    shv = d.get("shv", np.zeros((16, n)))
+    bulk = d.get("bulk", np.zeros(n))
+    dx, dy, deta, xmin, ymin, etamin = grid
+    with open(path, "w") as f:
+        f.write(f"#0 {dx!r} {dy!r} {deta!r} 0 {xmin!r} {ymin!r} {etamin!r}\n")
+        for i in range(n):
+            vals = [d["x"][i], d["y"][i], 0.0, d["e"][i] * hc, d["rhoB"][i], d["rhoS"][i], d["rhoQ"][i],
+                    d["ux"][i], d["uy"][i], 0.0, bulk[i] * hc, shv[5][i] * hc, shv[6][i] * hc, shv[7][i] * hc,
+                    shv[10][i] * hc, shv[11][i] * hc, shv[15][i] * hc]
+            f.write(" ".join("%.17g" % float(v) for v in vals) + "\n")
