@@ -268,6 +268,83 @@ static void init_analytic(EosMember& m, int kind, double c, double T0, double mu
   for (int a = 1; a < 4; ++a) { m.mins[a] = -1e10; m.maxs[a] = 1e10; }
 }
 
+// EoS_Interpolator::load_flat_table (src/eos_interpolator.cpp:291-527): the
+// flat (s, nB) table in MeV units -> fm units; log10-uniform axes (checked to
+// 1e-6); dw/ds and dw/dB from the baryon-only Schur complement of the
+// susceptibilities (dw/dS = dw/dQ = 0).  Fills e->pre on the device and the
+// host copy e->h_pre (variables, then the s axis).
+static int make_preinverted(ccake_dev* dev, const ccake_eos_desc* desc, ccake_eos* e)
+{
+  const int Ns = desc->dims[0], NB = desc->dims[1];
+  if (Ns < 2 || NB < 2 || !desc->table) { set_error("pre-inverted table needs dims[0], dims[1] >= 2 and a table"); return CCAKE_ERR_INVALID; }
+  const size_t nt = (size_t)Ns * NB;
+  const double* sB = desc->table;                 // s, nB, T, muB, muS, muQ, e, p, cs2, chiTT, chiTB, chiBB
+  auto in = [&](int k, size_t i) { return sB[(size_t)k * nt + i]; };
+  const double h1 = 1.0 / kHbarcMeVfm, h2 = h1 * h1, h3 = h2 * h1, h4 = h2 * h2;
+  std::vector<double> sax(Ns), nax(NB);
+  for (int is = 0; is < Ns; ++is) sax[is] = in(0, (size_t)is * NB) * h3;
+  for (int iB = 0; iB < NB; ++iB) nax[iB] = in(1, (size_t)iB) * h3;
+  auto axis = [](const std::vector<double>& a, double* l0, double* step) {
+    for (double x : a) if (!(x > 0.0)) return false;
+    *l0 = std::log10(a.front());
+    *step = (std::log10(a.back()) - *l0) / (a.size() - 1);
+    double dev_max = 0.0;
+    for (size_t i = 1; i + 1 < a.size(); ++i)
+      dev_max = std::max(dev_max, std::fabs(std::log10(a[i + 1]) - std::log10(a[i]) - *step));
+    return dev_max <= 1e-6;
+  };
+  Preinv& p = e->pre;
+  p.Ns = Ns; p.NB = NB;
+  if (!axis(sax, &p.ls0, &p.dls) || !axis(nax, &p.lb0, &p.dlb)) {
+    set_error("pre-inverted table: s and nB axes must be positive and uniform in log10");
+    return CCAKE_ERR_INVALID;
+  }
+  p.s_first = sax.front(); p.s_last = sax.back(); p.nB_first = nax.front();
+  e->h_pre.assign((size_t)PI_NUM * nt + Ns, 0.0);
+  double* v = e->h_pre.data();
+  const double EPS = 1e-30, TINY = 1e-25;
+  for (int is = 0; is < Ns; ++is)
+    for (int iB = 0; iB < NB; ++iB) {
+      const size_t i = (size_t)is * NB + iB;
+      const double T = in(2, i) * h1, muB = in(3, i) * h1;
+      const double cTT = in(9, i) * h2, cTB = in(10, i) * h2, cBB = in(11, i) * h2;
+      const double s_c = sax[is], nB_c = nax[iB];
+      const double dentr_dt = cTT - (cTB * cTB) / (cBB + EPS);
+      const double dentr_dmub = cTB - (cTT * cBB) / (cTB + EPS);
+      const double db_dt = cTB - (cBB * cTT) / (cTB + EPS);
+      const double db_dmub = cBB - (cTB * cTB) / (cTT + EPS);
+      double dwds = T + s_c / dentr_dt;
+      if (std::fabs(nB_c) > TINY) dwds += nB_c / dentr_dmub;
+      double dwdB = muB;
+      if (std::fabs(nB_c) > TINY) dwdB += s_c / db_dt + nB_c / db_dmub;
+      v[PI_T * nt + i] = T; v[PI_MUB * nt + i] = muB;
+      v[PI_MUS * nt + i] = in(4, i) * h1; v[PI_MUQ * nt + i] = in(5, i) * h1;
+      v[PI_E * nt + i] = in(6, i) * h4; v[PI_P * nt + i] = in(7, i) * h4;
+      v[PI_CS2 * nt + i] = in(8, i);
+      v[PI_DWDS * nt + i] = dwds; v[PI_DWDB * nt + i] = dwdB;
+      v[PI_DWDQ * nt + i] = 0.0; v[PI_DWDS_S * nt + i] = 0.0;
+    }
+  std::copy(sax.begin(), sax.end(), v + (size_t)PI_NUM * nt);
+  const size_t bytes = e->h_pre.size() * sizeof(double);
+  {
+    std::lock_guard<std::mutex> lk(dev->mu);
+    if (dev->budget && dev->used + bytes > dev->budget) { set_error("HBM budget exceeded"); return CCAKE_ERR_OOM; }
+    dev->used += bytes;
+  }
+  e->bytes = bytes;
+  if (hipMalloc((void**)&e->d_pre, bytes) != hipSuccess) {
+    std::lock_guard<std::mutex> lk(dev->mu);
+    dev->used -= bytes;
+    e->bytes = 0;
+    set_error("hipMalloc failed");
+    return CCAKE_ERR_OOM;
+  }
+  CCAKE_HIP(hipMemcpy(e->d_pre, v, bytes, hipMemcpyHostToDevice));
+  p.v = e->d_pre;
+  p.s_axis = e->d_pre + (size_t)PI_NUM * nt;
+  return 0;
+}
+
 int ccake_eos_create(ccake_dev* dev, const ccake_eos_desc* desc, ccake_eos** out)
 {
   if (!dev || !desc || !out) return CCAKE_ERR_INVALID;
@@ -282,6 +359,14 @@ int ccake_eos_create(ccake_dev* dev, const ccake_eos_desc* desc, ccake_eos** out
     ch.n = 2;
     init_analytic(ch.m[0], M_CONF, c0, 1, 1, 1, 1, 0, 0);
     init_analytic(ch.m[1], M_DIAG, c0, 1, 1, 1, 1, 0, 0);
+    std::lock_guard<std::mutex> lk(dev->mu);
+    dev->live_eos++;
+    *out = e;
+    return 0;
+  }
+  if (desc->kind == CCAKE_EOS_PREINVERTED) {
+    const int rc = make_preinverted(dev, desc, e);
+    if (rc) { if (e->d_pre) (void)hipFree(e->d_pre); delete e; return rc; }
     std::lock_guard<std::mutex> lk(dev->mu);
     dev->live_eos++;
     *out = e;
@@ -362,6 +447,7 @@ int ccake_eos_destroy(ccake_eos* e)
   }
   if (e->d_tab) (void)hipFree(e->d_tab);
   if (e->d_tab4) (void)hipFree(e->d_tab4);
+  if (e->d_pre) (void)hipFree(e->d_pre);
   delete[] e->h_tab;
   delete e;
   return 0;
@@ -370,6 +456,26 @@ int ccake_eos_destroy(ccake_eos* e)
 int ccake_eos_efreeze(ccake_eos* e, double T_fo, double* e_out)
 {
   if (!e || !e_out) return CCAKE_ERR_INVALID;
+  if (e->chain.kind == CCAKE_EOS_PREINVERTED) {
+    // EoS_Interpolator::efreeze_at_T (src/eos_interpolator.cpp:865-892): T along
+    // the smallest-nB column, bisection, linear in the bracketing cell
+    const Preinv& p = e->pre;
+    const size_t nt = (size_t)p.Ns * p.NB;
+    const double* T = e->h_pre.data() + (size_t)PI_T * nt;
+    const double* E = e->h_pre.data() + (size_t)PI_E * nt;
+    auto at = [&](const double* a, int is) { return a[(size_t)is * p.NB]; };
+    if (T_fo <= at(T, 0)) { *e_out = at(E, 0); return 0; }
+    if (T_fo >= at(T, p.Ns - 1)) { *e_out = at(E, p.Ns - 1); return 0; }
+    int lo = 0, hi = p.Ns - 1;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) / 2;
+      if (at(T, mid) <= T_fo) lo = mid; else hi = mid;
+    }
+    const double Tl = at(T, lo), Th = at(T, hi);
+    const double alpha = (Th > Tl) ? (T_fo - Tl) / (Th - Tl) : 0.0;
+    *e_out = (1.0 - alpha) * at(E, lo) + alpha * at(E, hi);
+    return 0;
+  }
   double r[17], x[4] = { T_fo, 0, 0, 0 };
   member_fields(e->chain.m[0], e->h_tab, x, r);
   *e_out = r[X_E];
@@ -379,6 +485,10 @@ int ccake_eos_efreeze(ccake_eos* e, double T_fo, double* e_out)
 int ccake_eos_full_thermo(ccake_eos* e, int member, const double tbqs[4], double out17[17])
 {
   if (!e || !tbqs || !out17) return CCAKE_ERR_INVALID;
+  if (e->chain.kind == CCAKE_EOS_PREINVERTED) {
+    set_error("a pre-inverted EoS has no (T, mu) evaluation");
+    return CCAKE_ERR_UNSUPPORTED;
+  }
   return host_eos_full(e, member, tbqs, out17);
 }
 

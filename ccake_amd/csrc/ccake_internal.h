@@ -5,7 +5,9 @@
 #include <cstdint>
 #include <mutex>
 #include <string>
+#include <vector>
 #include "../../include/ccake_amd.h"
+#include "preinv_device.h"
 
 namespace ccake {
 
@@ -86,6 +88,11 @@ struct ccake_eos {
   int live_events = 0;       // events created on this EoS and not yet destroyed
   // host copy for host-side evaluation (efreeze / full_thermo)
   double* h_tab = nullptr;
+  // pre-inverted flat (s, nB) table (CCAKE_EOS_PREINVERTED): device view, its
+  // buffers (variables, then the s axis) and a host copy for efreeze_at_T
+  ccake::Preinv pre{};
+  double* d_pre = nullptr;
+  std::vector<double> h_pre;
 };
 
 struct ccake_event {

@@ -8,6 +8,7 @@
 // Threads walk the cell-sorted order, so the lanes of a wave hold nearby
 // particles with similar (T, mu) and share table cells.
 #include "eos_device.h"
+#include "preinv_device.h"
 #include <vector>
 #include <cstring>
 
@@ -76,6 +77,49 @@ __device__ __forceinline__ void transport(const Transport& P, double T, double w
 // MODE 0: s-inversion + reset_pi + transport + gradient gather (sorted order).
 // MODE 1: e-inversion of the inputs (id order), writes input s.
 // MODE 2: s-inversion only (id order).
+// reset_pi_tensor, transport coefficients and the gradient candidate record
+// of particle i (sorted position k) after its thermodynamics are known: the
+// tail of the fused EoS stage.
+__device__ __forceinline__ void stage_post(double* __restrict__ F, int npad, int i, int k, double t,
+                                           const Transport& P, double T, double w, double pr, double cs2,
+                                           double e, double* __restrict__ cand)
+{
+  auto col = [&](int c) -> double& { return F[(size_t)c * npad + i]; };
+  // ---- 05 reset_pi_tensor, D = 2 (fixed-size u has u^eta = 0) ----
+  const double gamma = col(CCAKE_F_GAMMA), sigma = col(CCAKE_F_SIGMA);
+  const double t2 = t * t;
+  const double ux = col(CCAKE_F_UX), uy = col(CCAKE_F_UY);
+  const double fc0 = -ux, fc1 = -uy;
+  const double bulk = col(CCAKE_F_EXT_BULK) * sigma;
+  col(CCAKE_F_BULK) = bulk;
+  const double s11 = col(CCAKE_F_EXT_SHV0 + 0) * sigma, s12 = col(CCAKE_F_EXT_SHV0 + 1) * sigma;
+  const double s13 = col(CCAKE_F_EXT_SHV0 + 2) * sigma, s22 = col(CCAKE_F_EXT_SHV0 + 3) * sigma;
+  const double s23 = col(CCAKE_F_EXT_SHV0 + 4) * sigma;
+  const double s00 = (fc0 * fc0 * s11 + fc1 * fc1 * s22 + 2. * fc1 * fc0 * s12) / (gamma * gamma);
+  const double s33 = (s00 - s11 - s22) / t2;
+  const double s01 = -s11 * fc0 / gamma + -s12 * fc1 / gamma;
+  const double s02 = -s12 * fc0 / gamma + -s22 * fc1 / gamma;
+  const double s03 = -s13 * fc0 / gamma + -s23 * fc1 / gamma;
+  const double S[16] = { s00, s01, s02, s03, s01, s11, s12, s13, s02, s12, s22, s23, s03, s13, s23, s33 };
+  for (int q = 0; q < 16; ++q) col(CCAKE_F_SHV0 + q) = S[q];
+
+  // ---- 08 transport coefficients ----
+  double tc[13];
+  transport(P, T, w, pr, cs2, tc);
+  for (int q = 0; q < 13; ++q) col(CCAKE_F_ETA_PI + q) = tc[q];
+
+  // ---- gradient candidate record: x, y, m_s, p/sl^2, e, Pi/sl^2, v, pi^{0i}/sl^2, pi^{ij}/sl^2
+  const double sl = col(CCAKE_F_SIGMA_LAB);
+  const double isl2 = 1.0 / (sl * sl);
+  double* o = cand + (size_t)k * kCandG;
+  o[0] = col(CCAKE_F_X); o[1] = col(CCAKE_F_Y); o[2] = col(CCAKE_F_MASS_S);
+  o[3] = pr * isl2; o[4] = e; o[5] = bulk * isl2;
+  o[6] = col(CCAKE_F_VX); o[7] = col(CCAKE_F_VY);
+  o[8] = s01 * isl2; o[9] = s02 * isl2;
+  o[10] = s11 * isl2; o[11] = s12 * isl2; o[12] = s22 * isl2;
+  o[13] = sl; o[14] = 0.0; o[15] = 0.0;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256) void k_eos(EosChain ch, double* __restrict__ F, int npad, int n,
                                              int32_t* __restrict__ I, const int32_t* __restrict__ perm,
@@ -113,39 +157,46 @@ __global__ __launch_bounds__(256) void k_eos(EosChain ch, double* __restrict__ F
   if (MODE == 1) { col(CCAKE_F_IN_S) = th[5]; return; }
   if (MODE == 2) return;
 
-  // ---- 05 reset_pi_tensor, D = 2 (fixed-size u has u^eta = 0) ----
-  const double gamma = col(CCAKE_F_GAMMA), sigma = col(CCAKE_F_SIGMA);
-  const double t2 = t * t;
-  const double ux = col(CCAKE_F_UX), uy = col(CCAKE_F_UY);
-  const double fc0 = -ux, fc1 = -uy;
-  const double bulk = col(CCAKE_F_EXT_BULK) * sigma;
-  col(CCAKE_F_BULK) = bulk;
-  const double s11 = col(CCAKE_F_EXT_SHV0 + 0) * sigma, s12 = col(CCAKE_F_EXT_SHV0 + 1) * sigma;
-  const double s13 = col(CCAKE_F_EXT_SHV0 + 2) * sigma, s22 = col(CCAKE_F_EXT_SHV0 + 3) * sigma;
-  const double s23 = col(CCAKE_F_EXT_SHV0 + 4) * sigma;
-  const double s00 = (fc0 * fc0 * s11 + fc1 * fc1 * s22 + 2. * fc1 * fc0 * s12) / (gamma * gamma);
-  const double s33 = (s00 - s11 - s22) / t2;
-  const double s01 = -s11 * fc0 / gamma + -s12 * fc1 / gamma;
-  const double s02 = -s12 * fc0 / gamma + -s22 * fc1 / gamma;
-  const double s03 = -s13 * fc0 / gamma + -s23 * fc1 / gamma;
-  const double S[16] = { s00, s01, s02, s03, s01, s11, s12, s13, s02, s12, s22, s23, s03, s13, s23, s33 };
-  for (int q = 0; q < 16; ++q) col(CCAKE_F_SHV0 + q) = S[q];
+  stage_post(F, npad, i, k, t, P, th[0], th[11], th[9], th[10], th[4], cand);
+}
 
-  // ---- 08 transport coefficients ----
-  double tc[13];
-  transport(P, th[0], th[11], th[9], th[10], tc);
-  for (int q = 0; q < 13; ++q) col(CCAKE_F_ETA_PI + q) = tc[q];
-
-  // ---- gradient candidate record: x, y, m_s, p/sl^2, e, Pi/sl^2, v, pi^{0i}/sl^2, pi^{ij}/sl^2
-  const double sl = col(CCAKE_F_SIGMA_LAB);
-  const double isl2 = 1.0 / (sl * sl);
-  double* o = cand + (size_t)k * kCandG;
-  o[0] = col(CCAKE_F_X); o[1] = col(CCAKE_F_Y); o[2] = col(CCAKE_F_MASS_S);
-  o[3] = th[9] * isl2; o[4] = th[4]; o[5] = bulk * isl2;
-  o[6] = col(CCAKE_F_VX); o[7] = col(CCAKE_F_VY);
-  o[8] = s01 * isl2; o[9] = s02 * isl2;
-  o[10] = s11 * isl2; o[11] = s12 * isl2; o[12] = s22 * isl2;
-  o[13] = sl; o[14] = 0.0; o[15] = 0.0;
+// The EoS stage with the pre-inverted table (online_inverter_enabled =
+// false): MODE 0 fill_thermodynamics and the fused tail, MODE 2 the lookup
+// only (SPHWorkstation::update_all_particle_thermodynamics,
+// src/sph_workstation.cpp:1730-1734), MODE 1 the initial e -> s inversion
+// (invert_e_to_s_host and query_thermo_host, src/sph_workstation.cpp:290-305).
+// Only the fields the reference's lookups write change.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_preinv(Preinv pre, double* __restrict__ F, int npad, int n,
+                                                const int32_t* __restrict__ perm, double t, Transport P,
+                                                double* __restrict__ cand)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = (MODE == 0) ? perm[k] : k;
+  auto col = [&](int c) -> double& { return F[(size_t)c * npad + i]; };
+  if (MODE == 1) {
+    const double rhoB = col(CCAKE_F_IN_B);
+    const double s = preinv_e_to_s(pre, col(CCAKE_F_IN_E), rhoB);
+    double o[10];
+    preinv_query(pre, s, rhoB, o);
+    col(CCAKE_F_T) = o[0]; col(CCAKE_F_MUB) = o[1]; col(CCAKE_F_MUS) = o[2]; col(CCAKE_F_MUQ) = o[3];
+    col(CCAKE_F_E) = o[4]; col(CCAKE_F_P) = o[5]; col(CCAKE_F_CS2) = o[6];
+    col(CCAKE_F_S) = s; col(CCAKE_F_RHOB) = rhoB; col(CCAKE_F_RHOS) = 0.0; col(CCAKE_F_RHOQ) = 0.0;
+    col(CCAKE_F_W) = o[9];
+    col(CCAKE_F_DWDS) = o[7]; col(CCAKE_F_DWDB) = o[8]; col(CCAKE_F_DWDS_S) = 0.0; col(CCAKE_F_DWDQ) = 0.0;
+    col(CCAKE_F_EOS_IS_TABLE) = 1.0;
+    col(CCAKE_F_IN_S) = s;
+    return;
+  }
+  double o[12];
+  preinv_fill(pre, col(CCAKE_F_S), col(CCAKE_F_RHOB), o);
+  col(CCAKE_F_T) = o[0]; col(CCAKE_F_MUB) = o[1]; col(CCAKE_F_MUS) = o[2]; col(CCAKE_F_MUQ) = o[3];
+  col(CCAKE_F_E) = o[4]; col(CCAKE_F_P) = o[5]; col(CCAKE_F_CS2) = o[6];
+  col(CCAKE_F_DWDS) = o[7]; col(CCAKE_F_DWDB) = o[8]; col(CCAKE_F_DWDS_S) = o[9]; col(CCAKE_F_DWDQ) = o[10];
+  col(CCAKE_F_W) = o[11];
+  if (MODE == 2) return;
+  stage_post(F, npad, i, k, t, P, o[0], o[11], o[5], o[6], o[4], cand);
 }
 
 static Transport make_transport(const ccake_params& p)
@@ -175,6 +226,17 @@ int launch_eos(ccake_event* ev, int mode, int fuse_post)
   const dim3 g((n + kEosThreads - 1) / kEosThreads), b(kEosThreads);
   hipStream_t st = ev->stream;
   const Transport P = make_transport(ev->p);
+  if (ev->eos->chain.kind == CCAKE_EOS_PREINVERTED) {
+    const dim3 g2((n + 255) / 256), b2(256);
+    const Preinv& pr = ev->eos->pre;
+    if (mode == 1)
+      hipLaunchKernelGGL(k_preinv<1>, g2, b2, 0, st, pr, ev->F, ev->npad, n, ev->perm, ev->t, P, ev->cand_g);
+    else if (fuse_post)
+      hipLaunchKernelGGL(k_preinv<0>, g2, b2, 0, st, pr, ev->F, ev->npad, n, ev->perm, ev->t, P, ev->cand_g);
+    else
+      hipLaunchKernelGGL(k_preinv<2>, g2, b2, 0, st, pr, ev->F, ev->npad, n, ev->perm, ev->t, P, ev->cand_g);
+    return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+  }
   if (mode == 1)
     hipLaunchKernelGGL(k_eos<1>, g, b, 0, st, ev->eos->chain, ev->F, ev->npad, n, ev->I, ev->perm, ev->t, P,
                        ev->cand_g, ev->err);

@@ -83,3 +83,37 @@ def tanh_conformal_table(dims=(64, 32, 32, 32), T_range=(0.05, 0.8), mu_max=0.45
     for i, T in enumerate(axes[0]):
         tab[i] = tanh_conformal_fields(np.full(B.shape, T), B, Q, S)
     return tab.reshape(-1), list(dims), mins, maxs
+
+
+def preinverted_flat(fields_fm, s_axis, nB_axis):
+    """A flat (s, nB) pre-inverted table (the input of CCAKE_EOS_PREINVERTED,
+    the content of the reference's flat HDF5 file, src/eos_interpolator.cpp:
+    291-527) from fields given in fm units on the (s, nB) grid.
+    fields_fm: dict of (Ns, NB) arrays T, muB, muS, muQ [fm^-1], e, p [fm^-4],
+    cs2, chiTT, chiTB, chiBB [fm^-2]; s_axis, nB_axis in fm^-3.
+    Returns (dims, table): table is 12 x Ns x NB in MeV units (MeV^3, MeV,
+    MeV^4, MeV^2) in the order s, nB, T, muB, muS, muQ, e, p, cs2, chiTT,
+    chiTB, chiBB."""
+    hc = HBARC_MEVFM
+    Ns, NB = len(s_axis), len(nB_axis)
+    S, B = np.meshgrid(np.asarray(s_axis, float), np.asarray(nB_axis, float), indexing="ij")
+    f = {k: np.broadcast_to(np.asarray(v, float), (Ns, NB)) for k, v in fields_fm.items()}
+    planes = [S * hc ** 3, B * hc ** 3, f["T"] * hc, f["muB"] * hc, f["muS"] * hc, f["muQ"] * hc,
+              f["e"] * hc ** 4, f["p"] * hc ** 4, f["cs2"], f["chiTT"] * hc ** 2, f["chiTB"] * hc ** 2,
+              f["chiBB"] * hc ** 2]
+    return (Ns, NB, 1, 1), np.ascontiguousarray(np.stack(planes))
+
+
+def preinverted_conformal(Ns=600, NB=4, s_range=(1e-3, 1e4), nB_range=(1e-4, 1e-1), c=C_CONF):
+    """The conformal EoS at zero chemical potential (p = c T^4, s = 4 c T^3)
+    as a flat pre-inverted table: T = (s/4c)^(1/3) on every nB column,
+    chiTT = 12 c T^2, chiTB = 0, chiBB = 2 c T^2 (the mu^2 curvature of a
+    free gas), cs2 = 1/3."""
+    s = np.logspace(np.log10(s_range[0]), np.log10(s_range[1]), Ns)
+    nB = np.logspace(np.log10(nB_range[0]), np.log10(nB_range[1]), NB)
+    T = (s / (4.0 * c)) ** (1.0 / 3.0)
+    T2 = np.repeat(T[:, None], NB, axis=1)
+    z = np.zeros_like(T2)
+    fields = dict(T=T2, muB=z, muS=z, muQ=z, e=3.0 * c * T2 ** 4, p=c * T2 ** 4, cs2=np.full_like(T2, 1.0 / 3.0),
+                  chiTT=12.0 * c * T2 ** 2, chiTB=z, chiBB=2.0 * c * T2 ** 2)
+    return preinverted_flat(fields, s, nB)

@@ -108,7 +108,9 @@ enum ccake_ifield {
 /* EoS kinds; a chain is built as in src/eos_initialization.cpp:216-501. */
 enum ccake_eos_kind {
   CCAKE_EOS_CONFORMAL = 0,   /* chain: conformal -> conformal_diagonal            */
-  CCAKE_EOS_TABLE = 1        /* chain: table -> tanh_conformal -> conformal -> conformal_diagonal */
+  CCAKE_EOS_TABLE = 1,       /* chain: table -> tanh_conformal -> conformal -> conformal_diagonal */
+  CCAKE_EOS_PREINVERTED = 2  /* pre-inverted flat (s, nB) table, online_inverter_enabled = false
+                                (EoS_Interpolator, include/eos_interpolator.h) */
 };
 
 /* Transport-coefficient modes (src/transport_coefficients.cpp:9-283). */
@@ -151,7 +153,16 @@ typedef struct ccake_params {
  * uniform 4D grid over (T, muB, muQ, muS), T slowest, 17 fields per node in
  * the order p, s, B, S, Q, e, cs2, chiBB, chiQQ, chiSS, chiBQ, chiBS, chiQS,
  * chiTB, chiTQ, chiTS, chiTT (src/eos_table.cpp:30-34), already converted to
- * fm units (no 1/T^n normalisation). */
+ * fm units (no 1/T^n normalisation).
+ * For CCAKE_EOS_PREINVERTED (EoS_Interpolator::load_flat_table,
+ * src/eos_interpolator.cpp:291-527) dims[0] = Ns, dims[1] = NB and the table
+ * is the content of the reference's flat HDF5 file: 12 arrays of Ns*NB
+ * doubles (s slowest) in the order s, nB, T, muB, muS, muQ, e, p, cs2, chiTT,
+ * chiTB, chiBB, in its MeV units (MeV^3, MeV, MeV^4, MeV^2; cs2 plain).  The
+ * s and nB axes (row 0 / column 0 of the s and nB arrays) must be positive
+ * and uniform in log10 to 1e-6; mins, maxs and dims[2..3] are not read.
+ * Per step the lookup replaces the Newton solve; initialisation inverts e
+ * by bisection on the table; ccake_eos_efreeze reads T(s, smallest nB). */
 typedef struct ccake_eos_desc {
   int    kind;
   double T_freeze_out;        /* fm^-1; sets the fallback matching scale     */

@@ -99,6 +99,7 @@ def lib():
             "orc_eos_members": (C.c_int, [vp]),
             "orc_eos_full_thermo": (None, [vp, C.c_int, dp, dp]),
             "orc_eos_efreeze": (C.c_double, [vp, C.c_double]),
+            "orc_preinv_point": (None, [vp, C.c_int, C.c_double, C.c_double, dp]),
             "orc_eos_invert_point": (C.c_int, [vp, C.c_int, dp, dp, C.POINTER(C.c_int), dp]),
             "orc_event_create": (vp, [C.POINTER(OrcParams), vp, C.c_int]),
             "orc_event_destroy": (None, [vp]),
@@ -157,6 +158,8 @@ class Eos:
                 d.dims[a] = int(dims[a]); d.mins[a] = float(mins[a]); d.maxs[a] = float(maxs[a])
             d.table = self._table.ctypes.data_as(C.POINTER(C.c_double))
         self.h = lib().orc_eos_create(C.byref(d))
+        if not self.h:
+            raise ValueError("the oracle rejected this EoS description")
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -171,6 +174,13 @@ class Eos:
 
     def efreeze(self, T):
         return lib().orc_eos_efreeze(self.h, T)
+
+    def preinv_point(self, which, a, b):
+        """Pre-inverted table queries: 0 fill_thermodynamics, 1
+        query_thermo_host, 2 invert_e_to_s_host (returns s)."""
+        out = (C.c_double * 12)()
+        lib().orc_preinv_point(self.h, which, a, b, out)
+        return out[0] if which == 2 else np.array(out[:12 if which == 0 else 10])
 
     def invert_point(self, mode, target, seed, member=0):
         tg = (C.c_double * 4)(*target)

@@ -68,7 +68,11 @@ enum orc_field {
 
 enum orc_ifield { ORC_I_FREEZE = 0, ORC_I_BTRACK, ORC_I_EOS_ID, ORC_NUM_IFIELDS };
 
-enum { ORC_EOS_CONFORMAL = 0, ORC_EOS_TABLE = 1 };
+enum { ORC_EOS_CONFORMAL = 0, ORC_EOS_TABLE = 1, ORC_EOS_PREINVERTED = 2 };
+/* ORC_EOS_PREINVERTED: the flat (s, nB) pre-inverted table.  dims[0] = Ns,
+ * dims[1] = NB; table = 12 arrays of Ns*NB doubles (s slowest): s, nB, T,
+ * muB, muS, muQ, e, p, cs2, chiTT, chiTB, chiBB, in the MeV units of the
+ * reference's HDF5 file.  orc_eos_create returns NULL for a table it rejects. */
 
 /* Same layout as the product's ccake_params. */
 typedef struct orc_params {
@@ -120,6 +124,11 @@ void     orc_eos_destroy(orc_eos* e);
 int      orc_eos_members(orc_eos* e);
 void     orc_eos_full_thermo(orc_eos* e, int member, const double tbqs[4], double out[17]);
 double   orc_eos_efreeze(orc_eos* e, double T_fo);
+/* pre-inverted table point queries: which = 0 fill_thermodynamics
+ * (out: T, muB, muS, muQ, e, p, cs2, dwds, dwdB, dwdS, dwdQ, w), 1
+ * query_thermo_host (T, muB, muS, muQ, e, p, cs2, dwds, dwdB, w), 2
+ * invert_e_to_s_host (out[0] = s); (a, b) = (s or e, rhoB). */
+void     orc_preinv_point(orc_eos* e, int which, double a, double b, double out[12]);
 /* invert one point: mode 0 -> target (s,B,S,Q), mode 1 -> (e,B,S,Q).
  * seed/out tbqs = (T, muB, muQ, muS); *member in: start member, out: member used.
  * thermo31: the 31 thermodynamic_info doubles. Returns 1 on success. */

@@ -458,6 +458,7 @@ void orc_smooth_gradients(orc_event* ev)
  * first failure status (EoS_NO_SOLUTION) or OK.                         */
 int orc_eos_invert(orc_event* ev, int mode)
 {
+  if (ev->eos->pre) return orc_preinv_invert(ev, mode);   /* online_inverter_enabled = false */
   int fail = 0;
   #pragma omp parallel for schedule(dynamic, 64) num_threads(orc_nthreads()) reduction(|:fail)
   for (int i = 0; i < ev->n; ++i) {

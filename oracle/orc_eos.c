@@ -543,6 +543,12 @@ orc_eos* orc_eos_create(const orc_eos_desc* d)
 {
   orc_eos* e = (orc_eos*)calloc(1, sizeof(orc_eos));
   e->kind = d->kind;
+  if (d->kind == ORC_EOS_PREINVERTED) {      /* EoS_Interpolator, flat (s, nB) format */
+    e->n = 0;
+    e->pre = orc_preinv_create(d);
+    if (!e->pre) { free(e); return NULL; }
+    return e;
+  }
   const double Nc = 3.0, Nf = 2.5;
   const double c0 = ORC_PI*ORC_PI*(2.0*(Nc*Nc - 1.0) + (7.0/2.0)*Nc*Nf)/90.0;
   if (d->kind == ORC_EOS_CONFORMAL) {
@@ -592,6 +598,7 @@ void orc_eos_destroy(orc_eos* e)
 {
   if (!e) return;
   for (int k = 0; k < e->n; ++k) free(e->m[k].tab);
+  orc_preinv_destroy(e->pre);
   free(e);
 }
 
@@ -605,6 +612,7 @@ void orc_eos_full_thermo(orc_eos* e, int member, const double tbqs[4], double ou
 /* EquationOfState::efreeze: e(T_FO, 0, 0, 0) of the default member. */
 double orc_eos_efreeze(orc_eos* e, double T_fo)
 {
+  if (e->pre) return orc_preinv_efreeze(e->pre, T_fo);   /* EoS_Interpolator::efreeze_at_T */
   double r[17], pt[4] = { T_fo, 0, 0, 0 };
   orc_member_full(&e->m[0], pt, r);
   return r[X_E];

@@ -34,11 +34,27 @@ typedef struct orc_member {
   double* tab;                        /* node-major, 17 fields per node      */
 } orc_member;
 
+/* flat pre-inverted table (EoS_Interpolator, include/eos_interpolator.h),
+ * variables in the reference's eos_variables order */
+enum { PV_T = 0, PV_MUB, PV_MUQ, PV_MUS, PV_E, PV_P, PV_CS2, PV_DWDS, PV_DWDB, PV_DWDQ, PV_DWDS_S, PV_NUM };
+typedef struct orc_preinv {
+  int Ns, NB;
+  double ls0, dls, lb0, dlb;       /* log10 axis start and stride */
+  double *s_axis, *nB_axis;        /* fm^-3 */
+  double* v[PV_NUM];               /* Ns*NB each, s slowest, fm units */
+} orc_preinv;
+
 struct orc_eos {
   int kind;
   int n;
   orc_member m[4];
+  orc_preinv* pre;
 };
+
+orc_preinv* orc_preinv_create(const orc_eos_desc* d);
+void orc_preinv_destroy(orc_preinv* p);
+double orc_preinv_efreeze(const orc_preinv* p, double T_fm);
+int orc_preinv_invert(orc_event* ev, int mode);
 
 /* chi indices inside the 17-field thermo vector (src/eos_table.cpp:30-34) */
 enum { X_P = 0, X_S, X_B, X_SS, X_Q, X_E, X_CS2, X_BB, X_QQ, X_SSS, X_BQ, X_BS,
