@@ -75,6 +75,10 @@ PARAM_FIELDS = [
     ("delta_PiPi_mode", C.c_int), ("lambda_Pipi_mode", C.c_int), ("phi1_mode", C.c_int),
     ("phi3_mode", C.c_int),
     ("cells_per_rc", C.c_int),
+    ("using_diffusion", C.c_int), ("diffusion_mode", C.c_int), ("diffusion_relax_mode", C.c_int),
+    ("kappa_over_T2", C.c_double * 9), ("C_B", C.c_double),
+    ("diffusion_regulator", C.c_int), ("regulator_chi0", C.c_double), ("regulator_e0", C.c_double),
+    ("regulator_xi0", C.c_double), ("regulator_rq_max", C.c_double),
 ]
 
 
@@ -169,8 +173,9 @@ def _ok(st, what):
 
 def make_params(cfg):
     p = Params()
-    for name, _ in PARAM_FIELDS:
-        setattr(p, name, cfg[name])
+    for name, typ in PARAM_FIELDS:
+        v = cfg[name]
+        setattr(p, name, typ(*v) if hasattr(typ, "_length_") else v)
     return p
 
 

@@ -18,6 +18,8 @@ ZETA_DEFAULT, ZETA_CONSTANT, ZETA_CS2_DEPENDENT, ZETA_GAUSSIAN = 0, 1, 2, 3
 MODE_DEFAULT, MODE_DISABLED, MODE_IS = 0, 1, 2
 
 EOS_CONFORMAL, EOS_TABLE = 0, 1
+DIFF_KAPPA_DEFAULT, DIFF_KAPPA_B_COTH = 0, 2
+TAUQ_DEFAULT, TAUQ_CB_OVER_T = 0, 1
 
 
 def base_params(**kw):
@@ -37,6 +39,9 @@ def base_params(**kw):
         delta_PiPi_mode=MODE_IS, lambda_Pipi_mode=MODE_DISABLED, phi1_mode=MODE_DISABLED,
         phi3_mode=MODE_DISABLED,
         cells_per_rc=3,
+        using_diffusion=0, diffusion_mode=DIFF_KAPPA_DEFAULT, diffusion_relax_mode=TAUQ_DEFAULT,
+        kappa_over_T2=[0.0] * 9, C_B=0.0,
+        diffusion_regulator=0, regulator_chi0=10.0, regulator_e0=0.1, regulator_xi0=0.01, regulator_rq_max=1.0,
     )
     p.update(kw)
     # input.cpp:390-407: eta/s <= 1e-3 disables shear evolution

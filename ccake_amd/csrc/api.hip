@@ -125,12 +125,14 @@ static int time_derivatives(ccake_event* ev, int fuse_rk, double dt_stage, doubl
   prof_end(ev, CCAKE_K_SMOOTH_SUM);
   prof_begin(ev, CCAKE_K_EOS);
   if ((rc = launch_eos(ev, 0, 1))) return rc;                   // 04 + 05 + 08 + gather
+  if (ev->p.using_diffusion && (rc = launch_diff_prep(ev, 15, ev->perm))) return rc;  // q, 06, kappa/tau_q
   prof_end(ev, CCAKE_K_EOS);
   prof_begin(ev, CCAKE_K_GRAD);
   if ((rc = launch_grad_pairs(ev))) return rc;                  // 09 pair sums per slice
   prof_end(ev, CCAKE_K_GRAD);
   prof_begin(ev, CCAKE_K_GRAD_SUM);
   if ((rc = launch_grad_sum(ev))) return rc;                    // 09 gradients
+  if (ev->p.using_diffusion && (rc = launch_diff_grad(ev))) return rc;   // 09 diffusion sums
   prof_end(ev, CCAKE_K_GRAD_SUM);
   prof_begin(ev, CCAKE_K_EOM);
   if ((rc = launch_eom(ev, fuse_rk, dt_stage, t0))) return rc;  // 10 (+ RK update)
@@ -497,6 +499,10 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
 {
   if (!dev || !p || !eos || !out || n <= 0) return CCAKE_ERR_INVALID;
   if (p->dim != 2) { set_error("only D = 2 is built"); return CCAKE_ERR_UNSUPPORTED; }
+  if (p->using_diffusion && (p->diffusion_mode == 1 || p->diffusion_mode > 2 || p->diffusion_relax_mode > 1)) {
+    set_error("DNMR diffusion coefficients are not built (the reference returns zero matrices for them)");
+    return CCAKE_ERR_UNSUPPORTED;
+  }
   CCAKE_HIP(hipSetDevice(dev->gpu));
   ccake_event* ev = new ccake_event();
   ev->dev = dev; ev->eos = eos; ev->p = *p; ev->n = n;
@@ -537,10 +543,12 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
   rc |= dalloc(ev, &ev->part, (size_t)ev->nslices * kMaxAcc * np);
   rc |= dalloc(ev, &ev->cand_s, 8 * np);
   rc |= dalloc(ev, &ev->cand_g, 16 * np);
+  if (p->using_diffusion) rc |= dalloc(ev, &ev->cand_d, 16 * np);
+  ev->snap_cols = p->using_diffusion ? 45 : 33;
   rc |= dalloc(ev, &ev->red, 1024);
   rc |= dalloc(ev, &ev->err, 1);
   if (p->particlization) {
-    rc |= dalloc(ev, &ev->snaps, (size_t)6 * 33 * np);
+    rc |= dalloc(ev, &ev->snaps, (size_t)6 * ev->snap_cols * np);
     rc |= dalloc(ev, &ev->fo_rows, (size_t)n * CCAKE_FO_ROW);
     rc |= dalloc(ev, &ev->fo_count, 1);
   }
@@ -551,7 +559,7 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
   CCAKE_HIP(hipMemsetAsync(ev->err, 0, sizeof(int32_t), st));
   CCAKE_HIP(hipMemsetAsync(ev->pair_count, 0, 2 * sizeof(int64_t), st));
   if (ev->snaps) {
-    CCAKE_HIP(hipMemsetAsync(ev->snaps, 0, (size_t)6 * 33 * np * sizeof(double), st));
+    CCAKE_HIP(hipMemsetAsync(ev->snaps, 0, (size_t)6 * ev->snap_cols * np * sizeof(double), st));
     CCAKE_HIP(hipMemsetAsync(ev->fo_count, 0, sizeof(int), st));
   }
   // sph_mass = 1 and gamma = 1 until the caller overrides them
@@ -575,7 +583,7 @@ int ccake_event_destroy(ccake_event* ev)
   if (ev->stream) (void)hipStreamSynchronize(ev->stream);
   void* bufs[] = { ev->F, ev->I, ev->grid, ev->bbox, ev->key, ev->key_a, ev->key_b, ev->val_a, ev->val_b, ev->hist,
                    ev->scan_tmp, ev->cell_off, ev->pair_count, ev->part, ev->cand_s, ev->cand_g, ev->red,
-                   ev->err, ev->snaps, ev->fo_rows, ev->fo_count };
+                   ev->err, ev->snaps, ev->fo_rows, ev->fo_count, ev->cand_d };
   for (void* b : bufs) if (b) (void)hipFree(b);
   if (ev->stream) (void)hipStreamDestroy(ev->stream);
   {
@@ -672,15 +680,32 @@ int ccake_eos_invert(ccake_event* ev, int mode)
   return ccake_event_check(ev);
 }
 
-int ccake_reset_pi_tensor(ccake_event* ev) { return ev ? launch_reset_pi(ev) : CCAKE_ERR_INVALID; }
-int ccake_viscosities(ccake_event* ev) { return ev ? launch_viscosities(ev) : CCAKE_ERR_INVALID; }
+int ccake_reset_pi_tensor(ccake_event* ev)
+{
+  if (!ev) return CCAKE_ERR_INVALID;
+  int rc = launch_reset_pi(ev);
+  // with diffusion: q^mu from the extensive diffusion, then the regulator
+  if (!rc && ev->p.using_diffusion) rc = launch_diff_prep(ev, 1 | 8, nullptr);
+  return rc;
+}
+
+int ccake_viscosities(ccake_event* ev)
+{
+  if (!ev) return CCAKE_ERR_INVALID;
+  int rc = launch_viscosities(ev);
+  if (!rc && ev->p.using_diffusion) rc = launch_diff_prep(ev, 2, nullptr);
+  return rc;
+}
 
 int ccake_smooth_gradients(ccake_event* ev)
 {
   if (!ev) return CCAKE_ERR_INVALID;
   int rc = ensure_sorted(ev);
   if (!rc) rc = launch_gather_grad(ev);
-  return rc ? rc : launch_grad(ev);
+  if (!rc && ev->p.using_diffusion) rc = launch_diff_prep(ev, 4, ev->perm);
+  if (!rc) rc = launch_grad(ev);
+  if (!rc && ev->p.using_diffusion) rc = launch_diff_grad(ev);
+  return rc;
 }
 
 int ccake_eom(ccake_event* ev) { return ev ? launch_eom(ev, 0, 0.0, ev->t) : CCAKE_ERR_INVALID; }
@@ -737,7 +762,7 @@ int ccake_initialize(ccake_event* ev)
   ev->fo_step = 0;
   ev->taup = ev->taupp = 0.0;
   if (ev->snaps) {                                          // a new surface
-    CCAKE_HIP(hipMemsetAsync(ev->snaps, 0, (size_t)6 * 33 * ev->npad * sizeof(double), ev->stream));
+    CCAKE_HIP(hipMemsetAsync(ev->snaps, 0, (size_t)6 * ev->snap_cols * ev->npad * sizeof(double), ev->stream));
     CCAKE_HIP(hipMemsetAsync(ev->fo_count, 0, sizeof(int), ev->stream));
   }
   if ((rc = launch_init_particles(ev, 0))) return rc;       // process_initial_conditions
@@ -748,8 +773,9 @@ int ccake_initialize(ccake_event* ev)
   if ((rc = launch_init_particles(ev, 2))) return rc;       // init_particles
   if ((rc = launch_stage_begin(ev))) return rc;             // calculate_gamma_and_velocities
   if ((rc = ccake_smooth_fields(ev))) return rc;            // initial_smoothing
-  if ((rc = launch_init_particles(ev, 3))) return rc;       // extensive shear, bulk
+  if ((rc = launch_init_particles(ev, 3))) return rc;       // extensive shear, bulk, set_diffusion
   if ((rc = launch_reset_pi(ev))) return rc;
+  if (ev->p.using_diffusion && (rc = launch_diff_prep(ev, 1, nullptr))) return rc;   // q^mu
   if ((rc = launch_eos(ev, 2, 0))) return rc;               // update_all_particle_thermodynamics
   if ((rc = launch_freeze(ev, -1))) return rc;              // check_freeze_out_status
   return ccake_event_check(ev);

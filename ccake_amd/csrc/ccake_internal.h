@@ -25,7 +25,8 @@ enum PrivateField {
   P_FB_E0 = P_C_SHV0 + 5,                          // fback..fback4 energy
   P_FB_G0 = P_FB_E0 + 4,                           // fback..fback4 gradP_x
   P_FRZ1_G, P_FRZ2_G,                              // frz1 / frz2 gradP_x
-  NUM_COLUMNS
+  P_C_Q0,                                          // RK cache: 9 extensive diffusion
+  NUM_COLUMNS = P_C_Q0 + 9
 };
 
 // Uniform cell grid of one neighbour build (written on the device).
@@ -140,6 +141,8 @@ struct ccake_event {
   // gathered candidates (sorted order)
   double* cand_s = nullptr;     // 8 doubles per particle
   double* cand_g = nullptr;     // 16 doubles per particle
+  double* cand_d = nullptr;     // diffusion pair records, 16 doubles per particle (diffusion on)
+  int snap_cols = 0;            // columns per freeze-out snapshot (33, or 45 with diffusion)
   // reductions
   double* red = nullptr;
   int32_t* err = nullptr;       // device error flags
@@ -191,6 +194,18 @@ int launch_gather_grad(ccake_event* ev);
 int host_eos_full(const ccake_eos* eos, int member, const double* x, double* r17);
 // gradients
 int launch_grad(ccake_event* ev);
+// BSQ diffusion (k_diff.hip): prep what = 1 q^mu + regulator, 2 kappa_q / tau_q,
+// 4 pair records (perm: sorted order); the diffusion sums of the gradient pass
+struct DiffParams {
+  int mode = 0, relax = 0;
+  double k2[9] = {};
+  double C_B = 0.0;
+  int reg = 0;
+  double chi0 = 0, e0 = 0, xi0 = 1, rq_max = 0, fermi_offset = 0;
+};
+DiffParams make_diff_params(const ccake_params& p);
+int launch_diff_prep(ccake_event* ev, int what, const int32_t* perm);
+int launch_diff_grad(ccake_event* ev);
 int launch_grad_pairs(ccake_event* ev);
 int launch_grad_sum(ccake_event* ev);
 // EoM + RK

@@ -67,10 +67,200 @@ __device__ __forceinline__ void hydro_numbers(const double S[4][4], double s11, 
   hn[3] = fabs(bulk) / pressure;
 }
 
-template <bool SHEAR>
+// ---------------- calculate_MRF_diffusion (src/eom_default.cpp:1416-1822), D = 2 ----------------
+// Shared set-up: tau_q^{-1}, tau_q^{-1} tau_q, tau_q^{-1} kappa_q, q^mu and
+// its covariant form, the EoS derivatives d alpha_c / d(s, n_c') and
+// grad alpha.  Charges run B, S, Q.
+struct DiffCtx {
+  double itq[3][3], itt[3][3], itk[3][3];
+  double q[3][4], qc[3][4];
+  double das[3], dar[3][3];
+  double galpha[2][3];
+};
+
+// inverse of a 3x3 matrix by cofactors
+__device__ __forceinline__ void inv3_cof(const double A[3][3], double R[3][3])
+{
+  const double c00 = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+  const double c01 = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+  const double c02 = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+  const double det = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
+  R[0][0] = c00 / det; R[1][0] = c01 / det; R[2][0] = c02 / det;
+  R[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det;
+  R[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / det;
+  R[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) / det;
+  R[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / det;
+  R[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / det;
+  R[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / det;
+}
+
+// inverse of a 3x3 matrix through its explicit determinant (used for R_N)
+__device__ __forceinline__ void inv3_det(const double A[3][3], double R[3][3])
+{
+  const double det = A[0][0] * A[1][1] * A[2][2] + A[0][1] * A[1][2] * A[2][0] + A[0][2] * A[1][0] * A[2][1]
+                   - A[0][2] * A[1][1] * A[2][0] - A[0][1] * A[1][0] * A[2][2] - A[0][0] * A[1][2] * A[2][1];
+  R[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) / det;
+  R[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det;
+  R[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / det;
+  R[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) / det;
+  R[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / det;
+  R[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / det;
+  R[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) / det;
+  R[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) / det;
+  R[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / det;
+}
+
+__device__ __forceinline__ void diff_ctx(const Cols& col, double t2, DiffCtx& x)
+{
+  double tq[3][3], kq[3][3];
+  #pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    #pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      tq[c][k] = col(CCAKE_F_TAU_Q0 + 3 * c + k);
+      kq[c][k] = col(CCAKE_F_KAPPA_Q0 + 3 * c + k);
+      x.dar[c][k] = col(CCAKE_F_DALPHA0 + 4 * c + 1 + k);
+    }
+    x.das[c] = col(CCAKE_F_DALPHA0 + 4 * c);
+    #pragma unroll
+    for (int m = 0; m < 4; ++m) x.q[c][m] = col(CCAKE_F_Q0 + 4 * c + m);
+    x.qc[c][0] = x.q[c][0]; x.qc[c][1] = -x.q[c][1]; x.qc[c][2] = -x.q[c][2]; x.qc[c][3] = x.q[c][3] * -t2;
+    x.galpha[0][c] = col(CCAKE_F_GRAD_ALPHA0 + c);
+    x.galpha[1][c] = col(CCAKE_F_GRAD_ALPHA0 + 3 + c);
+  }
+  inv3_cof(tq, x.itq);
+  #pragma unroll
+  for (int a = 0; a < 3; ++a)
+    #pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      double tt = 0.0, tk = 0.0;
+      #pragma unroll
+      for (int k = 0; k < 3; ++k) { tt += x.itq[a][k] * tq[k][b]; tk += x.itq[a][k] * kq[k][b]; }
+      x.itt[a][b] = tt; x.itk[a][b] = tk;
+    }
+}
+
+// The q^0 MRFs: F_q0[c], R_q0[c][c'], M_q0[j][c].
+__device__ __forceinline__ void diff_q0(const DiffCtx& x, double t, double gamma, double sigma, double s,
+                                        const double rho[3], const double u[2], double divV, const double M_S[2],
+                                        double F_S, const double R_S[3], double F_q0[3], double R_q0[3][3],
+                                        double M_q0[2][3])
+{
+  const double g2 = gamma * gamma, gf = 0.0, ueta = 0.0;
+  const double uc[2] = { -u[0], -u[1] };
+  const double expf = gamma * divV + gf + gamma / t;
+  double F0D[3], F0a[3], A0a[3], R0a[3][3], M0a[3][2], M0D[3][2];
+  #pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    F0D[c] = g2 * gf * x.q[c][0] + 2. * g2 * x.qc[c][3] * ueta / t + t * gamma * x.qc[c][0] * ueta * ueta
+           + t * ueta * x.q[c][3];
+    F0a[c] = (1. - g2) * sigma * x.das[c] * F_S;
+    A0a[c] = -(1. - g2) * x.das[c] * s / gamma;
+    #pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      M0D[c][d] = g2 * (x.qc[c][d + 1] - x.qc[c][0] * uc[d] / gamma);
+      M0a[c][d] = (1. - g2) * sigma * x.das[c] * M_S[d];
+      F0a[c] += -x.galpha[d][c] * u[d] / gamma;
+    }
+    #pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      R0a[c][k] = (1. - g2) * sigma * (x.das[c] * R_S[k] + x.dar[c][k]);
+      A0a[c] += -(1. - g2) * x.dar[c][k] * rho[k] / gamma;
+    }
+  }
+  #pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    #pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      double m = 0.0;
+      #pragma unroll
+      for (int k = 0; k < 3; ++k)
+        m += (-x.itt[c][k] * M0D[k][d] + x.itk[c][k] * M0a[k][d] - x.itk[c][k] * A0a[k] * uc[d] / gamma) / (gamma)
+           + x.itt[c][k] * x.q[k][0] * uc[d] / gamma / gamma;
+      M_q0[d][c] = m;
+    }
+    double f = 0.0;
+    #pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      f += (-x.itt[c][k] * F0D[k] - x.itq[c][k] * x.q[k][0] + x.itk[c][k] * F0a[k] + x.itk[c][k] * A0a[k] * expf)
+           / (gamma)
+         - x.itt[c][k] * x.q[k][0] * expf / gamma;
+      double r = 0.0;
+      #pragma unroll
+      for (int kk = 0; kk < 3; ++kk) r += x.itk[c][kk] * R0a[kk][k] / (gamma);
+      R_q0[c][k] = r;
+    }
+    F_q0[c] = f;
+  }
+}
+
+// d(extensive q^i_c)/dt = F_q + M_q du/dt + R_q dN/dt, out[3 c + i].  F_D
+// keeps only its first two terms: the reference ends that statement with a
+// semicolon after them (src/eom_default.cpp:1744-1745).
+__device__ __forceinline__ void diff_dext_q(const DiffCtx& x, double t, double t2, double gamma, double sigma,
+                                            double s, const double rho[3], const double u[2], double divV,
+                                            const double M_S[2], double F_S, const double R_S[3], double du0,
+                                            double du1, const double dN[3], double* out)
+{
+  const double cmd[3] = { -1.0, -1.0, -1.0 / t / t };
+  const double gf = 0.0;
+  const double fsu[3] = { u[0], u[1], 0.0 };
+  const double fsuc[2] = { -fsu[0], -fsu[1] };
+  const double expf = gamma * divV + gf + gamma / t;
+  const double du[2] = { du0, du1 };
+  #pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double FD[3], Fa[3], Aa[3], MD[3][2], Ma[3][2], Ra[3][3];
+    #pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double ga = (d < 2) ? x.galpha[d][c] : 0.0;
+      FD[c] = fsu[d] * gamma * x.q[c][0] * gf + 2. * fsu[d] * x.qc[c][3] * fsu[2] * gamma / t;
+      Fa[c] = -fsu[d] * gamma * sigma * x.das[c] * F_S + cmd[d] * ga;
+      Aa[c] = fsu[d] * x.das[c] * s;
+      #pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        MD[c][j] = fsu[d] * gamma * (x.qc[c][j + 1] - x.qc[c][0] * fsuc[j] / gamma);
+        Ma[c][j] = -fsu[d] * gamma * sigma * x.das[c] * M_S[j];
+      }
+      #pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        Ra[c][k] = -fsu[d] * gamma * sigma * (x.das[c] * R_S[k] + x.dar[c][k]);
+        Aa[c] += fsu[d] * x.dar[c][k] * rho[k];
+      }
+    }
+    #pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double v = 0.0;
+      #pragma unroll
+      for (int k = 0; k < 3; ++k)
+        v += (-x.itt[c][k] * FD[k] - x.itq[c][k] * x.q[k][d + 1] + x.itk[c][k] * Fa[k]
+              + x.itk[c][k] * Aa[k] * expf) / (gamma * sigma);
+      #pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        double m = 0.0;
+        #pragma unroll
+        for (int k = 0; k < 3; ++k)
+          m += (-x.itt[c][k] * MD[k][j] + x.itk[c][k] * Ma[k][j] - x.itk[c][k] * Aa[k] * fsuc[j] / gamma)
+               / (gamma * sigma);
+        v += m * du[j];
+      }
+      #pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        double r = 0.0;
+        #pragma unroll
+        for (int kk = 0; kk < 3; ++kk) r += x.itk[c][kk] * Ra[kk][k] / (gamma * sigma);
+        v += r * dN[k];
+      }
+      out[3 * c + d] = v;
+    }
+  }
+}
+
+template <bool SHEAR, bool DIFF>
 __device__ __forceinline__ void eom_d2(const Cols& col, double t, double use_vort, double* du_out,
                                        double* d_s_out, double* dN, double* d_b_out, double* d_shv_out,
-                                       double* theta_out, double* snu_out, double* dez_out, double* hn_out)
+                                       double* theta_out, double* snu_out, double* dez_out, double* hn_out,
+                                       double* dq_out)
 {
   const double t2 = t * t;
   const double cmd3 = -1.0 / t / t;                      // contra metric (1,-1,-1,-1/t^2)
@@ -310,9 +500,31 @@ __device__ __forceinline__ void eom_d2(const Cols& col, double t, double use_vor
     }
   }
 
+  // ---------------- calculate_MRF_diffusion, compute_charge_derivative ----------------
+  // Without diffusion R_N = gamma sigma I, F_N = 0 and M_N = 0 (no sources),
+  // so the Schur terms R_u R_N^{-1} M_N and R_u R_N^{-1} F_N vanish.
+  const double rho[3] = { rB, rS, rQ };
+  DiffCtx dx;
+  double RNI[3][3], F_N[3], M_N[2][3];
+  if constexpr (DIFF) {
+    diff_ctx(col, t2, dx);
+    double F_q0[3], R_q0[3][3];
+    diff_q0(dx, t, gamma, sigma, s, rho, u, divV, M_S, F_S, R_S, F_q0, R_q0, M_N);
+    double R_N[3][3];
+    #pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      #pragma unroll
+      for (int k = 0; k < 3; ++k) R_N[c][k] = (c == k ? gamma * sigma : 0.0) + -R_q0[c][k];
+      F_N[c] = 0.0 + (-F_q0[c] + col(CCAKE_F_GRAD_QA0 + c) - col(CCAKE_F_DIV_QA0 + c) - dx.q[c][0] / t);
+    }
+    #pragma unroll
+    for (int d = 0; d < 2; ++d)
+      #pragma unroll
+      for (int c = 0; c < 3; ++c) M_N[d][c] = 0.0 + -M_N[d][c];
+    inv3_det(R_N, RNI);
+  }
+
   // ---------------- compute_velocity_derivative ----------------
-  // R_N = gamma sigma I, F_N = 0, M_N = 0 (no sources, no diffusion), so the
-  // Schur terms R_u R_N^{-1} M_N and R_u R_N^{-1} F_N vanish.
   const double gP[2] = { -col(CCAKE_F_GRADP0 + 0), -col(CCAKE_F_GRADP0 + 1) };
   const double gB[2] = { -col(CCAKE_F_GRADBULK0 + 0), -col(CCAKE_F_GRADBULK0 + 1) };
   const double rdw = rB * dwdB + rS * dwdS + rQ * dwdQ;
@@ -339,6 +551,35 @@ __device__ __forceinline__ void eom_d2(const Cols& col, double t, double use_vor
       Mu[a][1] += M_0i[a][1];
     }
   }
+  if constexpr (DIFF) {
+    // Schur complement of the charge block: M_u - R_u R_N^{-1} M_N, F_u + R_u R_N^{-1} F_N
+    const double dwdr[3] = { dwdB, dwdS, dwdQ };
+    double R_u[2][3];
+    #pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double R_w = sigma * dwdr[c] + sigma * dwds * R_S[c];
+      #pragma unroll
+      for (int a = 0; a < 2; ++a) R_u[a][c] = -u[a] * gamma * (R_w + 0.0 * sigma);
+    }
+    #pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      double afr = 0.0;
+      #pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        double amr = 0.0;
+        #pragma unroll
+        for (int c = 0; c < 3; ++c)
+          #pragma unroll
+          for (int d = 0; d < 3; ++d) amr += R_u[a][c] * RNI[c][d] * M_N[b][d];
+        Mu[a][b] = Mu[a][b] - amr;
+      }
+      #pragma unroll
+      for (int c = 0; c < 3; ++c)
+        #pragma unroll
+        for (int d = 0; d < 3; ++d) afr += R_u[a][c] * RNI[c][d] * F_N[d];
+      Fu[a] = Fu[a] + afr;
+    }
+  }
   const double det = Mu[0][0] * Mu[1][1] - Mu[0][1] * Mu[1][0];
   const double du0 = (Mu[1][1] / det) * Fu[0] + (-Mu[0][1] / det) * Fu[1];
   const double du1 = (-Mu[1][0] / det) * Fu[0] + (Mu[0][0] / det) * Fu[1];
@@ -346,7 +587,14 @@ __device__ __forceinline__ void eom_d2(const Cols& col, double t, double use_vor
 
   // ---------------- compute_derivatives ----------------
   dN[0] = 0.0; dN[1] = 0.0; dN[2] = 0.0;
-  *d_s_out = (M_S[0] * du0 + M_S[1] * du1) + F_S + (R_S[0] * 0.0 + R_S[1] * 0.0 + R_S[2] * 0.0);
+  if constexpr (DIFF) {
+    double rhs[3];
+    #pragma unroll
+    for (int c = 0; c < 3; ++c) rhs[c] = F_N[c] + ((0.0 + M_N[0][c] * du0) + M_N[1][c] * du1);
+    #pragma unroll
+    for (int c = 0; c < 3; ++c) dN[c] = ((0.0 + RNI[c][0] * rhs[0]) + RNI[c][1] * rhs[1]) + RNI[c][2] * rhs[2];
+  }
+  *d_s_out = (M_S[0] * du0 + M_S[1] * du1) + F_S + ((0.0 + R_S[0] * dN[0]) + R_S[1] * dN[1]) + R_S[2] * dN[2];
   *d_b_out = (M_eb[0] * du0 + M_eb[1] * du1) + F_eb;
   *snu_out = (M_snu[0] * du0 + M_snu[1] * du1) + F_snu;
   *theta_out = -1. * (uc[0] * du0 + uc[1] * du1) / gamma + gamma * divV + gamma / t;
@@ -362,10 +610,26 @@ __device__ __forceinline__ void eom_d2(const Cols& col, double t, double use_vor
   }
   // ---------------- compute_Ez_derivative ----------------
   const double p = col(CCAKE_F_P), e = col(CCAKE_F_E);
-  if (SHEAR) {
+  if (SHEAR || DIFF) {
     const double s11 = ((0.0 + sgM[0][0] * du0) + sgM[0][1] * du1) + sgF[0];
     const double s12 = ((0.0 + sgM[1][0] * du0) + sgM[1][1] * du1) + sgF[1];
     hydro_numbers(S, s11, s12, u[0], u[1], gamma, t2, tau_pi, tau_Pi, *theta_out, bulk, p, hn_out);
+  }
+  if constexpr (DIFF) {
+    diff_dext_q(dx, t, t2, gamma, sigma, s, rho, u, divV, M_S, F_S, R_S, du0, du1, dN, dq_out);
+    // diffusion Knudsen numbers tau_q^(cc) |grad alpha_c| and inverse
+    // Reynolds numbers |q_c| / |n_c| (compute_hydro_numbers, 1831-1995)
+    #pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double g0 = dx.galpha[0][c], g1 = dx.galpha[1][c];
+      const double ga2 = (0.0 + g0 * g0) + g1 * g1;
+      const double q_sq = dx.q[c][1] * dx.q[c][1] + dx.q[c][2] * dx.q[c][2] + t2 * dx.q[c][3] * dx.q[c][3]
+                        - dx.q[c][0] * dx.q[c][0];
+      const double q_mag = q_sq > 0.0 ? sqrt(q_sq) : 0.0;
+      const double n_abs = fabs(rho[c]);
+      hn_out[4 + c] = col(CCAKE_F_TAU_Q0 + 4 * c) * sqrt(ga2);
+      hn_out[7 + c] = (n_abs > 1e-15) ? q_mag / n_abs : 0.0;
+    }
   }
   *dez_out = ((e + p + bulk) * 0.0 * 0.0 + (p + bulk) / t2 + S[3][3]) * col(CCAKE_F_MASS_S) * t2 / col(CCAKE_F_SIGMA_LAB);
   (void)e;
@@ -374,7 +638,7 @@ __device__ __forceinline__ void eom_d2(const Cols& col, double t, double use_vor
 // mode 0: store derivatives; mode 1: store derivatives and apply the RK
 // update y = y0 + dt f (src/evolver.cpp:363-484); returns the negative
 // entropy flag through err.
-template <bool SHEAR, bool RK>
+template <bool SHEAR, bool RK, bool DIFF>
 __global__ __launch_bounds__(256) void k_eom(double* __restrict__ F, int npad, int n,
                                              const int32_t* __restrict__ freeze, double t, double use_vort,
                                              double dt, int32_t* __restrict__ err)
@@ -382,8 +646,8 @@ __global__ __launch_bounds__(256) void k_eom(double* __restrict__ F, int npad, i
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const Cols col{ F, npad, i };
-  double du[2], ds, dN[3], db, dshv[5] = { 0, 0, 0, 0, 0 }, theta, snu, dez, hn[4];
-  eom_d2<SHEAR>(col, t, use_vort, du, &ds, dN, &db, dshv, &theta, &snu, &dez, hn);
+  double du[2], ds, dN[3], db, dshv[5] = { 0, 0, 0, 0, 0 }, theta, snu, dez, hn[10], dq[9];
+  eom_d2<SHEAR, DIFF>(col, t, use_vort, du, &ds, dN, &db, dshv, &theta, &snu, &dez, hn, dq);
   col(CCAKE_F_DUDT0 + 0) = du[0]; col(CCAKE_F_DUDT0 + 1) = du[1]; col(CCAKE_F_DUDT0 + 2) = 0.0;
   col(CCAKE_F_DEXT_S) = ds;
   col(CCAKE_F_DEXT_B) = dN[0]; col(CCAKE_F_DEXT_SS) = dN[1]; col(CCAKE_F_DEXT_Q) = dN[2];
@@ -392,9 +656,13 @@ __global__ __launch_bounds__(256) void k_eom(double* __restrict__ F, int npad, i
   col(CCAKE_F_THETA) = theta;
   col(CCAKE_F_SHV_NABLA_U) = snu;
   col(CCAKE_F_DEZ) = dez;
-  if (SHEAR) {
+  if (SHEAR || DIFF) {
     col(CCAKE_F_SHEAR_KNUDSEN) = hn[0]; col(CCAKE_F_BULK_KNUDSEN) = hn[1];
     col(CCAKE_F_INV_REYNOLDS_SHEAR) = hn[2]; col(CCAKE_F_INV_REYNOLDS_BULK) = hn[3];
+  }
+  if (DIFF) {
+    for (int q = 0; q < 9; ++q) col(CCAKE_F_DEXT_Q0 + q) = dq[q];
+    for (int c = 0; c < 3; ++c) { col(CCAKE_F_KN_DIFF0 + c) = hn[4 + c]; col(CCAKE_F_RE_DIFF0 + c) = hn[7 + c]; }
   }
   if (!RK) return;
   double es = col(P_C_S) + dt * ds;
@@ -411,13 +679,15 @@ __global__ __launch_bounds__(256) void k_eom(double* __restrict__ F, int npad, i
   col(CCAKE_F_UX) = col(P_C_UX) + dt * du[0];
   col(CCAKE_F_UY) = col(P_C_UY) + dt * du[1];
   for (int q = 0; q < 5; ++q) col(CCAKE_F_EXT_SHV0 + q) = col(P_C_SHV0 + q) + dt * col(CCAKE_F_DEXT_SHV0 + q);
+  if (DIFF) for (int q = 0; q < 9; ++q) col(CCAKE_F_EXT_Q0 + q) = col(P_C_Q0 + q) + dt * dq[q];
   col(CCAKE_F_ETA) = 0.0;
   col(CCAKE_F_UETA) = 0.0;
 }
 
 // Stand-alone RK update from stored derivatives.
 __global__ __launch_bounds__(256) void k_rk(double* __restrict__ F, int npad, int n,
-                                            const int32_t* __restrict__ freeze, double dt, int32_t* __restrict__ err)
+                                            const int32_t* __restrict__ freeze, double dt, int32_t* __restrict__ err,
+                                            int diff)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -436,12 +706,13 @@ __global__ __launch_bounds__(256) void k_rk(double* __restrict__ F, int npad, in
   col(CCAKE_F_UX) = col(P_C_UX) + dt * col(CCAKE_F_DUDT0 + 0);
   col(CCAKE_F_UY) = col(P_C_UY) + dt * col(CCAKE_F_DUDT0 + 1);
   for (int q = 0; q < 5; ++q) col(CCAKE_F_EXT_SHV0 + q) = col(P_C_SHV0 + q) + dt * col(CCAKE_F_DEXT_SHV0 + q);
+  if (diff) for (int q = 0; q < 9; ++q) col(CCAKE_F_EXT_Q0 + q) = col(P_C_Q0 + q) + dt * col(CCAKE_F_DEXT_Q0 + q);
   col(CCAKE_F_ETA) = 0.0;
   col(CCAKE_F_UETA) = 0.0;
 }
 
 // Evolver::set_current_timestep_quantities (src/evolver.cpp:124-168)
-__global__ __launch_bounds__(256) void k_cache(double* __restrict__ F, int npad, int n)
+__global__ __launch_bounds__(256) void k_cache(double* __restrict__ F, int npad, int n, int diff)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -452,6 +723,7 @@ __global__ __launch_bounds__(256) void k_cache(double* __restrict__ F, int npad,
   col(P_C_S) = col(CCAKE_F_EXT_S); col(P_C_BULK) = col(CCAKE_F_EXT_BULK);
   col(P_C_B) = col(CCAKE_F_EXT_B); col(P_C_SS) = col(CCAKE_F_EXT_SS); col(P_C_Q) = col(CCAKE_F_EXT_Q);
   col(P_C_EZ) = col(CCAKE_F_EZ);
+  if (diff) for (int q = 0; q < 9; ++q) col(P_C_Q0 + q) = col(CCAKE_F_EXT_Q0 + q);
 }
 
 int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0)
@@ -462,12 +734,17 @@ int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0)
   hipStream_t st = ev->stream;
   const int32_t* fz = ev->I + (size_t)CCAKE_I_FREEZE * ev->npad;
   const double uv = ev->p.using_vorticity ? 1.0 : 0.0;
-  if (ev->p.using_shear) {
-    if (fuse_rk) hipLaunchKernelGGL((k_eom<true, true>), g, b, 0, st, ev->F, ev->npad, n, fz, ev->t, uv, dt, ev->err);
-    else hipLaunchKernelGGL((k_eom<true, false>), g, b, 0, st, ev->F, ev->npad, n, fz, ev->t, uv, dt, ev->err);
-  } else {
-    if (fuse_rk) hipLaunchKernelGGL((k_eom<false, true>), g, b, 0, st, ev->F, ev->npad, n, fz, ev->t, uv, dt, ev->err);
-    else hipLaunchKernelGGL((k_eom<false, false>), g, b, 0, st, ev->F, ev->npad, n, fz, ev->t, uv, dt, ev->err);
+  auto go = [&](auto kern) { hipLaunchKernelGGL(kern, g, b, 0, st, ev->F, ev->npad, n, fz, ev->t, uv, dt, ev->err); };
+  const int key = (ev->p.using_shear ? 4 : 0) | (fuse_rk ? 2 : 0) | (ev->p.using_diffusion ? 1 : 0);
+  switch (key) {
+    case 0: go(k_eom<false, false, false>); break;
+    case 1: go(k_eom<false, false, true>); break;
+    case 2: go(k_eom<false, true, false>); break;
+    case 3: go(k_eom<false, true, true>); break;
+    case 4: go(k_eom<true, false, false>); break;
+    case 5: go(k_eom<true, false, true>); break;
+    case 6: go(k_eom<true, true, false>); break;
+    default: go(k_eom<true, true, true>); break;
   }
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
@@ -477,14 +754,15 @@ int launch_rk(ccake_event* ev, double dt, double t0)
   (void)t0;
   const int n = ev->n;
   hipLaunchKernelGGL(k_rk, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n,
-                     ev->I + (size_t)CCAKE_I_FREEZE * ev->npad, dt, ev->err);
+                     ev->I + (size_t)CCAKE_I_FREEZE * ev->npad, dt, ev->err, ev->p.using_diffusion);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
 
 int launch_cache(ccake_event* ev)
 {
   const int n = ev->n;
-  hipLaunchKernelGGL(k_cache, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n);
+  hipLaunchKernelGGL(k_cache, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n,
+                     ev->p.using_diffusion);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
 

@@ -9,7 +9,7 @@ namespace ccake {
 // phase 3: calculate_extensive_shv (1392-1432) + set_bulk_Pi (1349-1384)
 __global__ __launch_bounds__(256) void k_init(double* __restrict__ F, int npad, int n, int32_t* __restrict__ I,
                                               int phase, double t0, double efcheck, int input_shear,
-                                              int bulk_from_trace)
+                                              int bulk_from_trace, int diffusion)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -57,6 +57,10 @@ __global__ __launch_bounds__(256) void k_init(double* __restrict__ F, int npad, 
       col(CCAKE_F_EXT_BULK) = 0.0;
       col(CCAKE_F_BULK) = 0.0;
     }
+    // set_diffusion (src/sph_workstation.cpp:1441-1459): extensive q^i = q^i / sigma
+    if (diffusion)
+      for (int d = 0; d < 3; ++d)
+        for (int c = 0; c < 3; ++c) col(CCAKE_F_EXT_Q0 + 3 * c + d) = col(CCAKE_F_Q0 + 4 * c + 1 + d) / sig;
   }
 }
 
@@ -64,15 +68,18 @@ int launch_init_particles(ccake_event* ev, int phase)
 {
   const int n = ev->n;
   hipLaunchKernelGGL(k_init, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n, ev->I, phase,
-                     ev->p.t0, ev->p.e_freeze, ev->p.input_initial_shear, ev->p.bulk_from_trace);
+                     ev->p.t0, ev->p.e_freeze, ev->p.input_initial_shear, ev->p.bulk_from_trace,
+                     ev->p.using_diffusion);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
 
 // ---- freeze-out hypersurface -------------------------------------------
 // Snapshot of one particle (FreezeOut's freeze_array members for D = 2,
-// include/freeze_out.h:29-35; the diffusion members are zero while diffusion
-// is off and are not stored): the F column of each of the kSnap entries.
+// include/freeze_out.h:29-35): the F column of each of the kSnap entries,
+// then, with diffusion on, the kSnapDiff components of q^mu for B, S, Q
+// (the diffusion members stay zero while diffusion is off and are not stored).
 constexpr int kSnap = 33;
+constexpr int kSnapDiff = 12;
 enum SnapIdx {
   SN_SHEAR = 0,                 // 3x3 pi^{ab}, a, b = 0..2, row-major
   SN_R = 9, SN_U = 11, SN_GRADP = 13, SN_GRADE = 15,
@@ -95,27 +102,34 @@ __device__ __forceinline__ int snap_col(int q)
     case 25: return CCAKE_F_MUQ;      case 26: return CCAKE_F_THETA;    case 27: return CCAKE_F_BULK;
     case 28: return CCAKE_F_SIGMA_LAB; case 29: return CCAKE_F_SHV0 + 15;
     case 30: return CCAKE_F_SHV_NABLA_U; case 31: return CCAKE_F_W;
-    default: return CCAKE_F_CS2;
+    case 32: return CCAKE_F_CS2;
+    default: return CCAKE_F_Q0 + (q - kSnap);
   }
 }
 // Snapshot sets: frz1, frz2, fback, fback2, fback3, fback4.
 enum SnapSet { SS_FRZ1 = 0, SS_FRZ2, SS_FB1, SS_FB2, SS_FB3, SS_FB4, SS_NUM };
 
 struct Snaps {
-  double* p; int npad; int i;
+  double* p; int npad; int i; int nc;    // nc: kSnap, or kSnap + kSnapDiff with diffusion
   __device__ __forceinline__ double& at(int set, int q) const
   {
-    return p[((size_t)set * kSnap + q) * npad + i];
+    return p[((size_t)set * nc + q) * npad + i];
   }
   __device__ __forceinline__ void copy(int dst, int src) const
   {
     #pragma unroll
     for (int q = 0; q < kSnap; ++q) at(dst, q) = at(src, q);
+    if (nc > kSnap)
+      #pragma unroll
+      for (int q = kSnap; q < kSnap + kSnapDiff; ++q) at(dst, q) = at(src, q);
   }
   __device__ __forceinline__ void take(int dst, const double* F) const
   {
     #pragma unroll
     for (int q = 0; q < kSnap; ++q) at(dst, q) = F[(size_t)snap_col(q) * npad + i];
+    if (nc > kSnap)
+      #pragma unroll
+      for (int q = kSnap; q < kSnap + kSnapDiff; ++q) at(dst, q) = F[(size_t)snap_col(q) * npad + i];
   }
 };
 
@@ -192,7 +206,8 @@ __device__ void interpolate_row(const Snaps& sn, const double* __restrict__ F, i
   c[19] = sn.at(set, SN_MUB); c[20] = sn.at(set, SN_MUS); c[21] = sn.at(set, SN_MUQ);
   c[22] = w; c[23] = cs2;
   c[24] = sn.at(set, SN_RHOB); c[25] = sn.at(set, SN_RHOS); c[26] = sn.at(set, SN_RHOQ);
-  for (int q = 27; q < 39; ++q) c[q] = 0.0;          // diffusion currents (off)
+  for (int q = 0; q < kSnapDiff; ++q)                 // diffout(charge, mu); zero with diffusion off
+    c[27 + q] = sn.nc > kSnap ? sn.at(set, kSnap + q) : 0.0;
   c[39] = F[(size_t)CCAKE_F_EOS_IS_TABLE * npad + i];
 }
 
@@ -204,7 +219,7 @@ __device__ void interpolate_row(const Snaps& sn, const double* __restrict__ F, i
 // live in F; with `snaps` the full snapshots follow the same shifts.
 // phase -1: status check only (initial smoothing); 0, 1, 2: the frzc stage.
 __global__ __launch_bounds__(256) void k_freeze(double* __restrict__ F, int npad, int n, int32_t* __restrict__ I,
-                                                double efcheck, int phase, double* __restrict__ snaps,
+                                                double efcheck, int phase, double* __restrict__ snaps, int snap_cols,
                                                 double* __restrict__ rows, int* __restrict__ count, int cap,
                                                 double taup, double taupp, double dt, int step)
 {
@@ -214,7 +229,7 @@ __global__ __launch_bounds__(256) void k_freeze(double* __restrict__ F, int npad
   int32_t& fz = I[(size_t)CCAKE_I_FREEZE * npad + i];
   int32_t& bt = I[(size_t)CCAKE_I_BTRACK * npad + i];
   const bool full = snaps != nullptr;
-  const Snaps sn{ snaps, npad, i };
+  const Snaps sn{ snaps, npad, i, snap_cols };
   const double e = col(CCAKE_F_E);
   switch (fz) {
     case 0: if (e <= efcheck) fz = 1; break;
@@ -277,7 +292,7 @@ int launch_freeze(ccake_event* ev, int phase)
 {
   const int n = ev->n;
   hipLaunchKernelGGL(k_freeze, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n, ev->I,
-                     ev->p.e_freeze, phase, ev->snaps, ev->fo_rows, ev->fo_count, n, ev->taup, ev->taupp,
+                     ev->p.e_freeze, phase, ev->snaps, ev->snap_cols, ev->fo_rows, ev->fo_count, n, ev->taup, ev->taupp,
                      ev->p.dt, ev->fo_step);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }

@@ -20,6 +20,7 @@
 #include <fstream>
 #include <functional>
 #include <iomanip>
+#include <iostream>
 #include <memory>
 #include <sstream>
 #include <stdexcept>
@@ -65,6 +66,18 @@ struct Settings {
   int phi6_mode = CCAKE_MODE_DISABLED, phi7_mode = CCAKE_MODE_DISABLED;
   int delta_PiPi_mode = CCAKE_MODE_IS, lambda_Pipi_mode = CCAKE_MODE_DISABLED;
   int phi1_mode = CCAKE_MODE_DISABLED, phi3_mode = CCAKE_MODE_DISABLED;
+  // BSQ diffusion (include/settings.h:195-215, src/input.cpp:570-640):
+  // on unless diffusionMode is "disabled"; modes "constant_over_T2" and
+  // "kappaB_coth" ("DNMR" is rejected); relaxation "constant_over_T" or
+  // "cb_over_T", anything else falls back to the former with a warning.
+  std::string diffusionMode = "disabled";
+  std::string diffusionRelaxMode = "default";
+  double constant_kappa_over_T2[3][3] = {};
+  double C_B = 0.0;
+  bool critical_scaling_diffusion = false;     // needs the critical-point table inputs: rejected
+  bool diffusion_regulator_enabled = false;
+  double diffusion_regulator_chi0 = 10.0, diffusion_regulator_e0 = 0.1;   // e0, xi0 in GeV/fm^3
+  double diffusion_regulator_xi0 = 0.01, diffusion_regulator_rq_max = 1.0;
   int cells_per_rc = 3;
   bool print_conservation_status = false;
   bool exit_on_error = false;
@@ -108,11 +121,11 @@ struct Particle {
   double T_ = 0, muB = 0, muS = 0, muQ = 0, e_ = 0, s = 0, rhoB = 0, rhoS = 0, rhoQ = 0, p_ = 0, cs2 = 0, w = 0;
   double Bulk = 0;
   double shv[4][4] = {};              // also the initial pi^{mu nu} an IC reader sets
-  double diffusion[3][4] = {};        // initial diffusion currents (read, not evolved)
+  double diffusion[3][4] = {};        // q^mu of B, S, Q: the initial currents, then the evolved ones
   // transport coefficients and diagnostics (include/hydrodynamic_info.h:29-62)
   double eta_pi = 0, zeta_Pi = 0, tau_Pi = 0, tau_pi = 0, theta = 0;
   double shear_knudsen = 0, bulk_knudsen = 0, inverse_reynolds_shear = 0, inverse_reynolds_bulk = 0;
-  double knudsen_diffusion[3] = {}, inverse_reynolds_diffusion[3] = {};   // zero: diffusion is off
+  double knudsen_diffusion[3] = {}, inverse_reynolds_diffusion[3] = {};   // zero with diffusion off
   int causality = 0;                  // check_causality is not on this path
   std::string eos_name = "default";   // the host name the reference prints (never set from the device)
   int Freeze = 0, btrack = 0, eos_id = 0;
@@ -210,6 +223,10 @@ class SystemState {
     up(CCAKE_F_BULK, [](const Particle<D>& q) { return q.Bulk; });
     for (int c = 0; c < 16; ++c)
       up(CCAKE_F_SHV0 + c, [c](const Particle<D>& q) { return q.shv[c / 4][c % 4]; });
+    // initial q^mu (zero unless an IC reader set it): initialize derives the
+    // extensive currents from it when diffusion is on
+    for (int c = 0; c < 12; ++c)
+      up(CCAKE_F_Q0 + c, [c](const Particle<D>& q) { return q.diffusion[c / 4][c % 4]; });
     check(ccake_event_set_time(ev, t), "ccake_event_set_time", settingsPtr.get());
   }
 
@@ -261,6 +278,14 @@ class SystemState {
     for (int c = 0; c < 16; ++c) {
       check(ccake_event_download(ev, CCAKE_F_SHV0 + c, a.data(), a.size()), "ccake_event_download", settingsPtr.get());
       for (int i = 0; i < n_particles; ++i) particles[i].shv[c / 4][c % 4] = a[i];
+    }
+    if (settingsPtr->diffusionMode != "disabled") {
+      for (int c = 0; c < 12; ++c)
+        dn(CCAKE_F_Q0 + c, [c](Particle<D>& q, double v) { q.diffusion[c / 4][c % 4] = v; });
+      for (int c = 0; c < 3; ++c) {
+        dn(CCAKE_F_KN_DIFF0 + c, [c](Particle<D>& q, double v) { q.knudsen_diffusion[c] = v; });
+        dn(CCAKE_F_RE_DIFF0 + c, [c](Particle<D>& q, double v) { q.inverse_reynolds_diffusion[c] = v; });
+      }
     }
     auto dni = [&](int f, auto set) {
       check(ccake_event_download_int(ev, f, b.data(), b.size()), "ccake_event_download_int", settingsPtr.get());
@@ -326,6 +351,34 @@ class SystemState {
     p.phi6_mode = s.phi6_mode; p.phi7_mode = s.phi7_mode; p.delta_PiPi_mode = s.delta_PiPi_mode;
     p.lambda_Pipi_mode = s.lambda_Pipi_mode; p.phi1_mode = s.phi1_mode; p.phi3_mode = s.phi3_mode;
     p.cells_per_rc = s.cells_per_rc;
+    // diffusion (src/transport_coefficients.cpp:82-115, src/input.cpp:577-640)
+    p.using_diffusion = s.diffusionMode != "disabled";
+    if (p.using_diffusion) {
+      if (s.diffusionMode == "DNMR" || s.critical_scaling_diffusion)
+        throw Error(CCAKE_ERR_UNSUPPORTED, "diffusion: DNMR kappa and critical scaling are not on this path");
+      if (s.diffusionMode == "kappaB_coth") {
+        if (!(s.baryon_charge_enabled && !s.strange_charge_enabled && !s.electric_charge_enabled))
+          throw Error(CCAKE_ERR_INVALID, "diffusion mode 'kappaB_coth' requires a baryon-only charge configuration");
+        p.diffusion_mode = CCAKE_DIFF_KAPPA_B_COTH;
+      } else {
+        if (s.diffusionMode != "constant_over_T2")
+          std::cout << "WARNING: Unknown diffusion mode: " << s.diffusionMode << "\nDefaulting to constant_over_T2.\n";
+        p.diffusion_mode = CCAKE_DIFF_KAPPA_DEFAULT;
+      }
+      if (s.diffusionRelaxMode == "cb_over_T") p.diffusion_relax_mode = CCAKE_TAUQ_CB_OVER_T;
+      else {
+        if (s.diffusionRelaxMode != "constant_over_T")
+          std::cout << "WARNING: Unknown diffusion relaxation mode: " << s.diffusionRelaxMode
+                    << "\nDefaulting to constant_over_T (tau_q = 0.2/T).\n";
+        p.diffusion_relax_mode = CCAKE_TAUQ_DEFAULT;
+      }
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) p.kappa_over_T2[3 * a + b] = s.constant_kappa_over_T2[a][b];
+      p.C_B = s.C_B;
+      p.diffusion_regulator = s.diffusion_regulator_enabled;
+      p.regulator_chi0 = s.diffusion_regulator_chi0; p.regulator_e0 = s.diffusion_regulator_e0;
+      p.regulator_xi0 = s.diffusion_regulator_xi0; p.regulator_rq_max = s.diffusion_regulator_rq_max;
+    }
     return p;
   }
 

@@ -94,7 +94,21 @@ enum ccake_field {
   /* Knudsen and inverse Reynolds numbers, written by the EoM when shear is on
      (EoM_default::compute_hydro_numbers, src/eom_default.cpp:1831-1995) */
   CCAKE_F_SHEAR_KNUDSEN, CCAKE_F_BULK_KNUDSEN, CCAKE_F_INV_REYNOLDS_SHEAR, CCAKE_F_INV_REYNOLDS_BULK,
-  CCAKE_NUM_FIELDS
+  /* BSQ diffusion: q^mu of B, S, Q (4 each); extensive q^i and its time derivative
+     (3 per charge); kappa_q and tau_q (3x3); the gradient-pass sums grad q^0, div q
+     (per charge) and grad alpha (direction-major, 2 x 3); Knudsen and inverse
+     Reynolds numbers per charge */
+  CCAKE_F_Q0,
+  CCAKE_F_EXT_Q0 = CCAKE_F_Q0 + 12,
+  CCAKE_F_DEXT_Q0 = CCAKE_F_EXT_Q0 + 9,
+  CCAKE_F_KAPPA_Q0 = CCAKE_F_DEXT_Q0 + 9,
+  CCAKE_F_TAU_Q0 = CCAKE_F_KAPPA_Q0 + 9,
+  CCAKE_F_GRAD_QA0 = CCAKE_F_TAU_Q0 + 9,
+  CCAKE_F_DIV_QA0 = CCAKE_F_GRAD_QA0 + 3,
+  CCAKE_F_GRAD_ALPHA0 = CCAKE_F_DIV_QA0 + 3,
+  CCAKE_F_KN_DIFF0 = CCAKE_F_GRAD_ALPHA0 + 6,
+  CCAKE_F_RE_DIFF0 = CCAKE_F_KN_DIFF0 + 3,
+  CCAKE_NUM_FIELDS = CCAKE_F_RE_DIFF0 + 3
 };
 
 /* Integer per-particle fields (upload_int / download_int). */
@@ -120,6 +134,9 @@ enum ccake_zeta_mode  { CCAKE_ZETA_DEFAULT = 0, CCAKE_ZETA_CONSTANT = 1,
                         CCAKE_ZETA_CS2_DEPENDENT = 2,
                         CCAKE_ZETA_GAUSSIAN = 3 /* zeta/s = A exp(-(T-T0)^2/(2 w^2)); not in the reference */ };
 enum ccake_onoff_mode { CCAKE_MODE_DEFAULT = 0, CCAKE_MODE_DISABLED = 1, CCAKE_MODE_IS = 2 };
+/* diffusion_mode / diffusion_relax_mode (src/transport_coefficients.cpp:82-115) */
+enum ccake_diff_mode  { CCAKE_DIFF_KAPPA_DEFAULT = 0, CCAKE_DIFF_KAPPA_DNMR = 1, CCAKE_DIFF_KAPPA_B_COTH = 2 };
+enum ccake_tauq_mode  { CCAKE_TAUQ_DEFAULT = 0, CCAKE_TAUQ_CB_OVER_T = 1 };
 
 /* Settings consumed by the time-step path (a POD restatement of the fields
  * of include/settings.h the path reads). */
@@ -147,6 +164,15 @@ typedef struct ccake_params {
   int    delta_PiPi_mode, lambda_Pipi_mode, phi1_mode, phi3_mode;
   /* neighbour search tuning: cells per interaction radius (1 = reference) */
   int    cells_per_rc;
+  /* BSQ diffusion (transport: src/transport_coefficients.cpp:571-620, 917-1059;
+     regulator: src/sph_workstation.cpp:108-218, PhysRevC.98.034916 Eqs. C6-C8) */
+  int    using_diffusion;
+  int    diffusion_mode;          /* 0 kappa = (kappa/T^2)_ij T^2, 2 baryon coth form (1, DNMR: unsupported) */
+  int    diffusion_relax_mode;    /* 0 tau_q = 0.2/T, 1 tau_q = C_B/T (with mode 2) */
+  double kappa_over_T2[9];        /* 3x3 row-major, charges (B, S, Q) */
+  double C_B;
+  int    diffusion_regulator;     /* regulator on/off */
+  double regulator_chi0, regulator_e0, regulator_xi0, regulator_rq_max;   /* e0, xi0 in GeV/fm^3 */
 } ccake_params;
 
 /* Description of an equation of state.  For CCAKE_EOS_TABLE the table is a
@@ -289,7 +315,8 @@ CCAKE_API int  ccake_reduce(ccake_event* ev, int which, double* out);
  * Output<2, EoM_default>::print_freeze_out (src/output.cpp:502-560):
  * divEener, divE[2], gsub, uout[2], swsub, bulksub, shear 00, 11, 22, 33, 12,
  * tlist, rsub[2], sFO, E, T (GeV), muB, muS, muQ, w, cs2, rhoB, rhoS, rhoQ,
- * the 12 diffusion components (zero) and the table-EoS flag.  Rows are
+ * the 12 diffusion components q^mu of B, S, Q (zero with diffusion off) and
+ * the table-EoS flag.  Rows are
  * ordered by step, then particle id, the order the reference writes them.
  * Replaces FreezeOut::bsqsvinterpolate's results with
  * Output::print_freeze_out / flush_freeze_out (include/freeze_out.h:638-930,

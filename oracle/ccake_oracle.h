@@ -15,12 +15,18 @@
  * brute force, closed forms, finite differences and the analytic ideal
  * Gubser solution, and by tests/test_gubser_viscous.py against the
  * reference's semi-analytic Israel-Stewart Gubser profiles (shear terms,
- * Gubser-mode tau_pi).  Parity unpinned:
+ * Gubser-mode tau_pi).  BSQ diffusion is pinned by tests/test_diffusion.py
+ * (at-rest Israel-Stewart relaxation, continuity and transverse-momentum
+ * conservation of a uniform moving fluid, SPH gradient identities, the
+ * regulator cap, transport special values).  Parity unpinned:
  *   - which root the reference's GSL hybrids solver returns where several
  *     exist (the inversion here is Newton with the same absolute stop);
- *   - the d alpha / d(s, rho) fields (not read by the path without diffusion);
+ *   - the values of the d alpha / d(s, rho) fields themselves (the diffusion
+ *     tests use them only as inputs or through their Euler relation);
  *   - the bulk-pressure terms of the equations of motion (zero in conformal
- *     flows, so no analytic solution exercises them).
+ *     flows, so no analytic solution exercises them);
+ *   - diffusion MRF terms multiplied by the geometric factor or u^eta (zero
+ *     for D = 2).
  */
 #ifndef CCAKE_ORACLE_H
 #define CCAKE_ORACLE_H
@@ -63,7 +69,21 @@ enum orc_field {
   ORC_F_EZ, ORC_F_DEZ,
   ORC_F_FRZ1_E, ORC_F_FRZ2_E,
   ORC_F_SHEAR_KNUDSEN, ORC_F_BULK_KNUDSEN, ORC_F_INV_REYNOLDS_SHEAR, ORC_F_INV_REYNOLDS_BULK,
-  ORC_NUM_FIELDS
+  /* BSQ diffusion: q^mu of B, S, Q (4 each); extensive q^i and its time derivative
+     (3 per charge); kappa_q and tau_q (3x3); the gradient-pass sums grad q^0, div q
+     (per charge) and grad alpha (direction-major, 2 x 3); Knudsen and inverse
+     Reynolds numbers per charge */
+  ORC_F_Q0,
+  ORC_F_EXT_Q0 = ORC_F_Q0 + 12,
+  ORC_F_DEXT_Q0 = ORC_F_EXT_Q0 + 9,
+  ORC_F_KAPPA_Q0 = ORC_F_DEXT_Q0 + 9,
+  ORC_F_TAU_Q0 = ORC_F_KAPPA_Q0 + 9,
+  ORC_F_GRAD_QA0 = ORC_F_TAU_Q0 + 9,
+  ORC_F_DIV_QA0 = ORC_F_GRAD_QA0 + 3,
+  ORC_F_GRAD_ALPHA0 = ORC_F_DIV_QA0 + 3,
+  ORC_F_KN_DIFF0 = ORC_F_GRAD_ALPHA0 + 6,
+  ORC_F_RE_DIFF0 = ORC_F_KN_DIFF0 + 3,
+  ORC_NUM_FIELDS = ORC_F_RE_DIFF0 + 3
 };
 
 enum orc_ifield { ORC_I_FREEZE = 0, ORC_I_BTRACK, ORC_I_EOS_ID, ORC_NUM_IFIELDS };
@@ -97,6 +117,13 @@ typedef struct orc_params {
   int    delta_pipi_mode, tau_pipi_mode, lambda_piPi_mode, phi6_mode, phi7_mode;
   int    delta_PiPi_mode, lambda_Pipi_mode, phi1_mode, phi3_mode;
   int    cells_per_rc;
+  int    using_diffusion;
+  int    diffusion_mode;
+  int    diffusion_relax_mode;
+  double kappa_over_T2[9];
+  double C_B;
+  int    diffusion_regulator;
+  double regulator_chi0, regulator_e0, regulator_xi0, regulator_rq_max;
 } orc_params;
 
 typedef struct orc_eos_desc {

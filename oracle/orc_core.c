@@ -59,7 +59,9 @@ static double distance(int D, const double* a, const double* b)
   X(shear[0][0]) X(shear[0][1]) X(shear[0][2]) X(shear[1][0]) X(shear[1][1]) X(shear[1][2]) \
   X(shear[2][0]) X(shear[2][1]) X(shear[2][2]) X(r[0]) X(r[1]) X(u[0]) X(u[1]) \
   X(gradP[0]) X(gradP[1]) X(gradE[0]) X(gradE[1]) X(s) X(e) X(rhoB) X(rhoS) X(rhoQ) \
-  X(T) X(muB) X(muS) X(muQ) X(theta) X(bulk) X(sigma_lab) X(shear33) X(shv_nabla_u) X(wfz) X(cs2fz)
+  X(T) X(muB) X(muS) X(muQ) X(theta) X(bulk) X(sigma_lab) X(shear33) X(shv_nabla_u) X(wfz) X(cs2fz) \
+  X(diff[0]) X(diff[1]) X(diff[2]) X(diff[3]) X(diff[4]) X(diff[5]) X(diff[6]) X(diff[7]) \
+  X(diff[8]) X(diff[9]) X(diff[10]) X(diff[11])
 
 static orc_snap* snap_alloc(int n)
 {
@@ -120,6 +122,7 @@ static void snap_take(orc_snap* f, orc_event* ev)
     f->shear33[i] = FLD(ev, ORC_F_SHV0 + 15)[i];
     f->shv_nabla_u[i] = FLD(ev, ORC_F_SHV_NABLA_U)[i];
     f->wfz[i] = FLD(ev, ORC_F_W)[i];     f->cs2fz[i] = FLD(ev, ORC_F_CS2)[i];
+    for (int k = 0; k < 12; ++k) f->diff[k][i] = FLD(ev, ORC_F_Q0 + k)[i];
   }
 }
 
@@ -136,6 +139,7 @@ orc_event* orc_event_create(const orc_params* p, orc_eos* eos, int n)
   ev->c_s = calloc((size_t)n+1, 8); ev->c_B = calloc((size_t)n+1, 8); ev->c_S = calloc((size_t)n+1, 8);
   ev->c_Q = calloc((size_t)n+1, 8); ev->c_bulk = calloc((size_t)n+1, 8); ev->c_E0 = calloc((size_t)n+1, 8);
   for (int k = 0; k < 5; ++k) ev->c_shv[k] = calloc((size_t)n+1, 8);
+  for (int k = 0; k < 9; ++k) ev->c_q[k] = calloc((size_t)n+1, 8);
   for (int k = 0; k < 4; ++k) { ev->fback_e[k] = calloc((size_t)n+1, 8); ev->fback_gradP0[k] = calloc((size_t)n+1, 8); }
   ev->frz1_gradP0 = calloc((size_t)n+1, 8); ev->frz2_gradP0 = calloc((size_t)n+1, 8);
   ev->nb_off = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
@@ -159,6 +163,7 @@ void orc_event_destroy(orc_event* ev)
   for (int d = 0; d < 3; ++d) { free(ev->c_u[d]); free(ev->c_r[d]); }
   free(ev->c_s); free(ev->c_B); free(ev->c_S); free(ev->c_Q); free(ev->c_bulk); free(ev->c_E0);
   for (int k = 0; k < 5; ++k) free(ev->c_shv[k]);
+  for (int k = 0; k < 9; ++k) free(ev->c_q[k]);
   for (int k = 0; k < 4; ++k) { free(ev->fback_e[k]); free(ev->fback_gradP0[k]); }
   free(ev->frz1_gradP0); free(ev->frz2_gradP0);
   free(ev->nb_off); free(ev->nb_idx);
@@ -397,6 +402,18 @@ void orc_smooth_gradients(orc_event* ev)
       s0a[d] = FLD(ev, ORC_F_SHV0 + 0*4 + d + 1)[a];
       for (int e2 = 0; e2 < D; ++e2) mina[d][e2] = FLD(ev, ORC_F_SHV0 + (d+1)*4 + e2 + 1)[a];
     }
+    /* diffusion sums (src/sph_workstation.cpp:909-967): grad q^0, div q, grad alpha */
+    const int using_diff = ev->p.using_diffusion;
+    double gqa[3] = {0,0,0}, dqa[3] = {0,0,0}, gal[3][3] = {{0}};
+    double al_a[3] = {0,0,0}, q0_a[3] = {0,0,0}, qv_a[3][3] = {{0}};
+    if (using_diff) {
+      const double Ta = FLD(ev, ORC_F_T)[a];
+      al_a[0] = FLD(ev, ORC_F_MUB)[a]/Ta; al_a[1] = FLD(ev, ORC_F_MUS)[a]/Ta; al_a[2] = FLD(ev, ORC_F_MUQ)[a]/Ta;
+      for (int c = 0; c < 3; ++c) {
+        q0_a[c] = FLD(ev, ORC_F_Q0 + 4*c)[a];
+        for (int d = 0; d < D; ++d) qv_a[c][d] = FLD(ev, ORC_F_Q0 + 4*c + d + 1)[a];
+      }
+    }
     for (int64_t k = ev->nb_off[a]; k < ev->nb_off[a+1]; ++k) {
       const int b = ev->nb_idx[k];
       pos_of(ev, b, pb);
@@ -414,6 +431,20 @@ void orc_smooth_gradients(orc_event* ev)
       double ssK[3];
       const double msl = m_b*sl_a;
       for (int d = 0; d < D; ++d) ssK[d] = msl*gK[d];
+      if (using_diff) {
+        const double Tb = FLD(ev, ORC_F_T)[b];
+        const double al_b[3] = { FLD(ev, ORC_F_MUB)[b]/Tb, FLD(ev, ORC_F_MUS)[b]/Tb, FLD(ev, ORC_F_MUQ)[b]/Tb };
+        double cva = 0.0;
+        for (int d = 0; d < D; ++d) cva += ssK[d]*va[d];
+        for (int c = 0; c < 3; ++c) {
+          gqa[c] += cva*(sigsqrb*FLD(ev, ORC_F_Q0 + 4*c)[b] + sigsqra*q0_a[c]);
+          double dv = 0.0;
+          for (int d = 0; d < D; ++d)
+            dv += sigsqrb*ssK[d]*FLD(ev, ORC_F_Q0 + 4*c + d + 1)[b] + sigsqra*ssK[d]*qv_a[c][d];
+          dqa[c] += dv;
+          for (int d = 0; d < D; ++d) gal[d][c] += (m_b/sl_a)*(al_b[c] - al_a[c])*gK[d];
+        }
+      }
       const double cP = sigsqrb*p_b + sigsqra*p_a;
       const double cE = (m_b/sl_a)*(e_b - e_a);
       const double cB = bulk_b/sl_b/sl_b + bulk_a/sl_a/sl_a;
@@ -438,6 +469,13 @@ void orc_smooth_gradients(orc_event* ev)
           }
           dS[j] += tb + ta;
         }
+      }
+    }
+    if (using_diff) {
+      for (int c = 0; c < 3; ++c) {
+        FLD(ev, ORC_F_GRAD_QA0 + c)[a] = gqa[c];
+        FLD(ev, ORC_F_DIV_QA0 + c)[a] = dqa[c];
+        for (int d = 0; d < D; ++d) FLD(ev, ORC_F_GRAD_ALPHA0 + 3*d + c)[a] = gal[d][c];
       }
     }
     for (int d = 0; d < 3; ++d) {
@@ -497,6 +535,8 @@ int orc_get_time_derivatives(orc_event* ev)
   int st = orc_eos_invert(ev, 0);          /* 04 */
   if (st) return st;
   orc_reset_pi_tensor(ev);                 /* 05 */
+  if (ev->p.using_diffusion && ev->p.diffusion_regulator)
+    orc_regulate_diffusion(ev);            /* 06 */
   orc_viscosities(ev);                     /* 08 */
   orc_smooth_gradients(ev);                /* 09 */
   orc_eom(ev);                             /* 10 */
@@ -508,6 +548,7 @@ void orc_cache_state(orc_event* ev)
 {
   for (int i = 0; i < ev->n; ++i) {
     for (int k = 0; k < 5; ++k) ev->c_shv[k][i] = FLD(ev, ORC_F_EXT_SHV0 + k)[i];
+    for (int k = 0; k < 9; ++k) ev->c_q[k][i] = FLD(ev, ORC_F_EXT_Q0 + k)[i];
     for (int d = 0; d < ev->p.dim; ++d) { ev->c_u[d][i] = FLD(ev, ORC_F_UX + d)[i]; ev->c_r[d][i] = FLD(ev, ORC_F_X + d)[i]; }
     ev->c_s[i] = FLD(ev, ORC_F_EXT_S)[i];
     ev->c_bulk[i] = FLD(ev, ORC_F_EXT_BULK)[i];
@@ -545,6 +586,8 @@ int orc_rk_update(orc_event* ev, double dt, double t0)
     }
     for (int k = 0; k < 5; ++k)
       FLD(ev, ORC_F_EXT_SHV0 + k)[i] = ev->c_shv[k][i] + dt*FLD(ev, ORC_F_DEXT_SHV0 + k)[i];
+    if (ev->p.using_diffusion)
+      for (int k = 0; k < 9; ++k) FLD(ev, ORC_F_EXT_Q0 + k)[i] = ev->c_q[k][i] + dt*FLD(ev, ORC_F_DEXT_Q0 + k)[i];
     for (int d = D; d < 3; ++d) { FLD(ev, ORC_F_X + d)[i] = 0.0; FLD(ev, ORC_F_UX + d)[i] = 0.0; }
   }
   ev->t = t0 + dt;
@@ -667,7 +710,7 @@ static void interpolate(orc_event* ev)
     c[19] = f->muB[i]; c[20] = f->muS[i]; c[21] = f->muQ[i];
     c[22] = w; c[23] = cs2;
     c[24] = f->rhoB[i]; c[25] = f->rhoS[i]; c[26] = f->rhoQ[i];
-    for (int q = 27; q < 39; ++q) c[q] = 0.0;            /* diffout: diffusion off */
+    for (int q = 0; q < 12; ++q) c[27 + q] = f->diff[q][i];   /* diffout(charge, mu) */
     c[39] = FLD(ev, ORC_F_EOS_IS_TABLE)[i];
     ev->fo_count++;
     fz[i] = 4;
@@ -841,6 +884,7 @@ int orc_initialize(orc_event* ev)
       FLD(ev, ORC_F_EXT_BULK)[i] = (FLD(ev, ORC_F_BULK)[i] - FLD(ev, ORC_F_P)[i])/(sl/g/t0);
     } else { FLD(ev, ORC_F_EXT_BULK)[i] = 0.0; FLD(ev, ORC_F_BULK)[i] = 0.0; }
   }
+  if (ev->p.using_diffusion) orc_set_diffusion(ev);     /* set_diffusion (initial_smoothing, 418-419) */
   orc_reset_pi_tensor(ev);
   st = orc_eos_invert(ev, 0);
   if (st) return st;

@@ -106,6 +106,17 @@ void orc_reset_pi_tensor(orc_event* ev)
     }
     for (int a = 1; a < 4; ++a) S[a][0] = S[0][a];
     for (int a = 0; a < 4; ++a) for (int b = 0; b < 4; ++b) FLD(ev, ORC_F_SHV0 + 4*a + b)[i] = S[a][b];
+    if (ev->p.using_diffusion) {
+      /* q^i = extensive q^i sigma, q^0 = -q^i u_i / gamma (eom_default.cpp:139-163) */
+      for (int c = 0; c < 3; ++c) {
+        double qv[3];
+        for (int d = 0; d < 3; ++d) qv[d] = FLD(ev, ORC_F_EXT_Q0 + 3*c + d)[i]*sigma;
+        double q0 = 0.0;
+        for (int d = 0; d < 3; ++d) q0 += -qv[d]*fc[d]/gamma;
+        FLD(ev, ORC_F_Q0 + 4*c)[i] = q0;
+        for (int d = 0; d < 3; ++d) FLD(ev, ORC_F_Q0 + 4*c + d + 1)[i] = qv[d];
+      }
+    }
   }
 }
 
@@ -186,6 +197,14 @@ void orc_viscosities(orc_event* ev)
     for (int k = 0; k < 31; ++k) th[k] = FLD(ev, ORC_F_T + k)[i];
     transport_point(&ev->p, th, out);
     for (int k = 0; k < 13; ++k) FLD(ev, ORC_F_ETA_PI + k)[i] = out[k];
+    if (ev->p.using_diffusion) {
+      double kq[3][3], tq[3][3];
+      orc_diffusion_transport(&ev->p, th, kq, tq);
+      for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) {
+        FLD(ev, ORC_F_KAPPA_Q0 + 3*a + b)[i] = kq[a][b];
+        FLD(ev, ORC_F_TAU_Q0 + 3*a + b)[i] = tq[a][b];
+      }
+    }
   }
 }
 
@@ -438,9 +457,24 @@ static void eom_particle(orc_event* ev, int i)
     }
   }
 
+  /* ---------------- calculate_MRF_diffusion (1416-1822) ---------------- */
+  const int using_diffusion = P->using_diffusion;
+  double F_q0[3] = {0,0,0}, R_q0[3][3] = {{0}}, M_q0[3][3] = {{0}};
+  double F_q[3][3] = {{0}}, M_q[3][3][3] = {{{0}}}, R_q[3][3][3] = {{{0}}};
+  if (using_diffusion)
+    orc_mrf_diffusion(ev, i, M_S, F_S, R_S, F_q0, R_q0, M_q0, F_q, M_q, R_q);
+
   /* ---------------- compute_charge_derivative (369-453) ---------------- */
   double R_N[3][3] = {{0}}, F_N[3] = {0,0,0}, M_N[3][3] = {{0}};
   for (int c = 0; c < 3; ++c) R_N[c][c] += gamma*sigma;
+  if (using_diffusion) {
+    for (int d = 0; d < D; ++d) for (int c = 0; c < 3; ++c) M_N[d][c] += -M_q0[d][c];
+    for (int c = 0; c < 3; ++c) {
+      for (int k = 0; k < 3; ++k) R_N[c][k] += -R_q0[c][k];
+      F_N[c] += -F_q0[c] + FLD(ev, ORC_F_GRAD_QA0 + c)[i] - FLD(ev, ORC_F_DIV_QA0 + c)[i]
+              - FLD(ev, ORC_F_Q0 + 4*c)[i]/t;
+    }
+  }
 
   /* ---------------- compute_velocity_derivative (459-618) ---------------- */
   double gradP[3] = {0,0,0}, gradBulk[3] = {0,0,0}, divshear[3] = {0,0,0}, gradshear[3] = {0,0,0};
@@ -523,6 +557,14 @@ static void eom_particle(orc_event* ev, int i)
   FLD(ev, ORC_F_DEXT_BULK)[i] = d_b;
   FLD(ev, ORC_F_THETA)[i] = theta;
   FLD(ev, ORC_F_SHV_NABLA_U)[i] = snu;
+  if (using_diffusion) {
+    for (int d = 0; d < 3; ++d) for (int c = 0; c < 3; ++c) {
+      double v = F_q[d][c];
+      for (int j = 0; j < D; ++j) v += M_q[d][c][j]*du_dt[j];
+      for (int k = 0; k < 3; ++k) v += R_q[d][c][k]*dN_dt[k];
+      FLD(ev, ORC_F_DEXT_Q0 + 3*c + d)[i] = v;
+    }
+  }
   if (using_shear) {
     int k = 0;
     for (int a = 0; a < 2; ++a) for (int b = a; b < 3; ++b) {
@@ -535,8 +577,8 @@ static void eom_particle(orc_event* ev, int i)
   /* ---------------- compute_Ez_derivative (828-862) ---------------- */
   const double p = FLD(ev, ORC_F_P)[i], e = FLD(ev, ORC_F_E)[i];
 
-  /* ---------------- compute_hydro_numbers (1831-1995), with shear on ---------------- */
-  if (using_shear && D == 2) {
+  /* ---------------- compute_hydro_numbers (1831-1995), with shear or diffusion on ---------------- */
+  if ((using_shear || using_diffusion) && D == 2) {
     /* the stored sigma tensor: (a+1, b+1) = M_sigma du/dt + F_sigma (784-808) */
     double stored[4][4] = {{0}};
     for (int a = 0; a < 2; ++a) for (int b = a; b < 3; ++b) {
@@ -574,6 +616,20 @@ static void eom_particle(orc_event* ev, int i)
     FLD(ev, ORC_F_BULK_KNUDSEN)[i] = tau_Pi*fabs(theta);
     FLD(ev, ORC_F_INV_REYNOLDS_SHEAR)[i] = sqrt(fabs(shv_norm))/p;
     FLD(ev, ORC_F_INV_REYNOLDS_BULK)[i] = fabs(bulk)/p;
+    /* diffusion Knudsen tau_q^(aa) |grad alpha_a| and inverse Reynolds |q_a|/|n_a| */
+    const double rho_n[3] = { rhoB, rhoS, rhoQ };
+    for (int c = 0; c < 3; ++c) {
+      double ga2 = 0.0;
+      for (int d = 0; d < D; ++d) { const double g = FLD(ev, ORC_F_GRAD_ALPHA0 + 3*d + c)[i]; ga2 += g*g; }
+      const double tqa = FLD(ev, ORC_F_TAU_Q0 + 4*c)[i];
+      const double q0 = FLD(ev, ORC_F_Q0 + 4*c)[i], q1 = FLD(ev, ORC_F_Q0 + 4*c + 1)[i];
+      const double q2 = FLD(ev, ORC_F_Q0 + 4*c + 2)[i], q3 = FLD(ev, ORC_F_Q0 + 4*c + 3)[i];
+      const double q_sq = q1*q1 + q2*q2 + t2*q3*q3 - q0*q0;
+      const double q_mag = q_sq > 0.0 ? sqrt(q_sq) : 0.0;
+      const double n_abs = fabs(rho_n[c]);
+      FLD(ev, ORC_F_KN_DIFF0 + c)[i] = tqa*sqrt(ga2);
+      FLD(ev, ORC_F_RE_DIFF0 + c)[i] = (n_abs > 1e-15) ? q_mag/n_abs : 0.0;
+    }
   }
   const double u3 = (D == 3) ? u[2] : 0.0;
   FLD(ev, ORC_F_DEZ)[i] = ((e + p + bulk)*u3*u3 + (p + bulk)/t2 + S[3][3])

@@ -51,6 +51,14 @@ struct orc_eos {
   orc_preinv* pre;
 };
 
+/* BSQ diffusion (orc_diff.c) */
+void orc_diffusion_transport(const orc_params* P, const double* th, double kappa[3][3], double tauq[3][3]);
+void orc_regulate_diffusion(orc_event* ev);
+void orc_set_diffusion(orc_event* ev);
+void orc_mrf_diffusion(orc_event* ev, int i, const double M_S_in[3], double F_S, const double R_S[3],
+                       double F_q0[3], double R_q0[3][3], double M_q0[3][3],
+                       double F_q[3][3], double M_q[3][3][3], double R_q[3][3][3]);
+
 orc_preinv* orc_preinv_create(const orc_eos_desc* d);
 void orc_preinv_destroy(orc_preinv* p);
 double orc_preinv_efreeze(const orc_preinv* p, double T_fm);
@@ -75,6 +83,7 @@ struct orc_event {
   double* c_r[3];
   double* c_s, *c_B, *c_S, *c_Q, *c_bulk, *c_E0;
   double* c_shv[5];
+  double* c_q[9];        /* extensive diffusion (evolver.cpp:132-152) */
   /* neighbour list (CSR, j ascending) */
   int64_t* nb_off;
   int32_t* nb_idx;
@@ -105,6 +114,7 @@ typedef struct orc_snap {
   double* T; double* muB; double* muS; double* muQ;
   double* theta; double* bulk; double* sigma_lab; double* shear33; double* shv_nabla_u;
   double* wfz; double* cs2fz;
+  double* diff[12];      /* q^mu of B, S, Q (freeze_array diff) */
 } orc_snap;
 
 #endif
