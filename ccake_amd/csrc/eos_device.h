@@ -23,6 +23,12 @@ constexpr double kRfTol = 1e-10;       // include/rootfinder.h:37
 constexpr int kNewtonSteps = 60;
 
 enum { M_TABLE = 0, M_TANH = 1, M_CONF = 2, M_DIAG = 3 };
+// Member-kind masks (bit k: kind k may occur).  The solver templates take the
+// mask of the chain they run on, so a chain without a table compiles without
+// the table's gathers and registers; the arithmetic is the same either way.
+constexpr int kAllMembers = 15;
+constexpr int kAnalyticMembers = (1 << M_TANH) | (1 << M_CONF) | (1 << M_DIAG);
+template <int KM, int K> constexpr bool has_kind() { return (KM >> K) & 1; }
 // 17-field order (src/eos_table.cpp:30-34)
 enum { X_P = 0, X_S, X_B, X_SS, X_Q, X_E, X_CS2, X_BB, X_QQ, X_SSS, X_BQ, X_BS, X_QS, X_TB, X_TQ, X_TS, X_TT };
 
@@ -176,22 +182,23 @@ CK_HD void table_fields(const EosMember& m, const double* tab, const double* x, 
   }
 }
 
+template <int KM = kAllMembers>
 CK_HD void member_fields(const EosMember& m, const double* tab, const double* x, double* r)
 {
-  switch (m.kind) {
-    case M_CONF: conf_fields(m, x, r); break;
-    case M_DIAG: diag_fields(m, x, r); break;
-    case M_TANH: tanh_fields(m, x, r); break;
-    default: table_fields(m, tab, x, r); break;
-  }
+  if (has_kind<KM, M_CONF>() && m.kind == M_CONF) conf_fields(m, x, r);
+  else if (has_kind<KM, M_DIAG>() && m.kind == M_DIAG) diag_fields(m, x, r);
+  else if (has_kind<KM, M_TANH>() && m.kind == M_TANH) tanh_fields(m, x, r);
+  else if (has_kind<KM, M_TABLE>()) table_fields(m, tab, x, r);
+  else for (int k = 0; k < 17; ++k) r[k] = NAN;          // a kind outside the mask: never dispatched
 }
 
 // Residual densities (s|e, B, S, Q) at x and their Jacobian in (T, muB, muQ, muS).
 // Table: packed tab5 (s, B, S, Q, e per node) and the derivative of the
 // multilinear interpolant; analytic members: Hessian of p.
+template <int KM = kAllMembers>
 CK_HD void member_dens(const EosMember& m, const double* tab5, int mode, const double* x, double* F, double* J)
 {
-  if (m.kind == M_TABLE) {
+  if (has_kind<KM, M_TABLE>() && m.kind == M_TABLE) {
     int ind[4]; double fr[4];
     if (__builtin_isnan(x[0]) || __builtin_isnan(x[1]) || __builtin_isnan(x[2]) || __builtin_isnan(x[3]) || !table_cell(m, x, ind, fr)) {
       for (int k = 0; k < 4; ++k) F[k] = NAN;
@@ -222,7 +229,7 @@ CK_HD void member_dens(const EosMember& m, const double* tab5, int mode, const d
     return;
   }
   double r[17];
-  member_fields(m, nullptr, x, r);
+  member_fields<KM & kAnalyticMembers>(m, nullptr, x, r);
   const double T = x[0], mB = x[1], mQ = x[2], mS = x[3];
   const double Js[4] = { r[X_TT], r[X_TB], r[X_TQ], r[X_TS] };
   const double JB[4] = { r[X_TB], r[X_BB], r[X_BQ], r[X_BS] };
@@ -255,11 +262,12 @@ CK_HD double residual(const double* F, const double* tgt, double* R)
 // One rootfinder attempt from x (src/rootfinder.cpp:206-416): damped Newton
 // that never leaves the member's range; one polishing step after the
 // absolute residual test passes.
+template <int KM = kAllMembers>
 CK_HD bool newton_attempt(const EosMember& m, const double* tab5, int mode, const double* tgt, double* x)
 {
   double F[4], J[16], R[4], dx[4], xn[4], Fn[4], Jn[16], Rn[4];
   if (!in_range(m, x)) return false;
-  member_dens(m, tab5, mode, x, F, J);
+  member_dens<KM>(m, tab5, mode, x, F, J);
   double res = residual(F, tgt, R);
   for (int it = 0; it < kNewtonSteps; ++it) {
     if (!__builtin_isfinite(res)) return false;
@@ -271,7 +279,7 @@ CK_HD bool newton_attempt(const EosMember& m, const double* tab5, int mode, cons
     for (int k = 0; k < 40; ++k) {
       for (int a = 0; a < 4; ++a) xn[a] = x[a] + lam * dx[a];
       if (in_range(m, xn)) {
-        member_dens(m, tab5, mode, xn, Fn, Jn);
+        member_dens<KM>(m, tab5, mode, xn, Fn, Jn);
         const double rn = residual(Fn, tgt, Rn);
         if (__builtin_isfinite(rn) && (rn < res || (conv && rn <= res))) {
           for (int a = 0; a < 4; ++a) { x[a] = xn[a]; R[a] = Rn[a]; }
@@ -292,6 +300,7 @@ CK_HD bool newton_attempt(const EosMember& m, const double* tab5, int mode, cons
 
 // Rootfinder::find_root: seed adjustment and perturbed retries
 // (src/rootfinder.cpp:421-715).  tgt = (s|e, B, S, Q); x = (T, muB, muQ, muS).
+template <int KM = kAllMembers>
 CK_HD bool find_root(const EosMember& m, const double* tab5, int mode, const double* tgt, double* x)
 {
   double sd[4] = { x[0], x[1], x[2], x[3] };
@@ -317,7 +326,7 @@ CK_HD bool find_root(const EosMember& m, const double* tab5, int mode, const dou
         break;
       default: break;
     }
-    if (newton_attempt(m, tab5, mode, tgt, y)) { for (int a = 0; a < 4; ++a) x[a] = y[a]; return true; }
+    if (newton_attempt<KM>(m, tab5, mode, tgt, y)) { for (int a = 0; a < 4; ++a) x[a] = y[a]; return true; }
   }
   return false;
 }
@@ -367,11 +376,12 @@ CK_HD double schur3(double lead, double a0, double a1, double a2,
 }
 
 // The 31 thermodynamic_info doubles at x for member `mi` of the chain.
+template <int KM = kAllMembers>
 CK_HD void set_thermo(const EosChain& ch, const double* tab, int mi, const double* x, double* th)
 {
   const EosMember& m = ch.m[mi];
   double r[17];
-  member_fields(m, tab, x, r);
+  member_fields<KM>(m, tab, x, r);
   const double T = x[0], mB = x[1], mQ = x[2], mS = x[3];
   const double s = r[X_S], B = r[X_B], S = r[X_SS], Q = r[X_Q];
   const double tt = r[X_TT], tb = r[X_TB], tq = r[X_TQ], ts = r[X_TS];
@@ -416,10 +426,10 @@ CK_HD void set_thermo(const EosChain& ch, const double* tab, int mi, const doubl
   const double aSQ = hQ ? -mS / T / T / t3Q() + (hS ? 1 / T / kQS() : 0.0) : 0.0;
   const double aQQ = hQ ? -mQ / T / T / t3Q() + 1 / T / kQQ() : 0.0;
   double chiBB0 = r[X_BB];
-  if (m.kind == M_TABLE && (mB != 0.0 || mQ != 0.0 || mS != 0.0)) {
+  if (has_kind<KM, M_TABLE>() && m.kind == M_TABLE && (mB != 0.0 || mQ != 0.0 || mS != 0.0)) {
     const double z[4] = { T, 0.0, 0.0, 0.0 };
     double r0[17];
-    member_fields(m, tab, z, r0);
+    member_fields<KM>(m, tab, z, r0);
     chiBB0 = r0[X_BB];
   }
   th[0] = T; th[1] = mB; th[2] = mS; th[3] = mQ;
@@ -438,6 +448,7 @@ CK_HD void set_thermo(const EosChain& ch, const double* tab, int mi, const doubl
 // (580 MeV, sgn), conformal-diagonal seed.  mode 1: (e, B, S, Q) from the
 // first member, seeds: (580 MeV, sgn), conformal-diagonal seed.  A solution
 // with cs2 outside [0, 1] moves on to the next member, seeded from it.
+template <int KM = kAllMembers>
 CK_HD bool invert_point(const EosChain& ch, const double* tab, const double* tab5, int mode,
                         const double* tgt, double* x, int* member)
 {
@@ -459,12 +470,12 @@ CK_HD bool invert_point(const EosChain& ch, const double* tab, const double* tab
       else if (kind == 1) { y[0] = x[0]; y[1] = 0.0; y[2] = 0.0; y[3] = 0.0; }
       else if (kind == 2) { y[0] = 580.0 / hc; y[1] = sgnd(tgt[1]); y[2] = sgnd(tgt[3]); y[3] = sgnd(tgt[2]); }
       else diag_seed(diag, mode, tgt, y);
-      if (find_root(m, tab5, mode, tgt, y)) { found = true; for (int a = 0; a < 4; ++a) res[a] = y[a]; }
+      if (find_root<KM>(m, tab5, mode, tgt, y)) { found = true; for (int a = 0; a < 4; ++a) res[a] = y[a]; }
     }
     if (found) {
       used = k;
       double r[17];
-      member_fields(m, tab, res, r);
+      member_fields<KM>(m, tab, res, r);
       if (r[X_CS2] < 0.0 || r[X_CS2] > 1.0) { for (int a = 0; a < 4; ++a) x[a] = res[a]; continue; }
       break;
     }

@@ -120,7 +120,9 @@ __device__ __forceinline__ void stage_post(double* __restrict__ F, int npad, int
   o[13] = sl; o[14] = 0.0; o[15] = 0.0;
 }
 
-template <int MODE>
+// KM: the member kinds the chain holds (kAllMembers, or kConformalChain for
+// the analytic conformal chain, which then compiles without the table code).
+template <int MODE, int KM>
 __global__ __launch_bounds__(256) void k_eos(EosChain ch, double* __restrict__ F, int npad, int n,
                                              int32_t* __restrict__ I, const int32_t* __restrict__ perm,
                                              double t, Transport P, double* __restrict__ cand,
@@ -138,20 +140,20 @@ __global__ __launch_bounds__(256) void k_eos(EosChain ch, double* __restrict__ F
   bool ok;
   if (MODE == 1) {
     tgt[0] = col(CCAKE_F_IN_E); tgt[1] = col(CCAKE_F_IN_B); tgt[2] = col(CCAKE_F_IN_SS); tgt[3] = col(CCAKE_F_IN_Q);
-    ok = invert_point(ch, ch.tab, ch.tab4, 1, tgt, x, &member);
+    ok = invert_point<KM>(ch, ch.tab, ch.tab4, 1, tgt, x, &member);
     if (!ok) {
       // src/sph_workstation.cpp:1825-1848: freeze and retry at zero charge
       freeze[i] = 4;
       const double z[4] = { tgt[0], 0.0, 0.0, 0.0 };
-      ok = invert_point(ch, ch.tab, ch.tab4, 1, z, x, &member);
+      ok = invert_point<KM>(ch, ch.tab, ch.tab4, 1, z, x, &member);
     }
   } else {
     tgt[0] = col(CCAKE_F_S); tgt[1] = col(CCAKE_F_RHOB); tgt[2] = col(CCAKE_F_RHOS); tgt[3] = col(CCAKE_F_RHOQ);
-    ok = invert_point(ch, ch.tab, ch.tab4, 0, tgt, x, &member);
+    ok = invert_point<KM>(ch, ch.tab, ch.tab4, 0, tgt, x, &member);
   }
   if (!ok) { atomicOr(err, DE_EOS_FAIL); return; }
   double th[31];
-  set_thermo(ch, ch.tab, member, x, th);
+  set_thermo<KM>(ch, ch.tab, member, x, th);
   for (int q = 0; q < 31; ++q) col(CCAKE_F_T + q) = th[q];
   eos_id[i] = member;
   if (MODE == 1) { col(CCAKE_F_IN_S) = th[5]; return; }
@@ -219,6 +221,14 @@ static Transport make_transport(const ccake_params& p)
 #define CCAKE_EOS_THREADS 64
 #endif
 constexpr int kEosThreads = CCAKE_EOS_THREADS;
+constexpr int kConformalChain = (1 << M_CONF) | (1 << M_DIAG);
+
+static int chain_kinds(const EosChain& ch)
+{
+  int km = 0;
+  for (int k = 0; k < ch.n; ++k) km |= 1 << ch.m[k].kind;
+  return km;
+}
 
 int launch_eos(ccake_event* ev, int mode, int fuse_post)
 {
@@ -237,15 +247,20 @@ int launch_eos(ccake_event* ev, int mode, int fuse_post)
       hipLaunchKernelGGL(k_preinv<2>, g2, b2, 0, st, pr, ev->F, ev->npad, n, ev->perm, ev->t, P, ev->cand_g);
     return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
   }
-  if (mode == 1)
-    hipLaunchKernelGGL(k_eos<1>, g, b, 0, st, ev->eos->chain, ev->F, ev->npad, n, ev->I, ev->perm, ev->t, P,
+  const int md = (mode == 1) ? 1 : (fuse_post ? 0 : 2);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, g, b, 0, st, ev->eos->chain, ev->F, ev->npad, n, ev->I, ev->perm, ev->t, P,
                        ev->cand_g, ev->err);
-  else if (fuse_post)
-    hipLaunchKernelGGL(k_eos<0>, g, b, 0, st, ev->eos->chain, ev->F, ev->npad, n, ev->I, ev->perm, ev->t, P,
-                       ev->cand_g, ev->err);
-  else
-    hipLaunchKernelGGL(k_eos<2>, g, b, 0, st, ev->eos->chain, ev->F, ev->npad, n, ev->I, ev->perm, ev->t, P,
-                       ev->cand_g, ev->err);
+  };
+  if (chain_kinds(ev->eos->chain) == kConformalChain) {
+    if (md == 1) go(k_eos<1, kConformalChain>);
+    else if (md == 0) go(k_eos<0, kConformalChain>);
+    else go(k_eos<2, kConformalChain>);
+  } else {
+    if (md == 1) go(k_eos<1, kAllMembers>);
+    else if (md == 0) go(k_eos<0, kAllMembers>);
+    else go(k_eos<2, kAllMembers>);
+  }
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
 
