@@ -120,11 +120,8 @@ static int time_derivatives(ccake_event* ev, int fuse_rk, double dt_stage, doubl
   prof_begin(ev, CCAKE_K_SMOOTH);
   if ((rc = launch_smooth_pairs(ev, 0, true))) return rc;       // 03 pair sums per slice
   prof_end(ev, CCAKE_K_SMOOTH);
-  prof_begin(ev, CCAKE_K_SMOOTH_SUM);
-  if ((rc = launch_smooth_sum(ev, 0, true))) return rc;         // 03 fields (+ btrack)
-  prof_end(ev, CCAKE_K_SMOOTH_SUM);
   prof_begin(ev, CCAKE_K_EOS);
-  if ((rc = launch_eos(ev, 0, 1))) return rc;                   // 04 + 05 + 08 + gather
+  if ((rc = launch_eos(ev, 0, 2))) return rc;                   // 03 fields (+ btrack), 04 + 05 + 08 + gather
   if (ev->p.using_diffusion && (rc = launch_diff_prep(ev, 15, ev->perm))) return rc;  // q, 06, kappa/tau_q
   prof_end(ev, CCAKE_K_EOS);
   prof_begin(ev, CCAKE_K_GRAD);
@@ -161,13 +158,18 @@ static int freeze_step(ccake_event* ev)
 static int rk2_step(ccake_event* ev, double dt)
 {
   int rc;
-  prof_begin(ev, CCAKE_K_RK);
-  if ((rc = launch_cache(ev))) return rc;
-  prof_end(ev, CCAKE_K_RK);
+  // the RK cache: taken inside the first stage's EoM kernel, or up front when
+  // the diffusion regulator may rewrite the evolved currents within the stage
+  const bool sep_cache = ev->p.using_diffusion != 0;
+  if (sep_cache) {
+    prof_begin(ev, CCAKE_K_RK);
+    if ((rc = launch_cache(ev))) return rc;
+    prof_end(ev, CCAKE_K_RK);
+  }
   const double t0 = ev->t;
   const double adt = fabs(dt);
   bool sorted = false;
-  if ((rc = time_derivatives(ev, 1, 0.5 * dt, t0, true, &sorted))) return rc;
+  if ((rc = time_derivatives(ev, sep_cache ? 1 : 2, 0.5 * dt, t0, true, &sorted))) return rc;
   const double d0 = ev->sort_disp;                 // bound at the step's start positions x0
   ev->sort_disp = d0 + 0.5 * adt;                  // x0 + dt/2 v
   ev->t = t0 + 0.5 * dt;

@@ -187,7 +187,10 @@ int launch_smooth(ccake_event* ev, int mode, bool count_pairs = true);  // mode 
 int launch_smooth_pairs(ccake_event* ev, int mode, bool count_pairs);   // pair part (mode 2: whole count walk)
 int launch_smooth_sum(ccake_event* ev, int mode, bool count_pairs);     // slice sums + fields (modes 0, 1)
 // EoS + per-particle physics
-int launch_eos(ccake_event* ev, int mode, int fuse_post);   // fuse_post: reset_pi + transport + gradient gather
+// fuse_post: 0 EoS only, 1 with the stage's per-particle tail, 2 also with
+// the smoothing sum pass folded in front (mode 0 only)
+int launch_eos(ccake_event* ev, int mode, int fuse_post);
+double smooth_norm2(double hT);   // fuse_post: reset_pi + transport + gradient gather
 int launch_reset_pi(ccake_event* ev);
 int launch_viscosities(ccake_event* ev);
 int launch_gather_grad(ccake_event* ev);

@@ -635,13 +635,30 @@ __device__ __forceinline__ void eom_d2(const Cols& col, double t, double use_vor
   (void)e;
 }
 
+// Evolver::set_current_timestep_quantities (src/evolver.cpp:124-168) for
+// one particle: the evolved variables into the RK cache columns.
+__device__ __forceinline__ void cache_state(const Cols& col, int diff)
+{
+  for (int q = 0; q < 5; ++q) col(P_C_SHV0 + q) = col(CCAKE_F_EXT_SHV0 + q);
+  col(P_C_UX) = col(CCAKE_F_UX); col(P_C_UY) = col(CCAKE_F_UY);
+  col(P_C_X) = col(CCAKE_F_X); col(P_C_Y) = col(CCAKE_F_Y);
+  col(P_C_S) = col(CCAKE_F_EXT_S); col(P_C_BULK) = col(CCAKE_F_EXT_BULK);
+  col(P_C_B) = col(CCAKE_F_EXT_B); col(P_C_SS) = col(CCAKE_F_EXT_SS); col(P_C_Q) = col(CCAKE_F_EXT_Q);
+  col(P_C_EZ) = col(CCAKE_F_EZ);
+  if (diff) for (int q = 0; q < 9; ++q) col(P_C_Q0 + q) = col(CCAKE_F_EXT_Q0 + q);
+}
+
 // mode 0: store derivatives; mode 1: store derivatives and apply the RK
 // update y = y0 + dt f (src/evolver.cpp:363-484); returns the negative
-// entropy flag through err.
+// entropy flag through err.  With `cache` (the first stage of a step) the
+// RK cache is taken from the current state first, as k_cache would at the
+// start of the step: nothing in the stage before this kernel changes the
+// evolved variables (the diffusion regulator does, so events with diffusion
+// keep the separate k_cache).
 template <bool SHEAR, bool RK, bool DIFF>
 __global__ __launch_bounds__(256) void k_eom(double* __restrict__ F, int npad, int n,
                                              const int32_t* __restrict__ freeze, double t, double use_vort,
-                                             double dt, int32_t* __restrict__ err)
+                                             double dt, int32_t* __restrict__ err, int cache)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -665,6 +682,7 @@ __global__ __launch_bounds__(256) void k_eom(double* __restrict__ F, int npad, i
     for (int c = 0; c < 3; ++c) { col(CCAKE_F_KN_DIFF0 + c) = hn[4 + c]; col(CCAKE_F_RE_DIFF0 + c) = hn[7 + c]; }
   }
   if (!RK) return;
+  if (cache) cache_state(col, DIFF);
   double es = col(P_C_S) + dt * ds;
   if (es < 0.0 && freeze[i] > 3) es = 1.e-3;
   else if (es < 0.0) atomicOr(err, DE_NEG_ENTROPY);
@@ -716,14 +734,7 @@ __global__ __launch_bounds__(256) void k_cache(double* __restrict__ F, int npad,
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const Cols col{ F, npad, i };
-  for (int q = 0; q < 5; ++q) col(P_C_SHV0 + q) = col(CCAKE_F_EXT_SHV0 + q);
-  col(P_C_UX) = col(CCAKE_F_UX); col(P_C_UY) = col(CCAKE_F_UY);
-  col(P_C_X) = col(CCAKE_F_X); col(P_C_Y) = col(CCAKE_F_Y);
-  col(P_C_S) = col(CCAKE_F_EXT_S); col(P_C_BULK) = col(CCAKE_F_EXT_BULK);
-  col(P_C_B) = col(CCAKE_F_EXT_B); col(P_C_SS) = col(CCAKE_F_EXT_SS); col(P_C_Q) = col(CCAKE_F_EXT_Q);
-  col(P_C_EZ) = col(CCAKE_F_EZ);
-  if (diff) for (int q = 0; q < 9; ++q) col(P_C_Q0 + q) = col(CCAKE_F_EXT_Q0 + q);
+  cache_state(Cols{ F, npad, i }, diff);
 }
 
 int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0)
@@ -734,7 +745,12 @@ int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0)
   hipStream_t st = ev->stream;
   const int32_t* fz = ev->I + (size_t)CCAKE_I_FREEZE * ev->npad;
   const double uv = ev->p.using_vorticity ? 1.0 : 0.0;
-  auto go = [&](auto kern) { hipLaunchKernelGGL(kern, g, b, 0, st, ev->F, ev->npad, n, fz, ev->t, uv, dt, ev->err); };
+  // fuse_rk: 0 derivatives only, 1 with the RK update, 2 with the RK cache
+  // taken first and the update (the first stage of a step)
+  const int cache = fuse_rk == 2 ? 1 : 0;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, g, b, 0, st, ev->F, ev->npad, n, fz, ev->t, uv, dt, ev->err, cache);
+  };
   const int key = (ev->p.using_shear ? 4 : 0) | (fuse_rk ? 2 : 0) | (ev->p.using_diffusion ? 1 : 0);
   switch (key) {
     case 0: go(k_eom<false, false, false>); break;

@@ -120,17 +120,84 @@ __device__ __forceinline__ void stage_post(double* __restrict__ F, int npad, int
   o[13] = sl; o[14] = 0.0; o[15] = 0.0;
 }
 
+// The smoothing sum pass folded into the stage's EoS kernel (MODE 3): the
+// slices' partial sums of sorted position k in slice order, the smoothed
+// fields, btrack and the stage's pair total, as k_smooth_sum<0, S> writes
+// them, then the EoS stage of MODE 0 on the fields just written.
+struct SmoothSum {
+  const double* part = nullptr;
+  int nslices = 0;
+  double norm2 = 0.0;
+  unsigned long long* pair_total = nullptr;
+};
+
+__device__ __forceinline__ void smooth_sum_into(const SmoothSum& sm, double* __restrict__ F, int npad, int n,
+                                                int k, int id, double t, int32_t* __restrict__ btrack)
+{
+  constexpr int NACC = 6;
+  const bool act = k < n;
+  double acc[NACC];
+  #pragma unroll
+  for (int f = 0; f < NACC; ++f) acc[f] = 0.0;
+  if (act) {
+    // blocks of 8 slices, every load of a block issued before its adds (a
+    // clamped slice index past the last slice adds an exact 0.0)
+    const int ns = sm.nslices;
+    for (int s0 = 0; s0 < ns; s0 += 8) {
+      double v[8][NACC];
+      #pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int sli = s0 + u < ns ? s0 + u : ns - 1;
+        #pragma unroll
+        for (int f = 0; f < NACC; ++f) v[u][f] = sm.part[((size_t)sli * NACC + f) * npad + k];
+      }
+      #pragma unroll
+      for (int u = 0; u < 8; ++u)
+        #pragma unroll
+        for (int f = 0; f < NACC; ++f) acc[f] += (s0 + u < ns) ? v[u][f] : 0.0;
+    }
+  }
+  const int cnt = act ? (int)acc[1] - 1 : 0;    // neighbours, self excluded
+  if (sm.pair_total) {
+    // one pair of atomics per wave
+    unsigned long long tot = (unsigned long long)cnt;
+    for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
+    if ((threadIdx.x & (kWave - 1)) == 0 && tot) { atomicAdd(sm.pair_total, tot); atomicAdd(sm.pair_total + 1, tot); }
+  }
+  if (!act) return;
+  const int b = btrack[id];
+  btrack[id] = (b == -1) ? -1 : cnt;
+  auto col = [&](int c) -> double& { return F[(size_t)c * npad + id]; };
+  const double sl = acc[0] * sm.norm2;
+  const double gam = col(CCAKE_F_GAMMA);
+  col(CCAKE_F_SIGMA_LAB) = sl;
+  col(CCAKE_F_SIGMA) = sl / gam / t;            // get_LRF (src/eom_default.cpp:198-203)
+  const double ss = acc[2] * sm.norm2, sB = acc[3] * sm.norm2;
+  const double sS = acc[4] * sm.norm2, sQ = acc[5] * sm.norm2;
+  col(CCAKE_F_SM_S) = ss; col(CCAKE_F_SM_B) = sB; col(CCAKE_F_SM_SS) = sS; col(CCAKE_F_SM_Q) = sQ;
+  col(CCAKE_F_S) = ss / gam / t;
+  col(CCAKE_F_RHOB) = sB / gam / t;
+  col(CCAKE_F_RHOS) = sS / gam / t;
+  col(CCAKE_F_RHOQ) = sQ / gam / t;
+}
+
 // KM: the member kinds the chain holds (kAllMembers, or kConformalChain for
 // the analytic conformal chain, which then compiles without the table code).
+// MODE 0: the stage (EoS, reset_pi_tensor, transport, gradient records);
+// 1: the initial e inversion; 2: EoS only; 3: MODE 0 after the smoothing sum.
 template <int MODE, int KM>
 __global__ __launch_bounds__(256) void k_eos(EosChain ch, double* __restrict__ F, int npad, int n,
                                              int32_t* __restrict__ I, const int32_t* __restrict__ perm,
                                              double t, Transport P, double* __restrict__ cand,
-                                             int32_t* __restrict__ err)
+                                             int32_t* __restrict__ err, SmoothSum sm)
 {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (MODE == 3) {
+    const int id = k < n ? perm[k] : 0;
+    smooth_sum_into(sm, F, npad, n, k, id, t, I + (size_t)CCAKE_I_BTRACK * npad);
+  }
   if (k >= n) return;
-  const int i = (MODE == 0) ? perm[k] : k;
+  const int i = (MODE == 0 || MODE == 3) ? perm[k] : k;
   auto col = [&](int c) -> double& { return F[(size_t)c * npad + i]; };
   int32_t* eos_id = I + (size_t)CCAKE_I_EOS_ID * npad;
   int32_t* freeze = I + (size_t)CCAKE_I_FREEZE * npad;
@@ -237,6 +304,11 @@ int launch_eos(ccake_event* ev, int mode, int fuse_post)
   hipStream_t st = ev->stream;
   const Transport P = make_transport(ev->p);
   if (ev->eos->chain.kind == CCAKE_EOS_PREINVERTED) {
+    if (fuse_post == 2) {                 // the lookup kernel keeps the sum pass separate
+      const int rc = launch_smooth_sum(ev, 0, true);
+      if (rc) return rc;
+      fuse_post = 1;
+    }
     const dim3 g2((n + 255) / 256), b2(256);
     const Preinv& pr = ev->eos->pre;
     if (mode == 1)
@@ -247,18 +319,27 @@ int launch_eos(ccake_event* ev, int mode, int fuse_post)
       hipLaunchKernelGGL(k_preinv<2>, g2, b2, 0, st, pr, ev->F, ev->npad, n, ev->perm, ev->t, P, ev->cand_g);
     return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
   }
-  const int md = (mode == 1) ? 1 : (fuse_post ? 0 : 2);
+  const int md = (mode == 1) ? 1 : (fuse_post == 2 ? 3 : (fuse_post ? 0 : 2));
+  SmoothSum sm;
+  if (md == 3) {
+    sm.part = ev->part;
+    sm.nslices = ev->nslices;
+    sm.norm2 = smooth_norm2(ev->p.hT);
+    sm.pair_total = (unsigned long long*)ev->pair_count;
+  }
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, g, b, 0, st, ev->eos->chain, ev->F, ev->npad, n, ev->I, ev->perm, ev->t, P,
-                       ev->cand_g, ev->err);
+                       ev->cand_g, ev->err, sm);
   };
   if (chain_kinds(ev->eos->chain) == kConformalChain) {
     if (md == 1) go(k_eos<1, kConformalChain>);
     else if (md == 0) go(k_eos<0, kConformalChain>);
+    else if (md == 3) go(k_eos<3, kConformalChain>);
     else go(k_eos<2, kConformalChain>);
   } else {
     if (md == 1) go(k_eos<1, kAllMembers>);
     else if (md == 0) go(k_eos<0, kAllMembers>);
+    else if (md == 3) go(k_eos<3, kAllMembers>);
     else go(k_eos<2, kAllMembers>);
   }
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;

@@ -266,6 +266,8 @@ static int smooth_sum_dispatch(ccake_event* ev, double norm2, int32_t* bt, unsig
 }
 
 // Sum part of a smoothing pass (modes 0, 1).
+double smooth_norm2(double hT) { return make_w(hT).norm2; }
+
 int launch_smooth_sum(ccake_event* ev, int mode, bool count_pairs)
 {
   const KernelW W = make_w(ev->p.hT);
