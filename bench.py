@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", choices=["gubser", "lumpy40k"], default="gubser")
+    ap.add_argument("--diffusion", action="store_true",
+                    help="BSQ diffusion on (diagonal kappa/T^2 = 0.1, regulator on, q = 0 at tau0)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle sample")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--cells-per-rc", type=int, default=3, help="neighbour-grid cells per cutoff radius")
@@ -244,6 +246,10 @@ def main():
     from ccake_amd import config
 
     d, cfg, table, wl = workload(args.workload)
+    if args.diffusion:
+        cfg = dict(cfg, using_diffusion=1, kappa_over_T2=[0.1, 0.0, 0.0, 0.0, 0.1, 0.0, 0.0, 0.0, 0.1],
+                   diffusion_regulator=1)
+        wl = dict(wl, workload=wl["workload"] + ", BSQ diffusion (kappa/T^2 = 0.1 I, regulator on)")
     cfg = dict(cfg, cells_per_rc=args.cells_per_rc)
     wl["cells_per_rc"] = args.cells_per_rc
     n = len(d["x"])
