@@ -159,6 +159,28 @@ CK_HD bool table_cell(const EosMember& m, const double* x, int* ind, double* fr)
   return true;
 }
 
+// One table field at x: the same weights and the same accumulation order as
+// table_fields, so the value is the one table_fields gives; NaN outside.
+CK_HD double table_field_one(const EosMember& m, const double* tab, const double* x, int field)
+{
+  int ind[4]; double fr[4];
+  if (__builtin_isnan(x[0]) || __builtin_isnan(x[1]) || __builtin_isnan(x[2]) || __builtin_isnan(x[3]) || !table_cell(m, x, ind, fr))
+    return NAN;
+  double r = 0.0;
+  for (int h = 0; h < 16; ++h) {
+    const int b0 = (h >> 3) & 1, b1 = (h >> 2) & 1, b2 = (h >> 1) & 1, b3 = h & 1;
+    double w = 1.0;
+    w *= fr[0] + b0 - 2.0 * b0 * fr[0];
+    w *= fr[1] + b1 - 2.0 * b1 * fr[1];
+    w *= fr[2] + b2 - 2.0 * b2 * fr[2];
+    w *= fr[3] + b3 - 2.0 * b3 * fr[3];
+    const size_t node = (((size_t)(ind[0] + b0) * m.dims[1] + (ind[1] + b1)) * m.dims[2] + (ind[2] + b2)) * m.dims[3]
+                        + (ind[3] + b3);
+    r += w * tab[node * 17 + field];
+  }
+  return r;
+}
+
 // Table fields at x: all 17 (tab, stride 17) into r; NaN outside.
 CK_HD void table_fields(const EosMember& m, const double* tab, const double* x, double* r)
 {
@@ -427,10 +449,8 @@ CK_HD void set_thermo(const EosChain& ch, const double* tab, int mi, const doubl
   const double aQQ = hQ ? -mQ / T / T / t3Q() + 1 / T / kQQ() : 0.0;
   double chiBB0 = r[X_BB];
   if (has_kind<KM, M_TABLE>() && m.kind == M_TABLE && (mB != 0.0 || mQ != 0.0 || mS != 0.0)) {
-    const double z[4] = { T, 0.0, 0.0, 0.0 };
-    double r0[17];
-    member_fields<KM>(m, tab, z, r0);
-    chiBB0 = r0[X_BB];
+    const double z[4] = { T, 0.0, 0.0, 0.0 };   // chi_BB at mu = 0: one field of the 17
+    chiBB0 = table_field_one(m, tab, z, X_BB);
   }
   th[0] = T; th[1] = mB; th[2] = mS; th[3] = mQ;
   th[4] = r[X_E]; th[5] = s; th[6] = B; th[7] = S; th[8] = Q;
