@@ -426,27 +426,35 @@ CK_HD void set_thermo(const EosChain& ch, const double* tab, int mi, const doubl
   auto kBQ = [&] { return schur3(bq, tb, bb, bs, tt, tb, ts, ts, bs, ss, tq, bq, qs, tq, qs, qq); };
   auto kSQ = [&] { return schur3(qs, ts, bs, ss, tt, tb, ts, tb, bb, bs, tq, bq, qs, tq, bq, qq); };
   auto kQQ = [&] { return schur3(qq, tq, bq, qs, tt, tb, ts, tb, bb, bs, ts, bs, ss, tq, bq, qs); };
+  // each complement once, under the condition of its uses (pure functions:
+  // the same values as evaluating them at every use)
+  const double vt2B = hB ? t2B() : 0.0, vt3B = hB ? t3B() : 0.0, vkBB = hB ? kBB() : 0.0;
+  const double vt2S = hS ? t2S() : 0.0, vt3S = hS ? t3S() : 0.0, vkSS = hS ? kSS() : 0.0;
+  const double vt2Q = hQ ? t2Q() : 0.0, vt3Q = hQ ? t3Q() : 0.0, vkQQ = hQ ? kQQ() : 0.0;
+  const double vkBQ = (hB && hQ) ? kBQ() : 0.0, vkBS = (hB && hS) ? kBS() : 0.0;
+  const double vkSB = (hS && hB) ? kSB() : 0.0, vkSQ = (hS && hQ) ? kSQ() : 0.0;
+  const double vkQB = (hQ && hB) ? kQB() : 0.0, vkQS = (hQ && hS) ? kQS() : 0.0;
   double ct = 0.0;
-  if (hB) ct += B / t2B();
-  if (hQ) ct += Q / t2Q();
-  if (hS) ct += S / t2S();
+  if (hB) ct += B / vt2B;
+  if (hQ) ct += Q / vt2Q;
+  if (hS) ct += S / vt2S;
   const double dwds = T + s / dsdt + ct;
   double dwdB = mB, dwdS = mS, dwdQ = mQ;
-  if (hB) { double c = s / t3B() + B / kBB(); if (hQ) c += Q / kBQ(); if (hS) c += S / kBS(); dwdB += c; }
-  if (hS) { double c = s / t3S() + S / kSS(); if (hB) c += B / kSB(); if (hQ) c += Q / kSQ(); dwdS += c; }
-  if (hQ) { double c = s / t3Q() + Q / kQQ(); if (hB) c += B / kQB(); if (hS) c += S / kQS(); dwdQ += c; }
-  const double aBs = -mB / T / T / dsdt + (hB ? 1 / T / t2B() : 0.0);
-  const double aSs = -mS / T / T / dsdt + (hS ? 1 / T / t2S() : 0.0);
-  const double aQs = -mQ / T / T / dsdt + (hQ ? 1 / T / t2Q() : 0.0);
-  const double aBB = hB ? 1 / T / kBB() - mB / T / T / t3B() : 0.0;
-  const double aSB = hB ? -mS / T / T / t3B() + (hS ? 1 / T / kBS() : 0.0) : 0.0;
-  const double aQB = hB ? -mQ / T / T / t3B() + (hQ ? 1 / T / kBQ() : 0.0) : 0.0;
-  const double aBS = hS ? -mB / T / T / t3S() + (hB ? 1 / T / kSB() : 0.0) : 0.0;
-  const double aSS = hS ? -mS / T / T / t3S() + 1 / T / kSS() : 0.0;
-  const double aQS = hS ? -mQ / T / T / t3S() + (hQ ? 1 / T / kSQ() : 0.0) : 0.0;
-  const double aBQ = hQ ? -mB / T / T / t3Q() + (hB ? 1 / T / kQB() : 0.0) : 0.0;
-  const double aSQ = hQ ? -mS / T / T / t3Q() + (hS ? 1 / T / kQS() : 0.0) : 0.0;
-  const double aQQ = hQ ? -mQ / T / T / t3Q() + 1 / T / kQQ() : 0.0;
+  if (hB) { double c = s / vt3B + B / vkBB; if (hQ) c += Q / vkBQ; if (hS) c += S / vkBS; dwdB += c; }
+  if (hS) { double c = s / vt3S + S / vkSS; if (hB) c += B / vkSB; if (hQ) c += Q / vkSQ; dwdS += c; }
+  if (hQ) { double c = s / vt3Q + Q / vkQQ; if (hB) c += B / vkQB; if (hS) c += S / vkQS; dwdQ += c; }
+  const double aBs = -mB / T / T / dsdt + (hB ? 1 / T / vt2B : 0.0);
+  const double aSs = -mS / T / T / dsdt + (hS ? 1 / T / vt2S : 0.0);
+  const double aQs = -mQ / T / T / dsdt + (hQ ? 1 / T / vt2Q : 0.0);
+  const double aBB = hB ? 1 / T / vkBB - mB / T / T / vt3B : 0.0;
+  const double aSB = hB ? -mS / T / T / vt3B + (hS ? 1 / T / vkBS : 0.0) : 0.0;
+  const double aQB = hB ? -mQ / T / T / vt3B + (hQ ? 1 / T / vkBQ : 0.0) : 0.0;
+  const double aBS = hS ? -mB / T / T / vt3S + (hB ? 1 / T / vkSB : 0.0) : 0.0;
+  const double aSS = hS ? -mS / T / T / vt3S + 1 / T / vkSS : 0.0;
+  const double aQS = hS ? -mQ / T / T / vt3S + (hQ ? 1 / T / vkSQ : 0.0) : 0.0;
+  const double aBQ = hQ ? -mB / T / T / vt3Q + (hB ? 1 / T / vkQB : 0.0) : 0.0;
+  const double aSQ = hQ ? -mS / T / T / vt3Q + (hS ? 1 / T / vkQS : 0.0) : 0.0;
+  const double aQQ = hQ ? -mQ / T / T / vt3Q + 1 / T / vkQQ : 0.0;
   double chiBB0 = r[X_BB];
   if (has_kind<KM, M_TABLE>() && m.kind == M_TABLE && (mB != 0.0 || mQ != 0.0 || mS != 0.0)) {
     const double z[4] = { T, 0.0, 0.0, 0.0 };   // chi_BB at mu = 0: one field of the 17
