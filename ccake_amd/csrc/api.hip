@@ -127,12 +127,17 @@ static int time_derivatives(ccake_event* ev, int fuse_rk, double dt_stage, doubl
   prof_begin(ev, CCAKE_K_GRAD);
   if ((rc = launch_grad_pairs(ev))) return rc;                  // 09 pair sums per slice
   prof_end(ev, CCAKE_K_GRAD);
-  prof_begin(ev, CCAKE_K_GRAD_SUM);
-  if ((rc = launch_grad_sum(ev))) return rc;                    // 09 gradients
-  if (ev->p.using_diffusion && (rc = launch_diff_grad(ev))) return rc;   // 09 diffusion sums
-  prof_end(ev, CCAKE_K_GRAD_SUM);
+  // the diffusion pair pass reuses the partial-sum buffer, so with diffusion
+  // the gradient sums run first; otherwise they run inside the EoM kernel
+  const bool diff = ev->p.using_diffusion != 0;
+  if (diff) {
+    prof_begin(ev, CCAKE_K_GRAD_SUM);
+    if ((rc = launch_grad_sum(ev))) return rc;                  // 09 gradients
+    if ((rc = launch_diff_grad(ev))) return rc;                 // 09 diffusion sums
+    prof_end(ev, CCAKE_K_GRAD_SUM);
+  }
   prof_begin(ev, CCAKE_K_EOM);
-  if ((rc = launch_eom(ev, fuse_rk, dt_stage, t0))) return rc;  // 10 (+ RK update)
+  if ((rc = launch_eom(ev, fuse_rk, dt_stage, t0, !diff))) return rc;  // (09 gradient sums,) 10 (+ RK update)
   prof_end(ev, CCAKE_K_EOM);
   return 0;
 }

@@ -212,7 +212,10 @@ int launch_diff_grad(ccake_event* ev);
 int launch_grad_pairs(ccake_event* ev);
 int launch_grad_sum(ccake_event* ev);
 // EoM + RK
-int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0);
+// fold_grad_sum: the gradient pass's sum runs inside the EoM kernel (in a
+// time step, after k_grad_slice), which then walks particles in cell order
+int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0, bool fold_grad_sum = false);
+double grad_sum_scale(double hT);
 int launch_cache(ccake_event* ev);
 int launch_rk(ccake_event* ev, double dt, double t0);
 // misc

@@ -8,6 +8,7 @@
 // sign and slot 3 by -tau^2; the 2+1D geometric factor and delta_{i eta}
 // vanish.
 #include "ccake_internal.h"
+#include "grad_sum.h"
 
 namespace ccake {
 
@@ -655,13 +656,28 @@ __device__ __forceinline__ void cache_state(const Cols& col, int diff)
 // start of the step: nothing in the stage before this kernel changes the
 // evolved variables (the diffusion regulator does, so events with diffusion
 // keep the separate k_cache).
+// The gradient pass's sum folded in front (gs.perm set): thread k is sorted
+// position k and works on particle perm[k]; otherwise thread i is particle i.
+struct GradSum {
+  const int32_t* perm = nullptr;
+  const double* cand = nullptr;
+  const double* part = nullptr;
+  int nslices = 0;
+  double C = 0.0;
+};
+
 template <bool SHEAR, bool RK, bool DIFF>
 __global__ __launch_bounds__(256) void k_eom(double* __restrict__ F, int npad, int n,
                                              const int32_t* __restrict__ freeze, double t, double use_vort,
-                                             double dt, int32_t* __restrict__ err, int cache)
+                                             double dt, int32_t* __restrict__ err, int cache, GradSum gs)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int i = k;
+  if (gs.perm) {
+    i = gs.perm[k];
+    grad_sum_into<SHEAR, 0>(gs.cand, k, i, gs.C, gs.part, gs.nslices, npad, F);
+  }
   const Cols col{ F, npad, i };
   double du[2], ds, dN[3], db, dshv[5] = { 0, 0, 0, 0, 0 }, theta, snu, dez, hn[10], dq[9];
   eom_d2<SHEAR, DIFF>(col, t, use_vort, du, &ds, dN, &db, dshv, &theta, &snu, &dez, hn, dq);
@@ -737,9 +753,17 @@ __global__ __launch_bounds__(256) void k_cache(double* __restrict__ F, int npad,
   cache_state(Cols{ F, npad, i }, diff);
 }
 
-int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0)
+int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0, bool fold_grad_sum)
 {
   (void)t0;
+  GradSum gs;
+  if (fold_grad_sum) {
+    gs.perm = ev->perm;
+    gs.cand = ev->cand_g;
+    gs.part = ev->part;
+    gs.nslices = ev->nslices;
+    gs.C = grad_sum_scale(ev->p.hT);
+  }
   const int n = ev->n;
   const dim3 g((n + 255) / 256), b(256);
   hipStream_t st = ev->stream;
@@ -749,7 +773,7 @@ int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0)
   // taken first and the update (the first stage of a step)
   const int cache = fuse_rk == 2 ? 1 : 0;
   auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, g, b, 0, st, ev->F, ev->npad, n, fz, ev->t, uv, dt, ev->err, cache);
+    hipLaunchKernelGGL(kern, g, b, 0, st, ev->F, ev->npad, n, fz, ev->t, uv, dt, ev->err, cache, gs);
   };
   const int key = (ev->p.using_shear ? 4 : 0) | (fuse_rk ? 2 : 0) | (ev->p.using_diffusion ? 1 : 0);
   switch (key) {

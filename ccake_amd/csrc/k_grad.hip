@@ -13,17 +13,13 @@
 // only neighbour-side sums of w = m_b grad W_ab (17 accumulators) and
 // applies the owner terms once at the end.
 #include "pair_common.h"
+#include "grad_sum.h"
 
 namespace ccake {
-
-constexpr int kCandG = 16;
 
 // Pair part: one single-wave workgroup per (tile, slice) unit walks its
 // slice of the tile's candidates and stores the slice's neighbour-side sums;
 // k_grad_sum adds the slices in slice order and applies the owner terms.
-template <bool SHEAR>
-constexpr int grad_nacc() { return SHEAR ? 17 : 12; }
-
 template <bool SHEAR>
 __global__ __launch_bounds__(kWave) void k_grad_slice(const double* __restrict__ cand,
                                                       const int32_t* __restrict__ skey,
@@ -85,63 +81,17 @@ __global__ __launch_bounds__(kWave) void k_grad_slice(const double* __restrict__
 }
 
 // Sum pass: the slices' sums in slice order, scaled by C = kgrad / h^4,
-// then the owner terms, written by particle id.  S is the slice count, so
-// every load of a particle is in flight at once.
+// then the owner terms, written by particle id (grad_sum_into).  S is the
+// slice count, so every load of a particle is in flight at once.
 template <bool SHEAR, int S>
 __global__ __launch_bounds__(256) void k_grad_sum(const double* __restrict__ cand,
                                                   const int32_t* __restrict__ perm, int n, double C,
                                                   const double* __restrict__ part, int npad,
                                                   double* __restrict__ F)
 {
-  constexpr int NACC = grad_nacc<SHEAR>();
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  double acc[NACC];
-  #pragma unroll
-  for (int q = 0; q < NACC; ++q) acc[q] = 0.0;
-  #pragma unroll
-  for (int sli = 0; sli < S; ++sli)
-    #pragma unroll
-    for (int q = 0; q < NACC; ++q) acc[q] += part[((size_t)sli * NACC + q) * npad + k];
-  #pragma unroll
-  for (int q = 0; q < NACC; ++q) acc[q] *= C;
-  const double* me = cand + (size_t)k * kCandG;
-  const double vax = me[6], vay = me[7];
-
-  const double a1x = acc[0], a1y = acc[1], apx = acc[2], apy = acc[3], aex = acc[4], aey = acc[5];
-  const double abx = acc[6], aby = acc[7], vxx = acc[8], vxy = acc[9], vyx = acc[10], vyy = acc[11];
-  const int id = perm[k];
-  const double sla = me[13], isla = 1.0 / sla;
-  auto put = [&](int c, double v) { F[(size_t)c * npad + id] = v; };
-  put(CCAKE_F_GRADP0 + 0, sla * (apx + me[3] * a1x));
-  put(CCAKE_F_GRADP0 + 1, sla * (apy + me[3] * a1y));
-  put(CCAKE_F_GRADP0 + 2, 0.0);
-  put(CCAKE_F_GRADE0 + 0, (aex - me[4] * a1x) * isla);
-  put(CCAKE_F_GRADE0 + 1, (aey - me[4] * a1y) * isla);
-  put(CCAKE_F_GRADE0 + 2, 0.0);
-  put(CCAKE_F_GRADBULK0 + 0, sla * (abx + me[5] * a1x));
-  put(CCAKE_F_GRADBULK0 + 1, sla * (aby + me[5] * a1y));
-  put(CCAKE_F_GRADBULK0 + 2, 0.0);
-  put(CCAKE_F_GRADV0 + 0, (vxx - vax * a1x) * isla);
-  put(CCAKE_F_GRADV0 + 1, (vxy - vay * a1x) * isla);
-  put(CCAKE_F_GRADV0 + 2, 0.0);
-  put(CCAKE_F_GRADV0 + 3, (vyx - vax * a1y) * isla);
-  put(CCAKE_F_GRADV0 + 4, (vyy - vay * a1y) * isla);
-  put(CCAKE_F_GRADV0 + 5, 0.0);
-  put(CCAKE_F_GRADV0 + 6, 0.0); put(CCAKE_F_GRADV0 + 7, 0.0); put(CCAKE_F_GRADV0 + 8, 0.0);
-  if (SHEAR) {
-    const double acv = acc[SHEAR ? 12 : 0], gsx = acc[SHEAR ? 13 : 0], gsy = acc[SHEAR ? 14 : 0];
-    const double dsx = acc[SHEAR ? 15 : 0], dsy = acc[SHEAR ? 16 : 0];
-    put(CCAKE_F_GRADSHEAR0 + 0, sla * (gsx + me[8] * acv));
-    put(CCAKE_F_GRADSHEAR0 + 1, sla * (gsy + me[9] * acv));
-    put(CCAKE_F_DIVSHEAR0 + 0, sla * (dsx + me[10] * a1x + me[11] * a1y));
-    put(CCAKE_F_DIVSHEAR0 + 1, sla * (dsy + me[11] * a1x + me[12] * a1y));
-  } else {
-    put(CCAKE_F_GRADSHEAR0 + 0, 0.0); put(CCAKE_F_GRADSHEAR0 + 1, 0.0);
-    put(CCAKE_F_DIVSHEAR0 + 0, 0.0); put(CCAKE_F_DIVSHEAR0 + 1, 0.0);
-  }
-  put(CCAKE_F_GRADSHEAR0 + 2, 0.0);
-  put(CCAKE_F_DIVSHEAR0 + 2, 0.0);
+  grad_sum_into<SHEAR, S>(cand, k, perm[k], C, part, S, npad, F);
 }
 
 int launch_grad_pairs(ccake_event* ev)
@@ -177,10 +127,11 @@ static int grad_sum_dispatch(ccake_event* ev, double C)
   }
 }
 
+double grad_sum_scale(double h) { return -0.5 * (15.0 / (7.0 * kPi)) / pow(h, 4); }   // kgrad / h^4
+
 int launch_grad_sum(ccake_event* ev)
 {
-  const double h = ev->p.hT;
-  const double C = -0.5 * (15.0 / (7.0 * kPi)) / pow(h, 4);   // kgrad / h^4 = -kgrad2 / (2 h^4)
+  const double C = grad_sum_scale(ev->p.hT);
   const int rc = ev->p.using_shear ? grad_sum_dispatch<true>(ev, C) : grad_sum_dispatch<false>(ev, C);
   if (rc) return rc;
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
