@@ -99,10 +99,25 @@ static int ensure_sorted(ccake_event* ev)
 // ---- one RK stage: SPHWorkstation::get_time_derivatives ------------------
 // may_reuse: the stage may keep the last cell sort (ev->sort_disp bounds the
 // displacement since); *sorted reports whether it sorted.
+// Which sum passes run inside the next kernel (bit 1: smoothing sum in the
+// EoS kernel, bit 2: gradient sum in the EoM kernel).  The defaults follow
+// measurement: the smoothing fold pays off with every EoS chain; the
+// gradient fold pays off with the light ideal-fluid EoM kernel and costs
+// with the register-bound shear one (32k lumpy event: 0.837 ms per step
+// with bit 1 only, 0.860 with bit 2 only).  CCAKE_FOLD_SUMS overrides them
+// for tuning.
+static int fold_sums(const ccake_event* ev)
+{
+  static const int env = [] { const char* e = getenv("CCAKE_FOLD_SUMS"); return e ? atoi(e) : -1; }();
+  if (env >= 0) return env;
+  return 1 | (ev->p.using_shear ? 0 : 2);
+}
+
 static int time_derivatives(ccake_event* ev, int fuse_rk, double dt_stage, double t0, bool may_reuse,
                             bool* sorted = nullptr)
 {
   int rc;
+  const int fold = fold_sums(ev);
   const bool reuse = may_reuse && ev->sort_valid && ev->sort_disp <= ev->skin_frac * 2.0 * ev->p.hT;
   prof_begin(ev, CCAKE_K_BIN);
   if ((rc = launch_stage_begin(ev))) return rc;                 // 02 gamma, v (+ bbox)
@@ -120,8 +135,13 @@ static int time_derivatives(ccake_event* ev, int fuse_rk, double dt_stage, doubl
   prof_begin(ev, CCAKE_K_SMOOTH);
   if ((rc = launch_smooth_pairs(ev, 0, true))) return rc;       // 03 pair sums per slice
   prof_end(ev, CCAKE_K_SMOOTH);
+  if (!(fold & 1)) {
+    prof_begin(ev, CCAKE_K_SMOOTH_SUM);
+    if ((rc = launch_smooth_sum(ev, 0, true))) return rc;       // 03 fields (+ btrack)
+    prof_end(ev, CCAKE_K_SMOOTH_SUM);
+  }
   prof_begin(ev, CCAKE_K_EOS);
-  if ((rc = launch_eos(ev, 0, 2))) return rc;                   // 03 fields (+ btrack), 04 + 05 + 08 + gather
+  if ((rc = launch_eos(ev, 0, (fold & 1) ? 2 : 1))) return rc;  // (03 fields,) 04 + 05 + 08 + gather
   if (ev->p.using_diffusion && (rc = launch_diff_prep(ev, 15, ev->perm))) return rc;  // q, 06, kappa/tau_q
   prof_end(ev, CCAKE_K_EOS);
   prof_begin(ev, CCAKE_K_GRAD);
@@ -130,14 +150,15 @@ static int time_derivatives(ccake_event* ev, int fuse_rk, double dt_stage, doubl
   // the diffusion pair pass reuses the partial-sum buffer, so with diffusion
   // the gradient sums run first; otherwise they run inside the EoM kernel
   const bool diff = ev->p.using_diffusion != 0;
-  if (diff) {
+  const bool fold_grad = !diff && (fold & 2);
+  if (!fold_grad) {
     prof_begin(ev, CCAKE_K_GRAD_SUM);
     if ((rc = launch_grad_sum(ev))) return rc;                  // 09 gradients
-    if ((rc = launch_diff_grad(ev))) return rc;                 // 09 diffusion sums
+    if (diff && (rc = launch_diff_grad(ev))) return rc;         // 09 diffusion sums
     prof_end(ev, CCAKE_K_GRAD_SUM);
   }
   prof_begin(ev, CCAKE_K_EOM);
-  if ((rc = launch_eom(ev, fuse_rk, dt_stage, t0, !diff))) return rc;  // (09 gradient sums,) 10 (+ RK update)
+  if ((rc = launch_eom(ev, fuse_rk, dt_stage, t0, fold_grad))) return rc;  // (09 gradient sums,) 10 (+ RK update)
   prof_end(ev, CCAKE_K_EOM);
   return 0;
 }
