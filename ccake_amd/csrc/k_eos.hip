@@ -192,17 +192,17 @@ __global__ __launch_bounds__(256) void k_eos(EosChain ch, double* __restrict__ F
                                              int32_t* __restrict__ err, SmoothSum sm)
 {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (MODE == 3) {
-    const int id = k < n ? perm[k] : 0;
-    smooth_sum_into(sm, F, npad, n, k, id, t, I + (size_t)CCAKE_I_BTRACK * npad);
-  }
-  if (k >= n) return;
-  const int i = (MODE == 0 || MODE == 3) ? perm[k] : k;
+  const int kk = k < n ? k : n - 1;
+  const int i = (MODE == 0 || MODE == 3) ? perm[kk] : kk;
   auto col = [&](int c) -> double& { return F[(size_t)c * npad + i]; };
   int32_t* eos_id = I + (size_t)CCAKE_I_EOS_ID * npad;
   int32_t* freeze = I + (size_t)CCAKE_I_FREEZE * npad;
+  // the Newton seed and member are read before the smoothing sum's stores,
+  // so their loads overlap with the partial sums' loads
   double x[4] = { col(CCAKE_F_T), col(CCAKE_F_MUB), col(CCAKE_F_MUQ), col(CCAKE_F_MUS) };
   int member = eos_id[i];
+  if (MODE == 3) smooth_sum_into(sm, F, npad, n, k, i, t, I + (size_t)CCAKE_I_BTRACK * npad);
+  if (k >= n) return;
   double tgt[4];
   bool ok;
   if (MODE == 1) {
