@@ -65,8 +65,8 @@ def gubser_ideal_params(**kw):
 def lumpy_params(**kw):
     """Viscous BSQ lumpy event: table EoS, eta/s = 0.08, zeta/s a Gaussian
     of height 0.1 at T = 0.17 GeV (width 0.02 GeV), RK2, freeze-out at
-    T = 150 MeV.  tau0 = 0.6 fm/c and dt = 0.02 fm/c are this build's
-    choices (the workload leaves them open)."""
+    T = 150 MeV.  tau0 = 0.6 fm/c and dt = 0.02 fm/c are chosen here (the
+    workload description leaves them open)."""
     p = base_params(t0=0.6, dt=0.02, hT=0.3, particlization=1,
                     eta_mode=ETA_CONSTANT, constant_eta_over_s=0.08,
                     zeta_mode=ZETA_GAUSSIAN, zeta_gauss_A=0.1,

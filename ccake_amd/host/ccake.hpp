@@ -448,7 +448,7 @@ class FreezeOut {
 //  - print_system_state (63-83) downloads the particles and writes
 //    system_state_<k>.dat (print_system_state_to_txt, 102-172) when
 //    txt_evolution is set; k counts the calls.  HDF5 output (hdf_evolution)
-//    needs an HDF5 library, which this build does not link: asking for it
+//    needs an HDF5 library, which this library does not link: asking for it
 //    fails at construction.
 //  - freeze-out rows are buffered as text at every print_freeze_out and
 //    written to output_directory/freeze_out.dat by flush_freeze_out
