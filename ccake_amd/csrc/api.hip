@@ -119,18 +119,20 @@ static int time_derivatives(ccake_event* ev, int fuse_rk, double dt_stage, doubl
   int rc;
   const int fold = fold_sums(ev);
   const bool reuse = may_reuse && ev->sort_valid && ev->sort_disp <= ev->skin_frac * 2.0 * ev->p.hT;
-  prof_begin(ev, CCAKE_K_BIN);
-  if ((rc = launch_stage_begin(ev))) return rc;                 // 02 gamma, v (+ bbox)
-  prof_end(ev, CCAKE_K_BIN);
-  prof_begin(ev, CCAKE_K_SORT);
   if (reuse) {
-    if ((rc = launch_gather_smooth(ev))) return rc;             // candidate records at current positions
+    prof_begin(ev, CCAKE_K_SORT);
+    if ((rc = launch_stage_gather(ev))) return rc;              // 02 gamma, v; candidate records at current positions
+    prof_end(ev, CCAKE_K_SORT);
   } else {
+    prof_begin(ev, CCAKE_K_BIN);
+    if ((rc = launch_stage_begin(ev))) return rc;               // 02 gamma, v (+ bbox)
+    prof_end(ev, CCAKE_K_BIN);
+    prof_begin(ev, CCAKE_K_SORT);
     if ((rc = launch_neighbor_sort(ev))) return rc;             // 01 cell sort + candidate records
+    prof_end(ev, CCAKE_K_SORT);
     ev->sort_valid = true;
     ev->sort_disp = 0.0;
   }
-  prof_end(ev, CCAKE_K_SORT);
   if (sorted) *sorted = !reuse;
   prof_begin(ev, CCAKE_K_SMOOTH);
   if ((rc = launch_smooth_pairs(ev, 0, true))) return rc;       // 03 pair sums per slice

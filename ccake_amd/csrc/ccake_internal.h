@@ -185,7 +185,8 @@ int launch_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, s
 int launch_gather_smooth(ccake_event* ev);
 int launch_smooth(ccake_event* ev, int mode, bool count_pairs = true);  // mode 0 fields, 1 initial sigma, 2 counts
 int launch_smooth_pairs(ccake_event* ev, int mode, bool count_pairs);   // pair part (mode 2: whole count walk)
-int launch_smooth_sum(ccake_event* ev, int mode, bool count_pairs);     // slice sums + fields (modes 0, 1)
+int launch_smooth_sum(ccake_event* ev, int mode, bool count_pairs);
+int launch_stage_gather(ccake_event* ev);   // kept sort: gamma, v, smoothing records     // slice sums + fields (modes 0, 1)
 // EoS + per-particle physics
 // fuse_post: 0 EoS only, 1 with the stage's per-particle tail, 2 also with
 // the smoothing sum pass folded in front (mode 0 only)

@@ -19,6 +19,37 @@ __global__ __launch_bounds__(256) void k_gather_smooth(const double* __restrict_
   smooth_record(F, npad, perm[k], cand + (size_t)k * kCandS);
 }
 
+// A stage that keeps the cell sort (stage begin plus record gather in one
+// kernel): gamma and v of particle perm[k] (EoM_default::update_velocity,
+// src/eom_default.cpp:17-38, as k_stage_begin does), its smoothing record at
+// sorted position k, and the stage's pair count cleared.  The bounding box is
+// not needed: the kept grid stays.
+__global__ __launch_bounds__(256) void k_stage_gather(double* __restrict__ F, int npad, int n,
+                                                      const int32_t* __restrict__ perm,
+                                                      double* __restrict__ cand,
+                                                      unsigned long long* __restrict__ pair_count)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k == 0) pair_count[0] = 0ull;
+  if (k >= n) return;
+  const int i = perm[k];
+  const double ux = F[(size_t)CCAKE_F_UX * npad + i], uy = F[(size_t)CCAKE_F_UY * npad + i];
+  const double c = (-ux) * ux + (-uy) * uy;
+  const double g = sqrt(1.0 - c);
+  F[(size_t)CCAKE_F_GAMMA * npad + i] = g;
+  F[(size_t)CCAKE_F_VX * npad + i] = ux / g;
+  F[(size_t)CCAKE_F_VY * npad + i] = uy / g;
+  smooth_record(F, npad, i, cand + (size_t)k * kCandS);
+}
+
+int launch_stage_gather(ccake_event* ev)
+{
+  const int n = ev->n;
+  hipLaunchKernelGGL(k_stage_gather, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n,
+                     ev->perm, ev->cand_s, (unsigned long long*)ev->pair_count);
+  return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+}
+
 struct KernelW {
   double norm, norm2;   // src/kernel.cpp:16-17 (D = 2): W(0) = norm, norm / norm2 = 4
 };
