@@ -113,13 +113,27 @@ static int fold_sums(const ccake_event* ev)
   return 1 | (ev->p.using_shear ? 0 : 2);
 }
 
+// Within one stepping call, a stage that keeps the cell sort can be begun by
+// the previous stage's EoM kernel (`begun`).  next_disp: the sort bound B at
+// the next stage's start if this stage keeps the sort ([0]) or sorts ([1]);
+// negative when no stage follows in this call.  *begun_next reports whether
+// this stage's EoM kernel began the next one.
+struct NextStage {
+  bool begun = false;
+  double next_disp[2] = { -1.0, -1.0 };
+  bool begun_next = false;
+};
+
 static int time_derivatives(ccake_event* ev, int fuse_rk, double dt_stage, double t0, bool may_reuse,
-                            bool* sorted = nullptr)
+                            bool* sorted = nullptr, NextStage* ns = nullptr)
 {
   int rc;
   const int fold = fold_sums(ev);
-  const bool reuse = may_reuse && ev->sort_valid && ev->sort_disp <= ev->skin_frac * 2.0 * ev->p.hT;
-  if (reuse) {
+  const double bound = ev->skin_frac * 2.0 * ev->p.hT;
+  const bool reuse = may_reuse && ev->sort_valid && ev->sort_disp <= bound;
+  if (reuse && ns && ns->begun) {
+    // gamma, v and the smoothing records were written by the last EoM kernel
+  } else if (reuse) {
     prof_begin(ev, CCAKE_K_SORT);
     if ((rc = launch_stage_gather(ev))) return rc;              // 02 gamma, v; candidate records at current positions
     prof_end(ev, CCAKE_K_SORT);
@@ -159,8 +173,14 @@ static int time_derivatives(ccake_event* ev, int fuse_rk, double dt_stage, doubl
     if (diff && (rc = launch_diff_grad(ev))) return rc;         // 09 diffusion sums
     prof_end(ev, CCAKE_K_GRAD_SUM);
   }
+  bool begin_next = false;
+  if (ns) {
+    const double nd = ns->next_disp[reuse ? 0 : 1];
+    begin_next = fold_grad && fuse_rk && may_reuse && nd >= 0.0 && nd <= bound;
+    ns->begun_next = begin_next;
+  }
   prof_begin(ev, CCAKE_K_EOM);
-  if ((rc = launch_eom(ev, fuse_rk, dt_stage, t0, fold_grad))) return rc;  // (09 gradient sums,) 10 (+ RK update)
+  if ((rc = launch_eom(ev, fuse_rk, dt_stage, t0, fold_grad, begin_next))) return rc;  // (09,) 10 (+ RK, + next stage)
   prof_end(ev, CCAKE_K_EOM);
   return 0;
 }
@@ -183,7 +203,7 @@ static int freeze_step(ccake_event* ev)
   return 0;
 }
 
-static int rk2_step(ccake_event* ev, double dt)
+static int rk2_step(ccake_event* ev, double dt, bool more = false, bool* begun = nullptr)
 {
   int rc;
   // the RK cache: taken inside the first stage's EoM kernel, or up front when
@@ -197,14 +217,25 @@ static int rk2_step(ccake_event* ev, double dt)
   const double t0 = ev->t;
   const double adt = fabs(dt);
   bool sorted = false;
-  if ((rc = time_derivatives(ev, sep_cache ? 1 : 2, 0.5 * dt, t0, true, &sorted))) return rc;
+  NextStage s1;
+  s1.begun = begun && *begun;
+  s1.next_disp[0] = ev->sort_disp + 0.5 * adt;     // the bound update after stage 1, below
+  s1.next_disp[1] = 0.5 * adt;
+  if ((rc = time_derivatives(ev, sep_cache ? 1 : 2, 0.5 * dt, t0, true, &sorted, &s1))) return rc;
   const double d0 = ev->sort_disp;                 // bound at the step's start positions x0
   ev->sort_disp = d0 + 0.5 * adt;                  // x0 + dt/2 v
   ev->t = t0 + 0.5 * dt;
-  if ((rc = time_derivatives(ev, 1, dt, t0, true, &sorted))) return rc;
+  NextStage s2;
+  s2.begun = s1.begun_next;
+  if (more) {                                      // the bound update at the end of the step, below
+    s2.next_disp[0] = d0 + adt;
+    s2.next_disp[1] = 1.5 * adt;
+  }
+  if ((rc = time_derivatives(ev, 1, dt, t0, true, &sorted, &s2))) return rc;
   // end of step: x0 + dt v_mid, within dt of x0 and 1.5 dt of x0 + dt/2 v
   ev->sort_disp = sorted ? 1.5 * adt : d0 + adt;
   ev->t = t0 + dt;
+  if (begun) *begun = s2.begun_next;
   if (ev->p.particlization) {
     prof_begin(ev, CCAKE_K_MISC);
     if ((rc = freeze_step(ev))) return rc;
@@ -774,8 +805,9 @@ int ccake_run_steps(ccake_event* ev, double dt, int rk_order, int n_steps, int* 
   if (!ev) return CCAKE_ERR_INVALID;
   if (rk_order != 2) { set_error("only RK2 is built"); return CCAKE_ERR_UNSUPPORTED; }
   int done = 0;
+  bool begun = false;                  // the next stage was begun by the last EoM kernel
   for (int k = 0; k < n_steps; ++k) {
-    int rc = rk2_step(ev, dt);
+    int rc = rk2_step(ev, dt, k + 1 < n_steps, &begun);
     if (rc) { if (steps_done) *steps_done = done; return rc; }
     ++done;
   }

@@ -215,7 +215,10 @@ int launch_grad_sum(ccake_event* ev);
 // EoM + RK
 // fold_grad_sum: the gradient pass's sum runs inside the EoM kernel (in a
 // time step, after k_grad_slice), which then walks particles in cell order
-int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0, bool fold_grad_sum = false);
+// begin_next (with the fold and an RK update): also begin the next stage,
+// which must keep the cell sort (gamma, v, smoothing records)
+int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0, bool fold_grad_sum = false,
+               bool begin_next = false);
 double grad_sum_scale(double hT);
 int launch_cache(ccake_event* ev);
 int launch_rk(ccake_event* ev, double dt, double t0);

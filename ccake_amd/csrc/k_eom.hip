@@ -658,12 +658,18 @@ __device__ __forceinline__ void cache_state(const Cols& col, int diff)
 // keep the separate k_cache).
 // The gradient pass's sum folded in front (gs.perm set): thread k is sorted
 // position k and works on particle perm[k]; otherwise thread i is particle i.
+// With `next_cand` (cell-order walk only) the kernel also begins the next
+// stage when that stage keeps the cell sort: gamma and v from the updated u
+// and the smoothing record at the updated positions (k_stage_gather's work),
+// and the stage's pair count cleared.
 struct GradSum {
   const int32_t* perm = nullptr;
   const double* cand = nullptr;
   const double* part = nullptr;
   int nslices = 0;
   double C = 0.0;
+  double* next_cand = nullptr;
+  unsigned long long* pair_count = nullptr;
 };
 
 template <bool SHEAR, bool RK, bool DIFF>
@@ -716,6 +722,16 @@ __global__ __launch_bounds__(256) void k_eom(double* __restrict__ F, int npad, i
   if (DIFF) for (int q = 0; q < 9; ++q) col(CCAKE_F_EXT_Q0 + q) = col(P_C_Q0 + q) + dt * dq[q];
   col(CCAKE_F_ETA) = 0.0;
   col(CCAKE_F_UETA) = 0.0;
+  if (gs.next_cand) {
+    if (k == 0) gs.pair_count[0] = 0ull;
+    const double ux = col(CCAKE_F_UX), uy = col(CCAKE_F_UY);
+    const double c = (-ux) * ux + (-uy) * uy;
+    const double g = sqrt(1.0 - c);
+    col(CCAKE_F_GAMMA) = g;
+    col(CCAKE_F_VX) = ux / g;
+    col(CCAKE_F_VY) = uy / g;
+    smooth_record(F, npad, i, gs.next_cand + (size_t)k * kCandS);
+  }
 }
 
 // Stand-alone RK update from stored derivatives.
@@ -753,7 +769,7 @@ __global__ __launch_bounds__(256) void k_cache(double* __restrict__ F, int npad,
   cache_state(Cols{ F, npad, i }, diff);
 }
 
-int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0, bool fold_grad_sum)
+int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0, bool fold_grad_sum, bool begin_next)
 {
   (void)t0;
   GradSum gs;
@@ -763,6 +779,10 @@ int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0, bool fold_gra
     gs.part = ev->part;
     gs.nslices = ev->nslices;
     gs.C = grad_sum_scale(ev->p.hT);
+    if (begin_next && fuse_rk) {
+      gs.next_cand = ev->cand_s;
+      gs.pair_count = (unsigned long long*)ev->pair_count;
+    }
   }
   const int n = ev->n;
   const dim3 g((n + 255) / 256), b(256);

@@ -336,6 +336,37 @@ def test_gubser_steps_parity(gubser):
     ints_equal(gubser.g, gubser.o, "BTRACK")
 
 
+def test_run_steps_equals_single_steps_and_the_oracle(dev, orc):
+    """ccake_run_steps begins each kept-sort stage inside the previous
+    stage's EoM kernel (gamma, v, smoothing records); single calls to
+    ccake_advance_timestep begin only the second stage that way.  Both give
+    the same state bit for bit after 12 steps of ideal Gubser flow (sorts and
+    kept sorts both occur), and the run equals the oracle to 1e-12."""
+    cfg = config.gubser_ideal_params()
+    d = ic.gubser_ideal(spacing=0.1)
+    n = len(d["x"])
+    eos = ccake_amd.Eos(dev, 0)
+    a = ccake_amd.Event(dev, eos, cfg, n)
+    b = ccake_amd.Event(dev, eos, cfg, n)
+    for ev in (a, b):
+        ccake_amd.load_event(ev, d)
+        ev.initialize()
+    assert a.run_steps(cfg["dt"], 12) == 12
+    for _ in range(12):
+        b.advance_timestep(cfg["dt"], 2)
+    for f, lab in field_names():
+        assert np.array_equal(a.download(f), b.download(f)), lab
+    assert np.array_equal(a.download_int("CCAKE_I_BTRACK"), b.download_int("CCAKE_I_BTRACK"))
+    oeos = orc.Eos(0, T_FO)
+    o = orc.Event(cfg, oeos, n)
+    oracle_load(o, d)
+    assert o.initialize() == 0
+    for _ in range(12):
+        assert o.advance_timestep(cfg["dt"], 2) == 0
+    compare(a, o, STATE + DERIVS + ["EZ"], what="run_steps 12")
+    a.close(); b.close(); eos.close()
+
+
 def test_gubser_physics_and_conservation(dev):
     """120 RK2 steps (tau 1.0 -> 1.6): the energy density follows the
     analytic ideal Gubser solution within the SPH smoothing error present
