@@ -607,7 +607,7 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
   if (p->using_diffusion) rc |= dalloc(ev, &ev->cand_d, 16 * np);
   ev->snap_cols = p->using_diffusion ? 45 : 33;
   rc |= dalloc(ev, &ev->red, 1024);
-  rc |= dalloc(ev, &ev->err, 1);
+  rc |= dalloc(ev, &ev->err, 2);          // [0] error flags, [1] charge-check scratch
   if (p->particlization) {
     rc |= dalloc(ev, &ev->snaps, (size_t)6 * ev->snap_cols * np);
     rc |= dalloc(ev, &ev->fo_rows, (size_t)n * CCAKE_FO_ROW);
@@ -663,6 +663,7 @@ int ccake_event_get_time(ccake_event* ev, double* t) { if (!ev || !t) return CCA
 int ccake_event_upload(ccake_event* ev, int field, const double* host, size_t n)
 {
   if (!ev || !host || field < 0 || field >= CCAKE_NUM_FIELDS || n != (size_t)ev->n) return CCAKE_ERR_INVALID;
+  ev->no_charges = false;               // re-established by the next initialize
   if (field == CCAKE_F_X || field == CCAKE_F_Y || field == CCAKE_F_ETA) {
     for (size_t i = 0; i < n; ++i)
       if (!std::isfinite(host[i])) { set_error("non-finite particle position"); return CCAKE_ERR_INVALID; }
@@ -840,6 +841,14 @@ int ccake_initialize(ccake_event* ev)
   if (ev->p.using_diffusion && (rc = launch_diff_prep(ev, 1, nullptr))) return rc;   // q^mu
   if ((rc = launch_eos(ev, 2, 0))) return rc;               // update_all_particle_thermodynamics
   if ((rc = launch_freeze(ev, -1))) return rc;              // check_freeze_out_status
+  // charges stay zero without diffusion (dN/dt = 0): the smoothing pass can
+  // then skip the charge sums
+  ev->no_charges = false;
+  if (!ev->p.using_diffusion) {
+    bool any = true;
+    if ((rc = launch_charge_check(ev, &any))) return rc;
+    ev->no_charges = !any;
+  }
   return ccake_event_check(ev);
 }
 

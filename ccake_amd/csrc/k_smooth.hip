@@ -42,6 +42,35 @@ __global__ __launch_bounds__(256) void k_stage_gather(double* __restrict__ F, in
   smooth_record(F, npad, i, cand + (size_t)k * kCandS);
 }
 
+// Whether any particle carries a charge (the smoothing records' charge
+// entries, mass_X ext_X, are not all zero).
+__global__ __launch_bounds__(256) void k_any_charge(const double* __restrict__ F, int npad, int n,
+                                                    int32_t* __restrict__ flag)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool any = false;
+  if (i < n) {
+    auto f = [&](int c) { return F[(size_t)c * npad + i]; };
+    any = f(CCAKE_F_MASS_B) * f(CCAKE_F_EXT_B) != 0.0 || f(CCAKE_F_MASS_SS) * f(CCAKE_F_EXT_SS) != 0.0
+       || f(CCAKE_F_MASS_Q) * f(CCAKE_F_EXT_Q) != 0.0;
+  }
+  if (__ballot(any) != 0ull && (threadIdx.x & (kWave - 1)) == 0) atomicOr(flag, 1);
+}
+
+int launch_charge_check(ccake_event* ev, bool* any)
+{
+  const int n = ev->n;
+  int32_t* d = ev->err + 1;
+  CCAKE_HIP(hipMemsetAsync(d, 0, sizeof(int32_t), ev->stream));
+  hipLaunchKernelGGL(k_any_charge, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n, d);
+  CCAKE_HIP(hipGetLastError());
+  int32_t h = 0;
+  CCAKE_HIP(hipMemcpyAsync(&h, d, sizeof(int32_t), hipMemcpyDeviceToHost, ev->stream));
+  CCAKE_HIP(hipStreamSynchronize(ev->stream));
+  *any = h != 0;
+  return 0;
+}
+
 int launch_stage_gather(ccake_event* ev)
 {
   const int n = ev->n;
@@ -107,8 +136,12 @@ __global__ __launch_bounds__(64) void k_walk(const double* __restrict__ cand,
 // slice) unit walks its slice of the tile's candidates and stores the
 // slice's partial sums of m_b w(q) (the self pair included, W = norm2 w);
 // k_smooth_sum adds them in slice order.
+// MODE 0: sigma_lab, the entropy and the three charges; 3: the same sums for
+// an event whose charges are zero on every particle (the charge sums are
+// written as the exact zeros the full walk would give, and the records are
+// read four doubles at a time); 1: sigma_lab only (initial sigma).
 template <int MODE>
-constexpr int smooth_nacc() { return MODE == 0 ? 6 : 2; }
+constexpr int smooth_nacc() { return (MODE == 0 || MODE == 3) ? 6 : 2; }
 
 template <int MODE>
 __global__ __launch_bounds__(kWave) void k_smooth_slice(const double* __restrict__ cand,
@@ -138,7 +171,7 @@ __global__ __launch_bounds__(kWave) void k_smooth_slice(const double* __restrict
     // lanes of other cell rows sit this segment's walk out
     for_spans_slice(g, cell_off, seg, rcs, slice, nslices, in, [&](int j0, int j1) {
       // wave-uniform candidate records (scalar loads into SGPRs)
-      constexpr int NR = (MODE == 0) ? 7 : 3;
+      constexpr int NR = (MODE == 0) ? 7 : (MODE == 3 ? 4 : 3);
       walk_records<NR, kCandS, 2>(cand, xi, yi, j0, j1,
                                   [&](const double* c, double dx, double dy, int j, bool valid) {
         const double r2 = dist2_nocontract(dx, dy);
@@ -149,8 +182,9 @@ __global__ __launch_bounds__(kWave) void k_smooth_slice(const double* __restrict
         q_and_inverse(r2, hinv2, q, iq);
         const double w = valid ? spline_w(q) : 0.0;
         sl += c[2] * w;
+        if (MODE == 0 || MODE == 3) ss += c[MODE == 1 ? 0 : 3] * w;
         if (MODE == 0) {
-          ss += c[MODE == 0 ? 3 : 0] * w; sB += c[MODE == 0 ? 4 : 0] * w;
+          sB += c[MODE == 0 ? 4 : 0] * w;
           sS += c[MODE == 0 ? 5 : 0] * w; sQ += c[MODE == 0 ? 6 : 0] * w;
         }
       });
@@ -160,7 +194,7 @@ __global__ __launch_bounds__(kWave) void k_smooth_slice(const double* __restrict
   double* p = part + (size_t)slice * NACC * npad + k;
   p[0] = sl;
   p[(size_t)npad] = (double)cnt;
-  if (MODE == 0) {
+  if (MODE == 0 || MODE == 3) {
     p[(size_t)2 * npad] = ss; p[(size_t)3 * npad] = sB;
     p[(size_t)4 * npad] = sS; p[(size_t)5 * npad] = sQ;
   }
@@ -266,7 +300,10 @@ int launch_smooth_pairs(ccake_event* ev, int mode, bool count_pairs)
   }
   const dim3 grid(ntiles * ev->nslices), blk(kWave);
   const double hinv2 = 1.0 / (ev->p.hT * ev->p.hT);
-  if (mode == 0)
+  if (mode == 0 && ev->no_charges)
+    hipLaunchKernelGGL(k_smooth_slice<3>, grid, blk, 0, st, ev->cand_s, ev->skey, ev->cell_off, ev->grid, n, rc2,
+                       rcs, hinv2, ev->nslices, ev->part, ev->npad);
+  else if (mode == 0)
     hipLaunchKernelGGL(k_smooth_slice<0>, grid, blk, 0, st, ev->cand_s, ev->skey, ev->cell_off, ev->grid, n, rc2,
                        rcs, hinv2, ev->nslices, ev->part, ev->npad);
   else
