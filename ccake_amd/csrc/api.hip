@@ -664,6 +664,7 @@ int ccake_event_upload(ccake_event* ev, int field, const double* host, size_t n)
 {
   if (!ev || !host || field < 0 || field >= CCAKE_NUM_FIELDS || n != (size_t)ev->n) return CCAKE_ERR_INVALID;
   ev->no_charges = false;               // re-established by the next initialize
+  ev->no_bulk = false;
   if (field == CCAKE_F_X || field == CCAKE_F_Y || field == CCAKE_F_ETA) {
     for (size_t i = 0; i < n; ++i)
       if (!std::isfinite(host[i])) { set_error("non-finite particle position"); return CCAKE_ERR_INVALID; }
@@ -841,14 +842,15 @@ int ccake_initialize(ccake_event* ev)
   if (ev->p.using_diffusion && (rc = launch_diff_prep(ev, 1, nullptr))) return rc;   // q^mu
   if ((rc = launch_eos(ev, 2, 0))) return rc;               // update_all_particle_thermodynamics
   if ((rc = launch_freeze(ev, -1))) return rc;              // check_freeze_out_status
-  // charges stay zero without diffusion (dN/dt = 0): the smoothing pass can
-  // then skip the charge sums
-  ev->no_charges = false;
-  if (!ev->p.using_diffusion) {
-    bool any = true;
-    if ((rc = launch_charge_check(ev, &any))) return rc;
-    ev->no_charges = !any;
-  }
+  // charges stay zero without diffusion (dN/dt = 0), and the bulk pressure
+  // stays zero with zeta = 0 and no shear (dPi/dt = 0 at Pi = 0; with shear
+  // the lambda_Pipi and phi3 couplings can source it): the pair passes can
+  // then skip those sums
+  bool any_charge = true, any_bulk = true;
+  if ((rc = launch_charge_check(ev, &any_charge, &any_bulk))) return rc;
+  ev->no_charges = !ev->p.using_diffusion && !any_charge;
+  ev->no_bulk = ev->p.zeta_mode == CCAKE_ZETA_CONSTANT && ev->p.constant_zeta_over_s == 0.0 && !ev->p.using_shear
+                && !any_bulk;
   return ccake_event_check(ev);
 }
 

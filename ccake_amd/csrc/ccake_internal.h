@@ -127,6 +127,7 @@ struct ccake_event {
   // reuse of the cell sort: bound on |x - x_sorted| of every particle
   bool sort_valid = false;
   bool no_charges = false;      // every particle's charges are zero (set at initialize, cleared by uploads)
+  bool no_bulk = false;         // zeta = 0 and every particle's bulk pressure zero (likewise)
   double sort_disp = 0.0;
   double skin_frac = 0.04;      // re-sort when the bound would exceed skin_frac * rc (measured best at 40k)
   // freeze-out hypersurface (particlization on): 6 snapshot sets x 33 columns
@@ -188,7 +189,7 @@ int launch_smooth(ccake_event* ev, int mode, bool count_pairs = true);  // mode 
 int launch_smooth_pairs(ccake_event* ev, int mode, bool count_pairs);   // pair part (mode 2: whole count walk)
 int launch_smooth_sum(ccake_event* ev, int mode, bool count_pairs);
 int launch_stage_gather(ccake_event* ev);   // kept sort: gamma, v, smoothing records
-int launch_charge_check(ccake_event* ev, bool* any);     // slice sums + fields (modes 0, 1)
+int launch_charge_check(ccake_event* ev, bool* any_charge, bool* any_bulk);     // slice sums + fields (modes 0, 1)
 // EoS + per-particle physics
 // fuse_post: 0 EoS only, 1 with the stage's per-particle tail, 2 also with
 // the smoothing sum pass folded in front (mode 0 only)

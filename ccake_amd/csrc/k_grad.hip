@@ -20,7 +20,10 @@ namespace ccake {
 // Pair part: one single-wave workgroup per (tile, slice) unit walks its
 // slice of the tile's candidates and stores the slice's neighbour-side sums;
 // k_grad_sum adds the slices in slice order and applies the owner terms.
-template <bool SHEAR>
+// BULK false: the event's bulk pressure is zero on every particle (zeta = 0
+// and no initial bulk, checked at initialize), so the grad Pi sums are the
+// exact zeros the full walk would give and are not accumulated.
+template <bool SHEAR, bool BULK>
 __global__ __launch_bounds__(kWave) void k_grad_slice(const double* __restrict__ cand,
                                                       const int32_t* __restrict__ skey,
                                                       const int32_t* __restrict__ cell_off,
@@ -62,7 +65,7 @@ __global__ __launch_bounds__(kWave) void k_grad_slice(const double* __restrict__
         acc[0] += wx; acc[1] += wy;                    // sum m_b grad W / C
         acc[2] += c[3] * wx; acc[3] += c[3] * wy;      // p_b / sl_b^2
         acc[4] += c[4] * wx; acc[5] += c[4] * wy;      // e_b
-        acc[6] += c[5] * wx; acc[7] += c[5] * wy;      // Pi_b / sl_b^2
+        if (BULK) { acc[6] += c[5] * wx; acc[7] += c[5] * wy; }   // Pi_b / sl_b^2
         acc[8] += c[6] * wx; acc[9] += c[7] * wx;      // v_b d_x W
         acc[10] += c[6] * wy; acc[11] += c[7] * wy;    // v_b d_y W
         if (SHEAR) {
@@ -103,12 +106,15 @@ int launch_grad_pairs(ccake_event* ev)
   const int ntiles = (n + kTile - 1) / kTile;
   const dim3 grid(ntiles * ev->nslices), blk(kWave);
   const double rcs = search_radius(ev);
-  if (ev->p.using_shear)
-    hipLaunchKernelGGL(k_grad_slice<true>, grid, blk, 0, ev->stream, ev->cand_g, ev->skey, ev->cell_off,
-                       ev->grid, n, rc * rc, rcs, hinv2, ev->nslices, ev->part, ev->npad);
-  else
-    hipLaunchKernelGGL(k_grad_slice<false>, grid, blk, 0, ev->stream, ev->cand_g, ev->skey, ev->cell_off,
-                       ev->grid, n, rc * rc, rcs, hinv2, ev->nslices, ev->part, ev->npad);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, grid, blk, 0, ev->stream, ev->cand_g, ev->skey, ev->cell_off, ev->grid, n, rc * rc,
+                       rcs, hinv2, ev->nslices, ev->part, ev->npad);
+  };
+  if (ev->p.using_shear) {
+    if (ev->no_bulk) go(k_grad_slice<true, false>); else go(k_grad_slice<true, true>);
+  } else {
+    if (ev->no_bulk) go(k_grad_slice<false, false>); else go(k_grad_slice<false, true>);
+  }
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
 

@@ -42,22 +42,24 @@ __global__ __launch_bounds__(256) void k_stage_gather(double* __restrict__ F, in
   smooth_record(F, npad, i, cand + (size_t)k * kCandS);
 }
 
-// Whether any particle carries a charge (the smoothing records' charge
-// entries, mass_X ext_X, are not all zero).
+// Which particles carry a charge (bit 1: a smoothing record's charge entry,
+// mass_X ext_X, is not zero) or a bulk pressure (bit 2: ext_Pi is not zero).
 __global__ __launch_bounds__(256) void k_any_charge(const double* __restrict__ F, int npad, int n,
                                                     int32_t* __restrict__ flag)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool any = false;
+  bool ch = false, bu = false;
   if (i < n) {
     auto f = [&](int c) { return F[(size_t)c * npad + i]; };
-    any = f(CCAKE_F_MASS_B) * f(CCAKE_F_EXT_B) != 0.0 || f(CCAKE_F_MASS_SS) * f(CCAKE_F_EXT_SS) != 0.0
-       || f(CCAKE_F_MASS_Q) * f(CCAKE_F_EXT_Q) != 0.0;
+    ch = f(CCAKE_F_MASS_B) * f(CCAKE_F_EXT_B) != 0.0 || f(CCAKE_F_MASS_SS) * f(CCAKE_F_EXT_SS) != 0.0
+      || f(CCAKE_F_MASS_Q) * f(CCAKE_F_EXT_Q) != 0.0;
+    bu = f(CCAKE_F_EXT_BULK) != 0.0;
   }
-  if (__ballot(any) != 0ull && (threadIdx.x & (kWave - 1)) == 0) atomicOr(flag, 1);
+  const int bits = (__ballot(ch) != 0ull ? 1 : 0) | (__ballot(bu) != 0ull ? 2 : 0);
+  if (bits && (threadIdx.x & (kWave - 1)) == 0) atomicOr(flag, bits);
 }
 
-int launch_charge_check(ccake_event* ev, bool* any)
+int launch_charge_check(ccake_event* ev, bool* any_charge, bool* any_bulk)
 {
   const int n = ev->n;
   int32_t* d = ev->err + 1;
@@ -67,7 +69,8 @@ int launch_charge_check(ccake_event* ev, bool* any)
   int32_t h = 0;
   CCAKE_HIP(hipMemcpyAsync(&h, d, sizeof(int32_t), hipMemcpyDeviceToHost, ev->stream));
   CCAKE_HIP(hipStreamSynchronize(ev->stream));
-  *any = h != 0;
+  *any_charge = (h & 1) != 0;
+  *any_bulk = (h & 2) != 0;
   return 0;
 }
 
