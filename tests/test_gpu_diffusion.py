@@ -144,3 +144,39 @@ def test_freeze_out_rows_carry_the_diffusion_currents(dev, orc):
         assert err <= 1e-12, (c, err)
     assert np.max(np.abs(orr[:, 2 + 27:2 + 39])) > 0.0          # the currents are in the rows
     p.close()
+
+
+def test_phase_calls_equal_the_fused_stage_with_diffusion(dev, orc):
+    """The phase-by-phase C-ABI (build_neighbors ... eom, each phase its own
+    call) gives the derivatives of the fused stage, diffusion and shear on:
+    q^mu rebuilt and regulated in reset_pi_tensor, kappa/tau_q in
+    viscosities, the diffusion sums in smooth_gradients."""
+    d = ic.lumpy(spacing=0.1)
+    cfg = config.lumpy_params(particlization=0, using_diffusion=1, kappa_over_T2=KAPPA, diffusion_regulator=1)
+    q = initial_q(d)
+    n = len(d["x"])
+    eos = ccake_amd.Eos(dev, 0, T_FO)
+    evs = []
+    for _ in range(2):
+        ev = ccake_amd.Event(dev, eos, cfg, n)
+        ccake_amd.load_event(ev, d)
+        for k in range(12):
+            ev.upload(fid(f"Q0+{k}"), q[k])
+        ev.initialize()
+        evs.append(ev)
+    a, b = evs
+    a.get_time_derivatives()
+    for phase in ("build_neighbors", "gamma_velocity", "smooth_fields"):
+        getattr(b, phase)()
+    b.eos_invert(0)
+    for phase in ("reset_pi_tensor", "viscosities", "smooth_gradients", "eom"):
+        getattr(b, phase)()
+    worst = 0.0
+    for lab in STATE + TRANSPORT + DERIVS + DIFF + DIFF_DERIVS + ALPHA:
+        x, y = a.download(fid(lab)), b.download(fid(lab))
+        s = np.max(np.abs(x))
+        e = float(np.max(np.abs(x - y)) / s) if s > 0 else float(np.max(np.abs(x - y)))
+        worst = max(worst, e)
+        assert e <= 1e-12, (lab, e)
+    print("phase calls vs fused stage, worst", worst, flush=True)
+    a.close(); b.close(); eos.close()
