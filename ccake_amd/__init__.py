@@ -18,7 +18,11 @@ from . import config  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(_HERE)
-LIB_PATH = os.path.join(_HERE, "libccake_amd.so")
+# The gfx950 library.  CCAKE_AMD_LIBRARY names another build of the same
+# sources; the tests use it for the host emulation build (tests/hostemu),
+# which runs the kernels on the CPU under AddressSanitizer to check memory
+# accesses.
+LIB_PATH = os.environ.get("CCAKE_AMD_LIBRARY") or os.path.join(_HERE, "libccake_amd.so")
 HEADER = os.path.join(ROOT, "include", "ccake_amd.h")
 
 

@@ -244,10 +244,16 @@ static int rk2_step(ccake_event* ev, double dt, bool more = false, bool* begun =
   return 0;
 }
 
+// Bytes added past every event buffer (the host emulation build sets 0 so
+// that any access past a buffer's end is caught).
+#ifndef CCAKE_ALLOC_PAD
+#define CCAKE_ALLOC_PAD 256
+#endif
+
 template <class T>
 static int dalloc(ccake_event* ev, T** p, size_t count)
 {
-  const size_t bytes = count * sizeof(T) + 256;
+  const size_t bytes = count * sizeof(T) + CCAKE_ALLOC_PAD;
   ccake_dev* d = ev->dev;
   {
     std::lock_guard<std::mutex> lk(d->mu);

@@ -22,12 +22,20 @@ enum PrivateField {
   P_C_X, P_C_Y, P_C_ETA,                           // r0
   P_C_S, P_C_B, P_C_SS, P_C_Q, P_C_BULK, P_C_EZ,   // extensive s, B, S, Q, Pi; Ez
   P_C_SHV0,                                        // 5 extensive shear
-  P_FB_E0 = P_C_SHV0 + 5,                          // fback..fback4 energy
-  P_FB_G0 = P_FB_E0 + 4,                           // fback..fback4 gradP_x
-  P_FRZ1_G, P_FRZ2_G,                              // frz1 / frz2 gradP_x
+  P_FB_E0 = P_C_SHV0 + 5,                          // fback..fback4 energy (4 columns)
+  P_FB_G0 = P_FB_E0 + 4,                           // fback..fback4 gradP_x (4 columns)
+  P_FRZ1_G = P_FB_G0 + 4, P_FRZ2_G,                // frz1 / frz2 gradP_x
   P_C_Q0,                                          // RK cache: 9 extensive diffusion
   NUM_COLUMNS = P_C_Q0 + 9
 };
+// Multi-column members own disjoint column ranges: every enumerator after a
+// block of k columns starts k past it (an overlap would let one member's
+// writes land in another's column, or past the last column of the buffer).
+static_assert(P_C_X == P_C_UX + 3 && P_C_S == P_C_X + 3 && P_C_SHV0 == P_C_S + 6, "RK cache columns overlap");
+static_assert(P_FB_E0 == P_C_SHV0 + 5 && P_FB_G0 == P_FB_E0 + 4 && P_FRZ1_G == P_FB_G0 + 4,
+              "freeze-out fallback columns overlap");
+static_assert(P_FRZ2_G == P_FRZ1_G + 1 && P_C_Q0 == P_FRZ2_G + 1 && NUM_COLUMNS == P_C_Q0 + 9,
+              "private columns overlap");
 
 // Uniform cell grid of one neighbour build (written on the device).
 struct Grid {

@@ -140,7 +140,7 @@ __global__ __launch_bounds__(kWave) void k_diff_slice(const double* __restrict__
     for_spans_slice(g, cell_off, seg, rcs, slice, nslices, in, [&](int j0, int j1) {
       walk_records<15, kCandD, 1>(cand, xa, ya, j0, j1, [&](const double* c, double dx, double dy, int j, bool valid) {
         const double r2 = dist2_nocontract(dx, dy);
-        if (kSkipEmpty && __ballot(valid && r2 <= rc2) == 0ull) return;
+        if (kSkipEmpty && wave_none(valid && r2 <= rc2)) return;
         double q, iq;
         q_and_inverse(r2, hinv2, q, iq);
         const double gm = valid ? c[2] * spline_g(q, iq) : 0.0;

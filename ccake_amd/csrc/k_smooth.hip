@@ -180,7 +180,7 @@ __global__ __launch_bounds__(kWave) void k_smooth_slice(const double* __restrict
         const double r2 = dist2_nocontract(dx, dy);
         const bool nb = valid && r2 <= rc2;
         cnt += nb ? 1 : 0;
-        if (kSkipEmpty && __ballot(nb) == 0ull) return;
+        if (kSkipEmpty && wave_none(nb)) return;
         double q, iq;
         q_and_inverse(r2, hinv2, q, iq);
         const double w = valid ? spline_w(q) : 0.0;

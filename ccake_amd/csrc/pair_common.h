@@ -148,6 +148,21 @@ __device__ __forceinline__ void row_run(const Grid& g, const int32_t* __restrict
 #endif
 constexpr bool kSkipEmpty = CCAKE_PAIR_SKIP != 0;
 
+// The skip test: no lane of the wave has `p` set.  It runs inside the walk's
+// body, on the lanes of one segment.  The host emulation build (kernels
+// compiled for the CPU to check memory accesses) runs every candidate
+// instead, since its lanes do not execute the body in lockstep; the records
+// are read before the test either way, so the accesses are the same.
+__device__ __forceinline__ bool wave_none(bool p)
+{
+#ifdef CCAKE_HOST_EMU
+  (void)p;
+  return false;
+#else
+  return __ballot(p) == 0ull;
+#endif
+}
+
 template <class Body>
 __device__ __forceinline__ void for_spans_slice(const Grid& g, const int32_t* __restrict__ cell_off,
                                                 const Seg& s, double rcs, int slice, int nslices, bool in,
