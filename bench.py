@@ -1,27 +1,30 @@
 """Throughput of the per-timestep SPH path on one MI355X.
 
-Default workload: ideal 2+1D Gubser flow, FP64, 40,401 particles on the
-0.05 fm lattice over [-5, 5]^2, conformal EoS, h = 0.3 fm, RK2 with
-dt = 0.005 fm/c from tau0 = 1 fm/c.  `--workload lumpy40k` runs the viscous
-BSQ lumpy event (50 Gaussian blobs, charges as scaled blobs, the
+Default workload (`--workload lumpy`): the viscous BSQ 2+1D lumpy event on
+the 0.02 fm lattice over [-5, 5]^2 (251,001 lattice points; the 200,081
+above the reference's default energy cutoff of 0.15 GeV/fm^3 are kept, as
+its initial-condition readers do, which is what lets the event reach
+freeze-out), 50 Gaussian blobs with charges as scaled blobs, the
 64x32x32x32 tanh-modulated conformal table EoS with its fallback chain,
-eta/s = 0.08, Gaussian zeta/s, particlization at T = 150 MeV) on the
-0.05 fm lattice, keeping the points above the reference's default energy
-cutoff (0.15 GeV/fm^3).  The 251,001-particle lumpy event is
-not offered: its initialisation faulted the GPU and is held off it until
-the cause is found.
+eta/s = 0.08, Gaussian zeta/s (0.1 at T = 0.17 GeV, width 0.02 GeV),
+particlization at T = 150 MeV, h = 0.3 fm, RK2 with dt = 0.02 fm/c from
+tau0 = 0.6 fm/c.  `--workload gubser` runs ideal 2+1D Gubser flow on the
+0.05 fm lattice (40,401 particles, conformal EoS, dt = 0.005 fm/c);
+`--workload lumpy40k` the lumpy event on the 0.05 fm lattice.
 
 A "step" is one RK2 time step of the whole event (two derivative
 evaluations: neighbour build, smoothing, EoS inversion, transport,
 gradients, equations of motion, RK update, freeze-out check).  Inputs are
 resident in HBM before the timed region.
 
-Prints one JSON line with the throughput in particle-steps/s, the roofline
-of the dominant kernel (measured with HIP events on the event's stream
-over the timed region), the CPU oracle timed on a bounded sample on the
-host cores, and a served multi-event measurement ("serving"): a batch of
-ideal Gubser events of uneven size run one at a time (the plain path) and
-several at a time on separate streams, as events/s with p50/p99 latency.
+Prints one JSON line with the throughput in particle-steps/s over the timed
+steps; the whole event run from initialize to complete freeze-out
+(SPHWorkstation::continue_evolution) on the plain path, as events/s; the
+roofline of the dominant kernel (measured with HIP events on the event's
+stream over the timed region); the CPU oracle timed on a bounded sample of
+the same event on the host cores, as a parallel and a reference-faithful
+(serial EoS loop) variant with per-phase times; and a served multi-event
+measurement ("serving").
 """
 from __future__ import annotations
 
@@ -39,22 +42,28 @@ if ROOT not in sys.path:
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
 
-# Algorithmic work per unit, FMA = 2 flop, sqrt/div = 1 (frozen constants).
-FLOP_PER_PAIR_SMOOTH = 28
-FLOP_PER_PAIR_GRAD_SHEAR = 95
-FLOP_PER_PAIR_GRAD_IDEAL = 62
+# Algorithmic work per unit, FMA = 2 flop, sqrt/div = 1 (frozen constants),
+# per launched variant of the pair kernels.
+# Smoothing, per directed pair: distance 5, sqrt 1, q 1, W 7, sigma_lab
+# m W (2), then m ext W (3) for each of s, rho_B, rho_S, rho_Q: 28; the
+# charge-free variant (k_smooth_slice<3>, an event without charges) 19.
+FLOP_PER_PAIR_SMOOTH = {True: 28, False: 19}               # charges on / off
+# Gradients, per directed pair (2D): 95 with shear on, 62 with it off; the
+# bulk-free variant (no grad Pi sums: an ideal fluid with zeta = 0) skips 4.
+FLOP_PER_PAIR_GRAD = {(True, True): 95, (True, False): 91, (False, True): 62, (False, False): 58}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", choices=["gubser", "lumpy40k"], default="gubser")
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", choices=["lumpy", "gubser", "lumpy40k"], default="lumpy")
+    ap.add_argument("--no-event", action="store_true", help="skip the run of the whole event to freeze-out")
     ap.add_argument("--diffusion", action="store_true",
                     help="BSQ diffusion on (diagonal kappa/T^2 = 0.1, regulator on, q = 0 at tau0)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle sample")
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cells-per-rc", type=int, default=3, help="neighbour-grid cells per cutoff radius")
     ap.add_argument("--no-serve", action="store_true", help="skip the served multi-event measurement")
     ap.add_argument("--serve-events", type=int, default=24)
@@ -73,18 +82,34 @@ def workload(name):
             hT=0.3, dt=0.005, tau0=1.0, rk=2)
     # points at or below the reference's default energy cutoff, 0.15 GeV/fm^3,
     # are dropped as its initial-condition readers do
-    d = ic.lumpy(spacing=0.05, e_cutoff_gev=0.15)
+    spacing = 0.02 if name == "lumpy" else 0.05
+    d = ic.lumpy(spacing=spacing, e_cutoff_gev=0.15)
     cfg = config.lumpy_params()
     table = eos_tables.tanh_conformal_table()
     return d, cfg, table, dict(
-        workload="viscous BSQ 2+1D lumpy event, 0.05 fm lattice, e > 0.15 GeV/fm^3, 64x32x32x32 table EoS",
-        particles=len(d["x"]), hT=cfg["hT"], dt=cfg["dt"], tau0=cfg["t0"], rk=2,
-        eta_over_s=0.08, zeta_over_s="gaussian 0.1 at T=0.17 GeV, width 0.02 GeV", T_freeze_out_GeV=0.15)
+        workload=f"viscous BSQ 2+1D lumpy event, {spacing} fm lattice over [-5,5]^2, e > 0.15 GeV/fm^3, "
+                 "64x32x32x32 table EoS, run to freeze-out at T = 150 MeV",
+        lattice_points=len(ic.lattice(spacing, 5.0)[0]), particles=len(d["x"]), hT=cfg["hT"], dt=cfg["dt"],
+        tau0=cfg["t0"], rk=2, eta_over_s=0.08, zeta_over_s="gaussian 0.1 at T=0.17 GeV, width 0.02 GeV",
+        T_freeze_out_GeV=0.15)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(d, cfg, table, steps):
     """The oracle (OpenMP C restatement) on the same event: initialise
-    untimed, then time `steps` RK2 steps of the full event."""
+    untimed, then time `steps` RK2 steps of the full event twice: with every
+    per-particle loop parallel, and reference-faithful with the EoS inversion
+    as a serial loop (src/sph_workstation.cpp:1737-1762).  Per-phase wall
+    times use the reference's phase names."""
     import oracle
     T_FO = 150.0 / 197.3269804
     oracle.build()
@@ -102,14 +127,26 @@ def cpu_baseline(d, cfg, table, steps):
         ev.set("ORC_F_" + f, d[k])
     if ev.initialize() != 0:
         raise RuntimeError("oracle initialize failed")
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        if ev.advance_timestep(cfg["dt"], 2) != 0:
-            raise RuntimeError("oracle step failed")
-    sec = time.perf_counter() - t0
-    return dict(value=n * steps / sec, unit="particle-steps/s", cores=int(oracle.get_threads()), kind="port",
-                sample=f"{steps} RK2 step(s) of the full {n}-particle event after an untimed initialize "
-                       f"({sec:.1f} s on {oracle.get_threads()} OpenMP threads)")
+    variants = {}
+    for name, serial in (("parallel", False), ("reference_serial_eos", True)):
+        oracle.set_serial_eos(serial)
+        oracle.phase_reset()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            if ev.advance_timestep(cfg["dt"], 2) != 0:
+                raise RuntimeError("oracle step failed")
+        sec = time.perf_counter() - t0
+        variants[name] = dict(particle_steps_per_s=round(n * steps / sec, 1), seconds=round(sec, 3),
+                              phases_s={k: round(v[0], 4) for k, v in oracle.phase_times().items()})
+        print(f"bench: CPU {name}: {sec:.2f} s for {steps} step(s)", file=sys.stderr, flush=True)
+    oracle.set_serial_eos(False)
+    thr = int(oracle.get_threads())
+    return dict(value=variants["parallel"]["particle_steps_per_s"], unit="particle-steps/s", cores=thr,
+                kind="port", cpu=cpu_model(),
+                sample=f"{steps} RK2 step(s) of the full {n}-particle event after an untimed initialize, "
+                       f"each variant; {thr} OpenMP threads (the EoS loop on one thread in the "
+                       f"reference-faithful variant)",
+                variants=variants)
 
 
 # Compulsory HBM bytes per particle of the fused per-particle kernel k_eos<0>
@@ -119,6 +156,13 @@ def cpu_baseline(d, cfg, table, steps):
 # doubles (31 thermo, Pi, 16 pi^{mu nu}, 13 transport, 16-double record)
 # + 1 int32.
 BYTES_PER_PARTICLE_EOS = 24 * 8 + 2 * 4 + 77 * 8 + 4
+
+# The fused EoS kernel of a stage (k_eos<3, .>) also runs the smoothing sum
+# pass: S slices x 6 partial sums read, gamma and btrack read, btrack and ten
+# smoothed and local-rest-frame densities written.
+def bytes_per_particle_eos(nslices):
+    return BYTES_PER_PARTICLE_EOS + 48 * nslices + 8 + 8 + 4 + 80
+
 
 KERNEL_OF_GROUP = {"SMOOTH": "k_smooth_slice", "GRAD": "k_grad_slice", "EOS": "k_eos"}
 
@@ -140,32 +184,58 @@ def traffic_of(kernel, workload):
     return round(k["fetch_kib"] * 1024.0 * f_scale + k["write_kib"] * 1024.0 * w_scale)
 
 
-def make_roofline(prof, pairs, n, cfg, workload):
+def make_roofline(prof, pairs, n, cfg, workload, charges, bulk, nslices):
     """Roofline of the dominant kernel group (by measured time) among the
     three with a work model: the smoothing and gradient pair kernels are
-    FP64-VALU bound (work = flop per directed pair x pairs per launch), the
-    fused EoS kernel is HBM bound (work = compulsory bytes per particle x N)."""
+    FP64-VALU bound (work = flop per directed pair of the launched variant x
+    pairs per launch; no matrix work on this path, so the bound is the FP64
+    vector rate), the fused EoS kernel is HBM bound (work = compulsory bytes
+    per particle x N)."""
     cand = {g: prof[g] for g in KERNEL_OF_GROUP if prof[g][1]}
     dom = max(cand, key=lambda g: cand[g][0])
     ms = cand[dom][0] / cand[dom][1]
-    sec = ms * 1e-3
+    sec = max(ms * 1e-3, 1e-12)
     if dom == "EOS":
-        work = BYTES_PER_PARTICLE_EOS * n
-        achieved = work / sec / 1e9
+        bpp = bytes_per_particle_eos(nslices)
+        achieved = bpp * n / sec / 1e9
         r = dict(bound="hbm", achieved=round(achieved, 2), peak=PEAK_HBM_GBS, unit="GB/s",
                  frac=round(achieved / PEAK_HBM_GBS, 4),
-                 work=f"{BYTES_PER_PARTICLE_EOS} B/particle x {n} particles per launch")
+                 work=f"{bpp} B/particle x {n} particles per launch")
     else:
-        fpp = FLOP_PER_PAIR_SMOOTH if dom == "SMOOTH" else (
-            FLOP_PER_PAIR_GRAD_SHEAR if cfg["using_shear"] else FLOP_PER_PAIR_GRAD_IDEAL)
+        shear = bool(cfg["using_shear"])
+        fpp = FLOP_PER_PAIR_SMOOTH[charges] if dom == "SMOOTH" else FLOP_PER_PAIR_GRAD[(shear, bulk)]
         achieved = fpp * pairs / sec / 1e12
-        r = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
+        r = dict(bound="valu", achieved=round(achieved, 3), peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
                  frac=round(achieved / PEAK_FP64_TFLOPS, 4), engine="FP64 VALU (no matrix work on this path)",
                  work=f"{fpp} flop/pair x {pairs:.0f} pairs per launch")
     r["traffic"] = traffic_of(KERNEL_OF_GROUP[dom], workload)
     r["kernel"] = KERNEL_OF_GROUP[dom]
     r["avg_launch_ms"] = round(ms, 4)
     return r
+
+
+def whole_event(dev, eos, d, cfg, max_tau=30.0):
+    """One event from initialize until every particle has frozen out
+    (SPHWorkstation::continue_evolution checked after every step, as
+    BSQHydro::run does), wall-clocked on the host: the plain path's event
+    latency."""
+    import ccake_amd
+    n = len(d["x"])
+    ev = ccake_amd.Event(dev, eos, cfg, n)
+    ccake_amd.load_event(ev, d)
+    dev.sync()
+    t0 = time.perf_counter()
+    ev.initialize()
+    steps = 0
+    while ev.continue_evolution(max_tau=max_tau):
+        ev.advance_timestep(cfg["dt"], 2)
+        steps += 1
+    dev.sync()
+    sec = time.perf_counter() - t0
+    tau = ev.t
+    ev.close()
+    return dict(seconds=round(sec, 4), events_per_s=round(1.0 / sec, 4), steps=steps, tau_end=round(tau, 4),
+                particle_steps_per_s=round(n * steps / sec, 1))
 
 
 def serving_requests(k, steps, seed=0x5E4F):
@@ -302,7 +372,11 @@ def main():
     print(f"bench: {pairs_per_launch:.0f} directed pairs per stage, {sec / steps * 1e3:.3f} ms per step",
           file=sys.stderr, flush=True)
 
-    roofline = make_roofline(prof, pairs_per_launch, n, cfg, args.workload)
+    import numpy as np
+    charges = any(np.any(d[k] != 0) for k in ("rhoB", "rhoS", "rhoQ")) or bool(cfg["using_diffusion"])
+    bulk = not (cfg["zeta_mode"] == config.ZETA_CONSTANT and cfg["constant_zeta_over_s"] == 0.0
+                and not cfg["using_shear"])
+    roofline = make_roofline(prof, pairs_per_launch, n, cfg, args.workload, charges, bulk, ev.nslices())
     out = {
         "metric": "particle-steps/s",
         "value": n * steps / sec,
@@ -320,7 +394,12 @@ def main():
         "roofline": roofline,
         "stage_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items() if v[1]},
     }
-    ev.close(); eos.close()
+    ev.close()
+    if not args.no_event and cfg["particlization"]:
+        print("bench: whole event from initialize to freeze-out", file=sys.stderr, flush=True)
+        out["event_to_freeze_out"] = whole_event(dev, eos, d, cfg)
+        print(f"bench: event {json.dumps(out['event_to_freeze_out'])}", file=sys.stderr, flush=True)
+    eos.close()
     if not args.no_serve:
         print(f"bench: serving {args.serve_events} events, {args.serve_slots} slots", file=sys.stderr, flush=True)
         out["serving"] = serving_run(dev, args.serve_events, args.serve_slots)

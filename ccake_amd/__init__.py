@@ -143,6 +143,7 @@ ABI = {
     "ccake_profile_reset": (C.c_int, [_vp]),
     "ccake_last_pair_count": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
     "ccake_pair_total": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int64)]),
+    "ccake_event_pair_slices": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
 }
 
 _lib = None
@@ -379,6 +380,13 @@ class Event:
         _ok(lib().ccake_freeze_out_surface(self.h, rows.ctypes.data_as(_dp), n.value, C.byref(n)),
             "freeze_out_surface")
         return rows[: n.value]
+
+    def nslices(self, set_to=0):
+        """Slices per tile of the pair kernels; set_to > 0 changes it (up to
+        the count the event was created with)."""
+        v = C.c_int()
+        _ok(lib().ccake_event_pair_slices(self.h, int(set_to), C.byref(v)), "pair_slices")
+        return v.value
 
     def pair_total(self, reset=False):
         v = C.c_int64()

@@ -607,7 +607,8 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
     const double v = std::atof(ov);
     if (v >= 0.0 && v <= 1.0) ev->skin_frac = v;
   }
-  rc |= dalloc(ev, &ev->part, (size_t)ev->nslices * kMaxAcc * np);
+  ev->nslices_cap = ev->nslices;
+  rc |= dalloc(ev, &ev->part, (size_t)ev->nslices_cap * kMaxAcc * np);
   rc |= dalloc(ev, &ev->cand_s, 8 * np);
   rc |= dalloc(ev, &ev->cand_g, 16 * np);
   if (p->using_diffusion) rc |= dalloc(ev, &ev->cand_d, 16 * np);
@@ -961,6 +962,17 @@ int ccake_last_pair_count(ccake_event* ev, int64_t* n_pairs)
   CCAKE_HIP(hipMemcpyAsync(&v, ev->pair_count, sizeof(int64_t), hipMemcpyDeviceToHost, ev->stream));
   CCAKE_HIP(hipStreamSynchronize(ev->stream));
   *n_pairs = v;
+  return 0;
+}
+
+int ccake_event_pair_slices(ccake_event* ev, int set, int* nslices)
+{
+  if (!ev) return CCAKE_ERR_INVALID;
+  if (set > 0) {
+    if (set > ev->nslices_cap) { set_error("slice count above the event's partial-sum capacity"); return CCAKE_ERR_INVALID; }
+    ev->nslices = set;
+  }
+  if (nslices) *nslices = ev->nslices;
   return 0;
 }
 

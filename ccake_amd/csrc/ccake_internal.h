@@ -147,6 +147,7 @@ struct ccake_event {
   int fo_step = 0;                  // steps since initialize
   // pair-kernel partial sums: nslices x (up to 17) x npad, sorted order
   int nslices = 1;
+  int nslices_cap = 1;          // slices the partial-sum buffer holds
   double* part = nullptr;
   // gathered candidates (sorted order)
   double* cand_s = nullptr;     // 8 doubles per particle

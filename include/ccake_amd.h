@@ -341,6 +341,12 @@ CCAKE_API int  ccake_last_pair_count(ccake_event* ev, int64_t* n_pairs);
  * created or since the last call with reset != 0 (the work count behind the
  * pair kernels' throughput).  Diagnostic; no counterpart in the reference. */
 CCAKE_API int  ccake_pair_total(ccake_event* ev, int reset, int64_t* n_pairs);
+/* Slices per tile of the pair kernels (the work split of the smoothing and
+ * gradient passes).  set > 0 chooses a new count, at most the count the
+ * event was created with (its partial-sum buffer is sized for that); the
+ * current count is returned in *nslices.  Results do not depend on it beyond
+ * summation order.  Tuning; no counterpart in the reference. */
+CCAKE_API int  ccake_event_pair_slices(ccake_event* ev, int set, int* nslices);
 
 #ifdef __cplusplus
 }

@@ -133,6 +133,9 @@ def lib():
             "orc_reduce": (C.c_double, [vp, C.c_int]),
             "orc_set_threads": (None, [C.c_int]),
             "orc_get_threads": (C.c_int, []),
+            "orc_set_serial_eos": (None, [C.c_int]),
+            "orc_phase_reset": (None, []),
+            "orc_phase_read": (None, [dp, C.POINTER(C.c_longlong)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -265,3 +268,28 @@ def set_threads(n):
 
 def get_threads():
     return lib().orc_get_threads()
+
+
+# The reference's phase names (src/sph_workstation.cpp:1543-1583), indexed
+# as orc_phase_read returns them.
+PHASE_NAMES = {1: "01_neighbour_list", 2: "02_gamma_velocity", 3: "03_smooth_fields", 4: "04_thermodynamics",
+               5: "05_reset_pi_tensor", 6: "06_regulate_diffusion", 8: "08_viscosities",
+               9: "09_smooth_gradients", 10: "10_time_derivatives", 11: "rk_update", 12: "freeze_out"}
+
+
+def set_serial_eos(on):
+    """Run the EoS inversion loop on one thread (the reference's serial loop)."""
+    lib().orc_set_serial_eos(1 if on else 0)
+
+
+def phase_reset():
+    lib().orc_phase_reset()
+
+
+def phase_times():
+    """{phase name: (seconds, calls)} accumulated since phase_reset()."""
+    n = 13
+    s = (C.c_double * n)()
+    c = (C.c_longlong * n)()
+    lib().orc_phase_read(s, c)
+    return {PHASE_NAMES[k]: (s[k], c[k]) for k in PHASE_NAMES if c[k]}

@@ -199,6 +199,14 @@ double   orc_reduce(orc_event* ev, int which);
 /* thread count used by the OpenMP loops (0 = runtime default) */
 void     orc_set_threads(int n);
 int      orc_get_threads(void);
+/* Timing variants of the CPU baseline: the EoS inversion on one thread (the
+ * reference's serial host loop), and wall time per phase (index k = the
+ * reference's phase 0k of get_time_derivatives; 11 the RK cache and
+ * updates; 12 the freeze-out step).                                       */
+#define ORC_NUM_PHASES 13
+void     orc_set_serial_eos(int on);
+void     orc_phase_reset(void);
+void     orc_phase_read(double* seconds, long long* calls);
 
 #ifdef __cplusplus
 }

@@ -18,6 +18,33 @@ int orc_nthreads(void)
 }
 int orc_get_threads(void) { return orc_nthreads(); }
 
+/* The reference's EoS loop is a serial host loop
+ * (src/sph_workstation.cpp:1737-1762); orc_set_serial_eos(1) runs the
+ * oracle's EoS inversion on one thread to time that variant.            */
+static int g_serial_eos = 0;
+void orc_set_serial_eos(int on) { g_serial_eos = on != 0; }
+
+/* Wall time per phase of SPHWorkstation::get_time_derivatives, under the
+ * reference's phase names 01-10 (src/sph_workstation.cpp:1543-1583, the
+ * CCAKE_PHASE profiler of include/phase_profiler.h), plus the RK update and
+ * the freeze-out step of advance_timestep as phases 11 and 12.           */
+static double g_phase_s[ORC_NUM_PHASES];
+static long long g_phase_n[ORC_NUM_PHASES];
+void orc_phase_reset(void) { memset(g_phase_s, 0, sizeof g_phase_s); memset(g_phase_n, 0, sizeof g_phase_n); }
+void orc_phase_read(double* seconds, long long* calls)
+{
+  for (int k = 0; k < ORC_NUM_PHASES; ++k) { seconds[k] = g_phase_s[k]; calls[k] = g_phase_n[k]; }
+}
+static double now_s(void)
+{
+#ifdef _OPENMP
+  return omp_get_wtime();
+#else
+  return 0.0;
+#endif
+}
+#define PHASE(k, stmt) do { const double t_ = now_s(); stmt; g_phase_s[k] += now_s() - t_; g_phase_n[k]++; } while (0)
+
 /* ------------------------------------------------------------------ */
 /* SPHkernel<D> (src/kernel.cpp:10-118): cubic spline, q = r/h.        */
 static const double KNORM[4]  = { 0.0, 2./3., 10./(7*ORC_PI), 1./ORC_PI };
@@ -498,7 +525,7 @@ int orc_eos_invert(orc_event* ev, int mode)
 {
   if (ev->eos->pre) return orc_preinv_invert(ev, mode);   /* online_inverter_enabled = false */
   int fail = 0;
-  #pragma omp parallel for schedule(dynamic, 64) num_threads(orc_nthreads()) reduction(|:fail)
+  #pragma omp parallel for schedule(dynamic, 64) num_threads(g_serial_eos ? 1 : orc_nthreads()) reduction(|:fail)
   for (int i = 0; i < ev->n; ++i) {
     double x[4] = { FLD(ev, ORC_F_T)[i], FLD(ev, ORC_F_MUB)[i], FLD(ev, ORC_F_MUQ)[i], FLD(ev, ORC_F_MUS)[i] };
     int member = ev->I[ORC_I_EOS_ID][i];
@@ -529,17 +556,18 @@ int orc_eos_invert(orc_event* ev, int mode)
 /* SPHWorkstation::get_time_derivatives (src/sph_workstation.cpp:1509-1612) */
 int orc_get_time_derivatives(orc_event* ev)
 {
-  orc_build_neighbors(ev, 0);              /* 01 */
-  orc_gamma_velocity(ev);                  /* 02 */
-  orc_smooth_fields(ev);                   /* 03 */
-  int st = orc_eos_invert(ev, 0);          /* 04 */
+  int st = 0;
+  PHASE(1, orc_build_neighbors(ev, 0));    /* 01_neighbour_list */
+  PHASE(2, orc_gamma_velocity(ev));        /* 02_gamma_velocity */
+  PHASE(3, orc_smooth_fields(ev));         /* 03_smooth_fields */
+  PHASE(4, st = orc_eos_invert(ev, 0));    /* 04_thermodynamics */
   if (st) return st;
-  orc_reset_pi_tensor(ev);                 /* 05 */
+  PHASE(5, orc_reset_pi_tensor(ev));       /* 05_reset_pi_tensor */
   if (ev->p.using_diffusion && ev->p.diffusion_regulator)
-    orc_regulate_diffusion(ev);            /* 06 */
-  orc_viscosities(ev);                     /* 08 */
-  orc_smooth_gradients(ev);                /* 09 */
-  orc_eom(ev);                             /* 10 */
+    PHASE(6, orc_regulate_diffusion(ev));  /* 06_regulate_diffusion */
+  PHASE(8, orc_viscosities(ev));           /* 08_viscosities */
+  PHASE(9, orc_smooth_gradients(ev));      /* 09_smooth_gradients */
+  PHASE(10, orc_eom(ev));                  /* 10_time_derivatives */
   return ORC_OK;
 }
 
@@ -807,17 +835,17 @@ int orc_continue_evolution(orc_event* ev, double max_tau)
 int orc_advance_timestep(orc_event* ev, double dt, int rk_order)
 {
   if (rk_order != 2) return ORC_ERR_INVALID;
-  orc_cache_state(ev);
+  PHASE(11, orc_cache_state(ev));
   const double t0 = ev->t;
   int st = orc_get_time_derivatives(ev);
   if (st) return st;
-  st = orc_rk_update(ev, 0.5*dt, t0);
+  PHASE(11, st = orc_rk_update(ev, 0.5*dt, t0));
   if (st) return st;
   st = orc_get_time_derivatives(ev);
   if (st) return st;
-  st = orc_rk_update(ev, dt, t0);
+  PHASE(11, st = orc_rk_update(ev, dt, t0));
   if (st) return st;
-  if (ev->p.particlization) orc_freeze_out_particles(ev);
+  if (ev->p.particlization) PHASE(12, orc_freeze_out_particles(ev));
   return ORC_OK;
 }
 
