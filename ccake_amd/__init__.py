@@ -133,6 +133,9 @@ ABI = {
     "ccake_advance_timestep": (C.c_int, [_vp, C.c_double, C.c_int]),
     "ccake_run_steps": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     "ccake_initialize": (C.c_int, [_vp]),
+    "ccake_process_initial_conditions": (C.c_int, [_vp]),
+    "ccake_initialize_entropy_and_charge_densities": (C.c_int, [_vp]),
+    "ccake_initial_smoothing": (C.c_int, [_vp]),
     "ccake_freeze_out_particles": (C.c_int, [_vp]),
     "ccake_continue_evolution": (C.c_int, [_vp, C.c_double, C.POINTER(C.c_int)]),
     "ccake_freeze_out_surface": (C.c_int, [_vp, _dp, C.c_size_t, C.POINTER(C.c_size_t)]),
@@ -338,6 +341,9 @@ class Event:
     def rk_update(self, dt, t0): self._call("rk_update", float(dt), float(t0))
     def advance_timestep(self, dt, rk_order=2): self._call("advance_timestep", float(dt), int(rk_order))
     def initialize(self): self._call("initialize")
+    def process_initial_conditions(self): self._call("process_initial_conditions")
+    def initialize_entropy_and_charge_densities(self): self._call("initialize_entropy_and_charge_densities")
+    def initial_smoothing(self): self._call("initial_smoothing")
     def freeze_out_particles(self): self._call("freeze_out_particles")
     def check(self): self._call("event_check")
 

@@ -67,7 +67,7 @@ def build(verbose=False):
 
 HOST = os.path.join(HERE, "host")
 BIN = os.path.join(HERE, "bin")
-HOST_PROGRAMS = ["gubser_run", "ic_dump"]
+HOST_PROGRAMS = ["gubser_run", "ic_dump", "bsqhydro_run"]
 
 
 def build_host_programs():

@@ -824,26 +824,42 @@ int ccake_run_steps(ccake_event* ev, double dt, int rk_order, int n_steps, int* 
   return ccake_event_check(ev);
 }
 
-int ccake_initialize(ccake_event* ev)
+int ccake_process_initial_conditions(ccake_event* ev)
 {
   if (!ev) return CCAKE_ERR_INVALID;
-  int rc;
   ev->t = ev->p.t0;
   ev->frzc = 0;
   ev->fo_step = 0;
   ev->taup = ev->taupp = 0.0;
+  ev->init_phase = 1;
   if (ev->snaps) {                                          // a new surface
     CCAKE_HIP(hipMemsetAsync(ev->snaps, 0, (size_t)6 * ev->snap_cols * ev->npad * sizeof(double), ev->stream));
     CCAKE_HIP(hipMemsetAsync(ev->fo_count, 0, sizeof(int), ev->stream));
   }
-  if ((rc = launch_init_particles(ev, 0))) return rc;       // process_initial_conditions
-  if ((rc = ccake_build_neighbors(ev))) return rc;          // initialize_linklist
+  return launch_init_particles(ev, 0);
+}
+
+int ccake_initialize_entropy_and_charge_densities(ccake_event* ev)
+{
+  if (!ev) return CCAKE_ERR_INVALID;
+  if (ev->init_phase < 1) { set_error("process_initial_conditions first"); return CCAKE_ERR_INVALID; }
+  int rc;
+  if ((rc = ensure_sorted(ev))) return rc;                  // initialize_linklist, if the caller has not
   if ((rc = launch_eos(ev, 1, 0))) return rc;               // e -> s inversion
-  if ((rc = launch_init_particles(ev, 1))) return rc;
+  if ((rc = launch_init_particles(ev, 1))) return rc;       // freeze below e_freeze
   if ((rc = ccake_initial_sigma(ev))) return rc;            // calculate_intial_sigma
   if ((rc = launch_init_particles(ev, 2))) return rc;       // init_particles
+  ev->init_phase = 2;
+  return 0;
+}
+
+int ccake_initial_smoothing(ccake_event* ev)
+{
+  if (!ev) return CCAKE_ERR_INVALID;
+  if (ev->init_phase < 2) { set_error("initialize_entropy_and_charge_densities first"); return CCAKE_ERR_INVALID; }
+  int rc;
   if ((rc = launch_stage_begin(ev))) return rc;             // calculate_gamma_and_velocities
-  if ((rc = ccake_smooth_fields(ev))) return rc;            // initial_smoothing
+  if ((rc = ccake_smooth_fields(ev))) return rc;            // smooth_all_particle_fields
   if ((rc = launch_init_particles(ev, 3))) return rc;       // extensive shear, bulk, set_diffusion
   if ((rc = launch_reset_pi(ev))) return rc;
   if (ev->p.using_diffusion && (rc = launch_diff_prep(ev, 1, nullptr))) return rc;   // q^mu
@@ -858,7 +874,18 @@ int ccake_initialize(ccake_event* ev)
   ev->no_charges = !ev->p.using_diffusion && !any_charge;
   ev->no_bulk = ev->p.zeta_mode == CCAKE_ZETA_CONSTANT && ev->p.constant_zeta_over_s == 0.0 && !ev->p.using_shear
                 && !any_bulk;
+  ev->init_phase = 3;
   return ccake_event_check(ev);
+}
+
+int ccake_initialize(ccake_event* ev)
+{
+  if (!ev) return CCAKE_ERR_INVALID;
+  int rc;
+  if ((rc = ccake_process_initial_conditions(ev))) return rc;
+  if ((rc = ccake_build_neighbors(ev))) return rc;          // initialize_linklist
+  if ((rc = ccake_initialize_entropy_and_charge_densities(ev))) return rc;
+  return ccake_initial_smoothing(ev);                       // begins with calculate_gamma_and_velocities
 }
 
 int ccake_freeze_out_particles(ccake_event* ev)

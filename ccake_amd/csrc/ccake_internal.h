@@ -145,6 +145,7 @@ struct ccake_event {
   int* fo_count = nullptr;
   double taup = 0.0, taupp = 0.0;   // times of the last two freeze-out snapshots
   int fo_step = 0;                  // steps since initialize
+  int init_phase = 0;               // initialisation phases done (1, 2, 3)
   // pair-kernel partial sums: nslices x (up to 17) x npad, sorted order
   int nslices = 1;
   int nslices_cap = 1;          // slices the partial-sum buffer holds

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <functional>
 #include <iomanip>
@@ -192,15 +193,37 @@ class SystemState {
   }
   int n() const { return n_particles; }
 
-  // Creates the device event (the reference's Cabana AoSoA) for the
-  // particles added so far.
+  // SystemState::allocate_cabana_particles (src/system_state.cpp:58-63):
+  // the device event (the reference's Cabana AoSoA) for the particles added
+  // so far, filled from them (copy_host_to_device).  An existing event of
+  // the same size and settings is kept and refilled.  If
+  // SPHWorkstation::process_initial_conditions ran on the host particles,
+  // its device part runs here, so the device holds the processed particles
+  // as the reference's copy does.
   void allocate_cabana_particles()
   {
-    if (ev) { ccake_event_destroy(ev); ev = nullptr; }
     const ccake_params p = params();
-    check(ccake_event_create(dev, &p, eos, n_particles, &ev), "ccake_event_create", settingsPtr.get());
+    if (ev && (ev_n != n_particles || std::memcmp(&ev_params, &p, sizeof p) != 0)) {
+      ccake_event_destroy(ev);
+      ev = nullptr;
+    }
+    if (!ev) {
+      check(ccake_event_create(dev, &p, eos, n_particles, &ev), "ccake_event_create", settingsPtr.get());
+      ev_n = n_particles;
+      ev_params = p;
+    }
+    copy_host_to_device();
+    if (ic_processed)
+      check(ccake_process_initial_conditions(ev), "ccake_process_initial_conditions", settingsPtr.get());
   }
-  void initialize() { allocate_cabana_particles(); }
+  // SystemState::initialize (src/system_state.cpp:40-53)
+  void initialize()
+  {
+    number_of_elapsed_timesteps = 0;
+    t = settingsPtr->t0;
+  }
+  // set by SPHWorkstation::process_initial_conditions
+  bool ic_processed = false;
 
   // Initial-condition fields host -> device (the path's inputs).
   void copy_host_to_device()
@@ -386,6 +409,8 @@ class SystemState {
   ccake_dev* dev = nullptr;
   ccake_eos* eos = nullptr;
   ccake_event* ev = nullptr;
+  int ev_n = 0;                 // size and settings the event was created with
+  ccake_params ev_params{};
 };
 
 // Evolver<D> (src/evolver.cpp:97-517): the RK cache lives on the device.
@@ -555,28 +580,63 @@ class SPHWorkstation {
   SPHWorkstation(std::shared_ptr<Settings> s, std::shared_ptr<SystemState<D>> sys)
     : evolver(s, sys), freezePtr(std::make_shared<FreezeOut<D>>(sys)), settingsPtr(s), systemPtr(sys) {}
 
-  // SPHWorkstation::initialize (src/sph_workstation.cpp:29-110): e_freeze from
-  // the EoS at T_FO, then the device event.
+  // SPHWorkstation::initialize (src/sph_workstation.cpp:29-110):
+  // SystemState::initialize and e_freeze from the EoS at T_FO.
   void initialize()
   {
+    systemPtr->initialize();
     double ef = 0.0;
     check(ccake_eos_efreeze(systemPtr->equation_of_state(), settingsPtr->Freeze_Out_Temperature, &ef),
           "ccake_eos_efreeze", settingsPtr.get());
     systemPtr->efcheck = ef;
-    systemPtr->t = settingsPtr->t0;
-    systemPtr->allocate_cabana_particles();
-    systemPtr->copy_host_to_device();
   }
-  void setup_freeze_out() {}
+  void setup_freeze_out() {}   // the freeze-out state lives in the event (snapshots, status)
 
-  // The initialisation sequence of BSQHydro::initialize_hydrodynamics
-  // (src/BSQHydro.cpp:568-640) in one device call: process_initial_conditions,
-  // initialize_linklist, initialize_entropy_and_charge_densities,
-  // calculate_intial_sigma, init_particles, calculate_gamma_and_velocities,
-  // initial_smoothing, calculate_extensive_shv, set_bulk_Pi, reset_pi_tensor,
-  // update_all_particle_thermodynamics and the first freeze-out check.
+  // SPHWorkstation::process_initial_conditions (src/sph_workstation.cpp:1228-1340)
+  // on the host particles: gamma from u, extensive densities and SPH masses 1,
+  // the Newton seed T = 580 MeV, mu = 0, IDs.  The device event gets the same
+  // values when allocate_cabana_particles fills it.
+  void process_initial_conditions()
+  {
+    const double t2 = settingsPtr->t0 * settingsPtr->t0;
+    int id = 0;
+    for (Particle<D>& p : systemPtr->particles) {
+      p.gamma = TEOM<D>::gamma_calc(p.u, t2);
+      p.s_ext = p.rhoB_ext = p.rhoS_ext = p.rhoQ_ext = 1.0;
+      p.T_ = 580.0 / 197.3269804;
+      p.muB = p.muS = p.muQ = 0.0;
+      p.eos_name = "default";
+      p.ID = id++;
+    }
+    systemPtr->n_particles = (int)systemPtr->particles.size();
+    systemPtr->ic_processed = true;
+    if (systemPtr->event())       // an event already allocated: process it too
+      call(ccake_process_initial_conditions(ev()), "process_initial_conditions");
+  }
+  // SPHWorkstation::initialize_entropy_and_charge_densities (252-375)
+  void initialize_entropy_and_charge_densities()
+  {
+    call(ccake_initialize_entropy_and_charge_densities(ev()), "initialize_entropy_and_charge_densities");
+  }
+  // SPHWorkstation::initial_smoothing (394-457), with its initial energy
+  // conservation record
+  void initial_smoothing()
+  {
+    call(ccake_initial_smoothing(ev()), "initial_smoothing");
+    check(ccake_event_get_time(ev(), &systemPtr->t), "ccake_event_get_time", settingsPtr.get());
+    systemPtr->conservation_energy(true, settingsPtr->t0);
+  }
+
+  // The whole of BSQHydro::initialize_hydrodynamics (src/BSQHydro.cpp:568-640)
+  // in one device call (allocating and filling the event first if the caller
+  // has not): process_initial_conditions, initialize_linklist,
+  // initialize_entropy_and_charge_densities, calculate_gamma_and_velocities,
+  // initial_smoothing (calculate_extensive_shv, set_bulk_Pi, set_diffusion,
+  // reset_pi_tensor, update_all_particle_thermodynamics, the first
+  // freeze-out check), then the initial conservation records.
   void initialize_hydrodynamics()
   {
+    if (!systemPtr->event()) systemPtr->allocate_cabana_particles();
     check(ccake_initialize(ev()), "ccake_initialize", settingsPtr.get());
     check(ccake_event_get_time(ev(), &systemPtr->t), "ccake_event_get_time", settingsPtr.get());
     systemPtr->conservation_entropy(true);

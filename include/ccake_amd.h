@@ -286,10 +286,27 @@ CCAKE_API int  ccake_rk_update(ccake_event* ev, double dt, double t0);
 CCAKE_API int  ccake_advance_timestep(ccake_event* ev, double dt, int rk_order);
 /* Run `n_steps` timesteps; stops early when continue_evolution is false. */
 CCAKE_API int  ccake_run_steps(ccake_event* ev, double dt, int rk_order, int n_steps, int* steps_done);
-/* Initialisation once per event, after upload of positions, u, input e
- * (or s) and charges, pi and Pi (BSQHydro::initialize_hydrodynamics,
- * src/BSQHydro.cpp:568-646): process_initial_conditions,
- * initialize_entropy_and_charge_densities, initial_smoothing. */
+/* The initialisation phases, in the order BSQHydro::initialize_hydrodynamics
+ * calls them (src/BSQHydro.cpp:576-602), after upload of positions, u, input
+ * e and charges, pi and Pi:
+ * SPHWorkstation::process_initial_conditions (src/sph_workstation.cpp:1228-1340):
+ * gamma from u, extensive densities and SPH masses 1, the Newton seed
+ * (580 MeV, mu = 0), Freeze and btrack 0; the event's time to t0. */
+CCAKE_API int  ccake_process_initial_conditions(ccake_event* ev);
+/* SPHWorkstation::initialize_entropy_and_charge_densities (252-375): the
+ * e -> s inversion of the inputs (freezing particles below e_freeze, and
+ * those without a solution at their charges), calculate_intial_sigma and
+ * the extensive densities input / sigma.  Needs the neighbour build. */
+CCAKE_API int  ccake_initialize_entropy_and_charge_densities(ccake_event* ev);
+/* SPHWorkstation::initial_smoothing (394-457): gamma and v, the smoothing
+ * pass, calculate_extensive_shv, set_bulk_Pi, set_diffusion,
+ * reset_pi_tensor, update_all_particle_thermodynamics and the first
+ * check_freeze_out_status. */
+CCAKE_API int  ccake_initial_smoothing(ccake_event* ev);
+/* All of initialisation in one call: process_initial_conditions,
+ * the neighbour build, initialize_entropy_and_charge_densities,
+ * calculate_gamma_and_velocities and initial_smoothing
+ * (BSQHydro::initialize_hydrodynamics, src/BSQHydro.cpp:568-646). */
 CCAKE_API int  ccake_initialize(ccake_event* ev);
 /* FreezeOut::check_freeze_out_status + status transitions of bsqsvfreezeout
  * (include/freeze_out.h:322-542, 549-626). */

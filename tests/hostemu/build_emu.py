@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import concurrent.futures as cf
 import os
+import shutil
 import subprocess
 import sys
 
@@ -53,6 +54,12 @@ def build(asan=True):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
+    if asan:
+        # the same library under the product's name, for the C++ drivers of
+        # ccake_amd/bin run with LD_LIBRARY_PATH pointing here
+        lib_dir = os.path.join(OUT_DIR, "lib")
+        os.makedirs(lib_dir, exist_ok=True)
+        shutil.copyfile(out, os.path.join(lib_dir, "libccake_amd.so"))
     return out
 
 

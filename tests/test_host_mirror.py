@@ -76,3 +76,43 @@ def test_freeze_out_dat_through_cpp_call_surface(tmp_path, orc):
     assert got.shape == want.shape and len(got) > 300
     colmax = np.max(np.abs(want), axis=0)
     assert np.all(np.abs(got - want) <= 6e-6 * np.abs(want) + 1e-12 * colmax)
+
+
+BSQ = os.path.join(ROOT, "ccake_amd", "bin", "bsqhydro_run")
+
+
+def write_bsqhydro_inputs(path, spacing):
+    """A ccake-format initial condition of the lumpy event (every lattice
+    point; the reader applies the 0.15 GeV/fm^3 cutoff) and the table file."""
+    import numpy as np
+    from ccake_amd import eos_tables, ic
+    d = ic.lumpy(spacing=spacing)
+    ic.write_ccake(os.path.join(path, "ic.dat"), d, grid=(spacing, spacing, 0.0, -5.0, -5.0, 0.0))
+    tab, dims, mins, maxs = eos_tables.tanh_conformal_table()
+    with open(os.path.join(path, "table.bin"), "wb") as f:
+        np.asarray(dims, dtype=np.int32).tofile(f)
+        np.asarray(mins, dtype=np.float64).tofile(f)
+        np.asarray(maxs, dtype=np.float64).tofile(f)
+        np.ascontiguousarray(tab, dtype=np.float64).tofile(f)
+
+
+@pytest.mark.gpu
+def test_bsqhydro_sequence_equals_fused_initialize(tmp_path):
+    """The reference's driver sequence, BSQHydro::initialize_hydrodynamics
+    then BSQHydro::run's loop (src/BSQHydro.cpp:576-602, 745-797), call for
+    call on the C++ call surface, on the lumpy event (0.1 fm lattice, table
+    EoS, 8,077 particles after the reader's cutoff), run to complete
+    freeze-out: every field, every integer field and every freeze-out row
+    equal, bit for bit, to the same event initialised with the fused
+    SPHWorkstation::initialize_hydrodynamics."""
+    write_bsqhydro_inputs(str(tmp_path), 0.1)
+    r = subprocess.run([BSQ, "--ic", str(tmp_path / "ic.dat"), "--table", str(tmp_path / "table.bin"),
+                        "--output", str(tmp_path)], capture_output=True, text=True, timeout=600)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    assert d["particles"] == 8077
+    assert d["steps_sequence"] == d["steps_fused"] and d["t_sequence"] == d["t_fused"]
+    assert d["fields_differing"] == 0 and d["int_fields_differing"] == 0
+    assert d["rows_equal"] and d["rows"] == d["particles"] == d["frozen_out"]
+    assert d["freeze_out_lines_buffered"] > 0

@@ -78,3 +78,26 @@ def test_private_columns_disjoint():
         span = set(range(names[nm], names[nm] + widths.get(nm, 1)))
         assert not (span & used), f"{nm} overlaps another private column"
         used |= span
+
+
+def test_bsqhydro_sequence_on_host_emulation(emu_lib, tmp_path):
+    """The reference's driver sequence on the C++ call surface
+    (ccake_amd/bin/bsqhydro_run) against the host emulation build: the
+    phase-by-phase initialisation and 2 steps equal the fused one bit for
+    bit, with no invalid memory access."""
+    sys.path.insert(0, HERE)
+    from test_host_mirror import BSQ, write_bsqhydro_inputs
+    write_bsqhydro_inputs(str(tmp_path), 0.1)
+    env = dict(os.environ, LD_LIBRARY_PATH=os.path.join(EMU, "_build", "lib"),
+               LD_PRELOAD=subprocess.run(["g++", "-print-file-name=libasan.so"], capture_output=True,
+                                         text=True).stdout.strip(),
+               ASAN_OPTIONS="detect_leaks=0")
+    r = subprocess.run([BSQ, "--ic", str(tmp_path / "ic.dat"), "--table", str(tmp_path / "table.bin"),
+                        "--max-steps", "2", "--output", str(tmp_path)], capture_output=True, text=True,
+                       timeout=600, env=env)
+    print(r.stdout, r.stderr[-4000:])
+    assert r.returncode == 0 and "AddressSanitizer" not in r.stderr
+    import json
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    assert d["particles"] == 8077 and d["steps_sequence"] == d["steps_fused"] == 2
+    assert d["fields_differing"] == 0 and d["int_fields_differing"] == 0 and d["rows_equal"]
