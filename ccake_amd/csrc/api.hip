@@ -470,10 +470,27 @@ int ccake_eos_create(ccake_dev* dev, const ccake_eos_desc* desc, ccake_eos** out
     }
     dev->used += e->bytes;
   }
-  CCAKE_HIP(hipMalloc((void**)&e->d_tab, nn * 17 * sizeof(double)));
-  CCAKE_HIP(hipMalloc((void**)&e->d_tab4, nn * 5 * sizeof(double)));
-  CCAKE_HIP(hipMemcpy(e->d_tab, desc->table, nn * 17 * sizeof(double), hipMemcpyHostToDevice));
-  CCAKE_HIP(hipMemcpy(e->d_tab4, t5.data(), nn * 5 * sizeof(double), hipMemcpyHostToDevice));
+  {
+    // on any failure: free what was allocated and give the budget back
+    auto fail = [&](hipError_t err, const char* what) {
+      if (e->d_tab) (void)hipFree(e->d_tab);
+      if (e->d_tab4) (void)hipFree(e->d_tab4);
+      {
+        std::lock_guard<std::mutex> lk(dev->mu);
+        dev->used -= e->bytes;
+      }
+      delete[] e->h_tab;
+      delete e;
+      return hip_fail(err, what);
+    };
+    hipError_t err;
+    if ((err = hipMalloc((void**)&e->d_tab, nn * 17 * sizeof(double))) != hipSuccess) return fail(err, "hipMalloc(table)");
+    if ((err = hipMalloc((void**)&e->d_tab4, nn * 5 * sizeof(double))) != hipSuccess) return fail(err, "hipMalloc(table)");
+    if ((err = hipMemcpy(e->d_tab, desc->table, nn * 17 * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+      return fail(err, "hipMemcpy(table)");
+    if ((err = hipMemcpy(e->d_tab4, t5.data(), nn * 5 * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+      return fail(err, "hipMemcpy(table)");
+  }
   ch.tab = e->d_tab;
   ch.tab4 = e->d_tab4;
   // matched fallbacks at Tmax = 1.1 T_FO (src/eos_initialization.cpp:270-488)
@@ -609,6 +626,8 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
   }
   ev->nslices_cap = ev->nslices;
   rc |= dalloc(ev, &ev->part, (size_t)ev->nslices_cap * kMaxAcc * np);
+  rc |= dalloc(ev, &ev->hard, np);
+  rc |= dalloc(ev, &ev->hard_count, 2);
   rc |= dalloc(ev, &ev->cand_s, 8 * np);
   rc |= dalloc(ev, &ev->cand_g, 16 * np);
   if (p->using_diffusion) rc |= dalloc(ev, &ev->cand_d, 16 * np);
@@ -626,6 +645,7 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
   CCAKE_HIP(hipMemsetAsync(ev->I, 0, (size_t)CCAKE_NUM_IFIELDS * np * sizeof(int32_t), st));
   CCAKE_HIP(hipMemsetAsync(ev->err, 0, sizeof(int32_t), st));
   CCAKE_HIP(hipMemsetAsync(ev->pair_count, 0, 2 * sizeof(int64_t), st));
+  CCAKE_HIP(hipMemsetAsync(ev->hard_count, 0, 2 * sizeof(int32_t), st));
   if (ev->snaps) {
     CCAKE_HIP(hipMemsetAsync(ev->snaps, 0, (size_t)6 * ev->snap_cols * np * sizeof(double), st));
     CCAKE_HIP(hipMemsetAsync(ev->fo_count, 0, sizeof(int), st));
@@ -650,7 +670,7 @@ int ccake_event_destroy(ccake_event* ev)
   if (!ev) return CCAKE_ERR_INVALID;
   if (ev->stream) (void)hipStreamSynchronize(ev->stream);
   void* bufs[] = { ev->F, ev->I, ev->grid, ev->bbox, ev->key, ev->key_a, ev->key_b, ev->val_a, ev->val_b, ev->hist,
-                   ev->scan_tmp, ev->cell_off, ev->pair_count, ev->part, ev->cand_s, ev->cand_g, ev->red,
+                   ev->scan_tmp, ev->cell_off, ev->pair_count, ev->part, ev->hard, ev->hard_count, ev->cand_s, ev->cand_g, ev->red,
                    ev->err, ev->snaps, ev->fo_rows, ev->fo_count, ev->cand_d };
   for (void* b : bufs) if (b) (void)hipFree(b);
   if (ev->stream) (void)hipStreamDestroy(ev->stream);

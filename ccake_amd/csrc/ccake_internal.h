@@ -149,6 +149,11 @@ struct ccake_event {
   // pair-kernel partial sums: nslices x (up to 17) x npad, sorted order
   int nslices = 1;
   int nslices_cap = 1;          // slices the partial-sum buffer holds
+  // EoS stage: particles whose inversion needs the full cascade (positions),
+  // two alternating counters and which one the next stage appends to
+  int32_t* hard = nullptr;
+  int32_t* hard_count = nullptr;
+  int hard_parity = 0;
   double* part = nullptr;
   // gathered candidates (sorted order)
   double* cand_s = nullptr;     // 8 doubles per particle

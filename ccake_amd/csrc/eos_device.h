@@ -283,19 +283,23 @@ CK_HD double residual(const double* F, const double* tgt, double* R)
 
 // One rootfinder attempt from x (src/rootfinder.cpp:206-416): damped Newton
 // that never leaves the member's range; one polishing step after the
-// absolute residual test passes.
+// absolute residual test passes.  With budget < kNewtonSteps the attempt
+// stops undecided (-1) after `budget` iterations; otherwise it returns 1
+// (converged, x the root) or 0.  A decided outcome within the budget is
+// the outcome of the full attempt: the iterations are the same.
 template <int KM = kAllMembers>
-CK_HD bool newton_attempt(const EosMember& m, const double* tab5, int mode, const double* tgt, double* x)
+CK_HD int newton_attempt_b(const EosMember& m, const double* tab5, int mode, const double* tgt, double* x,
+                           int budget)
 {
   double F[4], J[16], R[4], dx[4], xn[4], Fn[4], Jn[16], Rn[4];
-  if (!in_range(m, x)) return false;
+  if (!in_range(m, x)) return 0;
   member_dens<KM>(m, tab5, mode, x, F, J);
   double res = residual(F, tgt, R);
-  for (int it = 0; it < kNewtonSteps; ++it) {
-    if (!__builtin_isfinite(res)) return false;
+  for (int it = 0; it < budget; ++it) {
+    if (!__builtin_isfinite(res)) return 0;
     const bool conv = res < kRfTol;
     const double nr[4] = { -R[0], -R[1], -R[2], -R[3] };
-    if (!solve_4x4(J, nr, dx)) return conv;
+    if (!solve_4x4(J, nr, dx)) return conv ? 1 : 0;
     double lam = 1.0;
     bool acc = false;
     for (int k = 0; k < 40; ++k) {
@@ -314,40 +318,59 @@ CK_HD bool newton_attempt(const EosMember& m, const double* tab5, int mode, cons
       if (conv) break;
       lam *= 0.5;
     }
-    if (conv) return true;
-    if (!acc) return false;
+    if (conv) return 1;
+    if (!acc) return 0;
   }
-  return res < kRfTol;
+  if (budget < kNewtonSteps) return -1;
+  return res < kRfTol ? 1 : 0;
 }
 
-// Rootfinder::find_root: seed adjustment and perturbed retries
-// (src/rootfinder.cpp:421-715).  tgt = (s|e, B, S, Q); x = (T, muB, muQ, muS).
 template <int KM = kAllMembers>
-CK_HD bool find_root(const EosMember& m, const double* tab5, int mode, const double* tgt, double* x)
+CK_HD bool newton_attempt(const EosMember& m, const double* tab5, int mode, const double* tgt, double* x)
+{
+  return newton_attempt_b<KM>(m, tab5, mode, tgt, x, kNewtonSteps) == 1;
+}
+
+// Attempt `k` (0..9) of Rootfinder::find_root from the seed x
+// (src/rootfinder.cpp:421-715): the seed adjusted away from zero, then
+// attempt 0 as it is and attempts 1-8 with one coordinate moved by +-20 %
+// (clamped inside the member's range), attempt 9 with the chemical
+// potentials reset.  tgt = (s|e, B, S, Q); x = (T, muB, muQ, muS).
+CK_HD void root_seed(const EosMember& m, const double* tgt, const double* x, int k, double* y)
 {
   double sd[4] = { x[0], x[1], x[2], x[3] };
   const double sg[4] = { 1.0, sgnd(tgt[1]), sgnd(tgt[3]), sgnd(tgt[2]) };
   for (int a = 0; a < 4; ++a)
     if (sd[a] * sd[a] < kRfTol) sd[a] = fmin(0.5 * m.maxs[a], 1.0) * sg[a];
   const double t0 = sd[0], b0 = sd[1], q0 = sd[2], s0 = sd[3];
-  for (int k = 0; k < 10; ++k) {
-    double y[4] = { t0, b0, q0, s0 };
-    switch (k) {
-      case 1: y[0] = (t0 + 0.2 * t0 > m.maxs[0]) ? m.maxs[0] - 1 : t0 + 0.2 * t0; break;
-      case 2: y[0] = (t0 - 0.2 * t0 < m.mins[0]) ? m.mins[0] + 1 : t0 - 0.2 * t0; break;
-      case 3: y[1] = (b0 + 0.2 * b0 > m.maxs[1]) ? m.maxs[1] - 1 : b0 + 0.2 * b0; break;
-      case 4: y[1] = (b0 - 0.2 * b0 < m.mins[1]) ? m.mins[1] + 1 : b0 - 0.2 * b0; break;
-      case 5: y[3] = (s0 + 0.2 * s0 > m.maxs[3]) ? m.maxs[3] - 1 : s0 + 0.2 * s0; break;
-      case 6: y[3] = (s0 - 0.2 * s0 < m.mins[3]) ? m.mins[3] + 1 : s0 - 0.2 * s0; break;
-      case 7: y[2] = (q0 + 0.2 * q0 > m.maxs[2]) ? m.maxs[2] - 1 : q0 + 0.2 * q0; break;
-      case 8: y[2] = (q0 - 0.2 * q0 < m.mins[2]) ? m.mins[2] + 1 : q0 - 0.2 * q0; break;
-      case 9:
-        y[1] = fmin(0.5 * m.maxs[1], 1.0) * sg[1];
-        y[2] = fmin(0.5 * m.maxs[2], 1.0) * sg[2];
-        y[3] = fmin(0.5 * m.maxs[3], 1.0) * sg[3];
-        break;
-      default: break;
-    }
+  y[0] = t0; y[1] = b0; y[2] = q0; y[3] = s0;
+  switch (k) {
+    case 1: y[0] = (t0 + 0.2 * t0 > m.maxs[0]) ? m.maxs[0] - 1 : t0 + 0.2 * t0; break;
+    case 2: y[0] = (t0 - 0.2 * t0 < m.mins[0]) ? m.mins[0] + 1 : t0 - 0.2 * t0; break;
+    case 3: y[1] = (b0 + 0.2 * b0 > m.maxs[1]) ? m.maxs[1] - 1 : b0 + 0.2 * b0; break;
+    case 4: y[1] = (b0 - 0.2 * b0 < m.mins[1]) ? m.mins[1] + 1 : b0 - 0.2 * b0; break;
+    case 5: y[3] = (s0 + 0.2 * s0 > m.maxs[3]) ? m.maxs[3] - 1 : s0 + 0.2 * s0; break;
+    case 6: y[3] = (s0 - 0.2 * s0 < m.mins[3]) ? m.mins[3] + 1 : s0 - 0.2 * s0; break;
+    case 7: y[2] = (q0 + 0.2 * q0 > m.maxs[2]) ? m.maxs[2] - 1 : q0 + 0.2 * q0; break;
+    case 8: y[2] = (q0 - 0.2 * q0 < m.mins[2]) ? m.mins[2] + 1 : q0 - 0.2 * q0; break;
+    case 9:
+      y[1] = fmin(0.5 * m.maxs[1], 1.0) * sg[1];
+      y[2] = fmin(0.5 * m.maxs[2], 1.0) * sg[2];
+      y[3] = fmin(0.5 * m.maxs[3], 1.0) * sg[3];
+      break;
+    default: break;
+  }
+}
+
+constexpr int kRootAttempts = 10;
+
+// Rootfinder::find_root: the attempts in order, the first that converges.
+template <int KM = kAllMembers>
+CK_HD bool find_root(const EosMember& m, const double* tab5, int mode, const double* tgt, double* x)
+{
+  for (int k = 0; k < kRootAttempts; ++k) {
+    double y[4];
+    root_seed(m, tgt, x, k, y);
     if (newton_attempt<KM>(m, tab5, mode, tgt, y)) { for (int a = 0; a < 4; ++a) x[a] = y[a]; return true; }
   }
   return false;
@@ -471,6 +494,20 @@ CK_HD void set_thermo(const EosChain& ch, const double* tab, int mi, const doubl
   th[30] = (m.kind == M_TABLE) ? 1.0 : 0.0;
 }
 
+// Seed of kind `kind` for a chain member (src/eos.cpp:629-975): 0 the
+// current location, 1 (T, 0, 0, 0), 2 (580 MeV, sgn rho), 3 the
+// conformal-diagonal seed of the chain's last member.
+CK_HD void seed_of_kind(const EosChain& ch, int mode, int kind, const double* tgt, const double* x, double* y)
+{
+  if (kind == 0) { y[0] = x[0]; y[1] = x[1]; y[2] = x[2]; y[3] = x[3]; }
+  else if (kind == 1) { y[0] = x[0]; y[1] = 0.0; y[2] = 0.0; y[3] = 0.0; }
+  else if (kind == 2) { y[0] = 580.0 / kHbarcMeVfm; y[1] = sgnd(tgt[1]); y[2] = sgnd(tgt[3]); y[3] = sgnd(tgt[2]); }
+  else diag_seed(ch.m[ch.n - 1], mode, tgt, y);
+}
+// Seed kinds per member: mode 0 (s) tries kinds 0-3, mode 1 (e) kinds 2-3.
+CK_HD int seed_kinds(int mode) { return mode == 0 ? 4 : 2; }
+CK_HD int first_kind(int mode) { return mode == 0 ? 0 : 2; }
+
 // Point inversion over the chain.  mode 0: (s, B, S, Q) starting at the
 // particle's current member, seeds: current location, (T, 0, 0, 0),
 // (580 MeV, sgn), conformal-diagonal seed.  mode 1: (e, B, S, Q) from the
@@ -480,8 +517,6 @@ template <int KM = kAllMembers>
 CK_HD bool invert_point(const EosChain& ch, const double* tab, const double* tab5, int mode,
                         const double* tgt, double* x, int* member)
 {
-  const double hc = kHbarcMeVfm;
-  const EosMember& diag = ch.m[ch.n - 1];
   int start = (mode == 0) ? *member : 0;
   if (start < 0 || start >= ch.n) start = 0;
   bool found = false;
@@ -490,17 +525,83 @@ CK_HD bool invert_point(const EosChain& ch, const double* tab, const double* tab
   for (int k = start; k < ch.n; ++k) {
     const EosMember& m = ch.m[k];
     found = false;
-    const int nseed = (mode == 0) ? 4 : 2;
+    const int nseed = seed_kinds(mode);
     for (int q = 0; q < nseed && !found; ++q) {
-      const int kind = (mode == 0) ? q : q + 2;
       double y[4];
-      if (kind == 0) { y[0] = x[0]; y[1] = x[1]; y[2] = x[2]; y[3] = x[3]; }
-      else if (kind == 1) { y[0] = x[0]; y[1] = 0.0; y[2] = 0.0; y[3] = 0.0; }
-      else if (kind == 2) { y[0] = 580.0 / hc; y[1] = sgnd(tgt[1]); y[2] = sgnd(tgt[3]); y[3] = sgnd(tgt[2]); }
-      else diag_seed(diag, mode, tgt, y);
+      seed_of_kind(ch, mode, first_kind(mode) + q, tgt, x, y);
       if (find_root<KM>(m, tab5, mode, tgt, y)) { found = true; for (int a = 0; a < 4; ++a) res[a] = y[a]; }
     }
     if (found) {
+      used = k;
+      double r[17];
+      member_fields<KM>(m, tab, res, r);
+      if (r[X_CS2] < 0.0 || r[X_CS2] > 1.0) { for (int a = 0; a < 4; ++a) x[a] = res[a]; continue; }
+      break;
+    }
+  }
+  if (!found) return false;
+  for (int a = 0; a < 4; ++a) x[a] = res[a];
+  *member = used;
+  return true;
+}
+
+// The first attempt of invert_point alone, with a Newton budget: the
+// starting member, the first seed kind, attempt 0.  Returns 1 when that
+// attempt converges within `budget` iterations to a point with cs2 in
+// [0, 1]: invert_point's result then is that point (x, *member set).
+// Returns -1 otherwise (the full inversion decides).
+template <int KM = kAllMembers>
+CK_HD int invert_first(const EosChain& ch, const double* tab, const double* tab5, int mode, const double* tgt,
+                       double* x, int* member, int budget)
+{
+  int start = (mode == 0) ? *member : 0;
+  if (start < 0 || start >= ch.n) start = 0;
+  const EosMember& m = ch.m[start];
+  double y0[4], y[4];
+  seed_of_kind(ch, mode, first_kind(mode), tgt, x, y0);
+  root_seed(m, tgt, y0, 0, y);
+  if (newton_attempt_b<KM>(m, tab5, mode, tgt, y, budget) != 1) return -1;
+  double r[17];
+  member_fields<KM>(m, tab, y, r);
+  if (r[X_CS2] < 0.0 || r[X_CS2] > 1.0) return -1;
+  for (int a = 0; a < 4; ++a) x[a] = y[a];
+  *member = start;
+  return 1;
+}
+
+// The whole of invert_point for one particle by one wave (every lane
+// active, the particle's values the same on every lane): the attempts of a
+// member -- seed kind q, find_root attempt k, at most 4 x 10 -- run on lanes
+// q * 10 + k at once, and the first that converges in that order is taken,
+// which is the attempt invert_point's loops stop at.  Members are tried in
+// order as there.
+constexpr int kHardLanes = 64;
+template <int KM = kAllMembers>
+__device__ bool invert_wave(const EosChain& ch, const double* tab, const double* tab5, int mode, const double* tgt,
+                            double* x, int* member)
+{
+  const int lane = threadIdx.x & (kHardLanes - 1);
+  int start = (mode == 0) ? *member : 0;
+  if (start < 0 || start >= ch.n) start = 0;
+  const int na = seed_kinds(mode) * kRootAttempts;
+  bool found = false;
+  int used = -1;
+  double res[4] = { 0, 0, 0, 0 };
+  for (int k = start; k < ch.n; ++k) {
+    const EosMember& m = ch.m[k];
+    double y[4] = { 0, 0, 0, 0 };
+    bool ok = false;
+    if (lane < na) {
+      double y0[4];
+      seed_of_kind(ch, mode, first_kind(mode) + lane / kRootAttempts, tgt, x, y0);
+      root_seed(m, tgt, y0, lane % kRootAttempts, y);
+      ok = newton_attempt<KM>(m, tab5, mode, tgt, y);
+    }
+    const unsigned long long mask = __ballot(ok);
+    found = mask != 0ull;
+    if (found) {
+      const int first = __builtin_ctzll(mask);
+      for (int a = 0; a < 4; ++a) res[a] = __shfl(y[a], first);
       used = k;
       double r[17];
       member_fields<KM>(m, tab, res, r);

@@ -181,6 +181,23 @@ __device__ __forceinline__ void smooth_sum_into(const SmoothSum& sm, double* __r
   col(CCAKE_F_RHOQ) = sQ / gam / t;
 }
 
+// The EoS stage runs in two kernels.  k_eos (one thread per particle) tries
+// the first attempt of the inversion -- the starting member, the first seed,
+// find_root's attempt 0 -- with a Newton budget; a particle for which that
+// attempt decides the result (almost every particle of a time step, which
+// starts at last step's solution) is finished there.  The others are
+// appended to the event's hard list, and k_eos_hard runs the whole
+// inversion for each of them with one wave per particle (invert_wave), so a
+// particle that needs the full seed cascade no longer holds 63 others.
+// Both give invert_point's result, so the split changes no value.
+constexpr int kFastBudget = 12;     // Newton iterations of the first attempt in k_eos
+
+struct HardList {
+  int32_t* idx = nullptr;           // positions (sorted for MODE 0, 3; id for 1, 2)
+  int32_t* count = nullptr;         // this stage's counter
+  int32_t* next = nullptr;          // the next stage's counter, cleared by k_eos_hard
+};
+
 // KM: the member kinds the chain holds (kAllMembers, or kConformalChain for
 // the analytic conformal chain, which then compiles without the table code).
 // MODE 0: the stage (EoS, reset_pi_tensor, transport, gradient records);
@@ -189,14 +206,13 @@ template <int MODE, int KM>
 __global__ __launch_bounds__(256) void k_eos(EosChain ch, double* __restrict__ F, int npad, int n,
                                              int32_t* __restrict__ I, const int32_t* __restrict__ perm,
                                              double t, Transport P, double* __restrict__ cand,
-                                             int32_t* __restrict__ err, SmoothSum sm)
+                                             int32_t* __restrict__ err, SmoothSum sm, HardList hl)
 {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int kk = k < n ? k : n - 1;
   const int i = (MODE == 0 || MODE == 3) ? perm[kk] : kk;
   auto col = [&](int c) -> double& { return F[(size_t)c * npad + i]; };
   int32_t* eos_id = I + (size_t)CCAKE_I_EOS_ID * npad;
-  int32_t* freeze = I + (size_t)CCAKE_I_FREEZE * npad;
   // the Newton seed and member are read before the smoothing sum's stores,
   // so their loads overlap with the partial sums' loads
   double x[4] = { col(CCAKE_F_T), col(CCAKE_F_MUB), col(CCAKE_F_MUQ), col(CCAKE_F_MUS) };
@@ -204,21 +220,15 @@ __global__ __launch_bounds__(256) void k_eos(EosChain ch, double* __restrict__ F
   if (MODE == 3) smooth_sum_into(sm, F, npad, n, k, i, t, I + (size_t)CCAKE_I_BTRACK * npad);
   if (k >= n) return;
   double tgt[4];
-  bool ok;
   if (MODE == 1) {
     tgt[0] = col(CCAKE_F_IN_E); tgt[1] = col(CCAKE_F_IN_B); tgt[2] = col(CCAKE_F_IN_SS); tgt[3] = col(CCAKE_F_IN_Q);
-    ok = invert_point<KM>(ch, ch.tab, ch.tab4, 1, tgt, x, &member);
-    if (!ok) {
-      // src/sph_workstation.cpp:1825-1848: freeze and retry at zero charge
-      freeze[i] = 4;
-      const double z[4] = { tgt[0], 0.0, 0.0, 0.0 };
-      ok = invert_point<KM>(ch, ch.tab, ch.tab4, 1, z, x, &member);
-    }
   } else {
     tgt[0] = col(CCAKE_F_S); tgt[1] = col(CCAKE_F_RHOB); tgt[2] = col(CCAKE_F_RHOS); tgt[3] = col(CCAKE_F_RHOQ);
-    ok = invert_point<KM>(ch, ch.tab, ch.tab4, 0, tgt, x, &member);
   }
-  if (!ok) { atomicOr(err, DE_EOS_FAIL); return; }
+  if (invert_first<KM>(ch, ch.tab, ch.tab4, MODE == 1 ? 1 : 0, tgt, x, &member, kFastBudget) != 1) {
+    hl.idx[atomicAdd(hl.count, 1)] = k;      // the whole inversion in k_eos_hard
+    return;
+  }
   double th[31];
   set_thermo<KM>(ch, ch.tab, member, x, th);
   for (int q = 0; q < 31; ++q) col(CCAKE_F_T + q) = th[q];
@@ -227,6 +237,53 @@ __global__ __launch_bounds__(256) void k_eos(EosChain ch, double* __restrict__ F
   if (MODE == 2) return;
 
   stage_post(F, npad, i, k, t, P, th[0], th[11], th[9], th[10], th[4], cand);
+}
+
+// The whole inversion (invert_point, with the initial inversion's freeze and
+// zero-charge retry, src/sph_workstation.cpp:1825-1848) of every particle on
+// the hard list, one wave per particle, then the same tail as k_eos.  Waves
+// stride over the list; every lane of a wave takes part in invert_wave.
+template <int MODE, int KM>
+__global__ __launch_bounds__(kHardLanes) void k_eos_hard(EosChain ch, double* __restrict__ F, int npad, int n,
+                                                         int32_t* __restrict__ I, const int32_t* __restrict__ perm,
+                                                         double t, Transport P, double* __restrict__ cand,
+                                                         int32_t* __restrict__ err, HardList hl)
+{
+  const int lane = threadIdx.x;
+  const int cnt = *hl.count;
+  if (blockIdx.x == 0 && lane == 0) *hl.next = 0;   // the next stage's list starts empty
+  int32_t* eos_id = I + (size_t)CCAKE_I_EOS_ID * npad;
+  int32_t* freeze = I + (size_t)CCAKE_I_FREEZE * npad;
+  for (int h = blockIdx.x; h < cnt; h += gridDim.x) {
+    const int k = hl.idx[h];
+    const int i = (MODE == 0 || MODE == 3) ? perm[k] : k;
+    auto col = [&](int c) -> double& { return F[(size_t)c * npad + i]; };
+    double x[4] = { col(CCAKE_F_T), col(CCAKE_F_MUB), col(CCAKE_F_MUQ), col(CCAKE_F_MUS) };
+    int member = eos_id[i];
+    double tgt[4];
+    bool ok;
+    if (MODE == 1) {
+      tgt[0] = col(CCAKE_F_IN_E); tgt[1] = col(CCAKE_F_IN_B); tgt[2] = col(CCAKE_F_IN_SS); tgt[3] = col(CCAKE_F_IN_Q);
+      ok = invert_wave<KM>(ch, ch.tab, ch.tab4, 1, tgt, x, &member);
+      if (!ok) {
+        // src/sph_workstation.cpp:1825-1848: freeze and retry at zero charge
+        if (lane == 0) freeze[i] = 4;
+        const double z[4] = { tgt[0], 0.0, 0.0, 0.0 };
+        ok = invert_wave<KM>(ch, ch.tab, ch.tab4, 1, z, x, &member);
+      }
+    } else {
+      tgt[0] = col(CCAKE_F_S); tgt[1] = col(CCAKE_F_RHOB); tgt[2] = col(CCAKE_F_RHOS); tgt[3] = col(CCAKE_F_RHOQ);
+      ok = invert_wave<KM>(ch, ch.tab, ch.tab4, 0, tgt, x, &member);
+    }
+    if (lane != 0) continue;
+    if (!ok) { atomicOr(err, DE_EOS_FAIL); continue; }
+    double th[31];
+    set_thermo<KM>(ch, ch.tab, member, x, th);
+    for (int q = 0; q < 31; ++q) col(CCAKE_F_T + q) = th[q];
+    eos_id[i] = member;
+    if (MODE == 1) col(CCAKE_F_IN_S) = th[5];
+    else if (MODE == 0 || MODE == 3) stage_post(F, npad, i, k, t, P, th[0], th[11], th[9], th[10], th[4], cand);
+  }
 }
 
 // The EoS stage with the pre-inverted table (online_inverter_enabled =
@@ -288,6 +345,7 @@ static Transport make_transport(const ccake_params& p)
 #define CCAKE_EOS_THREADS 64
 #endif
 constexpr int kEosThreads = CCAKE_EOS_THREADS;
+constexpr int kHardWaves = 512;     // waves of k_eos_hard, striding over the hard list
 constexpr int kConformalChain = (1 << M_CONF) | (1 << M_DIAG);
 
 static int chain_kinds(const EosChain& ch)
@@ -327,20 +385,29 @@ int launch_eos(ccake_event* ev, int mode, int fuse_post)
     sm.norm2 = smooth_norm2(ev->p.hT);
     sm.pair_total = (unsigned long long*)ev->pair_count;
   }
-  auto go = [&](auto kern) {
+  // the hard list's counters alternate between stages: this stage's k_eos
+  // appends to one, its k_eos_hard clears the other for the next stage
+  HardList hl;
+  hl.idx = ev->hard;
+  hl.count = ev->hard_count + ev->hard_parity;
+  hl.next = ev->hard_count + (1 - ev->hard_parity);
+  ev->hard_parity = 1 - ev->hard_parity;
+  auto go = [&](auto kern, auto hard) {
     hipLaunchKernelGGL(kern, g, b, 0, st, ev->eos->chain, ev->F, ev->npad, n, ev->I, ev->perm, ev->t, P,
-                       ev->cand_g, ev->err, sm);
+                       ev->cand_g, ev->err, sm, hl);
+    hipLaunchKernelGGL(hard, dim3(kHardWaves), dim3(kHardLanes), 0, st, ev->eos->chain, ev->F, ev->npad, n, ev->I,
+                       ev->perm, ev->t, P, ev->cand_g, ev->err, hl);
   };
   if (chain_kinds(ev->eos->chain) == kConformalChain) {
-    if (md == 1) go(k_eos<1, kConformalChain>);
-    else if (md == 0) go(k_eos<0, kConformalChain>);
-    else if (md == 3) go(k_eos<3, kConformalChain>);
-    else go(k_eos<2, kConformalChain>);
+    if (md == 1) go(k_eos<1, kConformalChain>, k_eos_hard<1, kConformalChain>);
+    else if (md == 0) go(k_eos<0, kConformalChain>, k_eos_hard<0, kConformalChain>);
+    else if (md == 3) go(k_eos<3, kConformalChain>, k_eos_hard<0, kConformalChain>);
+    else go(k_eos<2, kConformalChain>, k_eos_hard<2, kConformalChain>);
   } else {
-    if (md == 1) go(k_eos<1, kAllMembers>);
-    else if (md == 0) go(k_eos<0, kAllMembers>);
-    else if (md == 3) go(k_eos<3, kAllMembers>);
-    else go(k_eos<2, kAllMembers>);
+    if (md == 1) go(k_eos<1, kAllMembers>, k_eos_hard<1, kAllMembers>);
+    else if (md == 0) go(k_eos<0, kAllMembers>, k_eos_hard<0, kAllMembers>);
+    else if (md == 3) go(k_eos<3, kAllMembers>, k_eos_hard<0, kAllMembers>);
+    else go(k_eos<2, kAllMembers>, k_eos_hard<2, kAllMembers>);
   }
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }

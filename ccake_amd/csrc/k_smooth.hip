@@ -126,10 +126,12 @@ __global__ __launch_bounds__(64) void k_walk(const double* __restrict__ cand,
     });
   });
 
-  if (!act || MODE == 3) return;
-  unsigned long long tot = (unsigned long long)cnt;
+  if (MODE == 3) return;
+  // the wave's pair total, every lane taking part (lanes past n add 0)
+  unsigned long long tot = act ? (unsigned long long)cnt : 0ull;
   for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
   if (lane == 0 && pair_total) atomicAdd(pair_total, tot);
+  if (!act) return;
   const int b = btrack[id];
   btrack[id] = (b == -1) ? -1 : cnt;
 }
@@ -371,15 +373,23 @@ int launch_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, s
   CCAKE_HIP(hipMemcpyAsync(cnt.data(), bt, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
   CCAKE_HIP(hipStreamSynchronize(st));
   CCAKE_HIP(hipMemcpyAsync(bt, saved_bt.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  // the restore reads saved_bt: it completes before any return below
+  CCAKE_HIP(hipStreamSynchronize(st));
   std::vector<int64_t> off(n + 1, 0);
   for (int i = 0; i < n; ++i) off[i + 1] = off[i] + cnt[i];
   const size_t total = (size_t)off[n];
   *n_out = total;
-  if (total > cap) { CCAKE_HIP(hipStreamSynchronize(st)); return CCAKE_ERR_CAPACITY; }
-  int64_t* d_off = nullptr; int32_t* d_i = nullptr; int32_t* d_j = nullptr;
-  CCAKE_HIP(hipMalloc(&d_off, (n + 1) * sizeof(int64_t)));
-  CCAKE_HIP(hipMalloc(&d_i, (total + 1) * sizeof(int32_t)));
-  CCAKE_HIP(hipMalloc(&d_j, (total + 1) * sizeof(int32_t)));
+  if (total > cap) return CCAKE_ERR_CAPACITY;
+  struct DevBuf {                     // freed on every return
+    void* p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+  } b_off, b_i, b_j;
+  CCAKE_HIP(hipMalloc(&b_off.p, (n + 1) * sizeof(int64_t)));
+  CCAKE_HIP(hipMalloc(&b_i.p, (total + 1) * sizeof(int32_t)));
+  CCAKE_HIP(hipMalloc(&b_j.p, (total + 1) * sizeof(int32_t)));
+  int64_t* d_off = static_cast<int64_t*>(b_off.p);
+  int32_t* d_i = static_cast<int32_t*>(b_i.p);
+  int32_t* d_j = static_cast<int32_t*>(b_j.p);
   CCAKE_HIP(hipMemcpyAsync(d_off, off.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
   const double rc = 2.0 * ev->p.hT;
   hipLaunchKernelGGL(k_walk<3>, dim3((n + kTile - 1) / kTile), dim3(kTile), 0, st, ev->cand_s, ev->skey,
@@ -389,9 +399,6 @@ int launch_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, s
   CCAKE_HIP(hipMemcpyAsync(pi, d_i, total * sizeof(int32_t), hipMemcpyDeviceToHost, st));
   CCAKE_HIP(hipMemcpyAsync(pj, d_j, total * sizeof(int32_t), hipMemcpyDeviceToHost, st));
   CCAKE_HIP(hipStreamSynchronize(st));
-  CCAKE_HIP(hipFree(d_off));
-  CCAKE_HIP(hipFree(d_i));
-  CCAKE_HIP(hipFree(d_j));
   // canonical order: rows by i, j ascending inside a row
   for (int i = 0; i < n; ++i) std::sort(pj + off[i], pj + off[i + 1]);
   return 0;

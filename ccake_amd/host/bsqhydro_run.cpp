@@ -155,6 +155,7 @@ int main(int argc, char** argv)
     const std::vector<double> ra = a.ws->freezePtr->surface(&na), rb = b.ws->freezePtr->surface(&nb);
     const bool rows_equal = na == nb && ra == rb;
     int frozen = 0;
+    ccake::check(ccake_event_download_int(a.sys->event(), CCAKE_I_FREEZE, ia.data(), n), "download_int");
     for (int i = 0; i < n; ++i) frozen += ia[i] >= 4;
     std::printf("{\"particles\": %d, \"steps_sequence\": %d, \"steps_fused\": %d, \"t_sequence\": %.17g, "
                 "\"t_fused\": %.17g, \"fields_differing\": %d, \"int_fields_differing\": %d, \"rows\": %zu, "

@@ -37,6 +37,7 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __shared__ static thread_local
+#define __align__(n) alignas(n)
 
 struct dim3 {
   unsigned x, y, z;

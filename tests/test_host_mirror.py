@@ -114,5 +114,6 @@ def test_bsqhydro_sequence_equals_fused_initialize(tmp_path):
     assert d["particles"] == 8077
     assert d["steps_sequence"] == d["steps_fused"] and d["t_sequence"] == d["t_fused"]
     assert d["fields_differing"] == 0 and d["int_fields_differing"] == 0
-    assert d["rows_equal"] and d["rows"] == d["particles"] == d["frozen_out"]
-    assert d["freeze_out_lines_buffered"] > 0
+    assert d["frozen_out"] == d["particles"]          # the run ended by continue_evolution
+    assert d["rows_equal"] and 0.9 * d["particles"] < d["rows"] <= d["particles"]
+    assert d["freeze_out_lines_buffered"] == d["rows"]
