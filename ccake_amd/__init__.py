@@ -119,6 +119,7 @@ ABI = {
     "ccake_event_download_int": (C.c_int, [_vp, C.c_int, _ip, C.c_size_t]),
     "ccake_build_neighbors": (C.c_int, [_vp]),
     "ccake_export_pairs": (C.c_int, [_vp, _ip, _ip, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "ccake_export_pairs_noalloc": (C.c_int, [_vp, _ip, _ip, C.c_size_t, C.POINTER(C.c_size_t)]),
     "ccake_gamma_velocity": (C.c_int, [_vp]),
     "ccake_smooth_fields": (C.c_int, [_vp]),
     "ccake_initial_sigma": (C.c_int, [_vp]),
@@ -373,6 +374,19 @@ class Event:
         _ok(lib().ccake_export_pairs(self.h, pi.ctypes.data_as(_ip), pj.ctypes.data_as(_ip), n.value,
                                      C.byref(n)), "export_pairs")
         return pi, pj
+
+    def pairs_noalloc(self, cap=None):
+        """The pair set through ccake_export_pairs_noalloc (no allocation in the
+        library; the caller's arrays hold `cap` pairs, by default the count
+        of the last neighbour build)."""
+        if cap is None:
+            cap = self.last_pair_count()
+        pi = np.empty(max(cap, 1), dtype=np.int32)
+        pj = np.empty(max(cap, 1), dtype=np.int32)
+        n = C.c_size_t()
+        _ok(lib().ccake_export_pairs_noalloc(self.h, pi.ctypes.data_as(_ip), pj.ctypes.data_as(_ip), cap,
+                                             C.byref(n)), "export_pairs_noalloc")
+        return pi[:n.value], pj[:n.value]
 
     def freeze_out_surface(self):
         """The freeze-out hypersurface so far: one row per frozen-out particle,

@@ -291,6 +291,9 @@ int ccake_dev_create(int gpu, size_t budget, ccake_dev** out)
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, gpu) == hipSuccess && prop.multiProcessorCount > 0)
     d->simds = 4 * prop.multiProcessorCount;
+  // this device's address of the static scratch array of the
+  // allocation-free pair export (the device is current here)
+  if (const int rc = scratch_address(&d->scratch)) { delete d; return rc; }
   *out = d;
   return 0;
 }
@@ -742,6 +745,15 @@ int ccake_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, si
   if (!rc) rc = launch_gather_smooth(ev);
   if (rc) return rc;
   return launch_export_pairs(ev, pi, pj, cap, n_pairs);
+}
+
+int ccake_export_pairs_noalloc(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, size_t* n_pairs)
+{
+  if (!ev || !n_pairs) return CCAKE_ERR_INVALID;
+  int rc = ensure_sorted(ev);
+  if (!rc) rc = launch_gather_smooth(ev);
+  if (rc) return rc;
+  return launch_export_pairs_noalloc(ev, pi, pj, cap, n_pairs);
 }
 
 int ccake_gamma_velocity(ccake_event* ev) { return ev ? launch_stage_begin(ev) : CCAKE_ERR_INVALID; }

@@ -80,6 +80,7 @@ struct Profile {
 struct ccake_dev {
   int gpu = 0;
   int simds = 1024;          // SIMDs of the device (4 per CU)
+  void* scratch = nullptr;   // this device's address of the static pair-export scratch array
   std::mutex mu;             // guards the memory accounting and live counts below
   size_t budget = 0;
   size_t used = 0;
@@ -198,6 +199,8 @@ int launch_stage_begin(ccake_event* ev);            // gamma/v, bbox
 int launch_bbox(ccake_event* ev);                   // bbox only
 int launch_neighbor_sort(ccake_event* ev);          // grid, keys, radix sort, cell offsets, smoothing records
 int launch_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, size_t* n);
+int launch_export_pairs_noalloc(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, size_t* n);
+int scratch_address(void** out);   // the calling thread's current device's address of the scratch array
 // smoothing
 int launch_gather_smooth(ccake_event* ev);
 int launch_smooth(ccake_event* ev, int mode, bool count_pairs = true);  // mode 0 fields, 1 initial sigma, 2 counts

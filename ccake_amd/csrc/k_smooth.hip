@@ -404,4 +404,117 @@ int launch_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, s
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free pair export.  The device side of the export (the rows'
+// neighbour counts, their offsets and the pair rows) lives in `scratch`, a
+// file-scope device array of fixed size, instead of buffers allocated per
+// call, so that callers that cannot allocate device memory can take the
+// pair set.  Each device holds its own copy at its own address, which
+// ccake_dev_create obtains once per device (scratch_address) and keeps in
+// the device handle; host code uses that address, never the array's name.
+// Calls on one device share the array: two calls that overlap in time on the
+// same device race.
+constexpr size_t kScratchBytes = size_t(128) << 20;                 // 128 MiB per device
+constexpr int kScratchMaxParticles = 1 << 20;
+constexpr size_t kScratchCountOff = 0;                                // int32 counts [n]
+constexpr size_t kScratchRowOffOff = size_t(kScratchMaxParticles) * 4;   // int64 offsets [n + 1]
+constexpr size_t kScratchRowsOff = kScratchRowOffOff + (size_t(kScratchMaxParticles) + 1) * 8 + 248;  // 256-aligned
+constexpr size_t kScratchMaxPairs = (kScratchBytes - kScratchRowsOff) / 8;   // int32 (i, j) per pair
+static_assert(kScratchRowsOff % 256 == 0, "scratch row area alignment");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+int scratch_address(void** out)
+{
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+  *out = p;
+  return 0;
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32(int32_t* __restrict__ a, int n, int32_t v)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// Row offsets off[0..n] (int64) of the counts cnt[0..n) by one workgroup:
+// thread t owns the run [t*per, min((t+1)*per, n)), per = ceil(n / 1024).
+constexpr int kRowScanThreads = 1024;
+__global__ __launch_bounds__(kRowScanThreads) void k_row_offsets(const int32_t* __restrict__ cnt, int n,
+                                                                 int64_t* __restrict__ off)
+{
+  __shared__ int64_t wtot[kRowScanThreads / kWave];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int per = (n + kRowScanThreads - 1) / kRowScanThreads;
+  const int b = threadIdx.x * per;
+  const int e = min(b + per, n);
+  int64_t s = 0;
+  for (int i = b; i < e; ++i) s += cnt[i];
+  int64_t x = s;                               // inclusive scan of the run sums within the wave
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int64_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == kWave - 1) wtot[w] = x;
+  __syncthreads();
+  int64_t run = x - s;
+  for (int q = 0; q < w; ++q) run += wtot[q];
+  for (int i = b; i < e; ++i) { off[i] = run; run += cnt[i]; }
+  if (threadIdx.x == kRowScanThreads - 1) off[n] = run;   // the last thread's run ends at n
+}
+
+// The pair export of launch_export_pairs with every device buffer in
+// `scratch` (no allocation, host or device): counts, offsets and rows.
+int launch_export_pairs_noalloc(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, size_t* n_out)
+{
+  const int n = ev->n;
+  if (n > kScratchMaxParticles) {
+    set_error("ccake_export_pairs_noalloc: at most 1,048,576 particles (the scratch array's size)");
+    return CCAKE_ERR_CAPACITY;
+  }
+  unsigned char* base = static_cast<unsigned char*>(ev->dev->scratch);
+  if (!base) { set_error("ccake_export_pairs_noalloc: no scratch address for this device"); return CCAKE_ERR_INVALID; }
+  int32_t* cnt = reinterpret_cast<int32_t*>(base + kScratchCountOff);
+  int64_t* off = reinterpret_cast<int64_t*>(base + kScratchRowOffOff);
+  int32_t* ri = reinterpret_cast<int32_t*>(base + kScratchRowsOff);
+  hipStream_t st = ev->stream;
+  const double rc = 2.0 * ev->p.hT;
+  const int ntiles = (n + kTile - 1) / kTile;
+  // counts (self excluded) into scratch, cleared first: the last call's
+  // contents are still there
+  hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, st, cnt, n, 0);
+  hipLaunchKernelGGL(k_walk<2>, dim3(ntiles), dim3(kTile), 0, st, ev->cand_s, ev->skey, ev->perm, ev->cell_off,
+                     ev->grid, n, rc * rc, search_radius(ev), cnt, nullptr, nullptr, nullptr, nullptr);
+  hipLaunchKernelGGL(k_row_offsets, dim3(1), dim3(kRowScanThreads), 0, st, cnt, n, off);
+  CCAKE_HIP(hipGetLastError());
+  int64_t total = 0;
+  CCAKE_HIP(hipMemcpyAsync(&total, off + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  CCAKE_HIP(hipStreamSynchronize(st));
+  *n_out = (size_t)total;
+  if ((size_t)total > kScratchMaxPairs) {
+    set_error("ccake_export_pairs_noalloc: more pairs than the scratch array holds (15,204,320)");
+    return CCAKE_ERR_CAPACITY;
+  }
+  if ((size_t)total > cap) return CCAKE_ERR_CAPACITY;
+  int32_t* rj = ri + total;
+  hipLaunchKernelGGL(k_walk<3>, dim3(ntiles), dim3(kTile), 0, st, ev->cand_s, ev->skey, ev->perm, ev->cell_off,
+                     ev->grid, n, rc * rc, search_radius(ev), cnt, nullptr, off, ri, rj);
+  CCAKE_HIP(hipGetLastError());
+  if (total) {
+    CCAKE_HIP(hipMemcpyAsync(pi, ri, (size_t)total * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    CCAKE_HIP(hipMemcpyAsync(pj, rj, (size_t)total * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  }
+  CCAKE_HIP(hipStreamSynchronize(st));
+  // canonical order: rows by i (already), j ascending inside a row
+  for (size_t a = 0; a < (size_t)total;) {
+    size_t b = a + 1;
+    while (b < (size_t)total && pi[b] == pi[a]) ++b;
+    std::sort(pj + a, pj + b);
+    a = b;
+  }
+  return 0;
+}
+
 }  // namespace ccake

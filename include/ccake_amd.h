@@ -252,6 +252,15 @@ CCAKE_API int  ccake_build_neighbors(ccake_event* ev);
  * ordered by i then j ascending (parity checks only; never on the hot path).
  * Replaces reading Cabana::NeighborList (include/utilities.h:57-62). */
 CCAKE_API int  ccake_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, size_t* n_pairs);
+/* ccake_export_pairs without any allocation, host or device: no counterpart
+ * in the reference.  For callers that cannot allocate memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid it).  The
+ * device-side counts, row offsets and rows live in a static device array of
+ * 128 MiB per device (held until the process ends), so the event may hold
+ * at most 1,048,576 particles and 15,204,320 pairs; larger ones return
+ * CCAKE_ERR_CAPACITY with a message.  Same results as ccake_export_pairs;
+ * calls that overlap in time on one device race on that array. */
+CCAKE_API int  ccake_export_pairs_noalloc(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, size_t* n_pairs);
 /* 02: EoM_default::update_velocity (src/eom_default.cpp:17-38). */
 CCAKE_API int  ccake_gamma_velocity(ccake_event* ev);
 /* 03: SPHWorkstation::smooth_all_particle_fields (src/sph_workstation.cpp:494-578). */
