@@ -140,3 +140,83 @@ def test_bulk_ode_pin_rejects_planted_mistakes(bjorken, mistake):
     dev = max(abs(s_m / s_bad - 1), abs(Pi_m / Pi_bad - 1))
     print(mistake, f"deviation {dev:.3e}")
     assert dev > 1e-3
+
+
+# ---------------------------------------------------------------- gradients
+@pytest.fixture(scope="module")
+def gradients(orc):
+    """Linear profiles on the 0.05 fm lattice over [-3, 3]^2 (h = 0.3 fm,
+    shear on): each gradient pair sum of the oracle at the interior
+    particles (|x|, |y| < 2 fm, more than 2h from the edges)."""
+    x, y = lattice(3.0, 0.05)
+    n = len(x)
+    cfg = config.base_params(hT=0.3, t0=1.0, using_shear=1)
+    o = orc.Event(cfg, orc.Eos(0), n)
+    load(o, x, y, np.full_like(x, 1.0))
+    L = orc.lib()
+    assert L.orc_build_neighbors(o.h, 0) == 0
+    L.orc_smooth_fields(o.h)              # sigma_lab (masses 1, gamma 1)
+    prof = dict(
+        E=(2.0, 0.3, -0.5), P=(1.0, -0.2, 0.45), BULK=(-0.1, 0.05, 0.02),
+        VX=(0.01, 0.03, 0.07), VY=(-0.02, 0.05, -0.04),
+        PI11=(0.3, 0.11, -0.13), PI12=(0.1, 0.17, 0.19), PI22=(0.25, -0.23, 0.29),
+        PI01=(0.05, 0.31, -0.37), PI02=(-0.04, 0.41, 0.43))
+    lin = {k: c0 + cx * x + cy * y for k, (c0, cx, cy) in prof.items()}
+    o.set("ORC_F_E", lin["E"]); o.set("ORC_F_P", lin["P"]); o.set("ORC_F_BULK", lin["BULK"])
+    o.set("ORC_F_VX", lin["VX"]); o.set("ORC_F_VY", lin["VY"])
+    shv = np.zeros((16, n))
+    shv[5] = lin["PI11"]; shv[6] = shv[9] = lin["PI12"]; shv[10] = lin["PI22"]
+    shv[1] = shv[4] = lin["PI01"]; shv[2] = shv[8] = lin["PI02"]
+    for c in range(16):
+        o.set(ccake_amd.FIELDS["CCAKE_F_SHV0"] + c, shv[c])
+    L.orc_smooth_gradients(o.h)
+    inner = (np.abs(x) < 2.0) & (np.abs(y) < 2.0)
+    g = lambda name, c: o.field(ccake_amd.FIELDS["CCAKE_F_" + name] + c)[inner]
+    return dict(g=g, prof=prof, v=(lin["VX"][inner], lin["VY"][inner]))
+
+
+# the lattice's discretisation error of a first moment (h / a = 6): 5e-5
+TOL_GRAD = 5e-4
+
+
+def check(vals, want, scale):
+    err = float(np.max(np.abs(vals - want))) / scale
+    return err
+
+
+def test_linear_gradients_are_the_slopes(gradients):
+    g, pr = gradients["g"], gradients["prof"]
+    rows = []
+    for name, field in (("E", "GRADE0"), ("P", "GRADP0"), ("BULK", "GRADBULK0")):
+        for c, want in enumerate(pr[name][1:]):
+            rows.append((f"{field}+{c}", check(g(field, c), want, abs(want))))
+    # grad v: GRADV0 + 3 d + e = d_d v^e
+    for d in range(2):
+        for e, comp in enumerate(("VX", "VY")):
+            want = pr[comp][1 + d]
+            rows.append((f"GRADV0+{3 * d + e}", check(g("GRADV0", 3 * d + e), want, abs(want))))
+    # divshear_j = d_i pi^{j i}
+    div = (pr["PI11"][1] + pr["PI12"][2], pr["PI12"][1] + pr["PI22"][2])
+    for j in range(2):
+        rows.append((f"DIVSHEAR0+{j}", check(g("DIVSHEAR0", j), div[j], abs(div[j]))))
+    # gradshear_d = v^j d_j pi^{0 d} (v of the owner particle)
+    vx, vy = gradients["v"]
+    for d, comp in enumerate(("PI01", "PI02")):
+        want = vx * pr[comp][1] + vy * pr[comp][2]
+        rows.append((f"GRADSHEAR0+{d}", check(g("GRADSHEAR0", d), want, float(np.max(np.abs(want))))))
+    print(" ".join(f"{k}={v:.1e}" for k, v in rows))
+    assert all(v < TOL_GRAD for _, v in rows), rows
+
+
+def test_gradient_pins_reject_planted_mistakes(gradients):
+    """The answers of plausible mistakes -- a flipped sign, the transposed
+    velocity gradient, divshear over the other index pair, a factor
+    sigma_lab^2 -- are each far outside TOL_GRAD of the oracle's values."""
+    g, pr = gradients["g"], gradients["prof"]
+    sl2 = (1.0 / 0.05 ** 2) ** 2
+    far = 10 * TOL_GRAD
+    assert check(g("GRADE0", 0), -pr["E"][1], abs(pr["E"][1])) > far
+    assert check(g("GRADV0", 1), pr["VX"][2], abs(pr["VX"][2])) > far           # d_y v^x for d_x v^y
+    div_t = pr["PI11"][1] + pr["PI12"][1]                                       # d_j pi^{j x} summed wrongly
+    assert check(g("DIVSHEAR0", 0), div_t, abs(div_t)) > far
+    assert check(g("GRADP0", 0), pr["P"][1] * sl2, abs(pr["P"][1] * sl2)) > far
