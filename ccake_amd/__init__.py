@@ -134,6 +134,9 @@ ABI = {
     "ccake_advance_timestep": (C.c_int, [_vp, C.c_double, C.c_int]),
     "ccake_run_steps": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     "ccake_initialize": (C.c_int, [_vp]),
+    "ccake_allocate_k_values": (C.c_int, [_vp]),
+    "ccake_update_k": (C.c_int, [_vp, C.c_int]),
+    "ccake_update_rk4": (C.c_int, [_vp, C.c_double]),
     "ccake_process_initial_conditions": (C.c_int, [_vp]),
     "ccake_initialize_entropy_and_charge_densities": (C.c_int, [_vp]),
     "ccake_initial_smoothing": (C.c_int, [_vp]),

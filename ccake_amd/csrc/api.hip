@@ -251,6 +251,60 @@ static int rk2_step(ccake_event* ev, double dt, bool more = false, bool* begun =
 #endif
 
 template <class T>
+static int dalloc(ccake_event* ev, T** p, size_t count);
+
+// Evolver::allocate_k_values (src/evolver.cpp:519-526): the RK4
+// accumulator, allocated at an event's first fourth-order step and cleared
+// at every one (with a fill kernel: the step may be captured).
+__global__ void k_fill_f64(double* __restrict__ a, size_t n, double v)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+static int rk4_allocate_k(ccake_event* ev)
+{
+  const size_t cnt = (size_t)rk4_columns() * ev->npad;
+  if (!ev->k4) {
+    const int rc = dalloc(ev, &ev->k4, cnt);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, ev->stream, ev->k4, cnt, 0.0);
+  return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+}
+
+// Evolver::advance_timestep_rk4 (src/evolver.cpp:519-561) as the reference
+// runs it (not the classical method): the first stage's derivatives are not
+// kept; k = k2/6 + 2 k3/6 + 2 k4/6 + 2 k5/6 with k5 the derivatives at
+// t0 + dt after the fourth stage; y = y0 + dt k.  Every stage sorts afresh
+// (the stages' points lie up to dt apart), and none begins the next.
+static int rk4_step(ccake_event* ev, double dt)
+{
+  int rc;
+  if ((rc = launch_cache(ev))) return rc;                   // set_current_timestep_quantities
+  if ((rc = rk4_allocate_k(ev))) return rc;                 // allocate_k_values
+  const double t0 = ev->t;
+  if ((rc = time_derivatives(ev, 1, 0.5 * dt, t0, false))) return rc;   // step 1
+  ev->t = t0 + 0.5 * dt;
+  if ((rc = time_derivatives(ev, 1, 0.5 * dt, t0, false))) return rc;   // step 2
+  if ((rc = launch_rk4_update_k(ev, 1.0 / 6.0))) return rc;
+  if ((rc = time_derivatives(ev, 1, 0.5 * dt, t0, false))) return rc;   // step 3
+  if ((rc = launch_rk4_update_k(ev, 2.0 / 6.0))) return rc;
+  if ((rc = time_derivatives(ev, 1, dt, t0, false))) return rc;         // step 4
+  ev->t = t0 + dt;
+  if ((rc = launch_rk4_update_k(ev, 2.0 / 6.0))) return rc;
+  if ((rc = time_derivatives(ev, 0, 0.0, ev->t, false))) return rc;    // the last functional call
+  if ((rc = launch_rk4_update_k(ev, 2.0 / 6.0))) return rc;
+  if ((rc = launch_rk4_update(ev, dt))) return rc;                      // update_rk4
+  ev->sort_valid = false;
+  if (ev->p.particlization) {
+    prof_begin(ev, CCAKE_K_MISC);
+    if ((rc = freeze_step(ev))) return rc;
+    prof_end(ev, CCAKE_K_MISC);
+  }
+  return 0;
+}
+
+template <class T>
 static int dalloc(ccake_event* ev, T** p, size_t count)
 {
   const size_t bytes = count * sizeof(T) + CCAKE_ALLOC_PAD;
@@ -460,11 +514,20 @@ int ccake_eos_create(ccake_dev* dev, const ccake_eos_desc* desc, ccake_eos** out
   e->h_tab = new double[nn * 17];
   std::memcpy(e->h_tab, desc->table, nn * 17 * sizeof(double));
   std::vector<double> t5(nn * 5);
+  for (int q = 0; q < 5; ++q) { t.dmin[q] = INFINITY; t.dmax[q] = -INFINITY; }
+  bool finite = true;
   for (size_t k = 0; k < nn; ++k) {
     const double* c = desc->table + k * 17;
     t5[k * 5 + 0] = c[X_S]; t5[k * 5 + 1] = c[X_B]; t5[k * 5 + 2] = c[X_SS]; t5[k * 5 + 3] = c[X_Q];
     t5[k * 5 + 4] = c[X_E];
+    for (int q = 0; q < 5; ++q) {
+      const double v = t5[k * 5 + q];
+      if (!std::isfinite(v)) finite = false;
+      t.dmin[q] = std::min(t.dmin[q], v);
+      t.dmax[q] = std::max(t.dmax[q], v);
+    }
   }
+  if (!finite) for (int q = 0; q < 5; ++q) { t.dmin[q] = 1.0; t.dmax[q] = 0.0; }   // no bounds: never skipped
   e->bytes = nn * 22 * sizeof(double);
   {
     std::lock_guard<std::mutex> lk(dev->mu);
@@ -673,7 +736,7 @@ int ccake_event_destroy(ccake_event* ev)
   if (!ev) return CCAKE_ERR_INVALID;
   if (ev->stream) (void)hipStreamSynchronize(ev->stream);
   void* bufs[] = { ev->F, ev->I, ev->grid, ev->bbox, ev->key, ev->key_a, ev->key_b, ev->val_a, ev->val_b, ev->hist,
-                   ev->scan_tmp, ev->cell_off, ev->pair_count, ev->part, ev->hard, ev->hard_count, ev->cand_s, ev->cand_g, ev->red,
+                   ev->scan_tmp, ev->cell_off, ev->pair_count, ev->part, ev->hard, ev->hard_count, ev->k4, ev->cand_s, ev->cand_g, ev->red,
                    ev->err, ev->snaps, ev->fo_rows, ev->fo_count, ev->cand_d };
   for (void* b : bufs) if (b) (void)hipFree(b);
   if (ev->stream) (void)hipStreamDestroy(ev->stream);
@@ -835,20 +898,40 @@ int ccake_rk_update(ccake_event* ev, double dt, double t0)
 int ccake_advance_timestep(ccake_event* ev, double dt, int rk_order)
 {
   if (!ev) return CCAKE_ERR_INVALID;
-  if (rk_order != 2) { set_error("only RK2 is built"); return CCAKE_ERR_UNSUPPORTED; }
-  int rc = rk2_step(ev, dt);
+  if (rk_order != 2 && rk_order != 4) { set_error("Invalid Runge-Kutta order"); return CCAKE_ERR_INVALID; }
+  int rc = rk_order == 4 ? rk4_step(ev, dt) : rk2_step(ev, dt);
   if (rc) return rc;
+  return ccake_event_check(ev);
+}
+
+int ccake_allocate_k_values(ccake_event* ev) { return ev ? rk4_allocate_k(ev) : CCAKE_ERR_INVALID; }
+
+int ccake_update_k(ccake_event* ev, int n)
+{
+  if (!ev || n < 1 || n > 4) return CCAKE_ERR_INVALID;
+  if (!ev->k4) { set_error("allocate_k_values first"); return CCAKE_ERR_INVALID; }
+  static const double w[5] = { 0.0, 1.0 / 6.0, 2.0 / 6.0, 2.0 / 6.0, 1.0 / 6.0 };
+  return launch_rk4_update_k(ev, w[n]);
+}
+
+int ccake_update_rk4(ccake_event* ev, double dt)
+{
+  if (!ev) return CCAKE_ERR_INVALID;
+  if (!ev->k4) { set_error("allocate_k_values first"); return CCAKE_ERR_INVALID; }
+  const int rc = launch_rk4_update(ev, dt);
+  if (rc) return rc;
+  ev->sort_valid = false;               // positions moved from outside a step
   return ccake_event_check(ev);
 }
 
 int ccake_run_steps(ccake_event* ev, double dt, int rk_order, int n_steps, int* steps_done)
 {
   if (!ev) return CCAKE_ERR_INVALID;
-  if (rk_order != 2) { set_error("only RK2 is built"); return CCAKE_ERR_UNSUPPORTED; }
+  if (rk_order != 2 && rk_order != 4) { set_error("Invalid Runge-Kutta order"); return CCAKE_ERR_INVALID; }
   int done = 0;
   bool begun = false;                  // the next stage was begun by the last EoM kernel
   for (int k = 0; k < n_steps; ++k) {
-    int rc = rk2_step(ev, dt, k + 1 < n_steps, &begun);
+    int rc = rk_order == 4 ? rk4_step(ev, dt) : rk2_step(ev, dt, k + 1 < n_steps, &begun);
     if (rc) { if (steps_done) *steps_done = done; return rc; }
     ++done;
   }

@@ -58,6 +58,9 @@ struct EosMember {
   double mins[4], maxs[4];
   int dims[4];
   double gmin[4], gdel[4];
+  // table: smallest and largest node value of (s, B, S, Q, e), the bounds of
+  // the multilinear interpolant (dmin > dmax for members without them)
+  double dmin[5], dmax[5];
 };
 constexpr int kMaxMembers = 4;
 struct EosChain {
@@ -152,6 +155,7 @@ struct ccake_event {
   int nslices_cap = 1;          // slices the partial-sum buffer holds
   // EoS stage: particles whose inversion needs the full cascade (positions),
   // two alternating counters and which one the next stage appends to
+  double* k4 = nullptr;         // RK4 accumulator (allocated at the first fourth-order step)
   int32_t* hard = nullptr;
   int32_t* hard_count = nullptr;
   int hard_parity = 0;
@@ -242,6 +246,9 @@ int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0, bool fold_gra
                bool begin_next = false);
 double grad_sum_scale(double hT);
 int launch_cache(ccake_event* ev);
+int launch_rk4_update_k(ccake_event* ev, double w);   // k += w f (Evolver::update_k)
+int launch_rk4_update(ccake_event* ev, double dt);    // y = y0 + dt k (Evolver::update_rk4)
+int rk4_columns();
 int launch_rk(ccake_event* ev, double dt, double t0);
 // misc
 int launch_gamma_velocity(ccake_event* ev);

@@ -274,6 +274,21 @@ CK_HD bool in_range(const EosMember& m, const double* x)
   return true;
 }
 
+// A target the member cannot reach: the multilinear interpolant lies within
+// its node values, so a density more than the stopping tolerance outside
+// their range leaves |F| >= kRfTol everywhere in the table, and every
+// attempt on the member fails.  Skipping the member's attempts then gives
+// invert_point's result.
+CK_HD bool unreachable(const EosMember& m, int mode, const double* tgt)
+{
+  if (!(m.dmin[0] <= m.dmax[0])) return false;
+  const double v[4] = { tgt[0], tgt[1], tgt[2], tgt[3] };
+  const int idx[4] = { mode ? 4 : 0, 1, 2, 3 };    // packed (s, B, S, Q, e)
+  for (int a = 0; a < 4; ++a)
+    if (v[a] < m.dmin[idx[a]] - kRfTol || v[a] > m.dmax[idx[a]] + kRfTol) return true;
+  return false;
+}
+
 CK_HD double residual(const double* F, const double* tgt, double* R)
 {
   double s = 0.0;
@@ -508,42 +523,17 @@ CK_HD void seed_of_kind(const EosChain& ch, int mode, int kind, const double* tg
 CK_HD int seed_kinds(int mode) { return mode == 0 ? 4 : 2; }
 CK_HD int first_kind(int mode) { return mode == 0 ? 0 : 2; }
 
-// Point inversion over the chain.  mode 0: (s, B, S, Q) starting at the
-// particle's current member, seeds: current location, (T, 0, 0, 0),
-// (580 MeV, sgn), conformal-diagonal seed.  mode 1: (e, B, S, Q) from the
-// first member, seeds: (580 MeV, sgn), conformal-diagonal seed.  A solution
-// with cs2 outside [0, 1] moves on to the next member, seeded from it.
-template <int KM = kAllMembers>
-CK_HD bool invert_point(const EosChain& ch, const double* tab, const double* tab5, int mode,
-                        const double* tgt, double* x, int* member)
-{
-  int start = (mode == 0) ? *member : 0;
-  if (start < 0 || start >= ch.n) start = 0;
-  bool found = false;
-  int used = -1;
-  double res[4] = { 0, 0, 0, 0 };
-  for (int k = start; k < ch.n; ++k) {
-    const EosMember& m = ch.m[k];
-    found = false;
-    const int nseed = seed_kinds(mode);
-    for (int q = 0; q < nseed && !found; ++q) {
-      double y[4];
-      seed_of_kind(ch, mode, first_kind(mode) + q, tgt, x, y);
-      if (find_root<KM>(m, tab5, mode, tgt, y)) { found = true; for (int a = 0; a < 4; ++a) res[a] = y[a]; }
-    }
-    if (found) {
-      used = k;
-      double r[17];
-      member_fields<KM>(m, tab, res, r);
-      if (r[X_CS2] < 0.0 || r[X_CS2] > 1.0) { for (int a = 0; a < 4; ++a) x[a] = res[a]; continue; }
-      break;
-    }
-  }
-  if (!found) return false;
-  for (int a = 0; a < 4; ++a) x[a] = res[a];
-  *member = used;
-  return true;
-}
+// Point inversion over the chain (src/eos.cpp:629-975,
+// src/sph_workstation.cpp:1713-1851), "invert_point" below.  mode 0:
+// (s, B, S, Q) starting at the particle's current member, seeds: current
+// location, (T, 0, 0, 0), (580 MeV, sgn), conformal-diagonal seed.  mode 1:
+// (e, B, S, Q) from the first member, seeds: (580 MeV, sgn),
+// conformal-diagonal seed.  Within a member the seed kinds are tried in
+// order, each with find_root's attempts in order, and the first attempt
+// that converges is taken; a solution with cs2 outside [0, 1] moves on to
+// the next member, seeded from it.  The device runs it in two parts:
+// invert_first (the first attempt alone) and invert_wave (all of it, one
+// wave per particle).
 
 // The first attempt of invert_point alone, with a Newton budget: the
 // starting member, the first seed kind, attempt 0.  Returns 1 when that
@@ -557,6 +547,7 @@ CK_HD int invert_first(const EosChain& ch, const double* tab, const double* tab5
   int start = (mode == 0) ? *member : 0;
   if (start < 0 || start >= ch.n) start = 0;
   const EosMember& m = ch.m[start];
+  if (has_kind<KM, M_TABLE>() && m.kind == M_TABLE && unreachable(m, mode, tgt)) return -1;
   double y0[4], y[4];
   seed_of_kind(ch, mode, first_kind(mode), tgt, x, y0);
   root_seed(m, tgt, y0, 0, y);
@@ -589,6 +580,7 @@ __device__ bool invert_wave(const EosChain& ch, const double* tab, const double*
   double res[4] = { 0, 0, 0, 0 };
   for (int k = start; k < ch.n; ++k) {
     const EosMember& m = ch.m[k];
+    if (has_kind<KM, M_TABLE>() && m.kind == M_TABLE && unreachable(m, mode, tgt)) { found = false; continue; }
     double y[4] = { 0, 0, 0, 0 };
     bool ok = false;
     if (lane < na) {

@@ -769,6 +769,79 @@ __global__ __launch_bounds__(256) void k_cache(double* __restrict__ F, int npad,
   cache_state(Cols{ F, npad, i }, diff);
 }
 
+// ---- the reference's fourth-order step (Evolver::update_k and update_rk4,
+// src/evolver.cpp:171-341): the accumulator k has 24 columns in ev->k4,
+// K4_* below.  update_k(n) adds w_n times the stage's derivatives
+// (w = 1/6, 2/6, 2/6, 1/6 for n = 1..4; position from v); update_rk4 sets
+// y = y0 + dt k from the RK cache, with the negative-entropy rule, and
+// writes the extensive bulk pressure into the bulk field as well, as the
+// reference does (318).
+enum { K4_UX = 0, K4_UY, K4_X, K4_Y, K4_S, K4_B, K4_SS, K4_Q, K4_BULK, K4_EZ, K4_SHV0, K4_Q0 = K4_SHV0 + 5,
+       K4_NUM = K4_Q0 + 9 };
+
+__global__ __launch_bounds__(256) void k_rk4_update_k(double* __restrict__ F, int npad, int n, double* __restrict__ K,
+                                                      double w)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Cols col{ F, npad, i };
+  auto k = [&](int c) -> double& { return K[(size_t)c * npad + i]; };
+  for (int q = 0; q < 5; ++q) k(K4_SHV0 + q) += w * col(CCAKE_F_DEXT_SHV0 + q);
+  for (int q = 0; q < 9; ++q) k(K4_Q0 + q) += w * col(CCAKE_F_DEXT_Q0 + q);
+  k(K4_UX) += w * col(CCAKE_F_DUDT0 + 0); k(K4_UY) += w * col(CCAKE_F_DUDT0 + 1);
+  k(K4_X) += w * col(CCAKE_F_VX); k(K4_Y) += w * col(CCAKE_F_VY);
+  k(K4_B) += w * col(CCAKE_F_DEXT_B); k(K4_SS) += w * col(CCAKE_F_DEXT_SS); k(K4_Q) += w * col(CCAKE_F_DEXT_Q);
+  k(K4_S) += w * col(CCAKE_F_DEXT_S);
+  k(K4_BULK) += w * col(CCAKE_F_DEXT_BULK);
+  k(K4_EZ) += w * col(CCAKE_F_DEZ);
+}
+
+__global__ __launch_bounds__(256) void k_rk4_update(double* __restrict__ F, int npad, int n,
+                                                    const double* __restrict__ K, const int32_t* __restrict__ freeze,
+                                                    double dt, int32_t* __restrict__ err)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Cols col{ F, npad, i };
+  auto k = [&](int c) { return K[(size_t)c * npad + i]; };
+  double es = col(P_C_S) + dt * k(K4_S);
+  if (es < 0.0 && freeze[i] > 3) es = 1.e-3;
+  else if (es < 0.0) atomicOr(err, DE_NEG_ENTROPY);
+  const double bulk = col(P_C_BULK) + dt * k(K4_BULK);
+  col(CCAKE_F_EXT_S) = es;
+  col(CCAKE_F_EXT_BULK) = bulk;
+  col(CCAKE_F_EXT_B) = col(P_C_B) + dt * k(K4_B);
+  col(CCAKE_F_EXT_SS) = col(P_C_SS) + dt * k(K4_SS);
+  col(CCAKE_F_EXT_Q) = col(P_C_Q) + dt * k(K4_Q);
+  col(CCAKE_F_BULK) = bulk;
+  col(CCAKE_F_EZ) = col(P_C_EZ) + dt * k(K4_EZ);
+  col(CCAKE_F_X) = col(P_C_X) + dt * k(K4_X);
+  col(CCAKE_F_Y) = col(P_C_Y) + dt * k(K4_Y);
+  col(CCAKE_F_UX) = col(P_C_UX) + dt * k(K4_UX);
+  col(CCAKE_F_UY) = col(P_C_UY) + dt * k(K4_UY);
+  for (int q = 0; q < 5; ++q) col(CCAKE_F_EXT_SHV0 + q) = col(P_C_SHV0 + q) + dt * k(K4_SHV0 + q);
+  for (int q = 0; q < 9; ++q) col(CCAKE_F_EXT_Q0 + q) = col(P_C_Q0 + q) + dt * k(K4_Q0 + q);
+  col(CCAKE_F_ETA) = 0.0;
+  col(CCAKE_F_UETA) = 0.0;
+}
+
+int launch_rk4_update_k(ccake_event* ev, double w)
+{
+  const int n = ev->n;
+  hipLaunchKernelGGL(k_rk4_update_k, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n, ev->k4, w);
+  return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+}
+
+int launch_rk4_update(ccake_event* ev, double dt)
+{
+  const int n = ev->n;
+  hipLaunchKernelGGL(k_rk4_update, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n, ev->k4,
+                     ev->I + (size_t)CCAKE_I_FREEZE * ev->npad, dt, ev->err);
+  return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+}
+
+int rk4_columns() { return K4_NUM; }
+
 int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0, bool fold_grad_sum, bool begin_next)
 {
   (void)t0;

@@ -431,13 +431,43 @@ class Evolver {
     check(ccake_rk_update(systemPtr->event(), dt, t0), "ccake_rk_update", settingsPtr.get());
     systemPtr->t = t0 + dt;
   }
-  void execute_timestep(double dt, int rk_order, std::function<void(void)> time_derivatives_functional)
+  // Evolver::allocate_k_values / update_k / update_rk4 (src/evolver.cpp:171-341)
+  void allocate_k_values() { check(ccake_allocate_k_values(systemPtr->event()), "ccake_allocate_k_values", settingsPtr.get()); }
+  void update_k(int n) { check(ccake_update_k(systemPtr->event(), n), "ccake_update_k", settingsPtr.get()); }
+  void update_rk4(double dt) { check(ccake_update_rk4(systemPtr->event(), dt), "ccake_update_rk4", settingsPtr.get()); }
+
+  // Evolver::advance_timestep_rk2 (src/evolver.cpp:493-517)
+  void advance_timestep_rk2(double dt, std::function<void(void)> time_derivatives_functional)
   {
-    if (rk_order != 2) check(CCAKE_ERR_UNSUPPORTED, "Evolver::execute_timestep (rk_order != 2)", settingsPtr.get());
-    const double t0 = systemPtr->t;
     set_current_timestep_quantities();
+    const double t0 = systemPtr->t;
     step_rk(0.5 * dt, t0, time_derivatives_functional);
     step_rk(dt, t0, time_derivatives_functional);
+  }
+  // Evolver::advance_timestep_rk4 (src/evolver.cpp:519-561), the reference's
+  // sequence of calls as it is
+  void advance_timestep_rk4(double dt, std::function<void(void)> time_derivatives_functional)
+  {
+    set_current_timestep_quantities();
+    allocate_k_values();
+    const double t0 = systemPtr->t;
+    step_rk(0.5 * dt, t0, time_derivatives_functional);
+    step_rk(0.5 * dt, t0, time_derivatives_functional);
+    update_k(1);
+    step_rk(0.5 * dt, t0, time_derivatives_functional);
+    update_k(2);
+    step_rk(dt, t0, time_derivatives_functional);
+    update_k(3);
+    time_derivatives_functional();
+    update_k(3);
+    update_rk4(dt);
+  }
+  // Evolver::execute_timestep (src/evolver.cpp:97-117)
+  void execute_timestep(double dt, int rk_order, std::function<void(void)> time_derivatives_functional)
+  {
+    if (rk_order == 2) advance_timestep_rk2(dt, time_derivatives_functional);
+    else if (rk_order == 4) advance_timestep_rk4(dt, time_derivatives_functional);
+    else check(CCAKE_ERR_INVALID, "Evolver::execute_timestep: Invalid Runge-Kutta order", settingsPtr.get());
   }
 
  private:

@@ -13,7 +13,10 @@
 // --txt-evolution K the system state is written to OUTPUT/system_state_<k>.dat
 // after initialisation and every K steps (src/BSQHydro.cpp:689-691, 791-796).
 //
-//   gubser_run [--spacing 0.05] [--steps 120] [--dt 0.005] [--evolver-steps 0]
+// With --rk 4 every step is the reference's fourth-order step
+// (Evolver::advance_timestep_rk4, src/evolver.cpp:519-561).
+//
+//   gubser_run [--spacing 0.05] [--steps 120] [--dt 0.005] [--evolver-steps 0] [--rk 2]
 //              [--freeze-out-T MeV] [--ic FILE --e-cutoff GeV/fm^3]
 //              [--txt-evolution K] [--output DIR]
 #include <algorithm>
@@ -75,7 +78,7 @@ std::shared_ptr<ccake::SystemState<2>> make_system(std::shared_ptr<ccake::Settin
 int main(int argc, char** argv)
 {
   double a = 0.05;
-  int steps = 120, evolver_steps = 0;
+  int steps = 120, evolver_steps = 0, rk = 2;
   double dt = 0.005, t_fo_mev = 0.0, e_cutoff = 0.15;
   int txt_stride = 0;
   std::string out_dir = ".", ic_file;
@@ -89,10 +92,11 @@ int main(int argc, char** argv)
     else if (!std::strcmp(argv[k], "--ic")) ic_file = argv[k + 1];
     else if (!std::strcmp(argv[k], "--e-cutoff")) e_cutoff = std::atof(argv[k + 1]);
     else if (!std::strcmp(argv[k], "--txt-evolution")) txt_stride = std::atoi(argv[k + 1]);
+    else if (!std::strcmp(argv[k], "--rk")) rk = std::atoi(argv[k + 1]);
   }
   auto s = std::make_shared<ccake::Settings>();
   s->eos_type = "conformal";
-  s->t0 = 1.0; s->dt = dt; s->hT = 0.3;
+  s->t0 = 1.0; s->dt = dt; s->hT = 0.3; s->rk_order = rk;
   s->particlization_enabled = t_fo_mev > 0.0;
   if (s->particlization_enabled) s->Freeze_Out_Temperature = t_fo_mev / 197.3269804;
   s->using_shear = false;
@@ -141,8 +145,8 @@ int main(int argc, char** argv)
       w1.initialize(); w1.initialize_hydrodynamics();
       w2.initialize(); w2.initialize_hydrodynamics();
       for (int k = 0; k < evolver_steps; ++k) {
-        w1.advance_timestep(s->dt, 2);
-        w2.advance_timestep_with_evolver(s->dt, 2);
+        w1.advance_timestep(s->dt, s->rk_order);
+        w2.advance_timestep_with_evolver(s->dt, s->rk_order);
       }
       s1->copy_device_to_host();
       s2->copy_device_to_host();

@@ -290,9 +290,19 @@ CCAKE_API int  ccake_cache_state(ccake_event* ev);
  * particle; the status is returned by ccake_event_check. */
 CCAKE_API int  ccake_rk_update(ccake_event* ev, double dt, double t0);
 /* SPHWorkstation::advance_timestep (src/sph_workstation.cpp:2042-2073) with
- * Evolver::advance_timestep_rk2 (src/evolver.cpp:493-517) and, when
+ * Evolver::advance_timestep_rk2 (src/evolver.cpp:493-517), or with rk_order
+ * 4 advance_timestep_rk4 (519-561), and, when
  * particlization is on, freeze_out_particles (1474-1485). */
 CCAKE_API int  ccake_advance_timestep(ccake_event* ev, double dt, int rk_order);
+/* Evolver::allocate_k_values, update_k(n) and update_rk4(dt)
+ * (src/evolver.cpp:171-341, 519-561): the pieces of the reference's
+ * fourth-order step for a caller that drives it phase by phase.
+ * update_k(n) adds w_n (1/6, 2/6, 2/6, 1/6 for n = 1..4) times the last
+ * derivatives to k; update_rk4 sets y = y0 + dt k from the RK cache.
+ * ccake_advance_timestep(ev, dt, 4) runs the whole step. */
+CCAKE_API int  ccake_allocate_k_values(ccake_event* ev);
+CCAKE_API int  ccake_update_k(ccake_event* ev, int n);
+CCAKE_API int  ccake_update_rk4(ccake_event* ev, double dt);
 /* Run `n_steps` timesteps; stops early when continue_evolution is false. */
 CCAKE_API int  ccake_run_steps(ccake_event* ev, double dt, int rk_order, int n_steps, int* steps_done);
 /* The initialisation phases, in the order BSQHydro::initialize_hydrodynamics

@@ -191,6 +191,7 @@ void orc_event_destroy(orc_event* ev)
   free(ev->c_s); free(ev->c_B); free(ev->c_S); free(ev->c_Q); free(ev->c_bulk); free(ev->c_E0);
   for (int k = 0; k < 5; ++k) free(ev->c_shv[k]);
   for (int k = 0; k < 9; ++k) free(ev->c_q[k]);
+  free(ev->k4);
   for (int k = 0; k < 4; ++k) { free(ev->fback_e[k]); free(ev->fback_gradP0[k]); }
   free(ev->frz1_gradP0); free(ev->frz2_gradP0);
   free(ev->nb_off); free(ev->nb_idx);
@@ -832,8 +833,106 @@ int orc_continue_evolution(orc_event* ev, double max_tau)
 
 /* Evolver::advance_timestep_rk2 (src/evolver.cpp:493-517) inside
  * SPHWorkstation::advance_timestep (src/sph_workstation.cpp:2042-2073). */
+/* ------------------------------------------------------------------ */
+/* The reference's fourth-order step, Evolver::advance_timestep_rk4
+ * (src/evolver.cpp:519-561) with update_k (171-212) and update_rk4
+ * (215-341), restated as it is (it is not the classical method):
+ *   y1 = y0 + dt/2 f(y0, t0)                     (step 1, f not kept)
+ *   k2 = f(y1, t0+dt/2);  y2 = y0 + dt/2 k2;     k += k2 / 6
+ *   k3 = f(y2, t0+dt/2);  y3 = y0 + dt/2 k3;     k += 2 k3 / 6
+ *   k4 = f(y3, t0+dt/2);  y4 = y0 + dt k4, t = t0 + dt;  k += 2 k4 / 6
+ *   k5 = f(y4, t0+dt);                           k += 2 k5 / 6
+ *   y = y0 + dt k   (weights summing to 7/6), with the negative-entropy rule
+ * and the extensive bulk pressure written into the bulk field too (218-341).
+ * k of the position is accumulated from v (dr/dt).                       */
+enum { K4_U = 0, K4_R = 3, K4_S = 6, K4_B, K4_SS, K4_Q, K4_BULK, K4_E0, K4_SHV, K4_DQ = K4_SHV + 5, K4_N = K4_DQ + 9 };
+
+static double* k4col(orc_event* ev, int c) { return ev->k4 + (size_t)c * ((size_t)ev->n + 1); }
+
+static void rk4_accumulate(orc_event* ev, double w)
+{
+  const int D = ev->p.dim;
+  for (int i = 0; i < ev->n; ++i) {
+    for (int k = 0; k < 5; ++k) k4col(ev, K4_SHV + k)[i] += w*FLD(ev, ORC_F_DEXT_SHV0 + k)[i];
+    for (int k = 0; k < 9; ++k) k4col(ev, K4_DQ + k)[i] += w*FLD(ev, ORC_F_DEXT_Q0 + k)[i];
+    for (int d = 0; d < D; ++d) {
+      k4col(ev, K4_U + d)[i] += w*FLD(ev, ORC_F_DUDT0 + d)[i];
+      k4col(ev, K4_R + d)[i] += w*FLD(ev, ORC_F_VX + d)[i];
+    }
+    k4col(ev, K4_B)[i] += w*FLD(ev, ORC_F_DEXT_B)[i];
+    k4col(ev, K4_SS)[i] += w*FLD(ev, ORC_F_DEXT_SS)[i];
+    k4col(ev, K4_Q)[i] += w*FLD(ev, ORC_F_DEXT_Q)[i];
+    k4col(ev, K4_S)[i] += w*FLD(ev, ORC_F_DEXT_S)[i];
+    k4col(ev, K4_BULK)[i] += w*FLD(ev, ORC_F_DEXT_BULK)[i];
+    k4col(ev, K4_E0)[i] += w*FLD(ev, ORC_F_DEZ)[i];
+  }
+}
+
+static int rk4_update(orc_event* ev, double dt)
+{
+  const int D = ev->p.dim;
+  int fail = 0;
+  for (int i = 0; i < ev->n; ++i) {
+    double es = ev->c_s[i] + dt*k4col(ev, K4_S)[i];
+    const double rB = ev->c_B[i] + dt*k4col(ev, K4_B)[i];
+    const double rS = ev->c_S[i] + dt*k4col(ev, K4_SS)[i];
+    const double rQ = ev->c_Q[i] + dt*k4col(ev, K4_Q)[i];
+    const double bulk = ev->c_bulk[i] + dt*k4col(ev, K4_BULK)[i];
+    const double ez = ev->c_E0[i] + dt*k4col(ev, K4_E0)[i];
+    const int fr = ev->I[ORC_I_FREEZE][i];
+    if (es < 0.0 && fr > 3) es = 1.e-3;
+    else if (es < 0.0) fail = 1;
+    FLD(ev, ORC_F_EXT_S)[i] = es;
+    FLD(ev, ORC_F_EXT_BULK)[i] = bulk;
+    FLD(ev, ORC_F_EXT_B)[i] = rB;
+    FLD(ev, ORC_F_EXT_SS)[i] = rS;
+    FLD(ev, ORC_F_EXT_Q)[i] = rQ;
+    FLD(ev, ORC_F_BULK)[i] = bulk;          /* the reference writes the extensive value here (318) */
+    FLD(ev, ORC_F_EZ)[i] = ez;
+    for (int d = 0; d < D; ++d) {
+      FLD(ev, ORC_F_X + d)[i] = ev->c_r[d][i] + dt*k4col(ev, K4_R + d)[i];
+      FLD(ev, ORC_F_UX + d)[i] = ev->c_u[d][i] + dt*k4col(ev, K4_U + d)[i];
+    }
+    for (int k = 0; k < 5; ++k) FLD(ev, ORC_F_EXT_SHV0 + k)[i] = ev->c_shv[k][i] + dt*k4col(ev, K4_SHV + k)[i];
+    for (int k = 0; k < 9; ++k) FLD(ev, ORC_F_EXT_Q0 + k)[i] = ev->c_q[k][i] + dt*k4col(ev, K4_DQ + k)[i];
+    for (int d = D; d < 3; ++d) { FLD(ev, ORC_F_X + d)[i] = 0.0; FLD(ev, ORC_F_UX + d)[i] = 0.0; }
+  }
+  return fail ? ORC_ERR_NEG_ENTROPY : ORC_OK;
+}
+
+static int orc_advance_timestep_rk4(orc_event* ev, double dt)
+{
+  if (!ev->k4) ev->k4 = (double*)calloc((size_t)K4_N * ((size_t)ev->n + 1), sizeof(double));
+  else memset(ev->k4, 0, (size_t)K4_N * ((size_t)ev->n + 1) * sizeof(double));
+  PHASE(11, orc_cache_state(ev));
+  const double t0 = ev->t;
+  int st;
+  if ((st = orc_get_time_derivatives(ev))) return st;                 /* step 1 */
+  PHASE(11, st = orc_rk_update(ev, 0.5*dt, t0));
+  if (st) return st;
+  if ((st = orc_get_time_derivatives(ev))) return st;                 /* step 2 */
+  PHASE(11, st = orc_rk_update(ev, 0.5*dt, t0));
+  if (st) return st;
+  rk4_accumulate(ev, 1.0/6.0);
+  if ((st = orc_get_time_derivatives(ev))) return st;                 /* step 3 */
+  PHASE(11, st = orc_rk_update(ev, 0.5*dt, t0));
+  if (st) return st;
+  rk4_accumulate(ev, 2.0/6.0);
+  if ((st = orc_get_time_derivatives(ev))) return st;                 /* step 4 */
+  PHASE(11, st = orc_rk_update(ev, dt, t0));
+  if (st) return st;
+  rk4_accumulate(ev, 2.0/6.0);
+  if ((st = orc_get_time_derivatives(ev))) return st;                 /* the last functional call */
+  rk4_accumulate(ev, 2.0/6.0);
+  PHASE(11, st = rk4_update(ev, dt));
+  if (st) return st;
+  if (ev->p.particlization) PHASE(12, orc_freeze_out_particles(ev));
+  return ORC_OK;
+}
+
 int orc_advance_timestep(orc_event* ev, double dt, int rk_order)
 {
+  if (rk_order == 4) return orc_advance_timestep_rk4(ev, dt);
   if (rk_order != 2) return ORC_ERR_INVALID;
   PHASE(11, orc_cache_state(ev));
   const double t0 = ev->t;

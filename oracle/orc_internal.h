@@ -84,6 +84,9 @@ struct orc_event {
   double* c_s, *c_B, *c_S, *c_Q, *c_bulk, *c_E0;
   double* c_shv[5];
   double* c_q[9];        /* extensive diffusion (evolver.cpp:132-152) */
+  /* RK4 accumulator k (Evolver::k, src/evolver.cpp:171-212): u, r, s, B, S,
+   * Q, bulk, E0, 5 shear, 9 diffusion; allocated at the first RK4 step */
+  double* k4;
   /* neighbour list (CSR, j ascending) */
   int64_t* nb_off;
   int32_t* nb_idx;

@@ -466,6 +466,26 @@ def test_table_eos_parity(dev, orc, table):
     p.close()
 
 
+def test_rk4_parity(dev, orc):
+    """The reference's fourth-order step (src/evolver.cpp:519-561) on the
+    viscous lumpy event (conformal EoS with charges, shear and bulk; 10,201
+    particles): 3 steps on the GPU equal the oracle's, and the extensive
+    bulk pressure is in the bulk field after each step, as update_rk4 leaves
+    it."""
+    d = lumpy_small(0.1)
+    p = Pair(dev, orc, config.lumpy_params(), d)
+    p.initialize()
+    for _ in range(3):
+        p.g.advance_timestep(p.cfg["dt"], 4)
+        assert p.o.advance_timestep(p.cfg["dt"], 4) == 0
+    assert abs(p.g.t - p.o.t) < 1e-15
+    report_all(p.g, p.o, "rk4 3 steps")
+    compare_lumpy(p, STATE + DERIVS, what="rk4 3 steps")
+    assert np.array_equal(p.g.download("CCAKE_F_BULK"), p.g.download("CCAKE_F_EXT_BULK"))
+    ints_equal(p.g, p.o, "FREEZE")
+    p.close()
+
+
 def test_table_eos_long_run_parity(dev, orc, table):
     """The lumpy event on the table EoS with the reference's default energy
     cutoff (0.15 GeV/fm^3; 8,077 particles on the 0.1 fm lattice): 60 steps,

@@ -220,3 +220,68 @@ def test_gradient_pins_reject_planted_mistakes(gradients):
     div_t = pr["PI11"][1] + pr["PI12"][1]                                       # d_j pi^{j x} summed wrongly
     assert check(g("DIVSHEAR0", 0), div_t, abs(div_t)) > far
     assert check(g("GRADP0", 0), pr["P"][1] * sl2, abs(pr["P"][1] * sl2)) > far
+
+
+# ---------------------------------------------------------------- RK4
+def test_rk4_step_is_the_references_sequence(orc):
+    """One step of Evolver::advance_timestep_rk4 (src/evolver.cpp:519-561)
+    on an ideal conformal fluid at rest: in the interior nothing moves and
+    the extensive entropy is constant, so only E_z evolves, with the rate
+    g(t) = p(s_ext sigma_lab / t) m / sigma_lab (src/eom_default.cpp:855)
+    that depends on t alone.  The reference's step then gives exactly
+        E_z(t0 + dt) = E_z(t0) + dt (g(t0+dt/2)/6 + 2 g(t0+dt/2)/6
+                                     + 2 g(t0+dt/2)/6 + 2 g(t0+dt)/6),
+    weights summing to 7/6 (the first stage's derivative is not kept); the
+    classical step, dt (g(t0) + 4 g(t0+dt/2) + g(t0+dt)) / 6, differs by
+    far more than the rounding."""
+    x, y = lattice(3.0, 0.1)
+    cfg = config.gubser_ideal_params(dt=0.05)
+    eos = orc.Eos(0)
+    o = orc.Event(cfg, eos, len(x))
+    load(o, x, y, np.full_like(x, 10.0 / config.HBARC_GEVFM))
+    assert o.initialize() == 0
+    inner = (np.abs(x) < 1.0) & (np.abs(y) < 1.0)
+    t0, dt = o.t, cfg["dt"]
+    ez0 = o.field("ORC_F_EZ")[inner].copy()
+    sext = o.field("ORC_F_EXT_S")[inner].copy()
+    sl = o.field("ORC_F_SIGMA_LAB")[inner].copy()
+    m = o.field("ORC_F_MASS_S")[inner].copy()
+    assert o.advance_timestep(dt, 4) == 0
+    assert abs(o.t - (t0 + dt)) < 1e-15
+    assert np.max(np.abs(o.field("ORC_F_UX")[inner])) < 1e-10       # at rest to rounding
+    assert np.array_equal(o.field("ORC_F_EXT_S")[inner], sext)
+
+    def g(t):
+        out = np.empty(len(sext))
+        for k in range(len(sext)):
+            ok, _, _, th = eos.invert_point(0, (sext[k] * sl[k] / t, 0.0, 0.0, 0.0), (1.0, 0.0, 0.0, 0.0))
+            assert ok
+            out[k] = th[9] * m[k] / sl[k]
+        return out
+
+    gh, g1, g0 = g(t0 + 0.5 * dt), g(t0 + dt), g(t0)
+    want = ez0 + dt * (gh / 6.0 + 2.0 * gh / 6.0 + 2.0 * gh / 6.0 + 2.0 * g1 / 6.0)
+    classical = ez0 + dt * (g0 + 4.0 * gh + g1) / 6.0
+    got = o.field("ORC_F_EZ")[inner]
+    err = np.max(np.abs(got - want) / np.abs(want - ez0))
+    far = np.max(np.abs(classical - want) / np.abs(want - ez0))
+    print(f"RK4 E_z: relative error {err:.2e}; classical RK4 would be {far:.2e} away")
+    # the Newton roots of g's evaluations and the step's own differ at 1e-12
+    assert err < 1e-10 and far > 0.1
+
+
+def test_rk4_update_writes_the_extensive_bulk_into_bulk(orc):
+    """update_rk4 stores the updated extensive bulk pressure into the bulk
+    field as well (src/evolver.cpp:318): after a fourth-order step of a
+    bulk-viscous fluid the two fields are equal (and non-zero)."""
+    tab, dims, mins, maxs = eos_tables.tanh_conformal_table()
+    eos = orc.Eos(1, T_FO, dims, mins, maxs, tab)
+    cfg = config.lumpy_params(eta_mode=config.ETA_CONSTANT, constant_eta_over_s=0.0, using_shear=0, particlization=0,
+                              zeta_gauss_A=1.0)
+    x, y = lattice(2.0, 0.1)
+    o = orc.Event(cfg, eos, len(x))
+    load(o, x, y, np.full_like(x, 6.0 / config.HBARC_GEVFM), np.full_like(x, 0.1))
+    assert o.initialize() == 0
+    assert o.advance_timestep(cfg["dt"], 4) == 0
+    b, eb = o.field("ORC_F_BULK"), o.field("ORC_F_EXT_BULK")
+    assert np.any(eb != 0.0) and np.array_equal(b, eb)
