@@ -18,13 +18,13 @@
  * Gubser-mode tau_pi).  BSQ diffusion is pinned by tests/test_diffusion.py
  * (at-rest Israel-Stewart relaxation, continuity and transverse-momentum
  * conservation of a uniform moving fluid, SPH gradient identities, the
- * regulator cap, transport special values).  Parity unpinned:
+ * regulator cap, transport special values).  tests/test_oracle_pins.py pins
+ * the bulk-pressure terms (0+1D Bjorken interior against an ODE solve), the
+ * six gradient sums (linear profiles) and the fourth-order step (a closed
+ * form); the d alpha / d(s, rho) fields and the Knudsen numbers with flow
+ * are pinned in tests/test_oracle.py.  Parity unpinned:
  *   - which root the reference's GSL hybrids solver returns where several
  *     exist (the inversion here is Newton with the same absolute stop);
- *   - the values of the d alpha / d(s, rho) fields themselves (the diffusion
- *     tests use them only as inputs or through their Euler relation);
- *   - the bulk-pressure terms of the equations of motion (zero in conformal
- *     flows, so no analytic solution exercises them);
  *   - diffusion MRF terms multiplied by the geometric factor or u^eta (zero
  *     for D = 2).
  */

@@ -181,6 +181,35 @@ def test_enthalpy_derivatives_by_finite_differences(orc):
             assert abs(fd - th[idx]) <= 2e-6 * max(1.0, abs(th[idx])), (k, fd, th[idx])
 
 
+def test_alpha_derivatives_by_finite_differences(orc):
+    """d alpha_c / d(s, B, S, Q), alpha_c = mu_c / T for c = B, S, Q, each at
+    fixed values of the other three densities (the fields the diffusion MRFs
+    read, src/eos_derivatives.cpp): central differences along the inversion
+    on the conformal EoS with all three charges non-zero.  Pins the index
+    placement of the 12 fields (charge major, density minor) and their
+    Schur complements."""
+    eos = orc.Eos(0)
+    rng = np.random.default_rng(13)
+    for _ in range(6):
+        T = rng.uniform(0.6, 2.0); mu = rng.uniform(-0.8, 0.8, 3)
+        r = _thermo_at(eos, 0, T, *mu)
+        dens = np.array([r[1], r[2], r[3], r[4]])             # s, B, S, Q
+        ok, x, m, th = eos.invert_point(0, list(dens), [T, *mu], 0)
+        assert ok
+
+        def alpha_at(dv):
+            ok2, x2, _, th2 = eos.invert_point(0, list(dens + dv), [T, *mu], 0)
+            assert ok2
+            return np.array([th2[1], th2[2], th2[3]]) / th2[0]    # mu_B, mu_S, mu_Q over T
+        for k in range(4):
+            h = 1e-5 * max(abs(dens[k]), 1.0)
+            dv = np.zeros(4); dv[k] = h
+            fd = (alpha_at(dv) - alpha_at(-dv)) / (2 * h)
+            for c in range(3):
+                got = th[16 + 4 * c + k]
+                assert abs(fd[c] - got) <= 2e-6 * max(1.0, abs(got)), (c, k, fd[c], got)
+
+
 def test_tanh_conformal_cs2(orc):
     """cs2 of the tanh-conformal EoS: at mu = 0 it is s/(T chi_TT); at mu != 0
     the adiabatic derivative dp/de along fixed n/s, by finite differences."""
@@ -392,3 +421,63 @@ def test_freeze_status_machine(orc):
     # when e rose; 1 -> 3 below e_FO; 1 -> 0 above e_FO and falling; 4 stays 4
     assert list(ev.ifield("ORC_I_FREEZE")) == [0, 1, 3, 1, 3, 0, 4]
     assert np.array_equal(ev.field("ORC_F_FRZ2_E"), ev.field("ORC_F_E"))
+
+
+def test_hydro_numbers_closed_forms_with_flow(orc):
+    """compute_hydro_numbers (src/eom_default.cpp:1831-1995) on moving fluid
+    cells, u = (u^x, 0), gamma = sqrt(1 + u_x^2), v = u / gamma, with
+    prescribed d_i v^j, pi^{ij} and Pi at tau = 1.3.  The expansion rate is
+    theta = d_mu u^mu + u^tau / tau = dgamma/dt + gamma div v + gamma / tau
+    (the comoving dgamma/dt = u^x (du^x/dt) / gamma from the EoM's du/dt).
+    With u^y = 0 the projector's y row is the unit row, so the textbook
+    sigma^{yy} = d_y u^y - theta / 3 with d_y u^y = gamma d_y v^y.  The
+    reference's read-back keeps sigma^{yy} (into the (1,1) slot) and
+    sigma^{y eta} = 0 and rebuilds the rest from u: then
+    sigma_{mu nu} sigma^{mu nu} = 2 (sigma^{yy})^2 / gamma^4, so
+    Kn_shear = tau_pi sqrt(2) |sigma^{yy}| / gamma^2, and Kn_bulk =
+    tau_Pi |theta|.  The flow-dependent parts pinned here are the MRF
+    shear rate's u-terms and theta's dgamma/dt and gamma factors."""
+    rng = np.random.default_rng(12)
+    n = 6
+    cfg = config.base_params(t0=1.3, constant_eta_over_s=0.08)
+    ev = _event(orc, cfg, n)
+    ev.t = 1.3
+    t = 1.3
+    ux = rng.uniform(-0.8, 0.8, n)
+    gam = np.sqrt(1.0 + ux ** 2)
+    ev.set("ORC_F_GAMMA", gam)
+    ev.set("ORC_F_UX", ux); ev.set("ORC_F_UY", np.zeros(n))
+    ev.set("ORC_F_VX", ux / gam); ev.set("ORC_F_VY", np.zeros(n))
+    for f in ("T", "W", "S", "P", "SIGMA", "SIGMA_LAB", "MASS_S", "DWDS", "TAU_PI", "TAU_BULK", "ETA_PI",
+              "ZETA_PI", "DELTA_PIPI", "DELTA_BB"):
+        ev.set("ORC_F_" + f, rng.uniform(0.5, 2.0, n))
+    g = rng.normal(scale=0.3, size=(4, n))          # d_x v^x, d_x v^y, d_y v^x, d_y v^y
+    fid = orc.FIELDS
+    for k, c in enumerate((0, 1, 3, 4)):
+        ev.field(fid["ORC_F_GRADV0"] + c)[:] = g[k]
+    # a shear tensor orthogonal to u (pi^{0i} = v_j pi^{ji}) and traceless
+    pxx, pxy, pyy = rng.normal(scale=0.05, size=(3, n))
+    shv = np.zeros((16, n))
+    shv[5] = pxx; shv[6] = shv[9] = pxy; shv[10] = pyy
+    shv[1] = shv[4] = (ux / gam) * pxx
+    shv[2] = shv[8] = (ux / gam) * pxy
+    shv[0] = (ux / gam) ** 2 * pxx
+    shv[15] = (shv[0] - pxx - pyy) / t ** 2
+    for c in range(16):
+        ev.field(fid["ORC_F_SHV0"] + c)[:] = shv[c]
+    ev.set("ORC_F_BULK", rng.normal(scale=0.1, size=n))
+    ev.eom()
+    dux = ev.field(fid["ORC_F_DUDT0"])
+    assert np.all(np.abs(dux) > 1e-3)                # the EoM's du/dt is live
+    theta = ux * dux / gam + gam * (g[0] + g[3]) + gam / t
+    assert np.allclose(ev.field("ORC_F_THETA"), theta, rtol=1e-12)
+    s_yy = gam * g[3] - theta / 3.0
+    kn = ev.field("ORC_F_SHEAR_KNUDSEN")
+    assert np.allclose(kn, ev.field("ORC_F_TAU_PI") * np.sqrt(2.0) * np.abs(s_yy) / gam ** 2, rtol=1e-10)
+    assert np.allclose(ev.field("ORC_F_BULK_KNUDSEN"), ev.field("ORC_F_TAU_BULK") * np.abs(theta), rtol=1e-12)
+    # plausible mistakes fall far outside: gamma factors dropped, the
+    # Eulerian/comoving dgamma/dt term dropped, the at-rest formula
+    for wrong in (ev.field("ORC_F_TAU_PI") * np.sqrt(2.0) * np.abs(g[3] - theta / 3.0) / gam ** 2,
+                  ev.field("ORC_F_TAU_PI") * np.sqrt(2.0) * np.abs(s_yy),
+                  ev.field("ORC_F_TAU_PI") * np.sqrt(2.0) * np.abs(gam * g[3] - (theta - ux * dux / gam) / 3.0) / gam ** 2):
+        assert np.max(np.abs(wrong - kn) / kn) > 1e-3
