@@ -653,6 +653,10 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
     set_error("DNMR diffusion coefficients are not built (the reference returns zero matrices for them)");
     return CCAKE_ERR_UNSUPPORTED;
   }
+  if (p->pair_precision != 0 && p->pair_precision != 1) {
+    set_error("pair_precision must be 0 (FP64) or 1 (FP32 with compensated sums)");
+    return CCAKE_ERR_INVALID;
+  }
   CCAKE_HIP(hipSetDevice(dev->gpu));
   ccake_event* ev = new ccake_event();
   ev->dev = dev; ev->eos = eos; ev->p = *p; ev->n = n;
@@ -697,6 +701,10 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
   rc |= dalloc(ev, &ev->cand_s, 8 * np);
   rc |= dalloc(ev, &ev->cand_g, 16 * np);
   if (p->using_diffusion) rc |= dalloc(ev, &ev->cand_d, 16 * np);
+  if (p->pair_precision == 1) {
+    rc |= dalloc(ev, &ev->cand_s32, (size_t)kCandS32 * np);
+    rc |= dalloc(ev, &ev->cand_g32, (size_t)kCandG32 * np);
+  }
   ev->snap_cols = p->using_diffusion ? 45 : 33;
   rc |= dalloc(ev, &ev->red, 1024);
   rc |= dalloc(ev, &ev->err, 2);          // [0] error flags, [1] charge-check scratch
@@ -737,7 +745,7 @@ int ccake_event_destroy(ccake_event* ev)
   if (ev->stream) (void)hipStreamSynchronize(ev->stream);
   void* bufs[] = { ev->F, ev->I, ev->grid, ev->bbox, ev->key, ev->key_a, ev->key_b, ev->val_a, ev->val_b, ev->hist,
                    ev->scan_tmp, ev->cell_off, ev->pair_count, ev->part, ev->hard, ev->hard_count, ev->k4, ev->cand_s, ev->cand_g, ev->red,
-                   ev->err, ev->snaps, ev->fo_rows, ev->fo_count, ev->cand_d };
+                   ev->err, ev->snaps, ev->fo_rows, ev->fo_count, ev->cand_d, ev->cand_s32, ev->cand_g32 };
   for (void* b : bufs) if (b) (void)hipFree(b);
   if (ev->stream) (void)hipStreamDestroy(ev->stream);
   {

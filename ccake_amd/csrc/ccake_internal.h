@@ -164,6 +164,10 @@ struct ccake_event {
   double* cand_s = nullptr;     // 8 doubles per particle
   double* cand_g = nullptr;     // 16 doubles per particle
   double* cand_d = nullptr;     // diffusion pair records, 16 doubles per particle (diffusion on)
+  // mixed precision (pair_precision 1): FP32 candidate records next to the
+  // FP64 ones, kCandS32 and kCandG32 dwords per particle
+  uint32_t* cand_s32 = nullptr;
+  uint32_t* cand_g32 = nullptr;
   int snap_cols = 0;            // columns per freeze-out snapshot (33, or 45 with diffusion)
   // reductions
   double* red = nullptr;
@@ -250,6 +254,9 @@ int launch_rk4_update_k(ccake_event* ev, double w);   // k += w f (Evolver::upda
 int launch_rk4_update(ccake_event* ev, double dt);    // y = y0 + dt k (Evolver::update_rk4)
 int rk4_columns();
 int launch_rk(ccake_event* ev, double dt, double t0);
+// mixed-precision pair passes (pair_precision 1), in place of the FP64 ones
+int launch_smooth_pairs32(ccake_event* ev);
+int launch_grad_pairs32(ccake_event* ev);
 // misc
 int launch_gamma_velocity(ccake_event* ev);
 int launch_init_particles(ccake_event* ev, int phase);
@@ -271,6 +278,38 @@ __device__ __forceinline__ void smooth_record(const double* __restrict__ F, int 
   o[5] = f(CCAKE_F_MASS_SS) * f(CCAKE_F_EXT_SS);
   o[6] = f(CCAKE_F_MASS_Q) * f(CCAKE_F_EXT_Q);
   o[7] = 0.0;
+}
+
+// FP32 candidate records of the mixed-precision pair kernels: the position
+// stays FP64 (the pair predicate is evaluated in FP64 on it, so the pair set
+// is the FP64 path's), every other entry is rounded to FP32.  Dwords:
+//   smoothing (12): x, y (2 each), m_s, m_s*ext_s, m_B*ext_B, m_S*ext_S,
+//                   m_Q*ext_Q, 3 unused;
+//   gradients (16): x, y (2 each), m_s, p/sl^2, e, Pi/sl^2, v^x, v^y,
+//                   pi^{0x}/sl^2, pi^{0y}/sl^2, pi^{xx}/sl^2, pi^{xy}/sl^2,
+//                   pi^{yy}/sl^2, 1 unused.
+constexpr int kCandS32 = 12;
+constexpr int kCandG32 = 16;
+__device__ __forceinline__ void put_f64(uint32_t* o, double v)
+{
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  o[0] = (uint32_t)b;
+  o[1] = (uint32_t)(b >> 32);
+}
+__device__ __forceinline__ void put_f32(uint32_t* o, double v) { o[0] = __float_as_uint((float)v); }
+__device__ __forceinline__ void smooth_record32(const double* __restrict__ o64, uint32_t* __restrict__ o)
+{
+  put_f64(o, o64[0]);
+  put_f64(o + 2, o64[1]);
+  for (int q = 2; q < 7; ++q) put_f32(o + 2 + q, o64[q]);
+  o[9] = o[10] = o[11] = 0u;
+}
+__device__ __forceinline__ void grad_record32(const double* __restrict__ o64, uint32_t* __restrict__ o)
+{
+  put_f64(o, o64[0]);
+  put_f64(o + 2, o64[1]);
+  for (int q = 2; q < 13; ++q) put_f32(o + 2 + q, o64[q]);
+  o[15] = 0u;
 }
 
 }  // namespace ccake

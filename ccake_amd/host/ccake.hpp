@@ -80,6 +80,9 @@ struct Settings {
   double diffusion_regulator_chi0 = 10.0, diffusion_regulator_e0 = 0.1;   // e0, xi0 in GeV/fm^3
   double diffusion_regulator_xi0 = 0.01, diffusion_regulator_rq_max = 1.0;
   int cells_per_rc = 3;
+  // 0: FP64 pair sums; 1: FP32 pair terms with compensated accumulation
+  // (ccake_params::pair_precision; no counterpart in the reference)
+  int pair_precision = 0;
   bool print_conservation_status = false;
   bool exit_on_error = false;
   // initial conditions and outputs (include/settings.h, include/defaults.h:25-72)
@@ -402,6 +405,7 @@ class SystemState {
       p.regulator_chi0 = s.diffusion_regulator_chi0; p.regulator_e0 = s.diffusion_regulator_e0;
       p.regulator_xi0 = s.diffusion_regulator_xi0; p.regulator_rq_max = s.diffusion_regulator_rq_max;
     }
+    p.pair_precision = s.pair_precision;
     return p;
   }
 

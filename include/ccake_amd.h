@@ -142,7 +142,7 @@ enum ccake_tauq_mode  { CCAKE_TAUQ_DEFAULT = 0, CCAKE_TAUQ_CB_OVER_T = 1 };
  * of include/settings.h the path reads). */
 typedef struct ccake_params {
   int    dim;                 /* spatial dimension D; 2 is built            */
-  int    rk_order;            /* 2 (RK4 is CCAKE_ERR_UNSUPPORTED)            */
+  int    rk_order;            /* 2 or 4 (the reference's fourth-order step)   */
   double hT, hEta;            /* smoothing lengths, fm                       */
   double t0;                  /* tau_0, fm/c                                 */
   double dt;                  /* fm/c                                        */
@@ -173,6 +173,14 @@ typedef struct ccake_params {
   double C_B;
   int    diffusion_regulator;     /* regulator on/off */
   double regulator_chi0, regulator_e0, regulator_xi0, regulator_rq_max;   /* e0, xi0 in GeV/fm^3 */
+  /* Precision of the time step's smoothing and gradient pair sums (no
+     counterpart in the reference, which is FP64 throughout):
+     0 FP64 (the reference's arithmetic);
+     1 FP32 pair terms with compensated FP32 accumulation.  Positions, the
+       pair predicate (so the pair set), the slices' sums and everything
+       outside the pair loops (EoS inversion, EoM, RK accumulation, shear
+       update, initialisation, diffusion sums) stay FP64. */
+  int    pair_precision;
 } ccake_params;
 
 /* Description of an equation of state.  For CCAKE_EOS_TABLE the table is a

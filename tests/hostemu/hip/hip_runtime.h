@@ -187,6 +187,16 @@ inline T emu_readlane(T v, int r) { return hipemu::exchange(v, [r](int) { return
 #define __builtin_amdgcn_readfirstlane(v) emu_readfirstlane(v)
 #define __builtin_amdgcn_readlane(v, r) emu_readlane((v), (r))
 #define __builtin_amdgcn_rsq(x) (1.0 / std::sqrt(x))
+#define __builtin_amdgcn_rsqf(x) (1.0f / std::sqrt((float)(x)))
+inline long long __double_as_longlong(double v) { long long r; std::memcpy(&r, &v, 8); return r; }
+inline double __longlong_as_double(long long v) { double r; std::memcpy(&r, &v, 8); return r; }
+inline unsigned int __float_as_uint(float v) { unsigned int r; std::memcpy(&r, &v, 4); return r; }
+inline float __uint_as_float(unsigned int v) { float r; std::memcpy(&r, &v, 4); return r; }
+inline double __hiloint2double(int hi, int lo)
+{
+  const unsigned long long b = ((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo;
+  return __longlong_as_double((long long)b);
+}
 #define __builtin_amdgcn_sched_barrier(x) ((void)0)
 
 template <class T, class U>

@@ -49,6 +49,19 @@ def test_lumpy_table_event_on_host_emulation(emu_lib):
     assert "OK" in r.stdout
 
 
+def test_mixed_precision_on_host_emulation(emu_lib):
+    """The FP32 pair kernels (pair_precision 1) under AddressSanitizer: no
+    invalid access, and after one derivative evaluation and 2 steps of a
+    lumpy viscous BSQ event (2,042 particles) every compared field is within
+    1e-6 (whole-field relative) of the FP64 path, with equal btrack."""
+    env = dict(os.environ)
+    r = subprocess.run(["bash", os.path.join(EMU, "emu.sh"), os.path.join(EMU, "run_mixed.py"), "0.2", "2", "1e-6"],
+                       capture_output=True, text=True, timeout=600, env=env)
+    print(r.stdout[-4000:], r.stderr[-8000:])
+    assert r.returncode == 0 and "AddressSanitizer" not in r.stderr
+    assert "OK" in r.stdout
+
+
 def test_private_columns_disjoint():
     """Every multi-column private member owns its columns (the overlap that
     wrote past the field buffer is gone): parse the enumeration and check
