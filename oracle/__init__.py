@@ -37,6 +37,7 @@ PARAM_FIELDS = [
     ("kappa_over_T2", C.c_double * 9), ("C_B", C.c_double),
     ("diffusion_regulator", C.c_int), ("regulator_chi0", C.c_double), ("regulator_e0", C.c_double),
     ("regulator_xi0", C.c_double), ("regulator_rq_max", C.c_double),
+    ("pair_precision", C.c_int),
 ]
 
 
@@ -148,7 +149,7 @@ def lib():
 def make_params(cfg):
     p = OrcParams()
     for name, typ in PARAM_FIELDS:
-        v = cfg[name]
+        v = cfg.get(name, 0) if name == "pair_precision" else cfg[name]
         setattr(p, name, typ(*v) if hasattr(typ, "_length_") else v)
     return p
 

@@ -124,6 +124,7 @@ typedef struct orc_params {
   double C_B;
   int    diffusion_regulator;
   double regulator_chi0, regulator_e0, regulator_xi0, regulator_rq_max;
+  int    pair_precision;      /* the product's switch; the oracle is FP64 and ignores it */
 } orc_params;
 
 typedef struct orc_eos_desc {
