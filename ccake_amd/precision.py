@@ -5,7 +5,9 @@ The measure is per particle: |x - x_ref| / |x_ref| for fields that never
 change sign, and |x - x_ref| / max(|x_ref|, 1e-3 max|x_ref|) for signed
 fields (velocities, charge densities, shear and bulk pressure), whose
 relative error at a zero crossing means nothing.  Each field reports the
-largest, the 99th-percentile and the median per-particle error.
+largest, the 99th-percentile and the median per-particle error, and the
+whole-field error max|x - x_ref| / max|x_ref|.  The four-velocity also
+reports |u^mu - u_ref^mu| / u_ref^0 per particle ("u_mu").
 """
 from __future__ import annotations
 
@@ -35,6 +37,20 @@ def rel_errors(x: np.ndarray, ref: np.ndarray, signed: bool) -> np.ndarray:
         return np.where(d == 0, 0.0, d / scale)
 
 
+def whole_field(x: np.ndarray, ref: np.ndarray) -> float:
+    """max |x - ref| / max |ref| over the field."""
+    if ref.size == 0:
+        return 0.0
+    return float(np.max(np.abs(np.asarray(x) - np.asarray(ref))) / (np.max(np.abs(ref)) or 1.0))
+
+
+def four_velocity_errors(g, ux, uy, g_ref, ux_ref, uy_ref) -> np.ndarray:
+    """|u^mu - u_ref^mu| / u_ref^0 per particle (Euclidean over the three
+    components, against the time component, which is at least 1)."""
+    d = np.sqrt((g - g_ref) ** 2 + (ux - ux_ref) ** 2 + (uy - uy_ref) ** 2)
+    return d / g_ref
+
+
 def summarize(r: np.ndarray) -> dict:
     if r.size == 0:
         return {"max": 0.0, "p99": 0.0, "median": 0.0}
@@ -58,6 +74,8 @@ def surface_errors(rows_ref: np.ndarray, rows: np.ndarray) -> dict:
            "same_step": len(same), "fields": {}}
     for name, (c, signed) in SURFACE_FIELDS.items():
         out["fields"][name] = summarize(rel_errors(b[:, c], a[:, c], signed))
+        out["fields"][name]["whole"] = whole_field(b[:, c], a[:, c])
+    out["fields"]["u_mu"] = summarize(four_velocity_errors(b[:, 5], b[:, 6], b[:, 7], a[:, 5], a[:, 6], a[:, 7]))
     return out
 
 
