@@ -23,7 +23,9 @@ steps; the whole event run from initialize to complete freeze-out
 roofline of the dominant kernel (measured with HIP events on the event's
 stream over the timed region); the CPU oracle timed on a bounded sample of
 the same event on the host cores, as a parallel and a reference-faithful
-(serial EoS loop) variant with per-phase times; and a served multi-event
+(serial EoS loop) variant with per-phase times; the same workload with
+the mixed-precision pair sums, with the per-field error of its freeze-out
+surface against the FP64 run ("mixed_precision"); and a served multi-event
 measurement ("serving").
 """
 from __future__ import annotations
@@ -66,6 +68,7 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cells-per-rc", type=int, default=3, help="neighbour-grid cells per cutoff radius")
     ap.add_argument("--no-serve", action="store_true", help="skip the served multi-event measurement")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision measurement")
     ap.add_argument("--serve-events", type=int, default=24)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -214,11 +217,12 @@ def make_roofline(prof, pairs, n, cfg, workload, charges, bulk, nslices):
     return r
 
 
-def whole_event(dev, eos, d, cfg, max_tau=30.0):
+def whole_event(dev, eos, d, cfg, max_tau=30.0, rows=None):
     """One event from initialize until every particle has frozen out
     (SPHWorkstation::continue_evolution checked after every step, as
-    BSQHydro::run does), wall-clocked on the host: the plain path's event
-    latency."""
+    BSQHydro::run does), wall-clocked on the host: the event latency.
+    With `rows` (a list) the freeze-out hypersurface rows are appended to
+    it after the clock stops."""
     import ccake_amd
     n = len(d["x"])
     ev = ccake_amd.Event(dev, eos, cfg, n)
@@ -233,9 +237,54 @@ def whole_event(dev, eos, d, cfg, max_tau=30.0):
     dev.sync()
     sec = time.perf_counter() - t0
     tau = ev.t
+    if rows is not None:
+        rows.append(ev.freeze_out_surface())
     ev.close()
     return dict(seconds=round(sec, 4), events_per_s=round(1.0 / sec, 4), steps=steps, tau_end=round(tau, 4),
                 particle_steps_per_s=round(n * steps / sec, 1))
+
+
+def timed_steps(dev, eos, d, cfg, warmup, steps):
+    """ms per RK2 step of the event over `steps` steps after `warmup`."""
+    import ccake_amd
+    ev = ccake_amd.Event(dev, eos, cfg, len(d["x"]))
+    ccake_amd.load_event(ev, d)
+    ev.initialize()
+    if warmup:
+        ev.run_steps(cfg["dt"], warmup)
+    dev.sync()
+    t0 = time.perf_counter()
+    done = ev.run_steps(cfg["dt"], steps)
+    dev.sync()
+    sec = time.perf_counter() - t0
+    ev.close()
+    return sec / max(done, 1) * 1e3
+
+
+def mixed_precision(dev, eos, d, cfg, warmup, steps, fp64_ms, fp64_event, fp64_rows):
+    """The same workload with the mixed-precision pair sums (pair_precision
+    1: FP32 pair terms, compensated FP32 accumulation, FP64 elsewhere):
+    step time, the whole event to freeze-out, and the per-field error of its
+    freeze-out hypersurface against the FP64 run's (rows matched by particle
+    id; ccake_amd.precision gives the measures)."""
+    from ccake_amd import precision
+    mcfg = dict(cfg, pair_precision=1)
+    ms = timed_steps(dev, eos, d, mcfg, warmup, steps)
+    out = dict(ms_per_step=round(ms, 4), particle_steps_per_s=round(len(d["x"]) / ms * 1e3, 1),
+               step_speedup_vs_fp64=round(fp64_ms / ms, 3))
+    if fp64_event is not None:
+        rows = []
+        ev = whole_event(dev, eos, d, mcfg, rows=rows)
+        out["event_to_freeze_out"] = ev
+        out["event_speedup_vs_fp64"] = round(fp64_event["seconds"] / ev["seconds"], 3)
+        err = precision.surface_errors(fp64_rows[0], rows[0])
+        keep = ("T", "u_mu", "rho_B", "rho_S", "rho_Q", "s", "e", "pi_11", "pi_22", "pi_12", "Pi")
+        out["freeze_out_error_vs_fp64"] = dict(
+            rows=err["rows_ref"], same_step=err["same_step"], same_snapshot=err["same_snapshot"],
+            measure="per particle: relative (signed fields against 1e-3 of their largest magnitude); "
+                    "u_mu: |du^mu| / u^0; whole: max|d| / max|x|",
+            fields={k: {m: float(f"{v:.3g}") for m, v in err["fields"][k].items()} for k in keep})
+    return out
 
 
 def serving_requests(k, steps, seed=0x5E4F):
@@ -395,10 +444,16 @@ def main():
         "stage_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items() if v[1]},
     }
     ev.close()
+    fp64_rows = []
     if not args.no_event and cfg["particlization"]:
         print("bench: whole event from initialize to freeze-out", file=sys.stderr, flush=True)
-        out["event_to_freeze_out"] = whole_event(dev, eos, d, cfg)
+        out["event_to_freeze_out"] = whole_event(dev, eos, d, cfg, rows=fp64_rows)
         print(f"bench: event {json.dumps(out['event_to_freeze_out'])}", file=sys.stderr, flush=True)
+    if not args.no_mixed:
+        print("bench: mixed-precision pair sums", file=sys.stderr, flush=True)
+        out["mixed_precision"] = mixed_precision(dev, eos, d, cfg, args.warmup, steps, sec / steps * 1e3,
+                                                 out.get("event_to_freeze_out"), fp64_rows)
+        print(f"bench: mixed {json.dumps(out['mixed_precision'])}", file=sys.stderr, flush=True)
     eos.close()
     if not args.no_serve:
         print(f"bench: serving {args.serve_events} events, {args.serve_slots} slots", file=sys.stderr, flush=True)

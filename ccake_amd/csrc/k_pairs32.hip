@@ -14,18 +14,26 @@
 //
 // Accumulation.  Each of a lane's sums is kept as a block sum b and a
 // compensated pair (hi, lo): the pair terms are added to b, and every
-// kFoldEvery candidates (and at the end of every span) b is folded into
+// FOLD candidates (32 by default; and at the end of every span) b is folded into
 // (hi, lo) with the error-free TwoSum (Knuth):
 //   s = hi + b;  z = s - hi;  err = (hi - (s - z)) + (b - z);  hi = s;  lo += err.
-// A block holds at most kFoldEvery terms, so its rounding is that of a short
+// A block holds at most FOLD terms, so its rounding is that of a short
 // FP32 sum; the running total carries its rounding error in lo.  The slice's
 // partial sum written for the sum pass is (double)hi + (double)lo.
+#include <cstdlib>
+
 #include "pair_common.h"
 #include "grad_sum.h"
 
 namespace ccake {
 
-constexpr int kFoldEvery = 16;     // candidates per FP32 block between folds
+// Default candidates per FP32 block between folds.  On the 200k lumpy event
+// to freeze-out, 16, 32 and 64 gave 5.62, 5.49 and 5.43 ms per step and
+// freeze-out errors against FP64 (largest per particle, same snapshot) of
+// 4.0e-7, 7.9e-7 and 1.9e-6 in u^mu, with the charge densities' whole-field
+// errors 6.5e-7, 8.6e-7 and 1.2e-6: 32 keeps T, u^mu and the charges
+// within 1e-6.
+constexpr int kFoldEvery = 32;
 
 struct Comp {
   float b = 0.0f, hi = 0.0f, lo = 0.0f;
@@ -64,13 +72,13 @@ __global__ __launch_bounds__(256) void k_records32(const double* __restrict__ ca
 // walk_records over FP32 records: NR dwords of each record (wave-uniform
 // scalar loads), U records per step with the next U requested while the
 // current ones are evaluated (the same ordering as walk_records), and
-// fold() after every kFoldEvery candidates and at the end of the span.
-template <int NR, int STRIDE, int U, class Body, class Fold>
+// fold() after every FOLD candidates and at the end of the span.
+template <int NR, int STRIDE, int U, int FOLD, class Body, class Fold>
 __device__ __forceinline__ void walk_records32(const uint32_t* __restrict__ cand, double xi, double yi, int j0,
                                                int j1, Body&& body, Fold&& fold)
 {
   static_assert(U == 1 || U == 2 || U == 4, "one, two or four records per step");
-  static_assert(kFoldEvery % U == 0, "a fold falls between steps");
+  static_assert(FOLD % U == 0, "a fold falls between steps");
   if (j0 >= j1) return;
   uint32_t cur[U][NR];
   #pragma unroll
@@ -96,7 +104,7 @@ __device__ __forceinline__ void walk_records32(const uint32_t* __restrict__ cand
     #pragma unroll
     for (int u = 0; u < U; ++u) body(cur[u], dx[u], dy[u], j + u, j + u < j1);
     since += U;
-    if (since == kFoldEvery) { fold(); since = 0; }
+    if (since == FOLD) { fold(); since = 0; }
     #pragma unroll
     for (int u = 0; u < U; ++u)
       #pragma unroll
@@ -127,7 +135,7 @@ __device__ __forceinline__ float spline_g32(float q, float iq)
 // Smoothing pairs, MODE 0 (charges) or 3 (no charges: the charge sums are
 // the exact zeros the full walk gives).  Same partial-sum layout as
 // k_smooth_slice: sigma_lab, neighbour count (FP64 predicate), s, B, S, Q.
-template <int MODE>
+template <int MODE, int U, int FOLD>
 __global__ __launch_bounds__(kWave) void k_smooth_slice32(const double* __restrict__ cand64,
                                                           const uint32_t* __restrict__ cand,
                                                           const int32_t* __restrict__ skey,
@@ -155,7 +163,7 @@ __global__ __launch_bounds__(kWave) void k_smooth_slice32(const double* __restri
   for_segments(act, myrow, xi, yi, [&](const Seg& seg, bool in) {
     for_spans_slice(g, cell_off, seg, rcs, slice, nslices, in, [&](int j0, int j1) {
       constexpr int NR = (MODE == 0) ? 9 : 6;
-      walk_records32<NR, kCandS32, 2>(cand, xi, yi, j0, j1,
+      walk_records32<NR, kCandS32, U, FOLD>(cand, xi, yi, j0, j1,
                                       [&](const uint32_t* c, double dx, double dy, int j, bool valid) {
         const double r2 = dist2_nocontract(dx, dy);
         const bool nb = valid && r2 <= rc2;
@@ -189,7 +197,7 @@ __global__ __launch_bounds__(kWave) void k_smooth_slice32(const double* __restri
 
 // Gradient pairs: the 17 (12 without shear) neighbour-side sums of
 // w = m_b grad W_ab / C of k_grad_slice, same partial-sum layout.
-template <bool SHEAR, bool BULK>
+template <bool SHEAR, bool BULK, int FOLD>
 __global__ __launch_bounds__(kWave) void k_grad_slice32(const double* __restrict__ cand64,
                                                         const uint32_t* __restrict__ cand,
                                                         const int32_t* __restrict__ skey,
@@ -217,7 +225,7 @@ __global__ __launch_bounds__(kWave) void k_grad_slice32(const double* __restrict
   for_segments(act, myrow, xa, ya, [&](const Seg& seg, bool in) {
     for_spans_slice(g, cell_off, seg, rcs, slice, nslices, in, [&](int j0, int j1) {
       constexpr int NR = SHEAR ? 15 : 10;
-      walk_records32<NR, kCandG32, 2>(cand, xa, ya, j0, j1,
+      walk_records32<NR, kCandG32, 2, FOLD>(cand, xa, ya, j0, j1,
                                       [&](const uint32_t* c, double dx, double dy, int j, bool valid) {
         const double r2 = dist2_nocontract(dx, dy);
         if (kSkipEmpty && wave_none(valid && r2 <= rc2)) return;
@@ -255,6 +263,39 @@ __global__ __launch_bounds__(kWave) void k_grad_slice32(const double* __restrict
 }
 
 // ---------------------------------------------------------------------------
+// Tuning overrides: CCAKE_MP_FOLD (16, 32 or 64 candidates per block) and
+// CCAKE_MP_SMOOTH_U (2 or 4 smoothing records per step).
+static int mp_fold()
+{
+  static const int v = [] {
+    const char* e = std::getenv("CCAKE_MP_FOLD");
+    const int f = e ? std::atoi(e) : kFoldEvery;
+    return (f == 16 || f == 64) ? f : 32;
+  }();
+  return v;
+}
+static int mp_smooth_u()
+{
+  static const int v = [] {
+    const char* e = std::getenv("CCAKE_MP_SMOOTH_U");
+    return (e && std::atoi(e) == 4) ? 4 : 2;
+  }();
+  return v;
+}
+
+template <int MODE, int U>
+static void smooth32_fold(ccake_event* ev, dim3 grid, double rc2, double rcs, float hinv2)
+{
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, grid, dim3(kWave), 0, ev->stream, ev->cand_s, ev->cand_s32, ev->skey, ev->cell_off,
+                       ev->grid, ev->n, rc2, rcs, hinv2, ev->nslices, ev->part, ev->npad);
+  };
+  const int f = mp_fold();
+  if (f == 64) go(k_smooth_slice32<MODE, U, 64>);
+  else if (f == 32) go(k_smooth_slice32<MODE, U, 32>);
+  else go(k_smooth_slice32<MODE, U, 16>);
+}
+
 int launch_smooth_pairs32(ccake_event* ev)
 {
   const int n = ev->n;
@@ -263,15 +304,28 @@ int launch_smooth_pairs32(ccake_event* ev)
   const int ntiles = (n + kTile - 1) / kTile;
   hipStream_t st = ev->stream;
   hipLaunchKernelGGL(k_records32<0>, dim3((n + 255) / 256), dim3(256), 0, st, ev->cand_s, n, ev->cand_s32);
-  const dim3 grid(ntiles * ev->nslices), blk(kWave);
+  const dim3 grid(ntiles * ev->nslices);
   const float hinv2 = (float)(1.0 / (ev->p.hT * ev->p.hT));
-  if (ev->no_charges)
-    hipLaunchKernelGGL(k_smooth_slice32<3>, grid, blk, 0, st, ev->cand_s, ev->cand_s32, ev->skey, ev->cell_off,
-                       ev->grid, n, rc * rc, rcs, hinv2, ev->nslices, ev->part, ev->npad);
-  else
-    hipLaunchKernelGGL(k_smooth_slice32<0>, grid, blk, 0, st, ev->cand_s, ev->cand_s32, ev->skey, ev->cell_off,
-                       ev->grid, n, rc * rc, rcs, hinv2, ev->nslices, ev->part, ev->npad);
+  const bool four = mp_smooth_u() == 4;
+  if (ev->no_charges) {
+    if (four) smooth32_fold<3, 4>(ev, grid, rc * rc, rcs, hinv2); else smooth32_fold<3, 2>(ev, grid, rc * rc, rcs, hinv2);
+  } else {
+    if (four) smooth32_fold<0, 4>(ev, grid, rc * rc, rcs, hinv2); else smooth32_fold<0, 2>(ev, grid, rc * rc, rcs, hinv2);
+  }
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+}
+
+template <bool SHEAR, bool BULK>
+static void grad32_fold(ccake_event* ev, dim3 grid, double rc2, double rcs, float hinv2)
+{
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, grid, dim3(kWave), 0, ev->stream, ev->cand_g, ev->cand_g32, ev->skey, ev->cell_off,
+                       ev->grid, ev->n, rc2, rcs, hinv2, ev->nslices, ev->part, ev->npad);
+  };
+  const int f = mp_fold();
+  if (f == 64) go(k_grad_slice32<SHEAR, BULK, 64>);
+  else if (f == 32) go(k_grad_slice32<SHEAR, BULK, 32>);
+  else go(k_grad_slice32<SHEAR, BULK, 16>);
 }
 
 int launch_grad_pairs32(ccake_event* ev)
@@ -282,16 +336,14 @@ int launch_grad_pairs32(ccake_event* ev)
   const int ntiles = (n + kTile - 1) / kTile;
   hipStream_t st = ev->stream;
   hipLaunchKernelGGL(k_records32<1>, dim3((n + 255) / 256), dim3(256), 0, st, ev->cand_g, n, ev->cand_g32);
-  const dim3 grid(ntiles * ev->nslices), blk(kWave);
+  const dim3 grid(ntiles * ev->nslices);
   const float hinv2 = (float)(1.0 / (ev->p.hT * ev->p.hT));
-  auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, grid, blk, 0, st, ev->cand_g, ev->cand_g32, ev->skey, ev->cell_off, ev->grid, n,
-                       rc * rc, rcs, hinv2, ev->nslices, ev->part, ev->npad);
-  };
   if (ev->p.using_shear) {
-    if (ev->no_bulk) go(k_grad_slice32<true, false>); else go(k_grad_slice32<true, true>);
+    if (ev->no_bulk) grad32_fold<true, false>(ev, grid, rc * rc, rcs, hinv2);
+    else grad32_fold<true, true>(ev, grid, rc * rc, rcs, hinv2);
   } else {
-    if (ev->no_bulk) go(k_grad_slice32<false, false>); else go(k_grad_slice32<false, true>);
+    if (ev->no_bulk) grad32_fold<false, false>(ev, grid, rc * rc, rcs, hinv2);
+    else grad32_fold<false, true>(ev, grid, rc * rc, rcs, hinv2);
   }
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }

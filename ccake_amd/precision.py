@@ -61,17 +61,24 @@ def surface_errors(rows_ref: np.ndarray, rows: np.ndarray) -> dict:
     """Per-field errors of the hypersurface rows of a mixed-precision run
     against the FP64 run's rows of the same event.  Rows are matched by
     particle id; the fields are compared where both runs froze the particle
-    out in the same step (a particle that crosses e_FO a step earlier in one
-    run carries the state of another time, which is not a precision error
-    of the fields; their number is reported)."""
+    out in the same step and interpolated from the same snapshot (same row
+    time).  A particle that crosses e_FO a step earlier in one run, or whose
+    two candidate snapshots are nearly equally close to e_FO, carries the
+    state of another time in one run, which is not a precision error of the
+    fields; the counts of both kinds are reported."""
     ia = {int(r[0]): k for k, r in enumerate(rows_ref)}
     ib = {int(r[0]): k for k, r in enumerate(rows)}
     common = sorted(set(ia) & set(ib))
-    same = [i for i in common if rows_ref[ia[i], 1] == rows[ib[i], 1]]
+    step_same = [i for i in common if rows_ref[ia[i], 1] == rows[ib[i], 1]]
+    # the row's time tells which snapshot the interpolation took (the one
+    # whose energy is closer to e_FO): where the two runs took different
+    # snapshots the row describes another time, a discrete choice rather
+    # than a precision error of the fields
+    same = [i for i in step_same if rows_ref[ia[i], 15] == rows[ib[i], 15]]
     a = rows_ref[[ia[i] for i in same]] if same else np.zeros((0, rows_ref.shape[1]))
     b = rows[[ib[i] for i in same]] if same else np.zeros((0, rows_ref.shape[1]))
     out = {"rows_ref": int(len(rows_ref)), "rows": int(len(rows)), "matched": len(common),
-           "same_step": len(same), "fields": {}}
+           "same_step": len(step_same), "same_snapshot": len(same), "fields": {}}
     for name, (c, signed) in SURFACE_FIELDS.items():
         out["fields"][name] = summarize(rel_errors(b[:, c], a[:, c], signed))
         out["fields"][name]["whole"] = whole_field(b[:, c], a[:, c])
