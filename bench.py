@@ -10,7 +10,8 @@ eta/s = 0.08, Gaussian zeta/s (0.1 at T = 0.17 GeV, width 0.02 GeV),
 particlization at T = 150 MeV, h = 0.3 fm, RK2 with dt = 0.02 fm/c from
 tau0 = 0.6 fm/c.  `--workload gubser` runs ideal 2+1D Gubser flow on the
 0.05 fm lattice (40,401 particles, conformal EoS, dt = 0.005 fm/c);
-`--workload lumpy40k` the lumpy event on the 0.05 fm lattice.
+`--workload lumpy40k` the lumpy event on the 0.05 fm lattice, `--workload
+lumpy1m` on the 0.01 fm lattice (1,002,001 lattice points).
 
 A "step" is one RK2 time step of the whole event (two derivative
 evaluations: neighbour build, smoothing, EoS inversion, transport,
@@ -60,7 +61,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["lumpy", "gubser", "lumpy40k"], default="lumpy")
+    ap.add_argument("--workload", choices=["lumpy", "gubser", "lumpy40k", "lumpy1m"], default="lumpy")
     ap.add_argument("--no-event", action="store_true", help="skip the run of the whole event to freeze-out")
     ap.add_argument("--diffusion", action="store_true",
                     help="BSQ diffusion on (diagonal kappa/T^2 = 0.1, regulator on, q = 0 at tau0)")
@@ -85,7 +86,7 @@ def workload(name):
             hT=0.3, dt=0.005, tau0=1.0, rk=2)
     # points at or below the reference's default energy cutoff, 0.15 GeV/fm^3,
     # are dropped as its initial-condition readers do
-    spacing = 0.02 if name == "lumpy" else 0.05
+    spacing = {"lumpy": 0.02, "lumpy1m": 0.01}.get(name, 0.05)
     d = ic.lumpy(spacing=spacing, e_cutoff_gev=0.15)
     cfg = config.lumpy_params()
     table = eos_tables.tanh_conformal_table()
