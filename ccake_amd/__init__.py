@@ -143,6 +143,10 @@ ABI = {
     "ccake_initial_smoothing": (C.c_int, [_vp]),
     "ccake_freeze_out_particles": (C.c_int, [_vp]),
     "ccake_continue_evolution": (C.c_int, [_vp, C.c_double, C.POINTER(C.c_int)]),
+    "ccake_batch_create": (C.c_int, [_vp, C.POINTER(Params), _vp, C.c_int, _ip, C.POINTER(_vp)]),
+    "ccake_batch_events": (C.c_int, [_vp, C.POINTER(C.c_int), _ip]),
+    "ccake_batch_reduce": (C.c_int, [_vp, C.c_int, _dp]),
+    "ccake_batch_continue_evolution": (C.c_int, [_vp, C.c_double, _ip]),
     "ccake_freeze_out_surface": (C.c_int, [_vp, _dp, C.c_size_t, C.POINTER(C.c_size_t)]),
     "ccake_event_check": (C.c_int, [_vp]),
     "ccake_reduce": (C.c_int, [_vp, C.c_int, _dp]),
@@ -278,11 +282,18 @@ class Eos:
 class Event:
     """One particle system (one SystemState) resident on the device."""
 
-    def __init__(self, dev, eos, cfg, n):
-        self.dev, self.eos, self.cfg, self.n = dev, eos, cfg, int(n)
+    def __init__(self, dev, eos, cfg, n, _counts=None):
+        self.dev, self.eos, self.cfg = dev, eos, cfg
         self._p = make_params(cfg)
         h = C.c_void_p()
-        _ok(lib().ccake_event_create(dev.h, C.byref(self._p), eos.h, self.n, C.byref(h)), "ccake_event_create")
+        if _counts is None:
+            self.n = int(n)
+            _ok(lib().ccake_event_create(dev.h, C.byref(self._p), eos.h, self.n, C.byref(h)), "ccake_event_create")
+        else:
+            cnt = np.ascontiguousarray(_counts, dtype=np.int32)
+            self.n = int(cnt.sum())
+            _ok(lib().ccake_batch_create(dev.h, C.byref(self._p), eos.h, len(cnt),
+                                         cnt.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(h)), "ccake_batch_create")
         self.h = h
         dev._children.add(self)
         eos._events.add(self)
@@ -424,6 +435,59 @@ class Event:
 
 
 STATUS_CAPACITY = 7
+
+
+class Batch(Event):
+    """Many independent events stepped together (ccake_batch_create): each
+    kernel launches once per stage for all of them.  Event e owns the
+    particle ids self.offsets[e] .. self.offsets[e + 1]; columns are the
+    concatenation of the events' columns."""
+
+    def __init__(self, dev, eos, cfg, counts):
+        super().__init__(dev, eos, cfg, 0, _counts=counts)
+        k = C.c_int()
+        _ok(lib().ccake_batch_events(self.h, C.byref(k), None), "batch_events")
+        off = np.zeros(k.value + 1, dtype=np.int32)
+        _ok(lib().ccake_batch_events(self.h, C.byref(k), off.ctypes.data_as(C.POINTER(C.c_int32))), "batch_events")
+        self.n_events = k.value
+        self.offsets = off
+
+    def event_slice(self, e):
+        return slice(int(self.offsets[e]), int(self.offsets[e + 1]))
+
+    def batch_reduce(self, which):
+        w = REDUCTIONS[which] if isinstance(which, str) else int(which)
+        out = np.zeros(self.n_events)
+        _ok(lib().ccake_batch_reduce(self.h, w, out.ctypes.data_as(C.POINTER(C.c_double))), "batch_reduce")
+        return out
+
+    def batch_continue_evolution(self, max_tau=50.0):
+        out = np.zeros(self.n_events, dtype=np.int32)
+        _ok(lib().ccake_batch_continue_evolution(self.h, float(max_tau), out.ctypes.data_as(C.POINTER(C.c_int32))),
+            "batch_continue_evolution")
+        return out.astype(bool)
+
+    def surface_by_event(self):
+        """The freeze-out rows split by event, with each event's own
+        particle ids (row[0] - offset)."""
+        rows = self.freeze_out_surface()
+        ids = rows[:, 0].astype(np.int64)
+        ev = np.searchsorted(self.offsets, ids, side="right") - 1
+        out = []
+        for e in range(self.n_events):
+            r = rows[ev == e].copy()
+            r[:, 0] -= self.offsets[e]
+            out.append(r)
+        return out
+
+
+def concat_ics(ics):
+    """One initial-condition dict from several (the batch's columns)."""
+    keys = [k for k in ("x", "y", "e", "ux", "uy", "rhoB", "rhoS", "rhoQ", "bulk") if all(k in d for d in ics)]
+    out = {k: np.concatenate([np.asarray(d[k], dtype=np.float64) for d in ics]) for k in keys}
+    if all("shv" in d for d in ics):
+        out["shv"] = np.concatenate([np.asarray(d["shv"], dtype=np.float64) for d in ics], axis=1)
+    return out
 
 
 def load_event(ev, ic):

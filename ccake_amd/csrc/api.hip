@@ -4,6 +4,7 @@
 #include "eos_device.h"
 #include "pair_common.h"
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -645,9 +646,18 @@ int ccake_eos_full_thermo(ccake_eos* e, int member, const double tbqs[4], double
 }
 
 // ---------------------------------------------------------------- events
-int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, int n, ccake_event** out)
+static int event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, int nev, const int* counts,
+                        ccake_event** out)
 {
-  if (!dev || !p || !eos || !out || n <= 0) return CCAKE_ERR_INVALID;
+  if (!dev || !p || !eos || !out || nev <= 0 || !counts) return CCAKE_ERR_INVALID;
+  long long ntot = 0;
+  for (int e = 0; e < nev; ++e) {
+    if (counts[e] <= 0) { set_error("every event of a batch needs particles"); return CCAKE_ERR_INVALID; }
+    ntot += counts[e];
+  }
+  if (ntot > INT_MAX / 32) { set_error("too many particles"); return CCAKE_ERR_INVALID; }
+  if (nev > kMaxBatchEvents) { set_error("too many events in one batch"); return CCAKE_ERR_INVALID; }
+  const int n = (int)ntot;
   if (p->dim != 2) { set_error("only D = 2 is built"); return CCAKE_ERR_UNSUPPORTED; }
   if (p->using_diffusion && (p->diffusion_mode == 1 || p->diffusion_mode > 2 || p->diffusion_relax_mode > 1)) {
     set_error("DNMR diffusion coefficients are not built (the reference returns zero matrices for them)");
@@ -662,7 +672,13 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
   ev->dev = dev; ev->eos = eos; ev->p = *p; ev->n = n;
   ev->npad = (n + 255) / 256 * 256;
   ev->t = p->t0;
-  ev->cap_cells = 65536;
+  ev->nev = nev;
+  ev->h_off.assign(nev + 1, 0);
+  for (int e = 0; e < nev; ++e) ev->h_off[e + 1] = ev->h_off[e] + counts[e];
+  // one event: up to 65,536 cells (16-bit keys, two radix passes); a batch:
+  // up to 128 x 128 cells per event in one key space (three passes)
+  ev->cap_cells = nev == 1 ? 65536 : nev * kBatchCellsPerEvent;
+  ev->sort_passes = ev->cap_cells <= 65536 ? 2 : 3;
   ev->radix_blocks = (n + 1023) / 1024;
   {
     const hipError_t se = hipStreamCreateWithFlags(&ev->stream, hipStreamNonBlocking);
@@ -674,8 +690,13 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
   const int nscan_blocks = (nscan + 2047) / 2048 + 1;
   rc |= dalloc(ev, &ev->F, (size_t)NUM_COLUMNS * np);
   rc |= dalloc(ev, &ev->I, (size_t)CCAKE_NUM_IFIELDS * np);
-  rc |= dalloc(ev, &ev->grid, 1);
-  rc |= dalloc(ev, &ev->bbox, 4 * (np / 256 + 1));
+  rc |= dalloc(ev, &ev->grid, (size_t)nev);
+  rc |= dalloc(ev, &ev->bbox, 4 * std::max(np / 256 + 1, (size_t)nev));
+  if (nev > 1) {
+    rc |= dalloc(ev, &ev->ev_off, (size_t)nev + 1);
+    rc |= dalloc(ev, &ev->eid, np);
+    rc |= dalloc(ev, &ev->red_ev, (size_t)nev);
+  }
   rc |= dalloc(ev, &ev->key, np);
   rc |= dalloc(ev, &ev->key_a, np);
   rc |= dalloc(ev, &ev->key_b, np);
@@ -728,6 +749,14 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
   std::vector<double> ones(n, 1.0);
   for (int f : { (int)CCAKE_F_MASS_S, (int)CCAKE_F_MASS_B, (int)CCAKE_F_MASS_SS, (int)CCAKE_F_MASS_Q, (int)CCAKE_F_GAMMA })
     CCAKE_HIP(hipMemcpyAsync(col(ev, f), ones.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+  if (nev > 1) {
+    std::vector<int32_t> eid(n);
+    for (int e = 0; e < nev; ++e)
+      for (int i = ev->h_off[e]; i < ev->h_off[e + 1]; ++i) eid[i] = e;
+    CCAKE_HIP(hipMemcpyAsync(ev->ev_off, ev->h_off.data(), (nev + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    CCAKE_HIP(hipMemcpyAsync(ev->eid, eid.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    CCAKE_HIP(hipStreamSynchronize(st));
+  }
   CCAKE_HIP(hipStreamSynchronize(st));
   ev->perm = ev->val_a;
   ev->skey = ev->key_a;
@@ -739,13 +768,57 @@ int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, in
   return 0;
 }
 
+int ccake_event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, int n, ccake_event** out)
+{
+  if (n <= 0) return CCAKE_ERR_INVALID;
+  return event_create(dev, p, eos, 1, &n, out);
+}
+
+int ccake_batch_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, int n_events, const int* counts,
+                       ccake_event** out)
+{
+  return event_create(dev, p, eos, n_events, counts, out);
+}
+
+int ccake_batch_events(ccake_event* ev, int* n_events, int* offsets)
+{
+  if (!ev || !n_events) return CCAKE_ERR_INVALID;
+  *n_events = ev->nev;
+  if (offsets) std::copy(ev->h_off.begin(), ev->h_off.end(), offsets);
+  return 0;
+}
+
+int ccake_batch_reduce(ccake_event* ev, int which, double* out)
+{
+  if (!ev || !out || which < 0 || which >= CCAKE_NUM_REDUCTIONS) return CCAKE_ERR_INVALID;
+  if (ev->nev == 1) return launch_reduce(ev, which, out);
+  return launch_reduce_events(ev, which, out);
+}
+
+int ccake_batch_continue_evolution(ccake_event* ev, double max_tau, int* keep_going)
+{
+  if (!ev || !keep_going) return CCAKE_ERR_INVALID;
+  std::vector<double> frozen(ev->nev, 0.0);
+  if (ev->p.particlization) {
+    const int rc = ccake_batch_reduce(ev, CCAKE_R_FROZEN, frozen.data());
+    if (rc) return rc;
+  }
+  for (int e = 0; e < ev->nev; ++e) {
+    const int ne = ev->h_off[e + 1] - ev->h_off[e];
+    const int keep = ev->p.particlization ? ((int)(frozen[e] + 0.5) < ne) : 1;
+    keep_going[e] = keep && (ev->t <= max_tau);
+  }
+  return 0;
+}
+
 int ccake_event_destroy(ccake_event* ev)
 {
   if (!ev) return CCAKE_ERR_INVALID;
   if (ev->stream) (void)hipStreamSynchronize(ev->stream);
   void* bufs[] = { ev->F, ev->I, ev->grid, ev->bbox, ev->key, ev->key_a, ev->key_b, ev->val_a, ev->val_b, ev->hist,
                    ev->scan_tmp, ev->cell_off, ev->pair_count, ev->part, ev->hard, ev->hard_count, ev->k4, ev->cand_s, ev->cand_g, ev->red,
-                   ev->err, ev->snaps, ev->fo_rows, ev->fo_count, ev->cand_d, ev->cand_s32, ev->cand_g32 };
+                   ev->err, ev->snaps, ev->fo_rows, ev->fo_count, ev->cand_d, ev->cand_s32, ev->cand_g32,
+                   ev->ev_off, ev->eid, ev->red_ev };
   for (void* b : bufs) if (b) (void)hipFree(b);
   if (ev->stream) (void)hipStreamDestroy(ev->stream);
   {

@@ -38,14 +38,28 @@ static_assert(P_FRZ2_G == P_FRZ1_G + 1 && P_C_Q0 == P_FRZ2_G + 1 && NUM_COLUMNS 
               "private columns overlap");
 
 // Uniform cell grid of one neighbour build (written on the device).
+// Cell grid of one event.  A batch of events (ccake_batch_create) keeps one
+// Grid per event; their cells share one key space of rows of `stride`
+// cells, event e owning the key rows [row0, row0 + nc[1]), so the cell sort
+// orders particles event by event and no pair walk crosses events.  For a
+// single event stride = nc[0] and row0 = 0.
 struct Grid {
   double lo[2];       // grid origin
   double cell;        // cell edge
   double inv_cell;
   int nc[2];          // cells per axis
-  int ncells;
-  int pad;
+  int ncells;         // cells of the whole key space (cell_off holds ncells + 1 entries)
+  int stride;         // cells per key row
+  int row0;           // first key row of this event
+  int nev;            // events sharing the key space
 };
+
+// Batches: at most kMaxBatchEvents events, each with at most
+// kBatchAxisCells cells per axis, so the key space (at most 2^24 cells)
+// sorts in three 8-bit radix passes.
+constexpr int kMaxBatchEvents = 1024;
+constexpr int kBatchAxisCells = 128;
+constexpr int kBatchCellsPerEvent = kBatchAxisCells * kBatchAxisCells;
 
 // Device error flags (sticky until ccake_event_check).
 enum DevErr { DE_NEG_ENTROPY = 1, DE_EOS_FAIL = 2, DE_GRID_OVERFLOW = 4 };
@@ -123,6 +137,14 @@ struct ccake_event {
   int32_t* I = nullptr;         // CCAKE_NUM_IFIELDS x npad
   // neighbour build
   int cap_cells = 0;
+  int sort_passes = 2;          // 8-bit radix passes over the cell keys
+  // a batch of events (ccake_batch_create): event e owns the particle ids
+  // [off[e], off[e+1]) and grid[e]; a single event has nev = 1
+  int nev = 1;
+  std::vector<int> h_off;       // host copy of the offsets (nev + 1)
+  int32_t* ev_off = nullptr;    // device offsets (nev + 1)
+  int32_t* eid = nullptr;       // event of each particle id (batches only)
+  double* red_ev = nullptr;     // per-event reduction results (nev)
   ccake::Grid* grid = nullptr;  // device
   double* bbox = nullptr;       // device partials
   int32_t* key = nullptr;       // cell key per particle (id order)
@@ -262,6 +284,7 @@ int launch_gamma_velocity(ccake_event* ev);
 int launch_init_particles(ccake_event* ev, int phase);
 int launch_freeze(ccake_event* ev, int phase);
 int launch_reduce(ccake_event* ev, int which, double* out);
+int launch_reduce_events(ccake_event* ev, int which, double* out);   // one result per event of a batch
 
 // Smoothing candidate record of particle i (8 doubles): x, y, m_s,
 // m_s*ext_s, m_B*ext_B, m_S*ext_S, m_Q*ext_Q, 0.

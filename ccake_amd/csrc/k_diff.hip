@@ -121,7 +121,7 @@ __global__ __launch_bounds__(kWave) void k_diff_slice(const double* __restrict__
                                                       double rc2, double rcs, double hinv2, int nslices,
                                                       double* __restrict__ part, int npad)
 {
-  const Grid g = *gridp;
+  const Grid g0 = *gridp;
   const int lane = threadIdx.x;
   const int unit = xcd_tile(blockIdx.x, gridDim.x);
   const int tile = unit / nslices;
@@ -131,12 +131,13 @@ __global__ __launch_bounds__(kWave) void k_diff_slice(const double* __restrict__
   const int kk = act ? k : 0;
   const double* me = cand + (size_t)kk * kCandD;
   const double xa = me[0], ya = me[1];
-  const int myrow = act ? skey[kk] / g.nc[0] : INT_MAX;
+  const int myrow = act ? skey[kk] / g0.stride : INT_MAX;
 
   double acc[kDiffAcc];
   for (int q = 0; q < kDiffAcc; ++q) acc[q] = 0.0;
 
   for_segments(act, myrow, xa, ya, [&](const Seg& seg, bool in) {
+    const Grid g = event_grid(gridp, g0, seg.row);
     for_spans_slice(g, cell_off, seg, rcs, slice, nslices, in, [&](int j0, int j1) {
       walk_records<15, kCandD, 1>(cand, xa, ya, j0, j1, [&](const double* c, double dx, double dy, int j, bool valid) {
         const double r2 = dist2_nocontract(dx, dy);

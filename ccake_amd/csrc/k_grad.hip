@@ -32,7 +32,7 @@ __global__ __launch_bounds__(kWave) void k_grad_slice(const double* __restrict__
                                                       double* __restrict__ part, int npad)
 {
   constexpr int NACC = grad_nacc<SHEAR>();
-  const Grid g = *gridp;
+  const Grid g0 = *gridp;
   const int lane = threadIdx.x;
   const int unit = xcd_tile(blockIdx.x, gridDim.x);
   const int tile = unit / nslices;
@@ -43,12 +43,13 @@ __global__ __launch_bounds__(kWave) void k_grad_slice(const double* __restrict__
   const double* me = cand + (size_t)kk * kCandG;
   const double xa = me[0], ya = me[1];
   const double vax = me[6], vay = me[7];
-  const int myrow = act ? skey[kk] / g.nc[0] : INT_MAX;
+  const int myrow = act ? skey[kk] / g0.stride : INT_MAX;
 
   double acc[NACC];
   for (int q = 0; q < NACC; ++q) acc[q] = 0.0;
 
   for_segments(act, myrow, xa, ya, [&](const Seg& seg, bool in) {
+    const Grid g = event_grid(gridp, g0, seg.row);
     // lanes of other cell rows sit this segment's walk out
     for_spans_slice(g, cell_off, seg, rcs, slice, nslices, in, [&](int j0, int j1) {
       // Candidate records are wave-uniform (scalar loads into SGPRs); the

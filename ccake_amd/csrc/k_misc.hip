@@ -299,28 +299,33 @@ int launch_freeze(ccake_event* ev, int phase)
 
 // Deterministic two-pass reductions (SystemState::conservation_*,
 // src/system_state.cpp:738-885; frozen count, src/sph_workstation.cpp:2011-2017).
+// One particle's term of reduction `which`.
+__device__ __forceinline__ double reduce_term(const double* __restrict__ F, int npad, const int32_t* __restrict__ I,
+                                              int which, double t, int i)
+{
+  auto col = [&](int c) { return F[(size_t)c * npad + i]; };
+  switch (which) {
+    case CCAKE_R_ENTROPY: return col(CCAKE_F_EXT_S) * col(CCAKE_F_MASS_S);
+    case CCAKE_R_BARYON: return col(CCAKE_F_EXT_B) * col(CCAKE_F_MASS_B);
+    case CCAKE_R_STRANGE: return col(CCAKE_F_EXT_SS) * col(CCAKE_F_MASS_SS);
+    case CCAKE_R_CHARGE: return col(CCAKE_F_EXT_Q) * col(CCAKE_F_MASS_Q);
+    case CCAKE_R_ENERGY: {
+      const double w = col(CCAKE_F_W), p = col(CCAKE_F_P), bulk = col(CCAKE_F_BULK), g = col(CCAKE_F_GAMMA);
+      return ((w + bulk) * g * g - p - bulk + col(CCAKE_F_SHV0)) * col(CCAKE_F_MASS_S) * t / col(CCAKE_F_SIGMA_LAB);
+    }
+    case CCAKE_R_EZ: return col(CCAKE_F_EZ);
+    case CCAKE_R_FROZEN: return (I[(size_t)CCAKE_I_FREEZE * npad + i] >= 4) ? 1.0 : 0.0;
+    default: return 0.0;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ F, int npad, int n,
                                                 const int32_t* __restrict__ I, int which, double t,
                                                 double* __restrict__ part)
 {
   double acc = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    auto col = [&](int c) { return F[(size_t)c * npad + i]; };
-    switch (which) {
-      case CCAKE_R_ENTROPY: acc += col(CCAKE_F_EXT_S) * col(CCAKE_F_MASS_S); break;
-      case CCAKE_R_BARYON: acc += col(CCAKE_F_EXT_B) * col(CCAKE_F_MASS_B); break;
-      case CCAKE_R_STRANGE: acc += col(CCAKE_F_EXT_SS) * col(CCAKE_F_MASS_SS); break;
-      case CCAKE_R_CHARGE: acc += col(CCAKE_F_EXT_Q) * col(CCAKE_F_MASS_Q); break;
-      case CCAKE_R_ENERGY: {
-        const double w = col(CCAKE_F_W), p = col(CCAKE_F_P), bulk = col(CCAKE_F_BULK), g = col(CCAKE_F_GAMMA);
-        acc += ((w + bulk) * g * g - p - bulk + col(CCAKE_F_SHV0)) * col(CCAKE_F_MASS_S) * t / col(CCAKE_F_SIGMA_LAB);
-        break;
-      }
-      case CCAKE_R_EZ: acc += col(CCAKE_F_EZ); break;
-      case CCAKE_R_FROZEN: acc += (I[(size_t)CCAKE_I_FREEZE * npad + i] >= 4) ? 1.0 : 0.0; break;
-      default: break;
-    }
-  }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    acc += reduce_term(F, npad, I, which, t, i);
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
   __shared__ double s[4];
   if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
@@ -339,6 +344,21 @@ __global__ __launch_bounds__(256) void k_reduce_final(double* __restrict__ part,
   if (threadIdx.x == 0) part[np] = s[0] + s[1] + s[2] + s[3];
 }
 
+// A batch: workgroup e reduces the ids [off[e], off[e+1]) of event e.
+__global__ __launch_bounds__(256) void k_reduce_events(const double* __restrict__ F, int npad,
+                                                       const int32_t* __restrict__ I, const int32_t* __restrict__ off,
+                                                       int which, double t, double* __restrict__ out)
+{
+  const int e = blockIdx.x;
+  double acc = 0.0;
+  for (int i = off[e] + threadIdx.x; i < off[e + 1]; i += blockDim.x) acc += reduce_term(F, npad, I, which, t, i);
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  __shared__ double s[4];
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) out[e] = s[0] + s[1] + s[2] + s[3];
+}
+
 constexpr int kRedBlocks = 512;
 
 int launch_reduce(ccake_event* ev, int which, double* out)
@@ -348,6 +368,17 @@ int launch_reduce(ccake_event* ev, int which, double* out)
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, ev->red, kRedBlocks);
   CCAKE_HIP(hipGetLastError());
   CCAKE_HIP(hipMemcpyAsync(out, ev->red + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, st));
+  CCAKE_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+int launch_reduce_events(ccake_event* ev, int which, double* out)
+{
+  hipStream_t st = ev->stream;
+  hipLaunchKernelGGL(k_reduce_events, dim3(ev->nev), dim3(256), 0, st, ev->F, ev->npad, ev->I, ev->ev_off, which,
+                     ev->t, ev->red_ev);
+  CCAKE_HIP(hipGetLastError());
+  CCAKE_HIP(hipMemcpyAsync(out, ev->red_ev, ev->nev * sizeof(double), hipMemcpyDeviceToHost, st));
   CCAKE_HIP(hipStreamSynchronize(st));
   return 0;
 }

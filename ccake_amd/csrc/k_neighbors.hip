@@ -96,8 +96,75 @@ __global__ __launch_bounds__(256) void k_grid_setup(const double* __restrict__ b
     }
     Grid g;
     g.lo[0] = xmn; g.lo[1] = ymn; g.cell = cell; g.inv_cell = 1.0 / cell;
-    g.nc[0] = nx; g.nc[1] = ny; g.ncells = nx * ny; g.pad = 0;
+    g.nc[0] = nx; g.nc[1] = ny; g.ncells = nx * ny;
+    g.stride = nx; g.row0 = 0; g.nev = 1;
     *grid = g;
+  }
+}
+
+// A batch: the bounding box of each event (workgroup e reduces the ids
+// [off[e], off[e+1])).
+__global__ __launch_bounds__(256) void k_bbox_events(const double* __restrict__ F, int npad,
+                                                     const int32_t* __restrict__ off, double* __restrict__ bbox)
+{
+  const int e = blockIdx.x;
+  double xmn = INFINITY, xmx = -INFINITY, ymn = INFINITY, ymx = -INFINITY;
+  for (int i = off[e] + threadIdx.x; i < off[e + 1]; i += blockDim.x) {
+    const double x = F[(size_t)CCAKE_F_X * npad + i], y = F[(size_t)CCAKE_F_Y * npad + i];
+    xmn = fmin(xmn, x); xmx = fmax(xmx, x); ymn = fmin(ymn, y); ymx = fmax(ymx, y);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    xmn = fmin(xmn, __shfl_xor(xmn, o)); xmx = fmax(xmx, __shfl_xor(xmx, o));
+    ymn = fmin(ymn, __shfl_xor(ymn, o)); ymx = fmax(ymx, __shfl_xor(ymx, o));
+  }
+  __shared__ double s[4][4];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) { s[w][0] = xmn; s[w][1] = xmx; s[w][2] = ymn; s[w][3] = ymx; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; ++k) {
+      xmn = fmin(xmn, s[k][0]); xmx = fmax(xmx, s[k][1]); ymn = fmin(ymn, s[k][2]); ymx = fmax(ymx, s[k][3]);
+    }
+    double* b = bbox + 4 * e;
+    b[0] = xmn; b[1] = xmx; b[2] = ymn; b[3] = ymx;
+  }
+}
+
+// A batch: each event's grid as k_grid_setup lays out one event's, with the
+// cell doubled until both axes have at most kBatchAxisCells cells, then the
+// key rows event after event (row0) and the common row stride (the widest
+// event).  One workgroup; thread e lays out event e, thread 0 the rows.
+__global__ __launch_bounds__(256) void k_grid_events(const double* __restrict__ bbox, int nev, double rc,
+                                                     int cells_per_rc, Grid* __restrict__ grid,
+                                                     int32_t* __restrict__ err,
+                                                     unsigned long long* __restrict__ pair_count)
+{
+  if (threadIdx.x == 0) *pair_count = 0ull;
+  for (int e = threadIdx.x; e < nev; e += blockDim.x) {
+    double xmn = bbox[4 * e], xmx = bbox[4 * e + 1], ymn = bbox[4 * e + 2], ymx = bbox[4 * e + 3];
+    double cell = rc / (double)(cells_per_rc > 0 ? cells_per_rc : 1);
+    int nx = 1, ny = 1;
+    const double wx = xmx - xmn, wy = ymx - ymn;
+    if (!(wx >= 0.0 && wx < 1e300) || !(wy >= 0.0 && wy < 1e300)) {
+      atomicOr(err, DE_GRID_OVERFLOW);
+      xmn = 0.0; ymn = 0.0;
+    } else {
+      for (;;) {
+        const double fx = floor(wx / cell) + 1.0, fy = floor(wy / cell) + 1.0;
+        if (fx <= (double)kBatchAxisCells && fy <= (double)kBatchAxisCells) { nx = (int)fx; ny = (int)fy; break; }
+        cell *= 2.0;
+      }
+    }
+    Grid g;
+    g.lo[0] = xmn; g.lo[1] = ymn; g.cell = cell; g.inv_cell = 1.0 / cell;
+    g.nc[0] = nx; g.nc[1] = ny; g.ncells = 0; g.stride = 0; g.row0 = 0; g.nev = nev;
+    grid[e] = g;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int rows = 0, stride = 1;
+    for (int e = 0; e < nev; ++e) { grid[e].row0 = rows; rows += grid[e].nc[1]; stride = max(stride, grid[e].nc[0]); }
+    for (int e = 0; e < nev; ++e) { grid[e].stride = stride; grid[e].ncells = rows * stride; }
   }
 }
 
@@ -119,7 +186,8 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const int32_t* __re
                                                              int shift, int nblk, int32_t* __restrict__ hist,
                                                              const double* __restrict__ F, int npad,
                                                              const Grid* __restrict__ grid,
-                                                             int32_t* __restrict__ keys_out)
+                                                             int32_t* __restrict__ keys_out,
+                                                             const int32_t* __restrict__ eid)
 {
   __shared__ int32_t h[256];
   h[threadIdx.x] = 0;
@@ -130,10 +198,10 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const int32_t* __re
     if (idx >= n) continue;
     int key;
     if (KEYS) {
-      const Grid g = *grid;
+      const Grid g = grid[eid ? eid[idx] : 0];
       const int cx = cell_coord(F[(size_t)CCAKE_F_X * npad + idx], g.lo[0], g.inv_cell, g.nc[0]);
       const int cy = cell_coord(F[(size_t)CCAKE_F_Y * npad + idx], g.lo[1], g.inv_cell, g.nc[1]);
-      key = cy * g.nc[0] + cx;
+      key = (g.row0 + cy) * g.stride + cx;
       keys_out[idx] = key;
     } else {
       key = keys_in[idx];
@@ -344,30 +412,47 @@ static void scan_hist(ccake_event* ev, int nscan)
     scan_exclusive(ev, ev->hist, ev->hist, nscan);
 }
 
-// Grid, keys, stable two-pass radix sort (cap_cells <= 65536 keys fit 16
-// bits), cell offsets and the smoothing candidate records.
+// Grid, keys, stable LSD radix sort (ev->sort_passes 8-bit passes: two
+// while the key space fits 16 bits, three for a batch's larger one), cell
+// offsets and the smoothing candidate records.
 int launch_neighbor_sort(ccake_event* ev)
 {
   const int n = ev->n;
   hipStream_t st = ev->stream;
   const int nb = (n + 255) / 256;
   const double rc = 2.0 * ev->p.hT;
-  hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(256), 0, st, ev->bbox, nb, rc, ev->p.cells_per_rc,
-                     ev->cap_cells, ev->grid, ev->err, (unsigned long long*)ev->pair_count);
+  if (ev->nev > 1) {
+    hipLaunchKernelGGL(k_bbox_events, dim3(ev->nev), dim3(256), 0, st, ev->F, ev->npad, ev->ev_off, ev->bbox);
+    hipLaunchKernelGGL(k_grid_events, dim3(1), dim3(256), 0, st, ev->bbox, ev->nev, rc, ev->p.cells_per_rc,
+                       ev->grid, ev->err, (unsigned long long*)ev->pair_count);
+  } else {
+    hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(256), 0, st, ev->bbox, nb, rc, ev->p.cells_per_rc,
+                       ev->cap_cells, ev->grid, ev->err, (unsigned long long*)ev->pair_count);
+  }
   const int nblk = ev->radix_blocks;
   const int nscan = 256 * nblk;
-  hipLaunchKernelGGL(k_radix_hist<true>, dim3(nblk), dim3(kSortThreads), 0, st, nullptr, n, 0, nblk, ev->hist,
-                     ev->F, ev->npad, ev->grid, ev->key);
-  scan_hist(ev, nscan);
-  hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(kSortThreads), 0, st, ev->key, nullptr, n, 0, nblk,
-                     ev->hist, ev->key_a, ev->val_a);
-  hipLaunchKernelGGL(k_radix_hist<false>, dim3(nblk), dim3(kSortThreads), 0, st, ev->key_a, n, 8, nblk, ev->hist,
-                     nullptr, 0, nullptr, nullptr);
-  scan_hist(ev, nscan);
-  hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(kSortThreads), 0, st, ev->key_a, ev->val_a, n, 8, nblk,
-                     ev->hist, ev->key_b, ev->val_b);
-  ev->skey = ev->key_b;
-  ev->perm = ev->val_b;
+  int32_t* kin = ev->key;
+  int32_t* vin = nullptr;
+  int32_t* kbuf[2] = { ev->key_a, ev->key_b };
+  int32_t* vbuf[2] = { ev->val_a, ev->val_b };
+  for (int pass = 0; pass < ev->sort_passes; ++pass) {
+    const int shift = 8 * pass;
+    if (pass == 0)
+      hipLaunchKernelGGL(k_radix_hist<true>, dim3(nblk), dim3(kSortThreads), 0, st, nullptr, n, 0, nblk, ev->hist,
+                         ev->F, ev->npad, ev->grid, ev->key, ev->eid);
+    else
+      hipLaunchKernelGGL(k_radix_hist<false>, dim3(nblk), dim3(kSortThreads), 0, st, kin, n, shift, nblk, ev->hist,
+                         nullptr, 0, nullptr, nullptr, nullptr);
+    scan_hist(ev, nscan);
+    int32_t* kout = kbuf[pass & 1];
+    int32_t* vout = vbuf[pass & 1];
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(kSortThreads), 0, st, kin, vin, n, shift, nblk,
+                       ev->hist, kout, vout);
+    kin = kout;
+    vin = vout;
+  }
+  ev->skey = kin;
+  ev->perm = vin;
   const int nbc = (n + 255) / 256 + 1;
   hipLaunchKernelGGL(k_cells_records, dim3(nbc), dim3(256), 0, st, ev->skey, ev->perm, n, ev->grid, ev->cell_off,
                      ev->F, ev->npad, ev->cand_s);

@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kWave) void k_smooth_slice32(const double* __restri
 {
   static_assert(MODE == 0 || MODE == 3, "time-step smoothing modes");
   constexpr int NACC = 6;
-  const Grid g = *gridp;
+  const Grid g0 = *gridp;
   const int lane = threadIdx.x;
   const int unit = xcd_tile(blockIdx.x, gridDim.x);
   const int tile = unit / nslices;
@@ -155,12 +155,13 @@ __global__ __launch_bounds__(kWave) void k_smooth_slice32(const double* __restri
   const bool act = k < n;
   const int kk = act ? k : 0;
   const double xi = cand64[(size_t)kk * kCandS], yi = cand64[(size_t)kk * kCandS + 1];
-  const int myrow = act ? skey[kk] / g.nc[0] : INT_MAX;
+  const int myrow = act ? skey[kk] / g0.stride : INT_MAX;
 
   Comp sl, ss, sB, sS, sQ;
   int cnt = 0;
 
   for_segments(act, myrow, xi, yi, [&](const Seg& seg, bool in) {
+    const Grid g = event_grid(gridp, g0, seg.row);
     for_spans_slice(g, cell_off, seg, rcs, slice, nslices, in, [&](int j0, int j1) {
       constexpr int NR = (MODE == 0) ? 9 : 6;
       walk_records32<NR, kCandS32, U, FOLD>(cand, xi, yi, j0, j1,
@@ -207,7 +208,7 @@ __global__ __launch_bounds__(kWave) void k_grad_slice32(const double* __restrict
                                                         double* __restrict__ part, int npad)
 {
   constexpr int NACC = grad_nacc<SHEAR>();
-  const Grid g = *gridp;
+  const Grid g0 = *gridp;
   const int lane = threadIdx.x;
   const int unit = xcd_tile(blockIdx.x, gridDim.x);
   const int tile = unit / nslices;
@@ -218,11 +219,12 @@ __global__ __launch_bounds__(kWave) void k_grad_slice32(const double* __restrict
   const double* me = cand64 + (size_t)kk * kCandG;
   const double xa = me[0], ya = me[1];
   const float vax = (float)me[6], vay = (float)me[7];
-  const int myrow = act ? skey[kk] / g.nc[0] : INT_MAX;
+  const int myrow = act ? skey[kk] / g0.stride : INT_MAX;
 
   Comp acc[NACC];
 
   for_segments(act, myrow, xa, ya, [&](const Seg& seg, bool in) {
+    const Grid g = event_grid(gridp, g0, seg.row);
     for_spans_slice(g, cell_off, seg, rcs, slice, nslices, in, [&](int j0, int j1) {
       constexpr int NR = SHEAR ? 15 : 10;
       walk_records32<NR, kCandG32, 2, FOLD>(cand, xa, ya, j0, j1,

@@ -102,20 +102,21 @@ __global__ __launch_bounds__(64) void k_walk(const double* __restrict__ cand,
                                              int32_t* __restrict__ out_i, int32_t* __restrict__ out_j)
 {
   static_assert(MODE == 2 || MODE == 3, "walk modes");
-  const Grid g = *gridp;
+  const Grid g0 = *gridp;
   const int lane = threadIdx.x;
   const int k = blockIdx.x * kTile + lane;
   const bool act = k < n;
   const int kk = act ? k : 0;
   const double* me = cand + (size_t)kk * kCandS;
   const double xi = me[0], yi = me[1];
-  const int myrow = act ? skey[kk] / g.nc[0] : INT_MAX;
+  const int myrow = act ? skey[kk] / g0.stride : INT_MAX;
   const int id = act ? perm[kk] : 0;
   int cnt = 0;
   int64_t wpos = 0;
   if (MODE == 3 && act) wpos = row_off[id];
 
   for_segments(act, myrow, xi, yi, [&](const Seg& seg, bool in) {
+    const Grid g = event_grid(gridp, g0, seg.row);
     for_spans(g, cell_off, seg, rcs, [&](int j) {
       const double* c = cand + (size_t)j * kCandS;
       const double dx = xi - c[0], dy = yi - c[1];
@@ -157,7 +158,7 @@ __global__ __launch_bounds__(kWave) void k_smooth_slice(const double* __restrict
                                                         double* __restrict__ part, int npad)
 {
   constexpr int NACC = smooth_nacc<MODE>();
-  const Grid g = *gridp;
+  const Grid g0 = *gridp;
   const int lane = threadIdx.x;
   const int unit = xcd_tile(blockIdx.x, gridDim.x);
   const int tile = unit / nslices;
@@ -167,12 +168,13 @@ __global__ __launch_bounds__(kWave) void k_smooth_slice(const double* __restrict
   const int kk = act ? k : 0;
   const double* me = cand + (size_t)kk * kCandS;
   const double xi = me[0], yi = me[1];
-  const int myrow = act ? skey[kk] / g.nc[0] : INT_MAX;
+  const int myrow = act ? skey[kk] / g0.stride : INT_MAX;
 
   double sl = 0.0, ss = 0.0, sB = 0.0, sS = 0.0, sQ = 0.0;
   int cnt = 0;   // r2 <= rc2, the self pair included
 
   for_segments(act, myrow, xi, yi, [&](const Seg& seg, bool in) {
+    const Grid g = event_grid(gridp, g0, seg.row);
     // lanes of other cell rows sit this segment's walk out
     for_spans_slice(g, cell_off, seg, rcs, slice, nslices, in, [&](int j0, int j1) {
       // wave-uniform candidate records (scalar loads into SGPRs)

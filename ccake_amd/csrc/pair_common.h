@@ -58,6 +58,19 @@ struct Seg {
   double xmn, xmx, ymn, ymx;
 };
 
+// The grid of the event that owns key row `row` (wave-uniform): entry 0 for
+// a single event, else a binary search over the batch's first rows.
+__device__ __forceinline__ Grid event_grid(const Grid* __restrict__ gp, const Grid& g0, int row)
+{
+  if (g0.nev <= 1) return g0;
+  int lo = 0, hi = g0.nev - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (gp[mid].row0 <= row) lo = mid; else hi = mid - 1;
+  }
+  return gp[lo];
+}
+
 // Walk the candidate spans of one tile segment.  `body(j)` is called with a
 // wave-uniform sorted index j for every candidate in range of the segment's
 // bounding box (rows ascending, cells ascending within a row).
@@ -74,8 +87,9 @@ __device__ __forceinline__ void for_spans(const Grid& g, const int32_t* __restri
     const double hw = sqrt(fmax(rcs * rcs - dy * dy, 0.0));
     const int cx0 = cell_of(s.xmn - hw, g.lo[0], g.inv_cell, g.nc[0]);
     const int cx1 = cell_of(s.xmx + hw, g.lo[0], g.inv_cell, g.nc[0]);
-    const int beg = uni(cell_off[ry * g.nc[0] + cx0]);
-    const int end = uni(cell_off[ry * g.nc[0] + cx1 + 1]);
+    const int kr = (g.row0 + ry) * g.stride;
+    const int beg = uni(cell_off[kr + cx0]);
+    const int end = uni(cell_off[kr + cx1 + 1]);
     for (int j = beg; j < end; ++j) body(j);
   }
 }
@@ -127,8 +141,9 @@ __device__ __forceinline__ void row_run(const Grid& g, const int32_t* __restrict
   const double hw = sqrt(fmax(rcs * rcs - dy * dy, 0.0));
   const int cx0 = cell_of(s.xmn - hw, g.lo[0], g.inv_cell, g.nc[0]);
   const int cx1 = cell_of(s.xmx + hw, g.lo[0], g.inv_cell, g.nc[0]);
-  b = cell_off[ry * g.nc[0] + cx0];
-  e = cell_off[ry * g.nc[0] + cx1 + 1];
+  const int kr = (g.row0 + ry) * g.stride;
+  b = cell_off[kr + cx0];
+  e = cell_off[kr + cx1 + 1];
 }
 
 // Walk slice `slice` of `nslices` of a segment's flattened candidate list

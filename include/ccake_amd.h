@@ -338,8 +338,32 @@ CCAKE_API int  ccake_initialize(ccake_event* ev);
 /* FreezeOut::check_freeze_out_status + status transitions of bsqsvfreezeout
  * (include/freeze_out.h:322-542, 549-626). */
 CCAKE_API int  ccake_freeze_out_particles(ccake_event* ev);
-/* SPHWorkstation::continue_evolution (src/sph_workstation.cpp:1992-2024). */
+/* SPHWorkstation::continue_evolution (src/sph_workstation.cpp:1992-2024).
+ * For a batch: true while any of its events continues. */
 CCAKE_API int  ccake_continue_evolution(ccake_event* ev, double max_tau, int* keep_going);
+
+/* ----------------------------- batches ------------------------------ */
+/* Many independent events in one handle, stepped together: every stage
+ * launches each kernel once for all of them.  No counterpart in the
+ * reference, which runs one event per process (BSQHydro::run,
+ * src/BSQHydro.cpp:658-866); each event evolves exactly as it would alone
+ * (same pair sets, same arithmetic; only the grouping of its pair sums into
+ * slices can differ, by rounding).  Event e owns the particle ids
+ * [offsets[e], offsets[e+1]) of the batch, counts[e] of them: upload and
+ * download the concatenated columns with the ordinary event calls.  All
+ * events share the parameters, the EoS and the time (they start at t0
+ * together).  At most 1024 events; each event's cell grid has at most 128
+ * cells per axis (coarser cells for very wide events).  Freeze-out rows
+ * carry batch particle ids. */
+CCAKE_API int  ccake_batch_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, int n_events,
+                                  const int* counts, ccake_event** out);
+/* Number of events (1 for an ordinary event) and, when offsets != NULL,
+ * the n_events + 1 offsets. */
+CCAKE_API int  ccake_batch_events(ccake_event* ev, int* n_events, int* offsets);
+/* ccake_reduce per event: out[e] for each of the n_events events. */
+CCAKE_API int  ccake_batch_reduce(ccake_event* ev, int which, double* out);
+/* continue_evolution per event: keep_going[e] for each event. */
+CCAKE_API int  ccake_batch_continue_evolution(ccake_event* ev, double max_tau, int* keep_going);
 /* Sticky device-side error flags raised by kernels since the last call
  * (negative entropy, EoS failure); clears them. */
 CCAKE_API int  ccake_event_check(ccake_event* ev);

@@ -62,6 +62,19 @@ def test_mixed_precision_on_host_emulation(emu_lib):
     assert "OK" in r.stdout
 
 
+def test_batch_on_host_emulation(emu_lib):
+    """A batch of three lumpy events (2,042, 864 and 1,465 particles) under
+    AddressSanitizer: no invalid access in the batch's grid setup, key sort
+    and pair walks, and after initialization and 2 steps each event's fields
+    are within 1e-12 of the same event run alone, with equal btrack."""
+    env = dict(os.environ)
+    r = subprocess.run(["bash", os.path.join(EMU, "emu.sh"), os.path.join(EMU, "run_batch.py"), "0.2", "2", "1e-12"],
+                       capture_output=True, text=True, timeout=600, env=env)
+    print(r.stdout[-4000:], r.stderr[-8000:])
+    assert r.returncode == 0 and "AddressSanitizer" not in r.stderr
+    assert "OK" in r.stdout
+
+
 def test_private_columns_disjoint():
     """Every multi-column private member owns its columns (the overlap that
     wrote past the field buffer is gone): parse the enumeration and check
