@@ -26,8 +26,10 @@ stream over the timed region); the CPU oracle timed on a bounded sample of
 the same event on the host cores, as a parallel and a reference-faithful
 (serial EoS loop) variant with per-phase times; the same workload with
 the mixed-precision pair sums, with the per-field error of its freeze-out
-surface against the FP64 run ("mixed_precision"); and a served multi-event
-measurement ("serving").
+surface against the FP64 run ("mixed_precision"); a batch of lognormal
+lumpy events run to freeze-out one at a time and as one batch, in FP64 and
+mixed precision ("batching"); and a served multi-event measurement
+("serving").
 """
 from __future__ import annotations
 
@@ -70,6 +72,8 @@ def parse():
     ap.add_argument("--cells-per-rc", type=int, default=3, help="neighbour-grid cells per cutoff radius")
     ap.add_argument("--no-serve", action="store_true", help="skip the served multi-event measurement")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision measurement")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-event measurement")
+    ap.add_argument("--batch-events", type=int, default=16)
     ap.add_argument("--serve-events", type=int, default=24)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -337,6 +341,47 @@ def serving_run(dev, events, slots, steps=40):
     return out
 
 
+def batched_run(dev, eos, cfg, k, seed=0xC3):
+    """k lumpy events with lognormal sizes (median 80,000 lattice points,
+    truncated to [20,000, 300,000]; 0.02 fm lattice, energy cutoff), all
+    arriving at once: the plain path (each event alone, one after the other)
+    against one batch of all of them with the autotuned cell size and slice
+    count, in FP64 and with the mixed-precision pair sums.  Every event runs
+    to complete freeze-out; latency is each event's completion time."""
+    import numpy as np
+    from ccake_amd import batching, precision
+    ics = batching.lumpy_batch_ics(k, seed=seed)
+    n = [len(d["x"]) for d in ics]
+    ntot = int(sum(n))
+    tune = batching.autotune(dev, eos, cfg, ics)
+    tcfg = dict(cfg, cells_per_rc=tune["cells_per_rc"])
+    alone = batching.run_alone(dev, eos, cfg, ics)
+    out = dict(workload=f"{k} viscous BSQ lumpy events on the 0.02 fm lattice, lognormal sizes (median 80,000 "
+                        "lattice points, [20,000, 300,000]), e > 0.15 GeV/fm^3, table EoS, each run to freeze-out, "
+                        "all arriving at once",
+               particles_min=min(n), particles_max=max(n), particles_median=int(np.median(n)),
+               autotune=tune, plain=batching.latency_summary(alone, ntot))
+    for name, pp in (("batched", 0), ("batched_mixed", 1)):
+        run = batching.run_batch(dev, eos, dict(tcfg, pair_precision=pp), ics, slices=tune["slices"])
+        r = batching.latency_summary(run, ntot)
+        r["batch_steps"] = run["steps"]
+        r["speedup_vs_plain"] = round(r["events_per_s"] / out["plain"]["events_per_s"], 3)
+        same = all(a.shape == b.shape and np.array_equal(a[:, :2], b[:, :2]) for a, b in zip(alone["rows"], run["rows"]))
+        r["rows_same_ids_and_steps"] = bool(same)
+        if pp == 0 and same:
+            r["max_column_difference_vs_plain"] = float(max(
+                float(np.max(np.abs(b[:, 2:] - a[:, 2:])) / max(float(np.max(np.abs(a[:, 2:]))), 1e-300))
+                for a, b in zip(alone["rows"], run["rows"]) if len(a)))
+        if pp == 1:
+            errs = [precision.surface_errors(a, b)["fields"] for a, b in zip(alone["rows"], run["rows"])]
+            r["freeze_out_error_vs_fp64"] = {f: float(f"{max(e[f]['max'] for e in errs):.3g}")
+                                             for f in ("T", "u_mu", "s", "e")}
+            r["freeze_out_error_vs_fp64"].update({f + "_whole": float(f"{max(e[f]['whole'] for e in errs):.3g}")
+                                                  for f in ("rho_B", "rho_S", "rho_Q")})
+        out[name] = r
+    return out
+
+
 def copy_calibration(nbytes=256 << 20):
     """One plain device-to-device copy of known size through the HIP runtime
     (its copy kernel reads and writes `nbytes`); HBM byte counters of the
@@ -455,6 +500,10 @@ def main():
         out["mixed_precision"] = mixed_precision(dev, eos, d, cfg, args.warmup, steps, sec / steps * 1e3,
                                                  out.get("event_to_freeze_out"), fp64_rows)
         print(f"bench: mixed {json.dumps(out['mixed_precision'])}", file=sys.stderr, flush=True)
+    if not args.no_batch and args.workload == "lumpy":
+        print(f"bench: batch of {args.batch_events} lognormal lumpy events", file=sys.stderr, flush=True)
+        out["batching"] = batched_run(dev, eos, cfg, args.batch_events)
+        print(f"bench: batching {json.dumps(out['batching'])}", file=sys.stderr, flush=True)
     eos.close()
     if not args.no_serve:
         print(f"bench: serving {args.serve_events} events, {args.serve_slots} slots", file=sys.stderr, flush=True)
