@@ -137,7 +137,7 @@ __global__ __launch_bounds__(kWave) void k_diff_slice(const double* __restrict__
   for (int q = 0; q < kDiffAcc; ++q) acc[q] = 0.0;
 
   for_segments(act, myrow, xa, ya, [&](const Seg& seg, bool in) {
-    const Grid g = event_grid(gridp, g0, seg.row);
+    const Grid& g = gridp[event_index(gridp, g0, seg.row)];
     for_spans_slice(g, cell_off, seg, rcs, slice, nslices, in, [&](int j0, int j1) {
       walk_records<15, kCandD, 1>(cand, xa, ya, j0, j1, [&](const double* c, double dx, double dy, int j, bool valid) {
         const double r2 = dist2_nocontract(dx, dy);

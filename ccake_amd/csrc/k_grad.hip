@@ -49,7 +49,7 @@ __global__ __launch_bounds__(kWave) void k_grad_slice(const double* __restrict__
   for (int q = 0; q < NACC; ++q) acc[q] = 0.0;
 
   for_segments(act, myrow, xa, ya, [&](const Seg& seg, bool in) {
-    const Grid g = event_grid(gridp, g0, seg.row);
+    const Grid& g = gridp[event_index(gridp, g0, seg.row)];
     // lanes of other cell rows sit this segment's walk out
     for_spans_slice(g, cell_off, seg, rcs, slice, nslices, in, [&](int j0, int j1) {
       // Candidate records are wave-uniform (scalar loads into SGPRs); the

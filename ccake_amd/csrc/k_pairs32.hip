@@ -161,7 +161,7 @@ __global__ __launch_bounds__(kWave) void k_smooth_slice32(const double* __restri
   int cnt = 0;
 
   for_segments(act, myrow, xi, yi, [&](const Seg& seg, bool in) {
-    const Grid g = event_grid(gridp, g0, seg.row);
+    const Grid& g = gridp[event_index(gridp, g0, seg.row)];
     for_spans_slice(g, cell_off, seg, rcs, slice, nslices, in, [&](int j0, int j1) {
       constexpr int NR = (MODE == 0) ? 9 : 6;
       walk_records32<NR, kCandS32, U, FOLD>(cand, xi, yi, j0, j1,
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(kWave) void k_grad_slice32(const double* __restrict
   Comp acc[NACC];
 
   for_segments(act, myrow, xa, ya, [&](const Seg& seg, bool in) {
-    const Grid g = event_grid(gridp, g0, seg.row);
+    const Grid& g = gridp[event_index(gridp, g0, seg.row)];
     for_spans_slice(g, cell_off, seg, rcs, slice, nslices, in, [&](int j0, int j1) {
       constexpr int NR = SHEAR ? 15 : 10;
       walk_records32<NR, kCandG32, 2, FOLD>(cand, xa, ya, j0, j1,

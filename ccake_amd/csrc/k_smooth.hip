@@ -116,7 +116,7 @@ __global__ __launch_bounds__(64) void k_walk(const double* __restrict__ cand,
   if (MODE == 3 && act) wpos = row_off[id];
 
   for_segments(act, myrow, xi, yi, [&](const Seg& seg, bool in) {
-    const Grid g = event_grid(gridp, g0, seg.row);
+    const Grid& g = gridp[event_index(gridp, g0, seg.row)];
     for_spans(g, cell_off, seg, rcs, [&](int j) {
       const double* c = cand + (size_t)j * kCandS;
       const double dx = xi - c[0], dy = yi - c[1];
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kWave) void k_smooth_slice(const double* __restrict
   int cnt = 0;   // r2 <= rc2, the self pair included
 
   for_segments(act, myrow, xi, yi, [&](const Seg& seg, bool in) {
-    const Grid g = event_grid(gridp, g0, seg.row);
+    const Grid& g = gridp[event_index(gridp, g0, seg.row)];
     // lanes of other cell rows sit this segment's walk out
     for_spans_slice(g, cell_off, seg, rcs, slice, nslices, in, [&](int j0, int j1) {
       // wave-uniform candidate records (scalar loads into SGPRs)

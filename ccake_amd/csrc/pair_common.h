@@ -58,17 +58,20 @@ struct Seg {
   double xmn, xmx, ymn, ymx;
 };
 
-// The grid of the event that owns key row `row` (wave-uniform): entry 0 for
-// a single event, else a binary search over the batch's first rows.
-__device__ __forceinline__ Grid event_grid(const Grid* __restrict__ gp, const Grid& g0, int row)
+// The event that owns key row `row` (wave-uniform): 0 for a single event,
+// else a binary search over the batch's first rows.  Kernels then read that
+// event's grid through a reference (scalar loads): a by-value copy selected
+// between two sources cost the gradient walk 12 VGPRs and 64 bytes of
+// scratch per lane (one wave less per SIMD).
+__device__ __forceinline__ int event_index(const Grid* __restrict__ gp, const Grid& g0, int row)
 {
-  if (g0.nev <= 1) return g0;
+  if (g0.nev <= 1) return 0;
   int lo = 0, hi = g0.nev - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (gp[mid].row0 <= row) lo = mid; else hi = mid - 1;
   }
-  return gp[lo];
+  return lo;
 }
 
 // Walk the candidate spans of one tile segment.  `body(j)` is called with a
