@@ -181,6 +181,11 @@ typedef struct ccake_params {
        outside the pair loops (EoS inversion, EoM, RK accumulation, shear
        update, initialisation, diffusion sums) stay FP64. */
   int    pair_precision;
+  /* D = 3 only: the factorized kernel W_2D(r_perp, hT) W_1D(|d eta|, hEta)
+     (SPHkernel<3>::kernel / gradKernel, src/kernel.cpp:129-170; selected by
+     Settings::factorized_kernel, include/settings.h:114); 0 is the
+     isotropic 3D spline in (x, y, eta) with hT. */
+  int    factorized_kernel;
 } ccake_params;
 
 /* Description of an equation of state.  For CCAKE_EOS_TABLE the table is a

@@ -80,6 +80,39 @@ static double distance(int D, const double* a, const double* b)
   return sqrt(d);
 }
 
+/* The pair's kernel value: SPHkernel<D>::kernel(distance, hT), or for D = 3
+ * with the factorized kernel W_2D(r_perp, hT) W_1D(|d eta|, hEta)
+ * (src/kernel.cpp:137-145, chosen at src/sph_workstation.cpp:502, 531). */
+static double pair_kernel(const orc_params* P, const double* ri, const double* rj)
+{
+  if (P->dim == 3 && P->factorized_kernel)
+    return orc_kernel(2, distance(2, ri, rj), P->hT)*orc_kernel(1, fabs(ri[2] - rj[2]), P->hEta);
+  return orc_kernel(P->dim, distance(P->dim, ri, rj), P->hT);
+}
+static double self_kernel(const orc_params* P)
+{
+  if (P->dim == 3 && P->factorized_kernel) return orc_kernel(2, 0.0, P->hT)*orc_kernel(1, 0.0, P->hEta);
+  return orc_kernel(P->dim, 0.0, P->hT);
+}
+/* Its gradient with respect to r_a, rel = r_a - r_b: SPHkernel<D>::gradKernel,
+ * or the factorized D = 3 gradient (src/kernel.cpp:149-168)
+ * (grad_perp W_2D W_1D, W_2D dW_1D/d eta). */
+static void pair_grad_kernel(const orc_params* P, const double* rel, const double* pa, const double* pb,
+                             double* gK)
+{
+  if (P->dim == 3 && P->factorized_kernel) {
+    const double rp = sqrt(rel[0]*rel[0] + rel[1]*rel[1]);
+    const double de = fabs(rel[2]);
+    double g2[2], g1;
+    orc_grad_kernel(2, rel, rp, P->hT, g2);
+    orc_grad_kernel(1, rel + 2, de, P->hEta, &g1);
+    const double w2 = orc_kernel(2, rp, P->hT), w1 = orc_kernel(1, de, P->hEta);
+    gK[0] = g2[0]*w1; gK[1] = g2[1]*w1; gK[2] = w2*g1;
+    return;
+  }
+  orc_grad_kernel(P->dim, rel, distance(P->dim, pa, pb), P->hT, gK);
+}
+
 /* ------------------------------------------------------------------ */
 /* freeze_array storage (include/freeze_out.h:29-35), one member per array */
 #define SNAP_MEMBERS(X) \
@@ -348,9 +381,8 @@ static void pos_of(orc_event* ev, int i, double* r)
 /* SPHWorkstation::smooth_all_particle_fields (src/sph_workstation.cpp:494-578) */
 void orc_smooth_fields(orc_event* ev)
 {
-  const int D = ev->p.dim;
-  const double hT = ev->p.hT, t = ev->t;
-  const double kern0 = orc_kernel(D, 0.0, hT);
+  const double t = ev->t;
+  const double kern0 = self_kernel(&ev->p);
   double* ms = FLD(ev, ORC_F_MASS_S); double* mB = FLD(ev, ORC_F_MASS_B);
   double* mS = FLD(ev, ORC_F_MASS_SS); double* mQ = FLD(ev, ORC_F_MASS_Q);
   double* es = FLD(ev, ORC_F_EXT_S); double* eB = FLD(ev, ORC_F_EXT_B);
@@ -365,7 +397,7 @@ void orc_smooth_fields(orc_event* ev)
     for (int64_t k = ev->nb_off[i]; k < ev->nb_off[i+1]; ++k) {
       const int j = ev->nb_idx[k];
       pos_of(ev, j, rj);
-      const double kern = orc_kernel(D, distance(D, ri, rj), hT);
+      const double kern = pair_kernel(&ev->p, ri, rj);
       sl += ms[j]*kern;
       ss += ms[j]*es[j]*kern;
       sB += mB[j]*eB[j]*kern;
@@ -388,9 +420,8 @@ void orc_smooth_fields(orc_event* ev)
 /* SPHWorkstation::calculate_intial_sigma (src/sph_workstation.cpp:582-640) */
 void orc_initial_sigma(orc_event* ev)
 {
-  const int D = ev->p.dim;
-  const double hT = ev->p.hT, t = ev->t;
-  const double kern0 = orc_kernel(D, 0.0, hT);
+  const double t = ev->t;
+  const double kern0 = self_kernel(&ev->p);
   double* ms = FLD(ev, ORC_F_MASS_S);
   #pragma omp parallel for schedule(dynamic, 256) num_threads(orc_nthreads())
   for (int i = 0; i < ev->n; ++i) {
@@ -399,7 +430,7 @@ void orc_initial_sigma(orc_event* ev)
     for (int64_t k = ev->nb_off[i]; k < ev->nb_off[i+1]; ++k) {
       const int j = ev->nb_idx[k];
       pos_of(ev, j, rj);
-      sl += ms[j]*orc_kernel(D, distance(D, ri, rj), hT);
+      sl += ms[j]*pair_kernel(&ev->p, ri, rj);
     }
     FLD(ev, ORC_F_SIGMA_LAB)[i] = sl;
     FLD(ev, ORC_F_SIGMA)[i] = sl/FLD(ev, ORC_F_GAMMA)[i]/t;
@@ -447,8 +478,7 @@ void orc_smooth_gradients(orc_event* ev)
       pos_of(ev, b, pb);
       double rel[3], gK[3];
       for (int d = 0; d < D; ++d) rel[d] = pa[d] - pb[d];
-      const double r = distance(D, pa, pb);
-      orc_grad_kernel(D, rel, r, hT, gK);
+      pair_grad_kernel(&ev->p, rel, pa, pb, gK);
       const double p_b = FLD(ev, ORC_F_P)[b], e_b = FLD(ev, ORC_F_E)[b];
       const double sl_b = FLD(ev, ORC_F_SIGMA_LAB)[b];
       const double sigsqrb = 1.0/(sl_b*sl_b);

@@ -35,7 +35,10 @@ def flags(asan=True):
 
 
 def _compile(src, asan):
-    obj = os.path.join(OUT_DIR, os.path.splitext(os.path.basename(src))[0] + (".asan.o" if asan else ".o"))
+    # objects carry the process id, so concurrent builds (pytest-xdist
+    # workers) never link each other's half-written files
+    obj = os.path.join(OUT_DIR, os.path.splitext(os.path.basename(src))[0]
+                       + (".asan" if asan else "") + f".{os.getpid()}.o")
     path = src if os.path.isabs(src) else os.path.join(CSRC, src)
     cmd = ["g++"] + flags(asan) + ["-x", "c++", "-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -50,8 +53,11 @@ def build(asan=True):
         objs = list(ex.map(lambda s: _compile(s, asan),
                            SOURCES + [os.path.join(HERE, "emu_runtime.cpp")]))
     out = os.path.join(OUT_DIR, "libccake_emu.so" if asan else "libccake_emu_noasan.so")
-    cmd = ["g++", "-shared", "-pthread"] + (["-fsanitize=address"] if asan else []) + ["-o", out] + objs
+    tmp = f"{out}.{os.getpid()}.tmp"
+    cmd = ["g++", "-shared", "-pthread"] + (["-fsanitize=address"] if asan else []) + ["-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
+    for o in objs:
+        os.remove(o)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
     if asan:
@@ -59,7 +65,10 @@ def build(asan=True):
         # ccake_amd/bin run with LD_LIBRARY_PATH pointing here
         lib_dir = os.path.join(OUT_DIR, "lib")
         os.makedirs(lib_dir, exist_ok=True)
-        shutil.copyfile(out, os.path.join(lib_dir, "libccake_amd.so"))
+        ltmp = os.path.join(lib_dir, f"libccake_amd.so.{os.getpid()}.tmp")
+        shutil.copyfile(tmp, ltmp)
+        os.replace(ltmp, os.path.join(lib_dir, "libccake_amd.so"))
+    os.replace(tmp, out)
     return out
 
 

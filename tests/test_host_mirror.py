@@ -25,17 +25,30 @@ def test_gubser_through_cpp_call_surface():
     SPHWorkstation::advance_timestep: energy density within the SPH
     smoothing error of the analytic solution, entropy unchanged, E + E_z
     conserved to 1 percent; 5 steps through Evolver::execute_timestep with
-    get_time_derivatives as the functional agree with the fused device step."""
+    get_time_derivatives as the functional agree with the fused device step.
+    The same with the reference's RK4 step: off the analytic solution by
+    10-25 %, and the Evolver's RK4 equal to the fused device step."""
     r = subprocess.run([PROG, "--steps", "120", "--evolver-steps", "5"], capture_output=True, text=True, timeout=300)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0, r.stderr
     d = json.loads(r.stdout.strip().splitlines()[-1])
     assert d["particles"] == 40401
     assert abs(d["t"] - 1.6) < 1e-9
-    assert 0.1 < d["max_rel_err_e_r_lt_2.5"] < 0.25
+    assert d["max_rel_err_e_r_lt_2.5"] < 0.07
     assert abs(d["entropy_rel_change"]) < 1e-14
     assert abs(d["energy_rel_change"]) < 0.01
     assert 0.0 <= d["evolver_vs_fused_max_abs_diff"] < 1e-12
+    # the reference's fourth-order step (src/evolver.cpp:519-561) never adds
+    # the first stage's derivative and weighs the others 7/6 in all, so it
+    # leaves the analytic solution by far more than the SPH error
+    r4 = subprocess.run([PROG, "--steps", "120", "--rk", "4", "--evolver-steps", "3"], capture_output=True,
+                        text=True, timeout=300)
+    print(r4.stdout, r4.stderr, flush=True)
+    assert r4.returncode == 0, r4.stderr
+    d4 = json.loads(r4.stdout.strip().splitlines()[-1])
+    assert 0.1 < d4["max_rel_err_e_r_lt_2.5"] < 0.25
+    assert abs(d4["entropy_rel_change"]) < 1e-14
+    assert 0.0 <= d4["evolver_vs_fused_max_abs_diff"] < 1e-12
 
 
 @pytest.mark.gpu
