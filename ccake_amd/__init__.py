@@ -483,7 +483,8 @@ class Batch(Event):
 
 def concat_ics(ics):
     """One initial-condition dict from several (the batch's columns)."""
-    keys = [k for k in ("x", "y", "e", "ux", "uy", "rhoB", "rhoS", "rhoQ", "bulk") if all(k in d for d in ics)]
+    keys = [k for k in ("x", "y", "eta", "e", "ux", "uy", "ueta", "rhoB", "rhoS", "rhoQ", "bulk")
+            if all(k in d for d in ics)]
     out = {k: np.concatenate([np.asarray(d[k], dtype=np.float64) for d in ics]) for k in keys}
     if all("shv" in d for d in ics):
         out["shv"] = np.concatenate([np.asarray(d["shv"], dtype=np.float64) for d in ics], axis=1)
@@ -492,10 +493,15 @@ def concat_ics(ics):
 
 def load_event(ev, ic):
     """Upload an initial condition dict (x, y, e, ux, uy, rhoB, rhoS, rhoQ;
-    optionally bulk and shv, the initial Pi and pi^{mu nu} that initialize
-    reads with bulk_from_trace / input_initial_shear)."""
+    for a 3+1D event eta and ueta; optionally bulk and shv, the initial Pi
+    and pi^{mu nu} that initialize reads with bulk_from_trace /
+    input_initial_shear)."""
     ev.upload("CCAKE_F_X", ic["x"])
     ev.upload("CCAKE_F_Y", ic["y"])
+    if "eta" in ic:
+        ev.upload("CCAKE_F_ETA", ic["eta"])
+    if "ueta" in ic:
+        ev.upload("CCAKE_F_UETA", ic["ueta"])
     ev.upload("CCAKE_F_IN_E", ic["e"])
     ev.upload("CCAKE_F_UX", ic["ux"])
     ev.upload("CCAKE_F_UY", ic["uy"])

@@ -18,7 +18,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libccake_amd.so")
 OBJ = os.path.join(HERE, "_build")
-SOURCES = ["api.hip", "k_neighbors.hip", "k_smooth.hip", "k_eos.hip", "k_grad.hip", "k_eom.hip", "k_misc.hip", "k_diff.hip", "k_pairs32.hip"]
+SOURCES = ["api.hip", "k_neighbors.hip", "k_smooth.hip", "k_eos.hip", "k_grad.hip", "k_eom.hip", "k_misc.hip", "k_diff.hip",
+           "k_pairs32.hip", "k_milne3.hip"]
 ARCH = os.environ.get("CCAKE_OFFLOAD_ARCH", "gfx950")
 
 

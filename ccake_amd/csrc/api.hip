@@ -109,6 +109,7 @@ static int ensure_sorted(ccake_event* ev)
 // for tuning.
 static int fold_sums(const ccake_event* ev)
 {
+  if (ev->p.dim == 3) return 0;                 // the 3+1D stage runs its passes as kernels of their own
   static const int env = [] { const char* e = getenv("CCAKE_FOLD_SUMS"); return e ? atoi(e) : -1; }();
   if (env >= 0) return env;
   return 1 | (ev->p.using_shear ? 0 : 2);
@@ -158,6 +159,19 @@ static int time_derivatives(ccake_event* ev, int fuse_rk, double dt_stage, doubl
     prof_end(ev, CCAKE_K_SMOOTH_SUM);
   }
   prof_begin(ev, CCAKE_K_EOS);
+  if (ev->p.dim == 3) {
+    if ((rc = launch_eos(ev, 0, 0))) return rc;                 // 04
+    if ((rc = launch3_post(ev, 1 | 2 | 4))) return rc;          // 05 + 08 + gradient records
+    prof_end(ev, CCAKE_K_EOS);
+    prof_begin(ev, CCAKE_K_GRAD);
+    if ((rc = launch_grad(ev))) return rc;                      // 09
+    prof_end(ev, CCAKE_K_GRAD);
+    prof_begin(ev, CCAKE_K_EOM);
+    if ((rc = launch_eom(ev, fuse_rk, dt_stage, t0))) return rc;  // 10 (+ RK)
+    prof_end(ev, CCAKE_K_EOM);
+    if (ns) ns->begun_next = false;
+    return 0;
+  }
   if ((rc = launch_eos(ev, 0, (fold & 1) ? 2 : 1))) return rc;  // (03 fields,) 04 + 05 + 08 + gather
   if (ev->p.using_diffusion && (rc = launch_diff_prep(ev, 15, ev->perm))) return rc;  // q, 06, kappa/tau_q
   prof_end(ev, CCAKE_K_EOS);
@@ -264,6 +278,7 @@ __global__ void k_fill_f64(double* __restrict__ a, size_t n, double v)
 }
 static int rk4_allocate_k(ccake_event* ev)
 {
+  if (ev->p.dim == 3) { set_error("the fourth-order step is built for D = 2"); return CCAKE_ERR_UNSUPPORTED; }
   const size_t cnt = (size_t)rk4_columns() * ev->npad;
   if (!ev->k4) {
     const int rc = dalloc(ev, &ev->k4, cnt);
@@ -658,7 +673,14 @@ static int event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, i
   if (ntot > INT_MAX / 32) { set_error("too many particles"); return CCAKE_ERR_INVALID; }
   if (nev > kMaxBatchEvents) { set_error("too many events in one batch"); return CCAKE_ERR_INVALID; }
   const int n = (int)ntot;
-  if (p->dim != 2) { set_error("only D = 2 is built"); return CCAKE_ERR_UNSUPPORTED; }
+  if (p->dim != 2 && p->dim != 3) { set_error("D = 2 (2+1D) and D = 3 (3+1D Milne) are built"); return CCAKE_ERR_UNSUPPORTED; }
+  if (p->dim == 3) {
+    // the 3+1D path: one event per handle, FP64 pair sums, no diffusion, RK2
+    if (nev != 1) { set_error("batches are built for D = 2"); return CCAKE_ERR_UNSUPPORTED; }
+    if (p->using_diffusion) { set_error("BSQ diffusion is built for D = 2"); return CCAKE_ERR_UNSUPPORTED; }
+    if (p->pair_precision != 0) { set_error("mixed-precision pair sums are built for D = 2"); return CCAKE_ERR_UNSUPPORTED; }
+    if (p->factorized_kernel && !(p->hEta > 0.0)) { set_error("the factorized kernel needs hEta > 0"); return CCAKE_ERR_INVALID; }
+  }
   if (p->using_diffusion && (p->diffusion_mode == 1 || p->diffusion_mode > 2 || p->diffusion_relax_mode > 1)) {
     set_error("DNMR diffusion coefficients are not built (the reference returns zero matrices for them)");
     return CCAKE_ERR_UNSUPPORTED;
@@ -678,6 +700,7 @@ static int event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, i
   // one event: up to 65,536 cells (16-bit keys, two radix passes); a batch:
   // up to 128 x 128 cells per event in one key space (three passes)
   ev->cap_cells = nev == 1 ? 65536 : nev * kBatchCellsPerEvent;
+  if (p->dim == 3) ev->cap_cells = 1 << 24;                   // (x, y, eta) cells: three passes
   ev->sort_passes = ev->cap_cells <= 65536 ? 2 : 3;
   ev->radix_blocks = (n + 1023) / 1024;
   {
@@ -691,7 +714,7 @@ static int event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, i
   rc |= dalloc(ev, &ev->F, (size_t)NUM_COLUMNS * np);
   rc |= dalloc(ev, &ev->I, (size_t)CCAKE_NUM_IFIELDS * np);
   rc |= dalloc(ev, &ev->grid, (size_t)nev);
-  rc |= dalloc(ev, &ev->bbox, 4 * std::max(np / 256 + 1, (size_t)nev));
+  rc |= dalloc(ev, &ev->bbox, 6 * std::max(np / 256 + 1, (size_t)nev));
   if (nev > 1) {
     rc |= dalloc(ev, &ev->ev_off, (size_t)nev + 1);
     rc |= dalloc(ev, &ev->eid, np);
@@ -716,11 +739,12 @@ static int event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, i
     if (v >= 0.0 && v <= 1.0) ev->skin_frac = v;
   }
   ev->nslices_cap = ev->nslices;
-  rc |= dalloc(ev, &ev->part, (size_t)ev->nslices_cap * kMaxAcc * np);
+  // partial sums: 17 per slice in 2+1D, 28 in 3+1D (the 3D gradient sums)
+  rc |= dalloc(ev, &ev->part, (size_t)ev->nslices_cap * (p->dim == 3 ? 28 : kMaxAcc) * np);
   rc |= dalloc(ev, &ev->hard, np);
   rc |= dalloc(ev, &ev->hard_count, 2);
   rc |= dalloc(ev, &ev->cand_s, 8 * np);
-  rc |= dalloc(ev, &ev->cand_g, 16 * np);
+  rc |= dalloc(ev, &ev->cand_g, (size_t)(p->dim == 3 ? kCandG3 : 16) * np);
   if (p->using_diffusion) rc |= dalloc(ev, &ev->cand_d, 16 * np);
   if (p->pair_precision == 1) {
     rc |= dalloc(ev, &ev->cand_s32, (size_t)kCandS32 * np);
@@ -729,7 +753,7 @@ static int event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, i
   ev->snap_cols = p->using_diffusion ? 45 : 33;
   rc |= dalloc(ev, &ev->red, 1024);
   rc |= dalloc(ev, &ev->err, 2);          // [0] error flags, [1] charge-check scratch
-  if (p->particlization) {
+  if (p->particlization && p->dim == 2) {     // 3+1D runs the status machine without hypersurface rows
     rc |= dalloc(ev, &ev->snaps, (size_t)6 * ev->snap_cols * np);
     rc |= dalloc(ev, &ev->fo_rows, (size_t)n * CCAKE_FO_ROW);
     rc |= dalloc(ev, &ev->fo_count, 1);
@@ -894,6 +918,7 @@ int ccake_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, si
 int ccake_export_pairs_noalloc(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, size_t* n_pairs)
 {
   if (!ev || !n_pairs) return CCAKE_ERR_INVALID;
+  if (ev->p.dim == 3) { set_error("the allocation-free export is built for D = 2"); return CCAKE_ERR_UNSUPPORTED; }
   int rc = ensure_sorted(ev);
   if (!rc) rc = launch_gather_smooth(ev);
   if (rc) return rc;
@@ -1127,6 +1152,7 @@ int ccake_reduce(ccake_event* ev, int which, double* out)
 int ccake_freeze_out_surface(ccake_event* ev, double* rows, size_t cap_rows, size_t* n_rows)
 {
   if (!ev || !n_rows) return CCAKE_ERR_INVALID;
+  if (ev->p.dim == 3) { set_error("freeze-out hypersurface rows are built for D = 2"); return CCAKE_ERR_UNSUPPORTED; }
   if (!ev->fo_rows) { set_error("freeze-out surface needs particlization at event creation"); return CCAKE_ERR_INVALID; }
   int cnt = 0;
   CCAKE_HIP(hipMemcpyAsync(&cnt, ev->fo_count, sizeof(int), hipMemcpyDeviceToHost, ev->stream));

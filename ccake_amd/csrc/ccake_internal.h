@@ -42,16 +42,19 @@ static_assert(P_FRZ2_G == P_FRZ1_G + 1 && P_C_Q0 == P_FRZ2_G + 1 && NUM_COLUMNS 
 // Grid per event; their cells share one key space of rows of `stride`
 // cells, event e owning the key rows [row0, row0 + nc[1]), so the cell sort
 // orders particles event by event and no pair walk crosses events.  For a
-// single event stride = nc[0] and row0 = 0.
+// single event stride = nc[0] and row0 = 0.  A 3+1D event (dim 3) has one
+// key row per (y, eta) cell pair, row = c_eta * nc[1] + c_y, so the key is
+// (c_eta * nc[1] + c_y) * nc[0] + c_x; a 2+1D grid has nc[2] = 1.
 struct Grid {
-  double lo[2];       // grid origin
+  double lo[3];       // grid origin
   double cell;        // cell edge
   double inv_cell;
-  int nc[2];          // cells per axis
+  int nc[3];          // cells per axis
   int ncells;         // cells of the whole key space (cell_off holds ncells + 1 entries)
   int stride;         // cells per key row
   int row0;           // first key row of this event
   int nev;            // events sharing the key space
+  int dim;            // 2 or 3
 };
 
 // Batches: at most kMaxBatchEvents events, each with at most
@@ -279,6 +282,18 @@ int launch_rk(ccake_event* ev, double dt, double t0);
 // mixed-precision pair passes (pair_precision 1), in place of the FP64 ones
 int launch_smooth_pairs32(ccake_event* ev);
 int launch_grad_pairs32(ccake_event* ev);
+// 3+1D Milne path (k_milne3.hip, dim 3): the stages whose data layout or
+// geometry differ from 2+1D
+int launch3_stage_begin(ccake_event* ev, int do_gamma);     // gamma/v (tau^2 metric), bbox in (x, y, eta)
+int launch3_stage_gather(ccake_event* ev);                  // kept sort: gamma, v, smoothing records
+int launch3_smooth_pairs(ccake_event* ev, int mode, bool count_pairs);
+int launch3_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, size_t* n);
+int launch3_post(ccake_event* ev, int what);                // 1 reset_pi, 2 transport, 4 gradient records
+int launch3_grad(ccake_event* ev);                          // gradient pairs + sums
+int launch3_eom(ccake_event* ev, int fuse_rk, double dt);   // EoM (+ RK update, + cache first)
+int launch3_rk(ccake_event* ev, double dt);
+int launch3_cache(ccake_event* ev);
+double smooth_norm2_event(const ccake_event* ev);           // W = norm2 w(q) of the event's kernel
 // misc
 int launch_gamma_velocity(ccake_event* ev);
 int launch_init_particles(ccake_event* ev, int phase);
@@ -301,6 +316,25 @@ __device__ __forceinline__ void smooth_record(const double* __restrict__ F, int 
   o[5] = f(CCAKE_F_MASS_SS) * f(CCAKE_F_EXT_SS);
   o[6] = f(CCAKE_F_MASS_Q) * f(CCAKE_F_EXT_Q);
   o[7] = 0.0;
+}
+
+// 3+1D candidate records.  Smoothing (8 doubles): x, y, eta, m_s,
+// m_s*ext_s, m_B*ext_B, m_S*ext_S, m_Q*ext_Q.  Gradients (24): x, y, eta,
+// m_s, p/sl^2, e, Pi/sl^2, v^x, v^y, v^eta, pi^{0i}/sl^2 (3), the spatial
+// pi^{ij}/sl^2 (xx, xy, x eta, yy, y eta, eta eta), sl, 4 unused.
+constexpr int kCandG3 = 24;
+__device__ __forceinline__ void smooth_record3(const double* __restrict__ F, int npad, int i, double* __restrict__ o)
+{
+  auto f = [&](int c) { return F[(size_t)c * npad + i]; };
+  const double ms = f(CCAKE_F_MASS_S);
+  o[0] = f(CCAKE_F_X);
+  o[1] = f(CCAKE_F_Y);
+  o[2] = f(CCAKE_F_ETA);
+  o[3] = ms;
+  o[4] = ms * f(CCAKE_F_EXT_S);
+  o[5] = f(CCAKE_F_MASS_B) * f(CCAKE_F_EXT_B);
+  o[6] = f(CCAKE_F_MASS_SS) * f(CCAKE_F_EXT_SS);
+  o[7] = f(CCAKE_F_MASS_Q) * f(CCAKE_F_EXT_Q);
 }
 
 // FP32 candidate records of the mixed-precision pair kernels: the position

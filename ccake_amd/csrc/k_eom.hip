@@ -845,6 +845,7 @@ int rk4_columns() { return K4_NUM; }
 int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0, bool fold_grad_sum, bool begin_next)
 {
   (void)t0;
+  if (ev->p.dim == 3) return launch3_eom(ev, fuse_rk, dt);
   GradSum gs;
   if (fold_grad_sum) {
     gs.perm = ev->perm;
@@ -885,6 +886,7 @@ int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0, bool fold_gra
 int launch_rk(ccake_event* ev, double dt, double t0)
 {
   (void)t0;
+  if (ev->p.dim == 3) return launch3_rk(ev, dt);
   const int n = ev->n;
   hipLaunchKernelGGL(k_rk, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n,
                      ev->I + (size_t)CCAKE_I_FREEZE * ev->npad, dt, ev->err, ev->p.using_diffusion);
@@ -893,6 +895,7 @@ int launch_rk(ccake_event* ev, double dt, double t0)
 
 int launch_cache(ccake_event* ev)
 {
+  if (ev->p.dim == 3) return launch3_cache(ev);
   const int n = ev->n;
   hipLaunchKernelGGL(k_cache, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n,
                      ev->p.using_diffusion);

@@ -455,8 +455,72 @@ __global__ __launch_bounds__(256) void k_post(double* __restrict__ F, int npad, 
   }
 }
 
+// The per-particle tail of a 3+1D EoS stage (after k_eos in its EoS-only
+// mode): `what` bit 1 reset_pi_tensor (EoM_default::reset_pi_tensor,
+// src/eom_default.cpp:56-172, with the tau^2 metric and u^eta), bit 2 the
+// transport coefficients, bit 4 the gradient candidate record (kCandG3
+// layout) at sorted position k.  With perm, thread k works on perm[k].
+__global__ __launch_bounds__(256) void k3_post(double* __restrict__ F, int npad, int n,
+                                               const int32_t* __restrict__ perm, double t, Transport P,
+                                               double* __restrict__ cand, int what)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = perm ? perm[k] : k;
+  auto col = [&](int c) -> double& { return F[(size_t)c * npad + i]; };
+  if (what & 1) {
+    const double gamma = col(CCAKE_F_GAMMA), sigma = col(CCAKE_F_SIGMA);
+    const double t2 = t * t;
+    const double fc[3] = { -col(CCAKE_F_UX), -col(CCAKE_F_UY), col(CCAKE_F_UETA) * -t2 };
+    col(CCAKE_F_BULK) = col(CCAKE_F_EXT_BULK) * sigma;
+    double S[4][4];
+    for (int a = 0; a < 4; ++a) for (int b = 0; b < 4; ++b) S[a][b] = 0.0;
+    int q = 0;
+    for (int a = 0; a < 2; ++a) for (int b = a; b < 3; ++b) S[a + 1][b + 1] = col(CCAKE_F_EXT_SHV0 + q++) * sigma;
+    for (int a = 1; a < 4; ++a) for (int b = a + 1; b < 4; ++b) S[b][a] = S[a][b];
+    S[0][0] = (fc[0] * fc[0] * S[1][1] + fc[1] * fc[1] * S[2][2] + 2. * fc[1] * fc[0] * S[2][1]
+               + 2. * fc[2] * fc[0] * S[3][1] + 2. * fc[2] * fc[1] * S[3][2]
+               - fc[2] * fc[2] * (S[1][1] + S[2][2]) / t2) / (gamma * gamma - fc[2] * fc[2] / t2);
+    S[3][3] = (S[0][0] - S[1][1] - S[2][2]) / t2;
+    for (int a = 0; a < 3; ++a) {
+      double p0 = 0.0;
+      for (int b = 0; b < 3; ++b) p0 += -S[a + 1][b + 1] * fc[b] / gamma;
+      S[0][a + 1] = p0;
+    }
+    for (int a = 1; a < 4; ++a) S[a][0] = S[0][a];
+    for (int a = 0; a < 4; ++a) for (int b = 0; b < 4; ++b) col(CCAKE_F_SHV0 + 4 * a + b) = S[a][b];
+  }
+  if (what & 2) {
+    double tc[13];
+    transport(P, col(CCAKE_F_T), col(CCAKE_F_W), col(CCAKE_F_P), col(CCAKE_F_CS2), tc);
+    for (int q = 0; q < 13; ++q) col(CCAKE_F_ETA_PI + q) = tc[q];
+  }
+  if (what & 4) {
+    const double sl = col(CCAKE_F_SIGMA_LAB);
+    const double isl2 = 1.0 / (sl * sl);
+    double* o = cand + (size_t)k * kCandG3;
+    o[0] = col(CCAKE_F_X); o[1] = col(CCAKE_F_Y); o[2] = col(CCAKE_F_ETA); o[3] = col(CCAKE_F_MASS_S);
+    o[4] = col(CCAKE_F_P) * isl2; o[5] = col(CCAKE_F_E); o[6] = col(CCAKE_F_BULK) * isl2;
+    o[7] = col(CCAKE_F_VX); o[8] = col(CCAKE_F_VY); o[9] = col(CCAKE_F_VETA);
+    o[10] = col(CCAKE_F_SHV0 + 1) * isl2; o[11] = col(CCAKE_F_SHV0 + 2) * isl2; o[12] = col(CCAKE_F_SHV0 + 3) * isl2;
+    o[13] = col(CCAKE_F_SHV0 + 5) * isl2; o[14] = col(CCAKE_F_SHV0 + 6) * isl2; o[15] = col(CCAKE_F_SHV0 + 7) * isl2;
+    o[16] = col(CCAKE_F_SHV0 + 10) * isl2; o[17] = col(CCAKE_F_SHV0 + 11) * isl2;
+    o[18] = col(CCAKE_F_SHV0 + 15) * isl2;
+    o[19] = sl; o[20] = 0.0; o[21] = 0.0; o[22] = 0.0; o[23] = 0.0;
+  }
+}
+
+int launch3_post(ccake_event* ev, int what)
+{
+  const int n = ev->n;
+  hipLaunchKernelGGL(k3_post, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n,
+                     (what & 4) ? ev->perm : nullptr, ev->t, make_transport(ev->p), ev->cand_g, what);
+  return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+}
+
 static int post(ccake_event* ev, int what, const int32_t* perm)
 {
+  if (ev->p.dim == 3) return launch3_post(ev, what == 0 ? 1 : (what == 1 ? 2 : 4));
   const int n = ev->n;
   hipLaunchKernelGGL(k_post, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n, perm, ev->t,
                      make_transport(ev->p), ev->cand_g, what);

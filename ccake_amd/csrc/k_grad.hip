@@ -147,6 +147,7 @@ int launch_grad_sum(ccake_event* ev)
 
 int launch_grad(ccake_event* ev)
 {
+  if (ev->p.dim == 3) return launch3_grad(ev);
   int rc = launch_grad_pairs(ev);
   return rc ? rc : launch_grad_sum(ev);
 }

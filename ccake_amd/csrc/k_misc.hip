@@ -9,7 +9,7 @@ namespace ccake {
 // phase 3: calculate_extensive_shv (1392-1432) + set_bulk_Pi (1349-1384)
 __global__ __launch_bounds__(256) void k_init(double* __restrict__ F, int npad, int n, int32_t* __restrict__ I,
                                               int phase, double t0, double efcheck, int input_shear,
-                                              int bulk_from_trace, int diffusion)
+                                              int bulk_from_trace, int diffusion, int dim)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -17,7 +17,17 @@ __global__ __launch_bounds__(256) void k_init(double* __restrict__ F, int npad, 
   int32_t* freeze = I + (size_t)CCAKE_I_FREEZE * npad;
   if (phase == 0) {
     const double ux = col(CCAKE_F_UX), uy = col(CCAKE_F_UY);
-    col(CCAKE_F_GAMMA) = sqrt(1.0 - ((-ux) * ux + (-uy) * uy));
+    if (dim == 3) {
+      // gamma_calc(u, t0^2): u_cov = (-u^x, -u^y, -t0^2 u^eta)
+      const double ue = col(CCAKE_F_UETA);
+      double c = 0.0;
+      c += (-ux) * ux;
+      c += (-uy) * uy;
+      c += (ue * -(t0 * t0)) * ue;
+      col(CCAKE_F_GAMMA) = sqrt(1.0 - c);
+    } else {
+      col(CCAKE_F_GAMMA) = sqrt(1.0 - ((-ux) * ux + (-uy) * uy));
+    }
     col(CCAKE_F_EXT_S) = col(CCAKE_F_EXT_B) = col(CCAKE_F_EXT_SS) = col(CCAKE_F_EXT_Q) = 1.0;
     col(CCAKE_F_MASS_S) = col(CCAKE_F_MASS_B) = col(CCAKE_F_MASS_SS) = col(CCAKE_F_MASS_Q) = 1.0;
     col(CCAKE_F_T) = 580.0 / kHbarcMeVfm;
@@ -25,8 +35,7 @@ __global__ __launch_bounds__(256) void k_init(double* __restrict__ F, int npad, 
     I[(size_t)CCAKE_I_EOS_ID * npad + i] = 0;
     I[(size_t)CCAKE_I_BTRACK * npad + i] = 0;
     freeze[i] = 0;
-    col(CCAKE_F_ETA) = 0.0;
-    col(CCAKE_F_UETA) = 0.0;
+    if (dim != 3) { col(CCAKE_F_ETA) = 0.0; col(CCAKE_F_UETA) = 0.0; }
     col(CCAKE_F_EZ) = 0.0;
     col(CCAKE_F_FRZ1_E) = col(CCAKE_F_FRZ2_E) = 0.0;
     for (int q = 0; q < 4; ++q) { col(P_FB_E0 + q) = 0.0; col(P_FB_G0 + q) = 0.0; }
@@ -69,7 +78,7 @@ int launch_init_particles(ccake_event* ev, int phase)
   const int n = ev->n;
   hipLaunchKernelGGL(k_init, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n, ev->I, phase,
                      ev->p.t0, ev->p.e_freeze, ev->p.input_initial_shear, ev->p.bulk_from_trace,
-                     ev->p.using_diffusion);
+                     ev->p.using_diffusion, ev->p.dim);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
 

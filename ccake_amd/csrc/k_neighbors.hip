@@ -95,11 +95,60 @@ __global__ __launch_bounds__(256) void k_grid_setup(const double* __restrict__ b
       }
     }
     Grid g;
-    g.lo[0] = xmn; g.lo[1] = ymn; g.cell = cell; g.inv_cell = 1.0 / cell;
-    g.nc[0] = nx; g.nc[1] = ny; g.ncells = nx * ny;
-    g.stride = nx; g.row0 = 0; g.nev = 1;
+    g.lo[0] = xmn; g.lo[1] = ymn; g.lo[2] = 0.0; g.cell = cell; g.inv_cell = 1.0 / cell;
+    g.nc[0] = nx; g.nc[1] = ny; g.nc[2] = 1; g.ncells = nx * ny;
+    g.stride = nx; g.row0 = 0; g.nev = 1; g.dim = 2;
     *grid = g;
   }
+}
+
+// 3+1D: the bounding box in (x, y, eta) from the six-value partials of
+// k3_stage_begin, and a grid of cubic cells of edge rc / cells_per_rc over
+// it, doubled until the key space fits `cap` cells.
+__global__ __launch_bounds__(256) void k3_grid_setup(const double* __restrict__ bbox_part, int nparts,
+                                                     double rc, int cells_per_rc, int cap,
+                                                     Grid* __restrict__ grid, int32_t* __restrict__ err,
+                                                     unsigned long long* __restrict__ pair_count)
+{
+  if (threadIdx.x == 0) *pair_count = 0ull;
+  double mn[3] = { INFINITY, INFINITY, INFINITY }, mx[3] = { -INFINITY, -INFINITY, -INFINITY };
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+    const double* b = bbox_part + 6 * k;
+    for (int d = 0; d < 3; ++d) { mn[d] = fmin(mn[d], b[2 * d]); mx[d] = fmax(mx[d], b[2 * d + 1]); }
+  }
+  for (int o = 32; o > 0; o >>= 1)
+    for (int d = 0; d < 3; ++d) { mn[d] = fmin(mn[d], __shfl_xor(mn[d], o)); mx[d] = fmax(mx[d], __shfl_xor(mx[d], o)); }
+  __shared__ double s[4][6];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) for (int d = 0; d < 3; ++d) { s[w][2 * d] = mn[d]; s[w][2 * d + 1] = mx[d]; }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int k = 1; k < 4; ++k)
+    for (int d = 0; d < 3; ++d) { mn[d] = fmin(mn[d], s[k][2 * d]); mx[d] = fmax(mx[d], s[k][2 * d + 1]); }
+  double cell = rc / (double)(cells_per_rc > 0 ? cells_per_rc : 1);
+  int nc[3] = { 1, 1, 1 };
+  bool finite = true;
+  for (int d = 0; d < 3; ++d) {
+    const double wd = mx[d] - mn[d];
+    if (!(wd >= 0.0 && wd < 1e300)) finite = false;
+  }
+  if (!finite) {
+    atomicOr(err, DE_GRID_OVERFLOW);
+    mn[0] = mn[1] = mn[2] = 0.0;
+  } else {
+    for (;;) {
+      double f[3], tot = 1.0;
+      for (int d = 0; d < 3; ++d) { f[d] = floor((mx[d] - mn[d]) / cell) + 1.0; tot *= f[d]; }
+      if (tot <= (double)cap) { for (int d = 0; d < 3; ++d) nc[d] = (int)f[d]; break; }
+      cell *= 2.0;
+    }
+  }
+  Grid g;
+  for (int d = 0; d < 3; ++d) { g.lo[d] = mn[d]; g.nc[d] = nc[d]; }
+  g.cell = cell; g.inv_cell = 1.0 / cell;
+  g.ncells = nc[0] * nc[1] * nc[2];
+  g.stride = nc[0]; g.row0 = 0; g.nev = 1; g.dim = 3;
+  *grid = g;
 }
 
 // A batch: the bounding box of each event (workgroup e reduces the ids
@@ -156,8 +205,8 @@ __global__ __launch_bounds__(256) void k_grid_events(const double* __restrict__ 
       }
     }
     Grid g;
-    g.lo[0] = xmn; g.lo[1] = ymn; g.cell = cell; g.inv_cell = 1.0 / cell;
-    g.nc[0] = nx; g.nc[1] = ny; g.ncells = 0; g.stride = 0; g.row0 = 0; g.nev = nev;
+    g.lo[0] = xmn; g.lo[1] = ymn; g.lo[2] = 0.0; g.cell = cell; g.inv_cell = 1.0 / cell;
+    g.nc[0] = nx; g.nc[1] = ny; g.nc[2] = 1; g.ncells = 0; g.stride = 0; g.row0 = 0; g.nev = nev; g.dim = 2;
     grid[e] = g;
   }
   __syncthreads();
@@ -201,7 +250,9 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const int32_t* __re
       const Grid g = grid[eid ? eid[idx] : 0];
       const int cx = cell_coord(F[(size_t)CCAKE_F_X * npad + idx], g.lo[0], g.inv_cell, g.nc[0]);
       const int cy = cell_coord(F[(size_t)CCAKE_F_Y * npad + idx], g.lo[1], g.inv_cell, g.nc[1]);
-      key = (g.row0 + cy) * g.stride + cx;
+      // 3+1D: key rows run over (eta, y) cell pairs
+      const int cz = g.dim == 3 ? cell_coord(F[(size_t)CCAKE_F_ETA * npad + idx], g.lo[2], g.inv_cell, g.nc[2]) : 0;
+      key = (g.row0 + cz * g.nc[1] + cy) * g.stride + cx;
       keys_out[idx] = key;
     } else {
       key = keys_in[idx];
@@ -388,7 +439,8 @@ __global__ __launch_bounds__(256) void k_cells_records(const int32_t* __restrict
   const int prev = (k == 0) ? -1 : skey[k - 1];
   for (int c = prev + 1; c <= cur; ++c) cell_off[c] = k;
   if (k == n - 1) for (int c = cur + 1; c <= nc; ++c) cell_off[c] = n;
-  smooth_record(F, npad, perm[k], cand + (size_t)k * kCandS);
+  if (grid->dim == 3) smooth_record3(F, npad, perm[k], cand + (size_t)k * kCandS);
+  else smooth_record(F, npad, perm[k], cand + (size_t)k * kCandS);
 }
 
 // ---------------------------------------------------------------------------
@@ -400,9 +452,9 @@ static int stage_begin(ccake_event* ev, int do_gamma)
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
 
-int launch_stage_begin(ccake_event* ev) { return stage_begin(ev, 1); }
-int launch_bbox(ccake_event* ev) { return stage_begin(ev, 0); }
-int launch_gamma_velocity(ccake_event* ev) { return stage_begin(ev, 1); }
+int launch_stage_begin(ccake_event* ev) { return ev->p.dim == 3 ? launch3_stage_begin(ev, 1) : stage_begin(ev, 1); }
+int launch_bbox(ccake_event* ev) { return ev->p.dim == 3 ? launch3_stage_begin(ev, 0) : stage_begin(ev, 0); }
+int launch_gamma_velocity(ccake_event* ev) { return launch_stage_begin(ev); }
 
 static void scan_hist(ccake_event* ev, int nscan)
 {
@@ -421,7 +473,10 @@ int launch_neighbor_sort(ccake_event* ev)
   hipStream_t st = ev->stream;
   const int nb = (n + 255) / 256;
   const double rc = 2.0 * ev->p.hT;
-  if (ev->nev > 1) {
+  if (ev->p.dim == 3) {
+    hipLaunchKernelGGL(k3_grid_setup, dim3(1), dim3(256), 0, st, ev->bbox, nb, rc, ev->p.cells_per_rc,
+                       ev->cap_cells, ev->grid, ev->err, (unsigned long long*)ev->pair_count);
+  } else if (ev->nev > 1) {
     hipLaunchKernelGGL(k_bbox_events, dim3(ev->nev), dim3(256), 0, st, ev->F, ev->npad, ev->ev_off, ev->bbox);
     hipLaunchKernelGGL(k_grid_events, dim3(1), dim3(256), 0, st, ev->bbox, ev->nev, rc, ev->p.cells_per_rc,
                        ev->grid, ev->err, (unsigned long long*)ev->pair_count);

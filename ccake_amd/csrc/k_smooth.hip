@@ -12,11 +12,12 @@ namespace ccake {
 // Candidate records in sorted order (smooth_record).
 __global__ __launch_bounds__(256) void k_gather_smooth(const double* __restrict__ F, int npad, int n,
                                                        const int32_t* __restrict__ perm,
-                                                       double* __restrict__ cand)
+                                                       double* __restrict__ cand, int dim)
 {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  smooth_record(F, npad, perm[k], cand + (size_t)k * kCandS);
+  if (dim == 3) smooth_record3(F, npad, perm[k], cand + (size_t)k * kCandS);
+  else smooth_record(F, npad, perm[k], cand + (size_t)k * kCandS);
 }
 
 // A stage that keeps the cell sort (stage begin plus record gather in one
@@ -76,6 +77,7 @@ int launch_charge_check(ccake_event* ev, bool* any_charge, bool* any_bulk)
 
 int launch_stage_gather(ccake_event* ev)
 {
+  if (ev->p.dim == 3) return launch3_stage_gather(ev);
   const int n = ev->n;
   hipLaunchKernelGGL(k_stage_gather, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n,
                      ev->perm, ev->cand_s, (unsigned long long*)ev->pair_count);
@@ -284,7 +286,7 @@ int launch_gather_smooth(ccake_event* ev)
 {
   const int n = ev->n;
   hipLaunchKernelGGL(k_gather_smooth, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad,
-                     n, ev->perm, ev->cand_s);
+                     n, ev->perm, ev->cand_s, ev->p.dim);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
 
@@ -292,6 +294,7 @@ int launch_gather_smooth(ccake_event* ev)
 // (mode 2, which is complete by itself).
 int launch_smooth_pairs(ccake_event* ev, int mode, bool count_pairs)
 {
+  if (ev->p.dim == 3) return launch3_smooth_pairs(ev, mode, count_pairs);
   const int n = ev->n;
   const double rc = 2.0 * ev->p.hT;
   const double rc2 = rc * rc;
@@ -346,10 +349,10 @@ double smooth_norm2(double hT) { return make_w(hT).norm2; }
 
 int launch_smooth_sum(ccake_event* ev, int mode, bool count_pairs)
 {
-  const KernelW W = make_w(ev->p.hT);
+  const double norm2 = smooth_norm2_event(ev);
   int32_t* bt = ev->I + (size_t)CCAKE_I_BTRACK * ev->npad;
   unsigned long long* pc = count_pairs ? (unsigned long long*)ev->pair_count : nullptr;
-  const int rc = (mode == 0) ? smooth_sum_dispatch<0>(ev, W.norm2, bt, pc) : smooth_sum_dispatch<1>(ev, W.norm2, bt, pc);
+  const int rc = (mode == 0) ? smooth_sum_dispatch<0>(ev, norm2, bt, pc) : smooth_sum_dispatch<1>(ev, norm2, bt, pc);
   if (rc) return rc;
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
@@ -364,6 +367,7 @@ int launch_smooth(ccake_event* ev, int mode, bool count_pairs)
 // Parity export: counts are in btrack (sticky -1 rows are recounted here).
 int launch_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, size_t* n_out)
 {
+  if (ev->p.dim == 3) return launch3_export_pairs(ev, pi, pj, cap, n_out);
   const int n = ev->n;
   hipStream_t st = ev->stream;
   // fresh counts through btrack, which is saved and restored around the count

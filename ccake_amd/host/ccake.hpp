@@ -83,6 +83,8 @@ struct Settings {
   // 0: FP64 pair sums; 1: FP32 pair terms with compensated accumulation
   // (ccake_params::pair_precision; no counterpart in the reference)
   int pair_precision = 0;
+  // D = 3: W_2D(r_perp, hT) W_1D(|d eta|, hEta) (include/settings.h:114)
+  bool factorized_kernel = false;
   bool print_conservation_status = false;
   bool exit_on_error = false;
   // initial conditions and outputs (include/settings.h, include/defaults.h:25-72)
@@ -406,6 +408,7 @@ class SystemState {
       p.regulator_xi0 = s.diffusion_regulator_xi0; p.regulator_rq_max = s.diffusion_regulator_rq_max;
     }
     p.pair_precision = s.pair_precision;
+    p.factorized_kernel = s.factorized_kernel;
     return p;
   }
 

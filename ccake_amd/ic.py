@@ -79,6 +79,55 @@ def lumpy(seed=0xCCA4E001, spacing=0.02, half_width=5.0, n_blobs=50, sigma=0.4,
     return dict(x=x, y=y, e=e / HBARC_GEVFM, ux=z, uy=z.copy(), rhoB=rb, rhoS=rs, rhoQ=rq)
 
 
+def lattice3(spacing, half_width, eta_spacing, eta_max):
+    """(x, y, eta_s) lattice: the square lattice of `lattice` at every eta
+    node of [-eta_max, eta_max] (x fastest, eta slowest)."""
+    x2, y2 = lattice(spacing, half_width)
+    m = int(round(2 * eta_max / eta_spacing)) + 1
+    ge = np.round(-eta_max + eta_spacing * np.arange(m), 12) + 0.0
+    x = np.tile(x2, m)
+    y = np.tile(y2, m)
+    eta = np.repeat(ge, len(x2))
+    return x, y, eta
+
+
+def plateau(eta, eta_flat=2.0, eta_sigma=0.5):
+    """Longitudinal Gaussian-plateau profile: 1 for |eta| <= eta_flat/2,
+    exp(-(|eta| - eta_flat/2)^2 / (2 eta_sigma^2)) beyond."""
+    d = np.maximum(np.abs(eta) - 0.5 * eta_flat, 0.0)
+    return np.exp(-d * d / (2.0 * eta_sigma * eta_sigma))
+
+
+def milne_lumpy(seed=0xCCA4E001, spacing=0.1, half_width=3.0, eta_spacing=0.1, eta_max=4.0, eta_flat=2.0,
+                eta_sigma=0.5, n_blobs=20, sigma=0.4, center_sd=1.0, amp_median_gev=20.0, amp_logsd=0.5,
+                charge_frac=0.05, e_cutoff_gev=0.0):
+    """3+1D lumpy event in Milne coordinates (tau, x, y, eta_s): the
+    transverse lumpy profile of `lumpy` (same draws, same units) times the
+    longitudinal Gaussian plateau of `plateau`, on the (x, y, eta_s)
+    lattice of `lattice3`; charges the same blobs scaled; u = 0.  Points at
+    or below e_cutoff_gev (GeV/fm^3) are dropped."""
+    rng = np.random.Generator(np.random.MT19937(seed))
+    x, y, eta = lattice3(spacing, half_width, eta_spacing, eta_max)
+    et = np.zeros_like(x)
+    rb, rs, rq = np.zeros_like(x), np.zeros_like(x), np.zeros_like(x)
+    for _ in range(n_blobs):
+        cx = rng.normal(0.0, center_sd)
+        cy = rng.normal(0.0, center_sd)
+        A = float(np.exp(rng.normal(np.log(amp_median_gev), amp_logsd)))
+        fB, fS, fQ = rng.uniform(-charge_frac, charge_frac, size=3)
+        blob = A * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * sigma * sigma))
+        et += blob
+        rb += fB * blob
+        rs += fS * blob
+        rq += fQ * blob
+    h = plateau(eta, eta_flat, eta_sigma)
+    e, rb, rs, rq = et * h, rb * h, rs * h, rq * h
+    keep = e > e_cutoff_gev
+    x, y, eta, e, rb, rs, rq = x[keep], y[keep], eta[keep], e[keep], rb[keep], rs[keep], rq[keep]
+    z = np.zeros_like(x)
+    return dict(x=x, y=y, eta=eta, e=e / HBARC_GEVFM, ux=z, uy=z.copy(), ueta=z.copy(), rhoB=rb, rhoS=rs, rhoQ=rq)
+
+
 # ---------------------------------------------------------------- file formats
 # Readers of the reference's initial-condition files, the Python twins of
 # read_ccake / read_ICCING in ccake_amd/host/ccake_io.hpp: the same fields,

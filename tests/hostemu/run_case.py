@@ -6,6 +6,9 @@ Cases:
   table   lumpy viscous BSQ event on the 64x32^3 tanh-conformal table
   table_fo  the same event with the reference's default energy cutoff,
           particlization on, run step by step
+  milne   3+1D lumpy viscous BSQ event in Milne coordinates on the table EoS
+          (isotropic 3D kernel); milne_fact the same with the factorized kernel
+          W_2D(r_perp, hT) W_1D(|d eta|, hEta), hEta = 0.2 fm
 
 Usage: run_case.py CASE SPACING STEPS [--no-oracle] [--tol X]
 STEPS = -1 runs until continue_evolution ends the run (every particle
@@ -29,7 +32,7 @@ import numpy as np  # noqa: E402
 
 import ccake_amd  # noqa: E402
 from ccake_amd import config, eos_tables, ic  # noqa: E402
-from parity_metrics import DERIVS, STATE, field_err, pp_tol, whole_field  # noqa: E402
+from parity_metrics import DERIVS, DERIVS3, STATE, STATE3, field_err, pp_tol, whole_field  # noqa: E402
 
 T_FO = 150.0 / config.HBARC_MEVFM
 
@@ -54,7 +57,10 @@ def main():
         tab, dims, mins, maxs = table
         geos = ccake_amd.Eos(dev, 1, T_FO, dims, mins, maxs, tab)
         cfg = config.lumpy_params()
-        if case == "table_fo":
+        if case.startswith("milne"):
+            cfg = dict(cfg, dim=3, hEta=0.2, factorized_kernel=1 if case == "milne_fact" else 0, particlization=0)
+            d = ic.milne_lumpy(spacing=spacing, half_width=1.0, eta_spacing=spacing, eta_max=1.0, center_sd=0.5)
+        elif case == "table_fo":
             cfg = dict(cfg, particlization=1, e_freeze=geos.efreeze(T_FO))
             d = ic.lumpy(spacing=spacing, e_cutoff_gev=0.15)
         else:
@@ -83,9 +89,10 @@ def main():
         import oracle
         oeos = oracle.Eos(0, T_FO) if table is None else oracle.Eos(1, T_FO, *table[1:], table[0])
         oev = oracle.Event(cfg, oeos, n)
-        for k, f in [("x", "X"), ("y", "Y"), ("e", "IN_E"), ("ux", "UX"), ("uy", "UY"),
-                     ("rhoB", "IN_B"), ("rhoS", "IN_SS"), ("rhoQ", "IN_Q")]:
-            oev.set("ORC_F_" + f, d[k])
+        for k, f in [("x", "X"), ("y", "Y"), ("eta", "ETA"), ("e", "IN_E"), ("ux", "UX"), ("uy", "UY"),
+                     ("ueta", "UETA"), ("rhoB", "IN_B"), ("rhoS", "IN_SS"), ("rhoQ", "IN_Q")]:
+            if k in d:
+                oev.set("ORC_F_" + f, d[k])
         assert oev.initialize() == 0
         if steps >= 0:
             for _ in range(steps):
@@ -99,7 +106,8 @@ def main():
             if osteps != nsteps:
                 raise SystemExit(f"step counts differ: {nsteps} against {osteps}")
         errs, whole = {}, {}
-        for lab in STATE + (DERIVS if steps != 0 else []):
+        st, dv = (STATE3, DERIVS3) if case.startswith("milne") else (STATE, DERIVS)
+        for lab in st + (dv if steps != 0 else []):
             fid = ccake_amd.FIELDS["CCAKE_F_" + lab.split("+")[0]] + (int(lab.split("+")[1]) if "+" in lab else 0)
             a, b = ev.download(fid), oev.field(fid)
             errs[lab] = field_err(lab, a, b, tol)

@@ -85,3 +85,15 @@ DERIVS = ([f"GRADP0+{c}" for c in range(2)] + [f"GRADE0+{c}" for c in range(2)]
           + ["DEXT_S", "DEXT_B", "DEXT_SS", "DEXT_Q", "DEXT_BULK", "THETA", "SHV_NABLA_U", "DEZ"]
           + ["SHEAR_KNUDSEN", "BULK_KNUDSEN", "INV_REYNOLDS_SHEAR", "INV_REYNOLDS_BULK"]
           + [f"DEXT_SHV0+{c}" for c in range(5)])
+
+
+# 3+1D (D = 3): the eta components, three-component gradients and the 3x3
+# velocity gradient; the Knudsen and inverse-Reynolds diagnostics are 2+1D
+# only.
+STATE3 = STATE + ["ETA", "UETA", "VETA"]
+DERIVS3 = ([f"GRADP0+{c}" for c in range(3)] + [f"GRADE0+{c}" for c in range(3)]
+           + [f"GRADBULK0+{c}" for c in range(3)] + [f"GRADSHEAR0+{c}" for c in range(3)]
+           + [f"DIVSHEAR0+{c}" for c in range(3)] + [f"GRADV0+{c}" for c in range(9)]
+           + [f"DUDT0+{c}" for c in range(3)]
+           + ["DEXT_S", "DEXT_B", "DEXT_SS", "DEXT_Q", "DEXT_BULK", "THETA", "SHV_NABLA_U", "DEZ"]
+           + [f"DEXT_SHV0+{c}" for c in range(5)])

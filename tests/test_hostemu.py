@@ -62,6 +62,19 @@ def test_mixed_precision_on_host_emulation(emu_lib):
     assert "OK" in r.stdout
 
 
+@pytest.mark.parametrize("case", ["milne", "milne_fact"])
+def test_milne_on_host_emulation(emu_lib, case):
+    """The 3+1D path (1,331 particles of a Milne lumpy event on the table EoS,
+    the isotropic 3D kernel or the factorized one) under AddressSanitizer:
+    no invalid access in the (x, y, eta) grid, the three-pass key sort, the
+    3D pair walks and the D = 3 equations of motion; after initialization
+    and 2 steps every field within 1e-12 of the oracle (whole field; per
+    particle 1e-12 for positive fields, 1e-10 for signed ones)."""
+    r = run_case(case, "0.2", "2")
+    assert r.returncode == 0 and "AddressSanitizer" not in r.stderr
+    assert "OK" in r.stdout
+
+
 def test_batch_on_host_emulation(emu_lib):
     """A batch of three lumpy events (2,042, 864 and 1,465 particles) under
     AddressSanitizer: no invalid access in the batch's grid setup, key sort

@@ -131,3 +131,74 @@ def test_served_under_hbm_budget():
     assert dev.memory()[0] == 0
     eos.close()
     dev.close()
+
+
+# ------------------------------------------------------------------ continuous batching
+def test_poisson_arrivals_rate_and_order():
+    t = serve.poisson_arrivals(20000, 50.0, seed=3)
+    assert np.all(np.diff(t) > 0)
+    assert abs(20000 / t[-1] - 50.0) < 1.5          # the mean rate, within a few standard errors
+    assert np.array_equal(t[:5], serve.poisson_arrivals(5, 50.0, seed=3))
+
+
+def test_batch_server_forms_batches_from_arrived_requests(monkeypatch):
+    """Stand-in batches: a free worker takes every request that has arrived
+    (within max_events), later arrivals go to the next batch, and no request
+    is taken before it arrives."""
+    taken = []
+
+    def fake_run_batch(self, reqs, admitted, now, bid, publish):
+        taken.append((bid, [r.rid for r in reqs], admitted, [r.arrival for r in reqs]))
+        time.sleep(0.05)
+        for r in reqs:
+            publish(serve.SurfaceResult(r.rid, 1, 3, np.zeros((0, 42)), r.arrival, admitted, now(), bid))
+
+    monkeypatch.setattr(serve.BatchServer, "_run_batch", fake_run_batch)
+    arr = [0.0, 0.001, 0.002, 0.08, 0.081, 0.2]
+    reqs = [serve.Request(rid=k, ic={"x": np.zeros(10)}, params={}, arrival=a) for k, a in enumerate(arr)]
+    srv = serve.BatchServer(None, None, dict(particlization=1, dt=0.02), slots=1, max_events=2)
+    res = srv.run(reqs)
+    assert [r.rid for r in res] == list(range(6))
+    sizes = [len(t[1]) for t in taken]
+    assert max(sizes) <= 2 and sum(sizes) == 6
+    for bid, rids, admitted, arrivals in taken:
+        assert all(a <= admitted + 1e-3 for a in arrivals)
+    assert all(r.latency >= 0 for r in res)
+
+
+@pytest.mark.gpu
+def test_batch_server_results_equal_alone():
+    """Three lumpy events (0.1 fm lattice, 8,077, 3,345 and 5,746 particles)
+    arriving as a Poisson stream, served by two workers forming batches of
+    at most two events and run to freeze-out: every event's hypersurface has
+    the ids and steps of its run alone and columns within 1e-9 of them."""
+    from ccake_amd import eos_tables
+    t_fo = 150.0 / config.HBARC_MEVFM
+    dev = ccake_amd.Device(0)
+    tab, dims, mins, maxs = eos_tables.tanh_conformal_table()
+    eos = ccake_amd.Eos(dev, config.EOS_TABLE, t_fo, dims, mins, maxs, tab)
+    cfg = dict(config.lumpy_params(), particlization=1, e_freeze=eos.efreeze(t_fo))
+    ics = [ic.lumpy(seed=0xCCA4E001 + k, spacing=0.1, half_width=hw, e_cutoff_gev=0.15)
+           for k, hw in enumerate((5.0, 3.0, 4.0))]
+    arr = serve.poisson_arrivals(3, 20.0, seed=11)
+    reqs = [serve.Request(rid=k, ic=d, params=cfg, arrival=float(a)) for k, (d, a) in enumerate(zip(ics, arr))]
+    res = serve.BatchServer(dev, eos, cfg, slots=2, max_events=2).run(reqs)
+    assert [r.rid for r in res] == [0, 1, 2]
+    for r, d in zip(res, ics):
+        ev = ccake_amd.Event(dev, eos, cfg, len(d["x"]))
+        ccake_amd.load_event(ev, d)
+        ev.initialize()
+        k = 0
+        while ev.continue_evolution(max_tau=30.0):
+            ev.advance_timestep(cfg["dt"], 2)
+            k += 1
+        ra = ev.freeze_out_surface()
+        ev.close()
+        assert r.steps == k, (r.rid, r.steps, k)
+        assert r.rows.shape == ra.shape and np.array_equal(r.rows[:, :2], ra[:, :2]), r.rid
+        for c in range(2, ra.shape[1]):
+            s = float(np.max(np.abs(ra[:, c]))) or 1.0
+            assert float(np.max(np.abs(r.rows[:, c] - ra[:, c]))) / s <= 1e-9, (r.rid, c)
+        assert r.finished >= r.admitted >= r.arrival - 1e-3
+    eos.close()
+    dev.close()
