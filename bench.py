@@ -56,6 +56,12 @@ FLOP_PER_PAIR_SMOOTH = {True: 28, False: 19}               # charges on / off
 # Gradients, per directed pair (2D): 95 with shear on, 62 with it off; the
 # bulk-free variant (no grad Pi sums: an ideal fluid with zeta = 0) skips 4.
 FLOP_PER_PAIR_GRAD = {(True, True): 95, (True, False): 91, (False, True): 62, (False, False): 58}
+# 3+1D (D = 3, isotropic kernel), per directed pair: smoothing 37 (offsets 3,
+# distance 5, q and 1/q 9, spline 10, five accumulations 10); gradients 99
+# with shear on (offsets 3, distance 5, q and 1/q 9, spline gradient 9,
+# weight 4, 21 accumulations 39, shear 30), 69 with it off.
+FLOP_PER_PAIR_SMOOTH3 = 37
+FLOP_PER_PAIR_GRAD3 = {True: 99, False: 69}
 
 
 def parse():
@@ -75,6 +81,7 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the served multi-event measurement")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-event measurement")
+    ap.add_argument("--no-milne", action="store_true", help="skip the 3+1D Milne event and served 3+1D stream")
     ap.add_argument("--batch-events", type=int, default=16)
     ap.add_argument("--serve-events", type=int, default=24)
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -381,6 +388,141 @@ def batched_run(dev, eos, cfg, k, seed=0xC3):
             r["freeze_out_error_vs_fp64"].update({f + "_whole": float(f"{max(e[f]['whole'] for e in errs):.3g}")
                                                   for f in ("rho_B", "rho_S", "rho_Q")})
         out[name] = r
+    out["served_stream"] = served_batch_stream(dev, eos, cfg, ics, tune, out["plain"])
+    return out
+
+
+def served_batch_stream(dev, eos, cfg, ics, tune, plain, rate=50.0, slots=2, window=0.05):
+    """The same events as a Poisson stream (rate events/s) served by the
+    continuous-batching loop (serve.BatchServer: each free worker forms a
+    batch of the requests that have arrived, every event returned as soon as
+    it has frozen out), in FP64 and mixed precision, against the plain path's
+    events/s (each event alone, one after the other)."""
+    import numpy as np
+    from ccake_amd import serve
+    arr = serve.poisson_arrivals(len(ics), rate)
+    out = dict(workload=f"the batch's {len(ics)} events arriving as a Poisson stream at {rate:g} events/s, "
+                        f"{slots} workers forming batches (window {window * 1e3:g} ms), each event returned "
+                        "when it has frozen out", arrivals_span_s=round(float(arr[-1]), 3))
+    for name, pp in (("fp64", 0), ("mixed", 1)):
+        pcfg = dict(cfg, cells_per_rc=tune["cells_per_rc"], pair_precision=pp)
+        reqs = [serve.Request(rid=k, ic=d, params=pcfg, arrival=float(a)) for k, (d, a) in enumerate(zip(ics, arr))]
+        dev.sync()
+        t0 = time.perf_counter()
+        res = serve.BatchServer(dev, eos, pcfg, slots=slots, max_events=len(ics), slices=tune["slices"],
+                                window=window).run(reqs)
+        wall = time.perf_counter() - t0
+        lat = np.array([r.latency for r in res]) * 1e3
+        out[name] = dict(events_per_s=round(len(res) / wall, 4), wall_s=round(wall, 3),
+                         p50_ms=round(float(np.percentile(lat, 50)), 1), p99_ms=round(float(np.percentile(lat, 99)), 1),
+                         batches=len({r.batch for r in res}),
+                         speedup_vs_plain=round(len(res) / wall / plain["events_per_s"], 3))
+    return out
+
+
+def milne_workload(half_width=3.0, center_sd=1.0, seed=0xCCA4E001):
+    """3+1D lumpy event in Milne coordinates: 0.1 fm transverse lattice over
+    [-L, L]^2, eta_s in [-4, 4] at 0.1, 20 blobs (centres N(0, center_sd))
+    times a longitudinal Gaussian plateau (flat over |eta_s| <= 1, width
+    0.5), points above 0.15 GeV/fm^3; table EoS, viscous BSQ as the lumpy
+    2+1D event; isotropic kernel hT = 0.3 in (x, y, eta_s); two cells per
+    cutoff radius (measured fastest: 5.38 ms per step against 5.94 with 3)."""
+    from ccake_amd import config, ic
+    d = ic.milne_lumpy(seed=seed, spacing=0.1, half_width=half_width, eta_spacing=0.1, eta_max=4.0,
+                       center_sd=center_sd, e_cutoff_gev=0.15)
+    cfg = config.lumpy_params(dim=3, hEta=0.3, cells_per_rc=2)
+    return d, cfg
+
+
+def milne_run(dev, eos, steps=20, warmup=2):
+    """One 3+1D event: timed RK2 steps (particle-steps/s, per-stage times,
+    the gradient kernel's roofline) and the whole event from initialize to
+    complete freeze-out."""
+    import ccake_amd
+    from ccake_amd import config
+    d, cfg = milne_workload()
+    T_FO = 150.0 / config.HBARC_MEVFM
+    cfg = dict(cfg, e_freeze=eos.efreeze(T_FO))
+    n = len(d["x"])
+    ev = ccake_amd.Event(dev, eos, cfg, n)
+    ccake_amd.load_event(ev, d)
+    ev.initialize()
+    ev.run_steps(cfg["dt"], warmup)
+    dev.sync()
+    dev.profile(True)
+    dev.profile_reset()
+    ev.pair_total(reset=True)
+    dev.sync()
+    t0 = time.perf_counter()
+    done = ev.run_steps(cfg["dt"], steps)
+    dev.sync()
+    sec = time.perf_counter() - t0
+    dev.profile(False)
+    prof = {}
+    for k in ccake_amd.KERNELS:
+        if k != "CCAKE_NUM_KERNELS":
+            ms, cnt = dev.profile_read(k)
+            prof[k[len("CCAKE_K_"):]] = (ms, cnt)
+    pairs = ev.pair_total() / max(prof["SMOOTH"][1], 1)
+    ev.close()
+    gms = prof["GRAD"][0] / max(prof["GRAD"][1], 1)
+    sms = prof["SMOOTH"][0] / max(prof["SMOOTH"][1], 1)
+    fg = FLOP_PER_PAIR_GRAD3[bool(cfg["using_shear"])]
+    ach = fg * pairs / (gms * 1e-3) / 1e12
+    out = dict(workload="3+1D lumpy viscous BSQ event in Milne coordinates (tau, x, y, eta_s): 0.1 fm lattice over "
+                        "[-3, 3]^2, eta_s in [-4, 4] at 0.1, Gaussian plateau (flat |eta_s| <= 1, width 0.5), "
+                        "e > 0.15 GeV/fm^3, table EoS, isotropic 3D kernel hT = 0.3",
+               particles=n, steps=done, ms_per_step=round(sec / max(done, 1) * 1e3, 4),
+               particle_steps_per_s=round(n * done / sec, 1), pairs_per_stage=int(pairs),
+               neighbours_per_particle=round(pairs / n, 1),
+               stage_ms={k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items() if v[1]},
+               roofline=dict(kernel="k3_grad_slice", bound="valu", achieved=round(ach, 3), peak=PEAK_FP64_TFLOPS,
+                             unit="TFLOP/s", frac=round(ach / PEAK_FP64_TFLOPS, 4),
+                             work=f"{fg} flop/pair x {int(pairs)} pairs per launch", avg_launch_ms=round(gms, 4),
+                             smoothing_frac=round(FLOP_PER_PAIR_SMOOTH3 * pairs / (sms * 1e-3) / 1e12
+                                                  / PEAK_FP64_TFLOPS, 4)))
+    out["event_to_freeze_out"] = whole_event(dev, eos, d, cfg)
+    return out
+
+
+def milne_stream(dev, eos, k=8, rate=50.0, slots=2, seed=0x3D5):
+    """Served 3+1D stream: k Milne lumpy events with lognormal sizes (the
+    transverse half-width L ~ LogNormal(ln 3 fm, 0.35) in [2, 5] fm, blob
+    centres N(0, L/3)), arriving as a Poisson stream at `rate` events/s,
+    each run to complete freeze-out; served `slots` events at a time (one
+    stream each, admitted as slots free up) against one at a time.  The
+    served results must equal the sequential ones bit for bit."""
+    import numpy as np
+    from ccake_amd import config, ic, serve
+    rng = np.random.default_rng(seed)
+    T_FO = 150.0 / config.HBARC_MEVFM
+    arr = serve.poisson_arrivals(k, rate, seed=seed)
+    reqs = []
+    for rid in range(k):
+        hw = float(np.clip(np.exp(rng.normal(np.log(3.0), 0.35)), 2.0, 5.0))
+        hw = round(hw * 10) / 10
+        d = ic.milne_lumpy(seed=0xCCA4E001 + rid, spacing=0.1, half_width=hw, eta_spacing=0.1, eta_max=4.0,
+                           center_sd=hw / 3.0, e_cutoff_gev=0.15)
+        cfg = config.lumpy_params(dim=3, hEta=0.3, cells_per_rc=2, e_freeze=eos.efreeze(T_FO))
+        reqs.append(serve.Request(rid=rid, ic=d, params=cfg, arrival=float(arr[rid]), max_tau=30.0,
+                                  fields=("CCAKE_F_T", "CCAKE_F_E", "CCAKE_F_UX", "CCAKE_F_UY", "CCAKE_F_UETA")))
+    n = [len(r.ic["x"]) for r in reqs]
+    out = dict(workload=f"{k} 3+1D Milne lumpy events (0.1 fm and 0.1 in eta_s, eta_s in [-4, 4], lognormal "
+                        f"transverse sizes), Poisson arrivals at {rate:g} events/s, each run to freeze-out",
+               particles_min=min(n), particles_max=max(n), particles_median=int(np.median(n)),
+               particles_total=int(sum(n)))
+    runs = {}
+    for name, sl in (("sequential", 1), ("served", slots)):
+        dev.sync()
+        t0 = time.perf_counter()
+        res = serve.Server(dev, eos, slots=sl).run(reqs)
+        wall = time.perf_counter() - t0
+        runs[name] = res
+        out[name] = dict(slots=sl, **{kk: round(v, 4) if isinstance(v, float) else v
+                                      for kk, v in serve.summarize(res, wall).items()})
+    out["results_equal"] = bool(all(np.array_equal(a.fields[f], b.fields[f]) for a, b in
+                                    zip(runs["sequential"], runs["served"]) for f in a.fields))
+    out["speedup"] = round(out["served"]["events_per_s"] / out["sequential"]["events_per_s"], 3)
     return out
 
 
@@ -506,6 +648,13 @@ def main():
         print(f"bench: batch of {args.batch_events} lognormal lumpy events", file=sys.stderr, flush=True)
         out["batching"] = batched_run(dev, eos, cfg, args.batch_events)
         print(f"bench: batching {json.dumps(out['batching'])}", file=sys.stderr, flush=True)
+    if not args.no_milne and table is not None:
+        print("bench: 3+1D Milne event", file=sys.stderr, flush=True)
+        out["milne_3d"] = milne_run(dev, eos)
+        print(f"bench: milne {json.dumps(out['milne_3d'])}", file=sys.stderr, flush=True)
+        print("bench: served 3+1D stream", file=sys.stderr, flush=True)
+        out["milne_3d"]["served_stream"] = milne_stream(dev, eos)
+        print(f"bench: milne stream {json.dumps(out['milne_3d']['served_stream'])}", file=sys.stderr, flush=True)
     eos.close()
     if not args.no_serve:
         print(f"bench: serving {args.serve_events} events, {args.serve_slots} slots", file=sys.stderr, flush=True)

@@ -71,14 +71,42 @@ BIN = os.path.join(HERE, "bin")
 HOST_PROGRAMS = ["gubser_run", "ic_dump", "bsqhydro_run"]
 
 
+HDF5_PREFIX = os.environ.get("CCAKE_HDF5_PREFIX", "/opt/conda")
+
+
+HDF5_LIB = os.path.join(HERE, "libccake_hdf5.so")
+
+
+def build_hdf5_shim():
+    """libccake_hdf5.so: the HDF5 input and output of the C++ call surface
+    (host/ccake_hdf5.cpp) on the HDF5 C library, which only this library
+    links (with a run-time path to HDF5's directory): a program linking it
+    keeps finding its C++ runtime in the system directories.  Returns the
+    compile and link flags for the programs, or none when HDF5 is not
+    installed."""
+    inc, lib = os.path.join(HDF5_PREFIX, "include"), os.path.join(HDF5_PREFIX, "lib")
+    if not (os.path.exists(os.path.join(inc, "hdf5.h")) and os.path.exists(os.path.join(lib, "libhdf5.so"))):
+        return [], []
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-shared", "-fPIC", "-I", HOST, "-I", inc,
+           os.path.join(HOST, "ccake_hdf5.cpp"), "-o", HDF5_LIB + ".tmp", os.path.join(lib, "libhdf5.so"),
+           "-Wl,-rpath," + lib]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed on ccake_hdf5.cpp:\n" + r.stdout + r.stderr)
+    os.replace(HDF5_LIB + ".tmp", HDF5_LIB)
+    return ["-DCCAKE_WITH_HDF5"], ["-lccake_hdf5"]
+
+
 def build_host_programs():
     """C++ drivers of the host call surface (ccake_amd/host), linked against
-    the library with an rpath to it."""
+    the library (and libccake_hdf5.so when HDF5 is installed) with an rpath
+    to them."""
     os.makedirs(BIN, exist_ok=True)
+    h5c, h5l = build_hdf5_shim()
     for prog in HOST_PROGRAMS:
         out = os.path.join(BIN, prog)
-        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", HOST,
-               os.path.join(HOST, prog + ".cpp"), "-o", out + ".tmp", "-L", HERE, "-lccake_amd",
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", HOST] + h5c + [
+               os.path.join(HOST, prog + ".cpp"), "-o", out + ".tmp", "-L", HERE, "-lccake_amd"] + h5l + [
                "-Wl,-rpath,$ORIGIN/.."]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:

@@ -117,3 +117,32 @@ def preinverted_conformal(Ns=600, NB=4, s_range=(1e-3, 1e4), nB_range=(1e-4, 1e-
     fields = dict(T=T2, muB=z, muS=z, muQ=z, e=3.0 * c * T2 ** 4, p=c * T2 ** 4, cs2=np.full_like(T2, 1.0 / 3.0),
                   chiTT=12.0 * c * T2 ** 2, chiTB=z, chiBB=2.0 * c * T2 ** 2)
     return preinverted_flat(fields, s, nB)
+
+
+def load_flat_table_h5(path):
+    """The reference's flat (s, nB) pre-inverted table file (HDF5: /dimensions
+    = [Ns, NB], /data/{s, nB, T, muB, muS, muQ, e, p, cs2, chiTT, chiTB,
+    chiBB}, MeV units; EoS_Interpolator::load_flat_table,
+    src/eos_interpolator.cpp:291-527) read through libccake_hdf5.so:
+    (table, (Ns, NB)) with the 12 arrays concatenated in that order, as
+    ccake_amd.Eos takes them for a pre-inverted EoS."""
+    import ctypes as C
+    import os
+    lib_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libccake_hdf5.so")
+    if not os.path.exists(lib_path):
+        raise RuntimeError("libccake_hdf5.so is not built (no HDF5 library at build time)")
+    L = C.CDLL(lib_path)
+    f = L.ccake_h5_load_flat_table
+    f.restype = C.c_int
+    f.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                  C.c_char_p, C.c_size_t]
+    ns, nb = C.c_int(), C.c_int()
+    err = C.create_string_buffer(512)
+    st = f(path.encode(), None, 0, C.byref(ns), C.byref(nb), err, 512)
+    if st == 1:
+        raise ValueError(err.value.decode())
+    out = np.zeros(12 * ns.value * nb.value)
+    st = f(path.encode(), out.ctypes.data_as(C.POINTER(C.c_double)), out.size, C.byref(ns), C.byref(nb), err, 512)
+    if st != 0:
+        raise ValueError(err.value.decode() or f"status {st}")
+    return out, (ns.value, nb.value)

@@ -15,6 +15,9 @@
 #pragma once
 
 #include <cmath>
+#ifdef CCAKE_WITH_HDF5
+#include "ccake_hdf5.hpp"
+#endif
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -506,12 +509,14 @@ class FreezeOut {
   std::shared_ptr<SystemState<D>> systemPtr;
 };
 
-// Output<D, TEOM> (src/output.cpp): the text outputs of the run.
+// Output<D, TEOM> (src/output.cpp): the outputs of the run.
 //  - print_system_state (63-83) downloads the particles and writes
 //    system_state_<k>.dat (print_system_state_to_txt, 102-172) when
-//    txt_evolution is set; k counts the calls.  HDF5 output (hdf_evolution)
-//    needs an HDF5 library, which this library does not link: asking for it
-//    fails at construction.
+//    txt_evolution is set and a frame of output_directory/system_state.h5
+//    (print_system_state_to_HDF, 213-266, ccake_hdf5.hpp) when
+//    hdf_evolution is set; k counts the calls.  HDF5 output needs a program
+//    built with CCAKE_WITH_HDF5 and linked to libhdf5; without it asking for
+//    hdf_evolution fails at construction.
 //  - freeze-out rows are buffered as text at every print_freeze_out and
 //    written to output_directory/freeze_out.dat by flush_freeze_out
 //    (412-498), after a header line "# B S Q" with the totals passed at the
@@ -526,16 +531,63 @@ class Output {
   Output(std::shared_ptr<Settings> s, std::shared_ptr<SystemState<D>> sys)
     : output_directory(s->results_directory), settingsPtr(std::move(s)), systemPtr(std::move(sys))
   {
-    if (settingsPtr->hdf_evolution)
-      check(CCAKE_ERR_UNSUPPORTED, "Output (hdf_evolution: HDF5 output is not built)", settingsPtr.get());
+    if (settingsPtr->hdf_evolution) {
+#ifdef CCAKE_WITH_HDF5
+      // the run's attributes (src/output.cpp:33-47)
+      hdf5_file.initialize(output_directory + "/system_state.h5", { settingsPtr->hT, settingsPtr->e_cutoff },
+                           { "h", "e_cutoff" });
+#else
+      check(CCAKE_ERR_UNSUPPORTED, "Output (hdf_evolution: this program is built without HDF5)", settingsPtr.get());
+#endif
+    }
   }
 
   void print_system_state()
   {
     systemPtr->copy_device_to_host();
     if (settingsPtr->txt_evolution) print_system_state_to_txt();
+#ifdef CCAKE_WITH_HDF5
+    if (settingsPtr->hdf_evolution) print_system_state_to_HDF();
+#endif
     n_timesteps_output++;
   }
+
+#ifdef CCAKE_WITH_HDF5
+  // One frame: x, y (fm), T, mu (MeV), e (MeV/fm^3), s, B, S, Q (1/fm^3) and
+  // each particle's EoS tag, by particle id (src/output.cpp:213-266).
+  void print_system_state_to_HDF()
+  {
+    constexpr double hc = 197.3269804;        // include/constants.h:16
+    const int width = (int)std::ceil(std::log10(std::ceil(settingsPtr->max_tau / settingsPtr->dt)));
+    static const std::vector<std::string> names = { "x", "y", "T", "muB", "muS", "muQ", "e", "s", "B", "S", "Q" };
+    static const std::vector<std::string> units = { "fm", "fm", "MeV", "MeV", "MeV", "MeV", "MeV/fm^3", "1/fm^3",
+                                                    "1/fm^3", "1/fm^3", "1/fm^3" };
+    const SystemState<D>& sys = *systemPtr;
+    const size_t n = sys.particles.size();
+    std::vector<std::vector<double>> data(names.size(), std::vector<double>(n));
+    std::vector<int> labels(n);
+    for (size_t id = 0; id < n; ++id) {
+      const Particle<D>& p = sys.particles[id];
+      data[0][id] = p.r[0];
+      data[1][id] = p.r[1];
+      data[2][id] = p.T() * hc;
+      data[3][id] = p.muB * hc;
+      data[4][id] = p.muS * hc;
+      data[5][id] = p.muQ * hc;
+      data[6][id] = p.e() * hc;
+      data[7][id] = p.s;
+      data[8][id] = p.rhoB;
+      data[9][id] = p.rhoS;
+      data[10][id] = p.rhoQ;
+      // the reference's map of EoS names to tags; a name outside it is
+      // inserted with tag 0 by std::map::operator[]
+      const std::string nm = p.get_current_eos_name();
+      labels[id] = nm == "tanh_conformal" ? 1 : nm == "conformal" ? 2 : nm == "conformal_diagonal" ? 3 : 0;
+    }
+    hdf5_file.output_dataset(names, units, data, width, n_timesteps_output, labels, { sys.t, -10.0, -10.0 },
+                             { "Time", "e_2_X", "e_2_P" });
+  }
+#endif
 
   void print_system_state_to_txt()
   {
@@ -608,6 +660,9 @@ class Output {
   std::ostringstream fo_buffer;
   size_t rows_buffered = 0;
   int fo_particles_buffered = 0;
+#ifdef CCAKE_WITH_HDF5
+  Hdf5SystemState hdf5_file;
+#endif
 };
 
 // SPHWorkstation<D, TEOM> (src/sph_workstation.cpp).

@@ -10,6 +10,8 @@
 // src/BSQHydro.cpp:797) and written to OUTPUT/freeze_out.dat at the end (859-860).
 // With --ic FILE the particles come from a ccake-format initial-condition
 // file (BSQHydro::read_ccake) instead of the built-in lattice; with
+// --hdf-evolution K a frame of OUTPUT/system_state.h5 every K steps (and at the
+//                  start; with --txt-evolution the text files share its stride)
 // --txt-evolution K the system state is written to OUTPUT/system_state_<k>.dat
 // after initialisation and every K steps (src/BSQHydro.cpp:689-691, 791-796).
 //
@@ -18,7 +20,7 @@
 //
 //   gubser_run [--spacing 0.05] [--steps 120] [--dt 0.005] [--evolver-steps 0] [--rk 2]
 //              [--freeze-out-T MeV] [--ic FILE --e-cutoff GeV/fm^3]
-//              [--txt-evolution K] [--output DIR]
+//              [--txt-evolution K] [--hdf-evolution K] [--output DIR]
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -80,7 +82,7 @@ int main(int argc, char** argv)
   double a = 0.05;
   int steps = 120, evolver_steps = 0, rk = 2;
   double dt = 0.005, t_fo_mev = 0.0, e_cutoff = 0.15;
-  int txt_stride = 0;
+  int txt_stride = 0, hdf_stride = 0;
   std::string out_dir = ".", ic_file;
   for (int k = 1; k + 1 < argc; k += 2) {
     if (!std::strcmp(argv[k], "--spacing")) a = std::atof(argv[k + 1]);
@@ -92,6 +94,7 @@ int main(int argc, char** argv)
     else if (!std::strcmp(argv[k], "--ic")) ic_file = argv[k + 1];
     else if (!std::strcmp(argv[k], "--e-cutoff")) e_cutoff = std::atof(argv[k + 1]);
     else if (!std::strcmp(argv[k], "--txt-evolution")) txt_stride = std::atoi(argv[k + 1]);
+    else if (!std::strcmp(argv[k], "--hdf-evolution")) hdf_stride = std::atoi(argv[k + 1]);
     else if (!std::strcmp(argv[k], "--rk")) rk = std::atoi(argv[k + 1]);
   }
   auto s = std::make_shared<ccake::Settings>();
@@ -104,6 +107,8 @@ int main(int argc, char** argv)
   s->zeta_mode = CCAKE_ZETA_CONSTANT; s->constant_zeta_over_s = 0.0;
   s->results_directory = out_dir;
   s->txt_evolution = txt_stride > 0;
+  s->hdf_evolution = hdf_stride > 0;
+  const int out_stride = txt_stride > 0 ? txt_stride : hdf_stride;
   s->IC_file = ic_file;
   s->e_cutoff = e_cutoff;
 
@@ -119,10 +124,10 @@ int main(int argc, char** argv)
     ws.initialize();
     ws.initialize_hydrodynamics();
     ccake::Output<2, ccake::EoM_default> out(s, sys);
-    if (s->txt_evolution) out.print_system_state();
+    if (out_stride > 0) out.print_system_state();
     for (int k = 0; k < steps; ++k) {
       ws.advance_timestep(s->dt, s->rk_order);
-      if (s->txt_evolution && (k + 1) % txt_stride == 0) out.print_system_state();
+      if (out_stride > 0 && (k + 1) % out_stride == 0) out.print_system_state();
       if (s->particlization_enabled)
         out.print_freeze_out(ws.freezePtr, sys->Btotal0, sys->Qtotal0, sys->Stotal0);
     }

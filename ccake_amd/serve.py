@@ -188,12 +188,13 @@ class BatchServer:
     particles are frozen out (or tau passes max_tau).  An event's
     hypersurface is complete at that step and is returned right away, while
     the batch goes on with its other events; requests that arrive meanwhile
-    wait for the next free worker.  Every request shares `params` (the
+    wait for the next free worker, which waits up to `window` seconds after
+    the first of them arrived for more.  Every request shares `params` (the
     workload's settings, particlization on).  An event's rows equal the same
     event's run alone to rounding (test_serve.py)."""
 
     def __init__(self, dev, eos, params, slots=2, max_events=64, max_particles=4_000_000, max_tau=30.0,
-                 slices=None):
+                 slices=None, window=0.0):
         if slots < 1 or max_events < 1:
             raise ValueError("slots and max_events must be >= 1")
         if not params.get("particlization", 0):
@@ -202,13 +203,16 @@ class BatchServer:
         self.slots, self.max_events, self.max_particles = int(slots), int(max_events), int(max_particles)
         self.max_tau = float(max_tau)
         self.slices = slices
+        # a free worker waits up to `window` seconds after the first waiting
+        # request's arrival for more arrivals before it forms the batch
+        self.window = float(window)
 
     def _run_batch(self, reqs, admitted, now, bid, publish):
         ics = [r.ic for r in reqs]
         b = ccake_amd.Batch(self.dev, self.eos, self.params, [len(d["x"]) for d in ics])
         try:
             if self.slices:
-                b.nslices(self.slices)
+                b.nslices(min(self.slices, b.nslices()))    # at most the count the batch was sized for
             ccake_amd.load_event(b, ccake_amd.concat_ics(ics))
             b.initialize()
             done = [False] * len(reqs)
@@ -252,7 +256,9 @@ class BatchServer:
                 while True:
                     if state["error"] is not None or state["next"] >= len(pending):
                         return None
-                    wait = pending[state["next"]].arrival - now()
+                    first = pending[state["next"]].arrival
+                    last = pending[min(state["next"] + self.max_events, len(pending)) - 1].arrival
+                    wait = min(first + self.window, last) - now()
                     if wait <= 0:
                         break
                     lock.wait(timeout=wait)

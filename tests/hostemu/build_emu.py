@@ -60,14 +60,14 @@ def build(asan=True):
         os.remove(o)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
-    if asan:
-        # the same library under the product's name, for the C++ drivers of
-        # ccake_amd/bin run with LD_LIBRARY_PATH pointing here
-        lib_dir = os.path.join(OUT_DIR, "lib")
-        os.makedirs(lib_dir, exist_ok=True)
-        ltmp = os.path.join(lib_dir, f"libccake_amd.so.{os.getpid()}.tmp")
-        shutil.copyfile(tmp, ltmp)
-        os.replace(ltmp, os.path.join(lib_dir, "libccake_amd.so"))
+    # the same library under the product's name, for the C++ drivers of
+    # ccake_amd/bin run with LD_LIBRARY_PATH pointing here (lib, or
+    # lib_noasan for the build without AddressSanitizer)
+    lib_dir = os.path.join(OUT_DIR, "lib" if asan else "lib_noasan")
+    os.makedirs(lib_dir, exist_ok=True)
+    ltmp = os.path.join(lib_dir, f"libccake_amd.so.{os.getpid()}.tmp")
+    shutil.copyfile(tmp, ltmp)
+    os.replace(ltmp, os.path.join(lib_dir, "libccake_amd.so"))
     os.replace(tmp, out)
     return out
 
