@@ -78,11 +78,17 @@ def parse():
     ap.add_argument("--cells-per-rc", type=int, default=4,
                     help="neighbour-grid cells per cutoff radius (4 measured fastest on the 200k lumpy event: "
                          "6.29 ms per step against 6.56 with 3)")
+    ap.add_argument("--x-subcells", type=int, default=8,
+                    help="x cells per cell of a key row (8: measured fastest on the 200k lumpy event)")
     ap.add_argument("--no-serve", action="store_true", help="skip the served multi-event measurement")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-event measurement")
     ap.add_argument("--no-milne", action="store_true", help="skip the 3+1D Milne event and served 3+1D stream")
+    ap.add_argument("--no-1m", action="store_true", help="skip the 0.01 fm lattice event (FP64 and mixed)")
     ap.add_argument("--batch-events", type=int, default=16)
+    ap.add_argument("--large-batch", type=int, default=0,
+                    help="also run a batch of this many lognormal lumpy events in mixed precision (512: "
+                         "about 40M particles, several minutes)")
     ap.add_argument("--serve-events", type=int, default=24)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -281,7 +287,6 @@ def mixed_precision(dev, eos, d, cfg, warmup, steps, fp64_ms, fp64_event, fp64_r
     step time, the whole event to freeze-out, and the per-field error of its
     freeze-out hypersurface against the FP64 run's (rows matched by particle
     id; ccake_amd.precision gives the measures)."""
-    from ccake_amd import precision
     mcfg = dict(cfg, pair_precision=1)
     ms = timed_steps(dev, eos, d, mcfg, warmup, steps)
     out = dict(ms_per_step=round(ms, 4), particle_steps_per_s=round(len(d["x"]) / ms * 1e3, 1),
@@ -291,14 +296,83 @@ def mixed_precision(dev, eos, d, cfg, warmup, steps, fp64_ms, fp64_event, fp64_r
         ev = whole_event(dev, eos, d, mcfg, rows=rows)
         out["event_to_freeze_out"] = ev
         out["event_speedup_vs_fp64"] = round(fp64_event["seconds"] / ev["seconds"], 3)
-        err = precision.surface_errors(fp64_rows[0], rows[0])
-        keep = ("T", "u_mu", "rho_B", "rho_S", "rho_Q", "s", "e", "pi_11", "pi_22", "pi_12", "Pi")
-        out["freeze_out_error_vs_fp64"] = dict(
-            rows=err["rows_ref"], same_step=err["same_step"], same_snapshot=err["same_snapshot"],
-            measure="per particle: relative (signed fields against 1e-3 of their largest magnitude); "
-                    "u_mu: |du^mu| / u^0; whole: max|d| / max|x|",
-            fields={k: {m: float(f"{v:.3g}") for m, v in err["fields"][k].items()} for k in keep})
+        out["freeze_out_error_vs_fp64"] = freeze_out_errors(fp64_rows[0], rows[0])
     return out
+
+
+def freeze_out_errors(ref_rows, rows):
+    """Per-field error of a freeze-out hypersurface against the FP64 run's
+    (rows matched by particle id; ccake_amd.precision gives the measures)."""
+    from ccake_amd import precision
+    err = precision.surface_errors(ref_rows, rows)
+    keep = ("T", "u_mu", "rho_B", "rho_S", "rho_Q", "s", "e", "pi_11", "pi_22", "pi_12", "Pi")
+    return dict(
+        rows=err["rows_ref"], same_step=err["same_step"], same_snapshot=err["same_snapshot"],
+        measure="per particle: relative (signed fields against 1e-3 of their largest magnitude); "
+                "u_mu: |du^mu| / u^0; whole: max|d| / max|x|",
+        fields={k: {m: float(f"{v:.3g}") for m, v in err["fields"][k].items()} for k in keep})
+
+
+def lattice_1m(dev, eos, cfg):
+    """The lumpy event on the 0.01 fm lattice (1,002,001 points over
+    [-5, 5]^2, the same blobs, EoS and transport as the 0.02 fm event; h
+    stays 0.3 fm, so a particle has about 11,000 neighbours): FP64 and
+    mixed-precision pair sums, each from initialize to complete freeze-out,
+    and the per-field error of the mixed run's hypersurface against the
+    FP64 run's."""
+    d, _, _, wl = workload("lumpy1m")
+    n = len(d["x"])
+    out = dict(workload=wl["workload"], lattice_points=wl["lattice_points"], particles=n)
+    fp64_rows, rows = [], []
+    print(f"bench: 0.01 fm lattice, {n} particles, FP64 to freeze-out", file=sys.stderr, flush=True)
+    out["fp64"] = whole_event(dev, eos, d, cfg, rows=fp64_rows)
+    out["fp64"]["ms_per_step"] = round(out["fp64"]["seconds"] / max(out["fp64"]["steps"], 1) * 1e3, 3)
+    print(f"bench: 1m fp64 {json.dumps(out['fp64'])}; mixed precision to freeze-out", file=sys.stderr, flush=True)
+    out["mixed"] = whole_event(dev, eos, d, dict(cfg, pair_precision=1), rows=rows)
+    out["mixed"]["ms_per_step"] = round(out["mixed"]["seconds"] / max(out["mixed"]["steps"], 1) * 1e3, 3)
+    out["mixed"]["event_speedup_vs_fp64"] = round(out["fp64"]["seconds"] / out["mixed"]["seconds"], 3)
+    out["mixed"]["freeze_out_error_vs_fp64"] = freeze_out_errors(fp64_rows[0], rows[0])
+    return out
+
+
+def large_batch(dev, eos, cfg, k, sample=32, seed=0xC3):
+    """k lumpy events (lognormal sizes as in batched_run) in one batch with
+    the mixed-precision pair sums, run until every event has frozen out,
+    against the plain FP64 path (each event alone, one after the other)
+    timed on the first `sample` events of the same draw: the plain path's
+    events/s does not depend on how many events queue behind each other.
+    The cell size and slice count are tuned on the full batch."""
+    import numpy as np
+    from ccake_amd import batching, precision
+    print(f"bench: building {k} lumpy events", file=sys.stderr, flush=True)
+    ics = batching.lumpy_batch_ics(k, seed=seed)
+    n = [len(d["x"]) for d in ics]
+    ntot = int(sum(n))
+    print(f"bench: {ntot} particles; autotune", file=sys.stderr, flush=True)
+    tune = batching.autotune(dev, eos, dict(cfg, pair_precision=1), ics, cells_per_rc=(3, 4), slices=(1, 2, 4))
+    print(f"bench: autotune {json.dumps(tune)}; plain path on {sample} events", file=sys.stderr, flush=True)
+    alone = batching.run_alone(dev, eos, cfg, ics[:sample])
+    plain = batching.latency_summary(alone, int(sum(n[:sample])))
+    print(f"bench: plain {json.dumps(plain)}; batch", file=sys.stderr, flush=True)
+
+    def progress(steps, done):
+        print(f"bench: batch step {steps}, {done} of {k} events frozen out", file=sys.stderr, flush=True)
+    run = batching.run_batch(dev, eos, dict(cfg, cells_per_rc=tune["cells_per_rc"], pair_precision=1), ics,
+                             slices=tune["slices"], progress=progress)
+    r = batching.latency_summary(run, ntot)
+    r["batch_steps"] = run["steps"]
+    r["particle_steps_per_s"] = round(ntot * run["steps"] / run["wall_s"], 1)
+    r["speedup_vs_plain"] = round(r["events_per_s"] / plain["events_per_s"], 3)
+    errs = [precision.surface_errors(a, b)["fields"] for a, b in zip(alone["rows"], run["rows"][:sample])]
+    r["freeze_out_error_vs_fp64"] = {f: float(f"{max(e[f]['max'] for e in errs):.3g}") for f in ("T", "u_mu", "s", "e")}
+    r["freeze_out_error_vs_fp64"].update({f + "_whole": float(f"{max(e[f]['whole'] for e in errs):.3g}")
+                                          for f in ("rho_B", "rho_S", "rho_Q")})
+    r["freeze_out_error_events"] = f"the first {sample} events (their FP64 runs alone)"
+    return dict(workload=f"{k} viscous BSQ lumpy events on the 0.02 fm lattice, lognormal sizes (median 80,000 "
+                         "lattice points, [20,000, 300,000]), e > 0.15 GeV/fm^3, table EoS, mixed-precision pair "
+                         "sums, each run to freeze-out, all arriving at once",
+                particles_min=min(n), particles_max=max(n), particles_median=int(np.median(n)), autotune=tune,
+                plain_fp64_sample=plain, batched_mixed=r)
 
 
 def serving_requests(k, steps, seed=0x5E4F):
@@ -426,11 +500,13 @@ def milne_workload(half_width=3.0, center_sd=1.0, seed=0xCCA4E001):
     times a longitudinal Gaussian plateau (flat over |eta_s| <= 1, width
     0.5), points above 0.15 GeV/fm^3; table EoS, viscous BSQ as the lumpy
     2+1D event; isotropic kernel hT = 0.3 in (x, y, eta_s); two cells per
-    cutoff radius (measured fastest: 5.38 ms per step against 5.94 with 3)."""
+    cutoff radius and six x cells per cell (measured fastest: 4.64 ms per
+    step against 5.07 with cubic cells and 5.61 with three cells per
+    radius)."""
     from ccake_amd import config, ic
     d = ic.milne_lumpy(seed=seed, spacing=0.1, half_width=half_width, eta_spacing=0.1, eta_max=4.0,
                        center_sd=center_sd, e_cutoff_gev=0.15)
-    cfg = config.lumpy_params(dim=3, hEta=0.3, cells_per_rc=2)
+    cfg = config.lumpy_params(dim=3, hEta=0.3, cells_per_rc=2, x_subcells=6)
     return d, cfg
 
 
@@ -559,8 +635,9 @@ def main():
         cfg = dict(cfg, using_diffusion=1, kappa_over_T2=[0.1, 0.0, 0.0, 0.0, 0.1, 0.0, 0.0, 0.0, 0.1],
                    diffusion_regulator=1)
         wl = dict(wl, workload=wl["workload"] + ", BSQ diffusion (kappa/T^2 = 0.1 I, regulator on)")
-    cfg = dict(cfg, cells_per_rc=args.cells_per_rc)
+    cfg = dict(cfg, cells_per_rc=args.cells_per_rc, x_subcells=args.x_subcells)
     wl["cells_per_rc"] = args.cells_per_rc
+    wl["x_subcells"] = args.x_subcells
     n = len(d["x"])
     T_FO = 150.0 / config.HBARC_MEVFM
     print(f"bench: {wl['workload']}: {n} particles, warmup {args.warmup}, steps {args.steps}",
@@ -648,6 +725,12 @@ def main():
         print(f"bench: batch of {args.batch_events} lognormal lumpy events", file=sys.stderr, flush=True)
         out["batching"] = batched_run(dev, eos, cfg, args.batch_events)
         print(f"bench: batching {json.dumps(out['batching'])}", file=sys.stderr, flush=True)
+    if not args.no_1m and args.workload == "lumpy":
+        out["lattice_1m"] = lattice_1m(dev, eos, cfg)
+        print(f"bench: 1m {json.dumps(out['lattice_1m'])}", file=sys.stderr, flush=True)
+    if args.large_batch and args.workload == "lumpy":
+        out["large_batch"] = large_batch(dev, eos, cfg, args.large_batch)
+        print(f"bench: large batch {json.dumps(out['large_batch'])}", file=sys.stderr, flush=True)
     if not args.no_milne and table is not None:
         print("bench: 3+1D Milne event", file=sys.stderr, flush=True)
         out["milne_3d"] = milne_run(dev, eos)

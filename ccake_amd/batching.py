@@ -81,10 +81,12 @@ def autotune(dev, eos, cfg, ics, cells_per_rc=(2, 3, 4), slices=(4, 8, 12, 16), 
     return dict(cells_per_rc=best[0], slices=best[1], ms_per_step=round(best[2], 3), table=table)
 
 
-def run_batch(dev, eos, cfg, ics, slices=None, max_tau=30.0):
+def run_batch(dev, eos, cfg, ics, slices=None, max_tau=30.0, progress=None, every=50):
     """The events in one batch until every event has frozen out.  Returns
     the wall time, each event's completion time (the step after which its
-    continue flag turned false) and the batch's step count."""
+    continue flag turned false) and the batch's step count.  `progress`
+    (optional) is called as progress(steps, events_done) every `every`
+    batch steps."""
     b = make_batch(dev, eos, cfg, ics)
     if slices:
         b.nslices(slices)
@@ -103,6 +105,8 @@ def run_batch(dev, eos, cfg, ics, slices=None, max_tau=30.0):
             break
         b.advance_timestep(cfg["dt"], 2)
         steps += 1
+        if progress is not None and steps % every == 0:
+            progress(steps, sum(t is not None for t in done))
     wall = time.perf_counter() - t0
     rows = b.surface_by_event()
     b.close()

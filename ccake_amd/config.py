@@ -42,7 +42,7 @@ def base_params(**kw):
         using_diffusion=0, diffusion_mode=DIFF_KAPPA_DEFAULT, diffusion_relax_mode=TAUQ_DEFAULT,
         kappa_over_T2=[0.0] * 9, C_B=0.0,
         diffusion_regulator=0, regulator_chi0=10.0, regulator_e0=0.1, regulator_xi0=0.01, regulator_rq_max=1.0,
-        pair_precision=0, factorized_kernel=0,
+        pair_precision=0, factorized_kernel=0, x_subcells=8,
     )
     p.update(kw)
     # input.cpp:390-407: eta/s <= 1e-3 disables shear evolution

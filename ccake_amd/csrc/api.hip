@@ -701,7 +701,13 @@ static int event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, i
   // up to 128 x 128 cells per event in one key space (three passes)
   ev->cap_cells = nev == 1 ? 65536 : nev * kBatchCellsPerEvent;
   if (p->dim == 3) ev->cap_cells = 1 << 24;                   // (x, y, eta) cells: three passes
-  ev->sort_passes = ev->cap_cells <= 65536 ? 2 : 3;
+  {
+    // the largest key: ncells - 1 for one event; a batch adds the retired
+    // events' key, ncells (at most cap_cells)
+    const long long kmax = nev > 1 ? (long long)ev->cap_cells : (long long)ev->cap_cells - 1;
+    ev->sort_passes = kmax < (1LL << 16) ? 2 : (kmax < (1LL << 24) ? 3 : 4);
+  }
+  ev->n_live = n;
   ev->radix_blocks = (n + 1023) / 1024;
   {
     const hipError_t se = hipStreamCreateWithFlags(&ev->stream, hipStreamNonBlocking);
@@ -719,6 +725,7 @@ static int event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, i
     rc |= dalloc(ev, &ev->ev_off, (size_t)nev + 1);
     rc |= dalloc(ev, &ev->eid, np);
     rc |= dalloc(ev, &ev->red_ev, (size_t)nev);
+    rc |= dalloc(ev, &ev->live, (size_t)nev);
   }
   rc |= dalloc(ev, &ev->key, np);
   rc |= dalloc(ev, &ev->key_a, np);
@@ -779,6 +786,8 @@ static int event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, i
       for (int i = ev->h_off[e]; i < ev->h_off[e + 1]; ++i) eid[i] = e;
     CCAKE_HIP(hipMemcpyAsync(ev->ev_off, ev->h_off.data(), (nev + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
     CCAKE_HIP(hipMemcpyAsync(ev->eid, eid.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    ev->h_live.assign(nev, 1);
+    CCAKE_HIP(hipMemcpyAsync(ev->live, ev->h_live.data(), nev * sizeof(int32_t), hipMemcpyHostToDevice, st));
     CCAKE_HIP(hipStreamSynchronize(st));
   }
   CCAKE_HIP(hipStreamSynchronize(st));
@@ -819,6 +828,22 @@ int ccake_batch_reduce(ccake_event* ev, int which, double* out)
   return launch_reduce_events(ev, which, out);
 }
 
+// The live flags back to the device (every event live again after
+// process_initial_conditions; an ended event retired by the continue test).
+static int upload_live(ccake_event* ev)
+{
+  if (!ev->live) return 0;
+  int nl = 0;
+  for (int e = 0; e < ev->nev; ++e)
+    if (ev->h_live[e]) nl += ev->h_off[e + 1] - ev->h_off[e];
+  ev->n_live = nl;
+  ev->sort_valid = false;               // the retired particles move behind the live ones at the next sort
+  CCAKE_HIP(hipMemcpyAsync(ev->live, ev->h_live.data(), ev->nev * sizeof(int32_t), hipMemcpyHostToDevice,
+                           ev->stream));
+  CCAKE_HIP(hipStreamSynchronize(ev->stream));
+  return 0;
+}
+
 int ccake_batch_continue_evolution(ccake_event* ev, double max_tau, int* keep_going)
 {
   if (!ev || !keep_going) return CCAKE_ERR_INVALID;
@@ -827,12 +852,17 @@ int ccake_batch_continue_evolution(ccake_event* ev, double max_tau, int* keep_go
     const int rc = ccake_batch_reduce(ev, CCAKE_R_FROZEN, frozen.data());
     if (rc) return rc;
   }
+  bool retire = false;
   for (int e = 0; e < ev->nev; ++e) {
     const int ne = ev->h_off[e + 1] - ev->h_off[e];
     const int keep = ev->p.particlization ? ((int)(frozen[e] + 0.5) < ne) : 1;
     keep_going[e] = keep && (ev->t <= max_tau);
+    // an event that has ended stays ended: its frozen count cannot fall and
+    // tau only grows
+    if (ev->live && !keep_going[e] && ev->h_live[e]) { ev->h_live[e] = 0; retire = true; }
+    if (ev->live && !ev->h_live[e]) keep_going[e] = 0;
   }
-  return 0;
+  return retire ? upload_live(ev) : 0;
 }
 
 int ccake_event_destroy(ccake_event* ev)
@@ -842,7 +872,7 @@ int ccake_event_destroy(ccake_event* ev)
   void* bufs[] = { ev->F, ev->I, ev->grid, ev->bbox, ev->key, ev->key_a, ev->key_b, ev->val_a, ev->val_b, ev->hist,
                    ev->scan_tmp, ev->cell_off, ev->pair_count, ev->part, ev->hard, ev->hard_count, ev->k4, ev->cand_s, ev->cand_g, ev->red,
                    ev->err, ev->snaps, ev->fo_rows, ev->fo_count, ev->cand_d, ev->cand_s32, ev->cand_g32,
-                   ev->ev_off, ev->eid, ev->red_ev };
+                   ev->ev_off, ev->eid, ev->red_ev, ev->live };
   for (void* b : bufs) if (b) (void)hipFree(b);
   if (ev->stream) (void)hipStreamDestroy(ev->stream);
   {
@@ -1053,6 +1083,11 @@ int ccake_process_initial_conditions(ccake_event* ev)
   ev->fo_step = 0;
   ev->taup = ev->taupp = 0.0;
   ev->init_phase = 1;
+  if (ev->live) {                                           // a new run: every event live
+    ev->h_live.assign(ev->nev, 1);
+    const int rc = upload_live(ev);
+    if (rc) return rc;
+  }
   if (ev->snaps) {                                          // a new surface
     CCAKE_HIP(hipMemsetAsync(ev->snaps, 0, (size_t)6 * ev->snap_cols * ev->npad * sizeof(double), ev->stream));
     CCAKE_HIP(hipMemsetAsync(ev->fo_count, 0, sizeof(int), ev->stream));

@@ -47,8 +47,10 @@ static_assert(P_FRZ2_G == P_FRZ1_G + 1 && P_C_Q0 == P_FRZ2_G + 1 && NUM_COLUMNS 
 // (c_eta * nc[1] + c_y) * nc[0] + c_x; a 2+1D grid has nc[2] = 1.
 struct Grid {
   double lo[3];       // grid origin
-  double cell;        // cell edge
+  double cell;        // cell edge along y (and eta): one key row per y (eta) cell
   double inv_cell;
+  double cellx;       // cell edge along x: cell / x_subcells, so a row's candidate
+  double inv_cellx;   // runs are cut close to the cutoff disc without adding rows
   int nc[3];          // cells per axis
   int ncells;         // cells of the whole key space (cell_off holds ncells + 1 entries)
   int stride;         // cells per key row
@@ -197,6 +199,15 @@ struct ccake_event {
   // reductions
   double* red = nullptr;
   int32_t* err = nullptr;       // device error flags
+  // Retired events of a batch: an event whose continue flag has turned
+  // false (every particle frozen out, or tau past max_tau) has ended, as
+  // BSQHydro::run ends for an event alone.  Its particles get the key past
+  // every cell, so the cell sort places them after the live events'; the
+  // time step's sorted-order kernels stop at n_live and the per-id kernels
+  // that change state skip them, so an ended event keeps its final state.
+  int32_t* live = nullptr;      // device, per event: 1 live, 0 retired (batches only)
+  std::vector<int> h_live;
+  int n_live = 0;               // particles of the live events (n for a single event)
   int64_t last_pairs = 0;
   size_t bytes = 0;
 };
@@ -213,6 +224,20 @@ void prof_begin(ccake_event* ev, int k);
 void prof_end(ccake_event* ev, int k);
 
 inline double* col(ccake_event* ev, int f) { return ev->F + (size_t)f * ev->npad; }
+
+// The live flags of a batch's events for the per-id kernels; empty for a
+// single event.
+struct Live {
+  const int32_t* flag = nullptr;
+  const int32_t* eid = nullptr;
+  __device__ __forceinline__ bool retired(int i) const { return flag != nullptr && flag[eid[i]] == 0; }
+};
+inline Live live_of(const ccake_event* ev)
+{
+  Live l;
+  if (ev->live && ev->n_live < ev->n) { l.flag = ev->live; l.eid = ev->eid; }
+  return l;
+}
 
 // Search radius of the pair walks: rc with a rounding margin, widened by the
 // bound on how far particles have moved since the cell sort (a candidate's

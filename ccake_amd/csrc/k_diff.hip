@@ -213,7 +213,7 @@ DiffParams make_diff_params(const ccake_params& p)
 
 int launch_diff_prep(ccake_event* ev, int what, const int32_t* perm)
 {
-  const int n = ev->n;
+  const int n = perm ? ev->n_live : ev->n;   // sorted positions of the live events, or every particle id
   hipLaunchKernelGGL(k_diff_prep, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n, perm, ev->t,
                      make_diff_params(ev->p), ev->cand_d, what);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
@@ -227,7 +227,7 @@ static int diff_sum_dispatch(ccake_event* ev, double C)
     return CCAKE_ERR_INVALID;
   } else {
     if (ev->nslices != S) return diff_sum_dispatch<S + 1>(ev, C);
-    const int n = ev->n;
+    const int n = ev->n_live;
     hipLaunchKernelGGL((k_diff_sum<S>), dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->cand_d, ev->perm, n, C,
                        ev->part, ev->npad, ev->F);
     return 0;
@@ -236,7 +236,7 @@ static int diff_sum_dispatch(ccake_event* ev, double C)
 
 int launch_diff_grad(ccake_event* ev)
 {
-  const int n = ev->n;
+  const int n = ev->n_live;          // sorted positions of the live events' particles
   const double h = ev->p.hT;
   const double rc = 2.0 * h;
   const int ntiles = (n + kTile - 1) / kTile;

@@ -675,11 +675,13 @@ struct GradSum {
 template <bool SHEAR, bool RK, bool DIFF>
 __global__ __launch_bounds__(256) void k_eom(double* __restrict__ F, int npad, int n,
                                              const int32_t* __restrict__ freeze, double t, double use_vort,
-                                             double dt, int32_t* __restrict__ err, int cache, GradSum gs)
+                                             double dt, int32_t* __restrict__ err, int cache, GradSum gs,
+                                             Live live)
 {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   int i = k;
+  if (!gs.perm && live.retired(i)) return;      // an ended event of a batch keeps its state
   if (gs.perm) {
     i = gs.perm[k];
     grad_sum_into<SHEAR, 0>(gs.cand, k, i, gs.C, gs.part, gs.nslices, npad, F);
@@ -737,10 +739,10 @@ __global__ __launch_bounds__(256) void k_eom(double* __restrict__ F, int npad, i
 // Stand-alone RK update from stored derivatives.
 __global__ __launch_bounds__(256) void k_rk(double* __restrict__ F, int npad, int n,
                                             const int32_t* __restrict__ freeze, double dt, int32_t* __restrict__ err,
-                                            int diff)
+                                            int diff, Live live)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || live.retired(i)) return;
   const Cols col{ F, npad, i };
   double es = col(P_C_S) + dt * col(CCAKE_F_DEXT_S);
   if (es < 0.0 && freeze[i] > 3) es = 1.e-3;
@@ -762,10 +764,10 @@ __global__ __launch_bounds__(256) void k_rk(double* __restrict__ F, int npad, in
 }
 
 // Evolver::set_current_timestep_quantities (src/evolver.cpp:124-168)
-__global__ __launch_bounds__(256) void k_cache(double* __restrict__ F, int npad, int n, int diff)
+__global__ __launch_bounds__(256) void k_cache(double* __restrict__ F, int npad, int n, int diff, Live live)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || live.retired(i)) return;
   cache_state(Cols{ F, npad, i }, diff);
 }
 
@@ -780,10 +782,10 @@ enum { K4_UX = 0, K4_UY, K4_X, K4_Y, K4_S, K4_B, K4_SS, K4_Q, K4_BULK, K4_EZ, K4
        K4_NUM = K4_Q0 + 9 };
 
 __global__ __launch_bounds__(256) void k_rk4_update_k(double* __restrict__ F, int npad, int n, double* __restrict__ K,
-                                                      double w)
+                                                      double w, Live live)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || live.retired(i)) return;
   const Cols col{ F, npad, i };
   auto k = [&](int c) -> double& { return K[(size_t)c * npad + i]; };
   for (int q = 0; q < 5; ++q) k(K4_SHV0 + q) += w * col(CCAKE_F_DEXT_SHV0 + q);
@@ -798,10 +800,10 @@ __global__ __launch_bounds__(256) void k_rk4_update_k(double* __restrict__ F, in
 
 __global__ __launch_bounds__(256) void k_rk4_update(double* __restrict__ F, int npad, int n,
                                                     const double* __restrict__ K, const int32_t* __restrict__ freeze,
-                                                    double dt, int32_t* __restrict__ err)
+                                                    double dt, int32_t* __restrict__ err, Live live)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || live.retired(i)) return;
   const Cols col{ F, npad, i };
   auto k = [&](int c) { return K[(size_t)c * npad + i]; };
   double es = col(P_C_S) + dt * k(K4_S);
@@ -828,7 +830,8 @@ __global__ __launch_bounds__(256) void k_rk4_update(double* __restrict__ F, int 
 int launch_rk4_update_k(ccake_event* ev, double w)
 {
   const int n = ev->n;
-  hipLaunchKernelGGL(k_rk4_update_k, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n, ev->k4, w);
+  hipLaunchKernelGGL(k_rk4_update_k, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n, ev->k4, w,
+                     live_of(ev));
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
 
@@ -836,7 +839,7 @@ int launch_rk4_update(ccake_event* ev, double dt)
 {
   const int n = ev->n;
   hipLaunchKernelGGL(k_rk4_update, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n, ev->k4,
-                     ev->I + (size_t)CCAKE_I_FREEZE * ev->npad, dt, ev->err);
+                     ev->I + (size_t)CCAKE_I_FREEZE * ev->npad, dt, ev->err, live_of(ev));
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
 
@@ -858,7 +861,10 @@ int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0, bool fold_gra
       gs.pair_count = (unsigned long long*)ev->pair_count;
     }
   }
-  const int n = ev->n;
+  // sorted positions of the live events with the folded gradient sums,
+  // particle ids (ended events skipped by the kernel) without
+  const int n = fold_grad_sum ? ev->n_live : ev->n;
+  const Live live = live_of(ev);
   const dim3 g((n + 255) / 256), b(256);
   hipStream_t st = ev->stream;
   const int32_t* fz = ev->I + (size_t)CCAKE_I_FREEZE * ev->npad;
@@ -867,7 +873,7 @@ int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0, bool fold_gra
   // taken first and the update (the first stage of a step)
   const int cache = fuse_rk == 2 ? 1 : 0;
   auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, g, b, 0, st, ev->F, ev->npad, n, fz, ev->t, uv, dt, ev->err, cache, gs);
+    hipLaunchKernelGGL(kern, g, b, 0, st, ev->F, ev->npad, n, fz, ev->t, uv, dt, ev->err, cache, gs, live);
   };
   const int key = (ev->p.using_shear ? 4 : 0) | (fuse_rk ? 2 : 0) | (ev->p.using_diffusion ? 1 : 0);
   switch (key) {
@@ -889,7 +895,7 @@ int launch_rk(ccake_event* ev, double dt, double t0)
   if (ev->p.dim == 3) return launch3_rk(ev, dt);
   const int n = ev->n;
   hipLaunchKernelGGL(k_rk, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n,
-                     ev->I + (size_t)CCAKE_I_FREEZE * ev->npad, dt, ev->err, ev->p.using_diffusion);
+                     ev->I + (size_t)CCAKE_I_FREEZE * ev->npad, dt, ev->err, ev->p.using_diffusion, live_of(ev));
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
 
@@ -898,7 +904,7 @@ int launch_cache(ccake_event* ev)
   if (ev->p.dim == 3) return launch3_cache(ev);
   const int n = ev->n;
   hipLaunchKernelGGL(k_cache, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n,
-                     ev->p.using_diffusion);
+                     ev->p.using_diffusion, live_of(ev));
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
 

@@ -357,7 +357,7 @@ static int chain_kinds(const EosChain& ch)
 
 int launch_eos(ccake_event* ev, int mode, int fuse_post)
 {
-  const int n = ev->n;
+  const int n = ev->n_live;          // sorted positions of the live events' particles
   const dim3 g((n + kEosThreads - 1) / kEosThreads), b(kEosThreads);
   hipStream_t st = ev->stream;
   const Transport P = make_transport(ev->p);
@@ -521,7 +521,7 @@ int launch3_post(ccake_event* ev, int what)
 static int post(ccake_event* ev, int what, const int32_t* perm)
 {
   if (ev->p.dim == 3) return launch3_post(ev, what == 0 ? 1 : (what == 1 ? 2 : 4));
-  const int n = ev->n;
+  const int n = perm ? ev->n_live : ev->n;   // sorted positions of the live events, or every particle id
   hipLaunchKernelGGL(k_post, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n, perm, ev->t,
                      make_transport(ev->p), ev->cand_g, what);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;

@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void k_grad_sum(const double* __restrict__ can
 int launch_grad_pairs(ccake_event* ev)
 {
   if (ev->p.pair_precision == 1) return launch_grad_pairs32(ev);
-  const int n = ev->n;
+  const int n = ev->n_live;          // sorted positions of the live events' particles
   const double h = ev->p.hT;
   const double rc = 2.0 * h;
   const double hinv2 = 1.0 / (h * h);
@@ -128,7 +128,7 @@ static int grad_sum_dispatch(ccake_event* ev, double C)
     return CCAKE_ERR_INVALID;
   } else {
     if (ev->nslices != S) return grad_sum_dispatch<SHEAR, S + 1>(ev, C);
-    const int n = ev->n;
+    const int n = ev->n_live;
     hipLaunchKernelGGL((k_grad_sum<SHEAR, S>), dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->cand_g,
                        ev->perm, n, C, ev->part, ev->npad, ev->F);
     return 0;

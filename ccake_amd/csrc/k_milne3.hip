@@ -149,8 +149,8 @@ __device__ __forceinline__ void row_run3(const Grid& g, const int32_t* __restric
   const double d2 = dy * dy + dz * dz;
   if (d2 > rcs * rcs) return;
   const double hw = sqrt(fmax(rcs * rcs - d2, 0.0));
-  const int cx0 = cell_of(s.mn[0] - hw, g.lo[0], g.inv_cell, g.nc[0]);
-  const int cx1 = cell_of(s.mx[0] + hw, g.lo[0], g.inv_cell, g.nc[0]);
+  const int cx0 = cell_of(s.mn[0] - hw, g.lo[0], g.inv_cellx, g.nc[0]);
+  const int cx1 = cell_of(s.mx[0] + hw, g.lo[0], g.inv_cellx, g.nc[0]);
   const int kr = (rz * g.nc[1] + ry) * g.stride;
   b = cell_off[kr + cx0];
   e = cell_off[kr + cx1 + 1];
@@ -168,7 +168,10 @@ __device__ __forceinline__ void row_range3(const Grid& g, const Seg3& s, double 
 }
 
 // Slice `slice` of `nslices` of a segment's flattened candidate list, rows
-// resolved 64 at a time lane-parallel (as for_spans_slice in 2+1D).
+// resolved 64 at a time lane-parallel (as for_spans_slice in 2+1D).  The
+// body runs on every lane (wave-uniform control flow, so the compiler keeps
+// one copy of the accumulators): it masks the lanes outside the segment
+// (`in` false) itself.
 template <class Body>
 __device__ __forceinline__ void for_spans_slice3(const Grid& g, const int32_t* __restrict__ cell_off,
                                                  const Seg3& s, double rcs, int slice, int nslices, bool in,
@@ -200,7 +203,7 @@ __device__ __forceinline__ void for_spans_slice3(const Grid& g, const int32_t* _
       const int rlen = __builtin_amdgcn_readlane(len, r);
       const int t0 = lo > rbase ? lo - rbase : 0;
       const int t1 = hi - rbase < rlen ? hi - rbase : rlen;
-      if (t0 < t1 && in) body(rb + t0, rb + t1);
+      if (t0 < t1) body(rb + t0, rb + t1);
     }
     base += ctot;
   }
@@ -243,6 +246,41 @@ __device__ __forceinline__ void walk_records3(const double* __restrict__ cand, c
     body(cur, dx, dy, dz, j, true);
     #pragma unroll
     for (int q = 0; q < NR; ++q) cur[q] = nxt[q];
+  }
+}
+
+// Candidates j0 .. j1-1 one at a time for records too large to double
+// buffer in SGPRs: the head of a record (position and mass, 4 doubles) is
+// requested one candidate ahead, the rest of it (entries 4 .. NR-1) when
+// its candidate comes up, together with the next head.  pair(c, dx, dy,
+// dz, w) computes the kernel factors w[3] from the head and returns false
+// when no lane has the pair in range (the rest is then not needed); sums(c,
+// w) adds the terms that need the rest.  The wait for the rest falls after
+// the kernel arithmetic, and 2*4 + NR - 4 doubles are in flight at most
+// (46 SGPRs for the 19-double shear record, against 76 for two records).
+template <int NR, int STRIDE, class Pair, class Sums>
+__device__ __forceinline__ void walk_records3_split(const double* __restrict__ cand, const double* r, int j0,
+                                                    int j1, Pair&& pair, Sums&& sums)
+{
+  if (j0 >= j1) return;
+  constexpr int NH = 4;
+  double head[NH];
+  #pragma unroll
+  for (int q = 0; q < NH; ++q) head[q] = cand[(size_t)j0 * STRIDE + q];
+  for (int j = j0; j < j1; ++j) {
+    double c[NR];
+    #pragma unroll
+    for (int q = 0; q < NH; ++q) c[q] = head[q];
+    const double dx = r[0] - c[0], dy = r[1] - c[1], dz = r[2] - c[2];
+    __builtin_amdgcn_sched_barrier(0);
+    const int jn = j + 1 < j1 ? j + 1 : j1 - 1;
+    #pragma unroll
+    for (int q = NH; q < NR; ++q) c[q] = cand[(size_t)j * STRIDE + q];
+    #pragma unroll
+    for (int q = 0; q < NH; ++q) head[q] = cand[(size_t)jn * STRIDE + q];
+    __builtin_amdgcn_sched_barrier(0);
+    double w[3];
+    if (pair(c, dx, dy, dz, w)) sums(c, w);
   }
 }
 
@@ -327,7 +365,7 @@ __global__ __launch_bounds__(kWave) void k3_smooth_slice(const double* __restric
       walk_records3<NR, kCandS>(cand, r, j0, j1, [&](const double* c, double dx, double dy, double dz, int j,
                                                      bool valid) {
         const double r2 = dist2_3(dx, dy, dz);
-        const bool nb = valid && r2 <= rc2;
+        const bool nb = in && valid && r2 <= rc2;
         cnt += nb ? 1 : 0;
         if (kSkipEmpty && wave_none(nb)) return;
         const double w = pair_w3<FACT>(dx, dy, dz, r2, nb, hinv2, heinv2);
@@ -424,12 +462,10 @@ __global__ __launch_bounds__(kWave) void k3_grad_slice(const double* __restrict_
   for_segments3(act, myrow, r, [&](const Seg3& seg, bool in) {
     for_spans_slice3(g, cell_off, seg, rcs, slice, nslices, in, [&](int j0, int j1) {
       constexpr int NR = SHEAR ? 19 : 10;
-      walk_records3<NR, kCandG3>(cand, r, j0, j1, [&](const double* c, double dx, double dy, double dz, int j,
-                                                      bool valid) {
+      auto pair = [&](const double* c, double dx, double dy, double dz, double* w) -> bool {
         const double r2 = dist2_3(dx, dy, dz);
-        const bool nb = valid && r2 <= rc2;
-        if (kSkipEmpty && wave_none(nb)) return;
-        double w[3];
+        const bool nb = in && r2 <= rc2;
+        if (kSkipEmpty && wave_none(nb)) return false;
         if (FACT) {
           double qp, iqp, qe, iqe;
           q_and_inverse(dist2_perp(dx, dy), hinv2, qp, iqp);
@@ -441,9 +477,12 @@ __global__ __launch_bounds__(kWave) void k3_grad_slice(const double* __restrict_
         } else {
           double q, iq;
           q_and_inverse(r2, hinv2, q, iq);
-          const double gm = valid ? c[3] * spline_g(q, iq) : 0.0;
+          const double gm = nb ? c[3] * spline_g(q, iq) : 0.0;
           w[0] = gm * dx; w[1] = gm * dy; w[2] = gm * dz;
         }
+        return true;
+      };
+      auto sums = [&](const double* c, const double* w) {
         #pragma unroll
         for (int d = 0; d < 3; ++d) {
           acc[d] += w[d];
@@ -463,7 +502,8 @@ __global__ __launch_bounds__(kWave) void k3_grad_slice(const double* __restrict_
           acc[SHEAR ? 26 : 0] += w[0] * c[SHEAR ? 14 : 0] + w[1] * c[SHEAR ? 16 : 0] + w[2] * c[SHEAR ? 17 : 0];
           acc[SHEAR ? 27 : 0] += w[0] * c[SHEAR ? 15 : 0] + w[1] * c[SHEAR ? 17 : 0] + w[2] * c[SHEAR ? 18 : 0];
         }
-      });
+      };
+      walk_records3_split<NR, kCandG3>(cand, r, j0, j1, pair, sums);
     });
   });
   double* p = part + (size_t)slice * NACC * npad + k;

@@ -57,8 +57,30 @@ __global__ __launch_bounds__(256) void k_stage_begin(double* __restrict__ F, int
 
 // Reduce the partials and lay out the cell grid.  cell = rc / cells_per_rc,
 // doubled until the grid fits the key capacity.
+// x cells per row cell: the requested count, cut so that the key space
+// (cells already laid out, times the count) stays within `cap` cells.
+__device__ __forceinline__ int x_subdivision(int xsub, long long cap, long long cells)
+{
+  long long xs = xsub > 1 ? xsub : 1;
+  const long long room = cells > 0 ? cap / cells : 1;
+  if (xs > room) xs = room;
+  return xs < 1 ? 1 : (int)xs;
+}
+
+// Cells along x of width `wx` for x cells of cell / xs, at most xs times
+// the count for cells of `cell` (which the key space was sized for); a
+// position past the last cell is clamped into it by the sort and the walks
+// alike.
+__device__ __forceinline__ int x_cells(double wx, double cell, int xs, int nx_coarse, double& cellx)
+{
+  cellx = cell / (double)xs;
+  const double f = floor(wx / cellx) + 1.0;
+  const double lim = (double)xs * (double)nx_coarse;
+  return (int)(f < lim ? f : lim);
+}
+
 __global__ __launch_bounds__(256) void k_grid_setup(const double* __restrict__ bbox_part, int nparts,
-                                                    double rc, int cells_per_rc, int cap,
+                                                    double rc, int cells_per_rc, int xsub, int cap,
                                                     Grid* __restrict__ grid, int32_t* __restrict__ err,
                                                     unsigned long long* __restrict__ pair_count)
 {
@@ -81,6 +103,7 @@ __global__ __launch_bounds__(256) void k_grid_setup(const double* __restrict__ b
       xmn = fmin(xmn, s[k][0]); xmx = fmax(xmx, s[k][1]); ymn = fmin(ymn, s[k][2]); ymx = fmax(ymx, s[k][3]);
     }
     double cell = rc / (double)(cells_per_rc > 0 ? cells_per_rc : 1);
+    double cellx = cell;
     int nx = 1, ny = 1;
     const double wx = xmx - xmn, wy = ymx - ymn;
     if (!(wx >= 0.0 && wx < 1e300) || !(wy >= 0.0 && wy < 1e300)) {
@@ -93,9 +116,11 @@ __global__ __launch_bounds__(256) void k_grid_setup(const double* __restrict__ b
         if (fx * fy <= (double)cap) { nx = (int)fx; ny = (int)fy; break; }
         cell *= 2.0;
       }
+      nx = x_cells(wx, cell, x_subdivision(xsub, cap, (long long)nx * ny), nx, cellx);
     }
     Grid g;
     g.lo[0] = xmn; g.lo[1] = ymn; g.lo[2] = 0.0; g.cell = cell; g.inv_cell = 1.0 / cell;
+    g.cellx = cellx; g.inv_cellx = 1.0 / cellx;
     g.nc[0] = nx; g.nc[1] = ny; g.nc[2] = 1; g.ncells = nx * ny;
     g.stride = nx; g.row0 = 0; g.nev = 1; g.dim = 2;
     *grid = g;
@@ -106,7 +131,7 @@ __global__ __launch_bounds__(256) void k_grid_setup(const double* __restrict__ b
 // k3_stage_begin, and a grid of cubic cells of edge rc / cells_per_rc over
 // it, doubled until the key space fits `cap` cells.
 __global__ __launch_bounds__(256) void k3_grid_setup(const double* __restrict__ bbox_part, int nparts,
-                                                     double rc, int cells_per_rc, int cap,
+                                                     double rc, int cells_per_rc, int xsub, int cap,
                                                      Grid* __restrict__ grid, int32_t* __restrict__ err,
                                                      unsigned long long* __restrict__ pair_count)
 {
@@ -143,9 +168,13 @@ __global__ __launch_bounds__(256) void k3_grid_setup(const double* __restrict__ 
       cell *= 2.0;
     }
   }
+  double cellx = cell;
+  if (finite)
+    nc[0] = x_cells(mx[0] - mn[0], cell, x_subdivision(xsub, cap, (long long)nc[0] * nc[1] * nc[2]), nc[0], cellx);
   Grid g;
   for (int d = 0; d < 3; ++d) { g.lo[d] = mn[d]; g.nc[d] = nc[d]; }
   g.cell = cell; g.inv_cell = 1.0 / cell;
+  g.cellx = cellx; g.inv_cellx = 1.0 / cellx;
   g.ncells = nc[0] * nc[1] * nc[2];
   g.stride = nc[0]; g.row0 = 0; g.nev = 1; g.dim = 3;
   *grid = g;
@@ -184,7 +213,7 @@ __global__ __launch_bounds__(256) void k_bbox_events(const double* __restrict__ 
 // key rows event after event (row0) and the common row stride (the widest
 // event).  One workgroup; thread e lays out event e, thread 0 the rows.
 __global__ __launch_bounds__(256) void k_grid_events(const double* __restrict__ bbox, int nev, double rc,
-                                                     int cells_per_rc, Grid* __restrict__ grid,
+                                                     int cells_per_rc, int xsub, int cap, Grid* __restrict__ grid,
                                                      int32_t* __restrict__ err,
                                                      unsigned long long* __restrict__ pair_count)
 {
@@ -206,13 +235,28 @@ __global__ __launch_bounds__(256) void k_grid_events(const double* __restrict__ 
     }
     Grid g;
     g.lo[0] = xmn; g.lo[1] = ymn; g.lo[2] = 0.0; g.cell = cell; g.inv_cell = 1.0 / cell;
+    g.cellx = cell; g.inv_cellx = 1.0 / cell;
     g.nc[0] = nx; g.nc[1] = ny; g.nc[2] = 1; g.ncells = 0; g.stride = 0; g.row0 = 0; g.nev = nev; g.dim = 2;
     grid[e] = g;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     int rows = 0, stride = 1;
-    for (int e = 0; e < nev; ++e) { grid[e].row0 = rows; rows += grid[e].nc[1]; stride = max(stride, grid[e].nc[0]); }
+    for (int e = 0; e < nev; ++e) { rows += grid[e].nc[1]; stride = max(stride, grid[e].nc[0]); }
+    // x cells: one subdivision for the whole batch, cut so that the key
+    // space (rows times the widest event's x cells) stays within cap
+    const int xs = x_subdivision(xsub, cap, (long long)rows * stride);
+    rows = 0;
+    stride = 1;
+    for (int e = 0; e < nev; ++e) {
+      Grid& g = grid[e];
+      const double wx = bbox[4 * e + 1] - bbox[4 * e];
+      if (wx >= 0.0 && wx < 1e300) g.nc[0] = x_cells(wx, g.cell, xs, g.nc[0], g.cellx);
+      g.inv_cellx = 1.0 / g.cellx;
+      g.row0 = rows;
+      rows += g.nc[1];
+      stride = max(stride, g.nc[0]);
+    }
     for (int e = 0; e < nev; ++e) { grid[e].stride = stride; grid[e].ncells = rows * stride; }
   }
 }
@@ -229,14 +273,17 @@ __device__ __forceinline__ int cell_coord(double x, double lo, double inv, int n
 // out digit-major (hist[d*nblk + b]) so that one exclusive scan yields every
 // (digit, block) output base.
 // KEYS: the first pass also computes the cell key of every particle from
-// its position and the grid, and stores it (id order) for the scatter.
+// its position and the grid, and stores it (id order) for the scatter.  A
+// particle of a retired event of a batch (live[eid] == 0) gets the key
+// ncells, past every cell, so it sorts after the live events' particles.
 template <bool KEYS>
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const int32_t* __restrict__ keys_in, int n,
                                                              int shift, int nblk, int32_t* __restrict__ hist,
                                                              const double* __restrict__ F, int npad,
                                                              const Grid* __restrict__ grid,
                                                              int32_t* __restrict__ keys_out,
-                                                             const int32_t* __restrict__ eid)
+                                                             const int32_t* __restrict__ eid,
+                                                             const int32_t* __restrict__ live)
 {
   __shared__ int32_t h[256];
   h[threadIdx.x] = 0;
@@ -248,11 +295,12 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const int32_t* __re
     int key;
     if (KEYS) {
       const Grid g = grid[eid ? eid[idx] : 0];
-      const int cx = cell_coord(F[(size_t)CCAKE_F_X * npad + idx], g.lo[0], g.inv_cell, g.nc[0]);
+      const int cx = cell_coord(F[(size_t)CCAKE_F_X * npad + idx], g.lo[0], g.inv_cellx, g.nc[0]);
       const int cy = cell_coord(F[(size_t)CCAKE_F_Y * npad + idx], g.lo[1], g.inv_cell, g.nc[1]);
       // 3+1D: key rows run over (eta, y) cell pairs
       const int cz = g.dim == 3 ? cell_coord(F[(size_t)CCAKE_F_ETA * npad + idx], g.lo[2], g.inv_cell, g.nc[2]) : 0;
       key = (g.row0 + cz * g.nc[1] + cy) * g.stride + cx;
+      if (live && live[eid[idx]] == 0) key = g.ncells;
       keys_out[idx] = key;
     } else {
       key = keys_in[idx];
@@ -475,14 +523,14 @@ int launch_neighbor_sort(ccake_event* ev)
   const double rc = 2.0 * ev->p.hT;
   if (ev->p.dim == 3) {
     hipLaunchKernelGGL(k3_grid_setup, dim3(1), dim3(256), 0, st, ev->bbox, nb, rc, ev->p.cells_per_rc,
-                       ev->cap_cells, ev->grid, ev->err, (unsigned long long*)ev->pair_count);
+                       ev->p.x_subcells, ev->cap_cells, ev->grid, ev->err, (unsigned long long*)ev->pair_count);
   } else if (ev->nev > 1) {
     hipLaunchKernelGGL(k_bbox_events, dim3(ev->nev), dim3(256), 0, st, ev->F, ev->npad, ev->ev_off, ev->bbox);
     hipLaunchKernelGGL(k_grid_events, dim3(1), dim3(256), 0, st, ev->bbox, ev->nev, rc, ev->p.cells_per_rc,
-                       ev->grid, ev->err, (unsigned long long*)ev->pair_count);
+                       ev->p.x_subcells, ev->cap_cells, ev->grid, ev->err, (unsigned long long*)ev->pair_count);
   } else {
     hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(256), 0, st, ev->bbox, nb, rc, ev->p.cells_per_rc,
-                       ev->cap_cells, ev->grid, ev->err, (unsigned long long*)ev->pair_count);
+                       ev->p.x_subcells, ev->cap_cells, ev->grid, ev->err, (unsigned long long*)ev->pair_count);
   }
   const int nblk = ev->radix_blocks;
   const int nscan = 256 * nblk;
@@ -494,10 +542,10 @@ int launch_neighbor_sort(ccake_event* ev)
     const int shift = 8 * pass;
     if (pass == 0)
       hipLaunchKernelGGL(k_radix_hist<true>, dim3(nblk), dim3(kSortThreads), 0, st, nullptr, n, 0, nblk, ev->hist,
-                         ev->F, ev->npad, ev->grid, ev->key, ev->eid);
+                         ev->F, ev->npad, ev->grid, ev->key, ev->eid, live_of(ev).flag);
     else
       hipLaunchKernelGGL(k_radix_hist<false>, dim3(nblk), dim3(kSortThreads), 0, st, kin, n, shift, nblk, ev->hist,
-                         nullptr, 0, nullptr, nullptr, nullptr);
+                         nullptr, 0, nullptr, nullptr, nullptr, nullptr);
     scan_hist(ev, nscan);
     int32_t* kout = kbuf[pass & 1];
     int32_t* vout = vbuf[pass & 1];

@@ -14,7 +14,7 @@
 //
 // Accumulation.  Each of a lane's sums is kept as a block sum b and a
 // compensated pair (hi, lo): the pair terms are added to b, and every
-// FOLD candidates (32 by default; and at the end of every span) b is folded into
+// FOLD candidates (16 by default; and at the end of every span) b is folded into
 // (hi, lo) with the error-free TwoSum (Knuth):
 //   s = hi + b;  z = s - hi;  err = (hi - (s - z)) + (b - z);  hi = s;  lo += err.
 // A block holds at most FOLD terms, so its rounding is that of a short
@@ -31,9 +31,11 @@ namespace ccake {
 // to freeze-out, 16, 32 and 64 gave 5.62, 5.49 and 5.43 ms per step and
 // freeze-out errors against FP64 (largest per particle, same snapshot) of
 // 4.0e-7, 7.9e-7 and 1.9e-6 in u^mu, with the charge densities' whole-field
-// errors 6.5e-7, 8.6e-7 and 1.2e-6: 32 keeps T, u^mu and the charges
-// within 1e-6.
-constexpr int kFoldEvery = 32;
+// errors 6.5e-7, 8.6e-7 and 1.2e-6.  In a batch of 512 events of 24k to
+// 216k particles, 32 left one event's u^mu 1.8e-6 off its FP64 run alone;
+// 16 halves the errors for 2 % of the step and keeps T, u^mu and the
+// charges within 1e-6.
+constexpr int kFoldEvery = 16;
 
 struct Comp {
   float b = 0.0f, hi = 0.0f, lo = 0.0f;
@@ -272,7 +274,7 @@ static int mp_fold()
   static const int v = [] {
     const char* e = std::getenv("CCAKE_MP_FOLD");
     const int f = e ? std::atoi(e) : kFoldEvery;
-    return (f == 16 || f == 64) ? f : 32;
+    return (f == 32 || f == 64) ? f : 16;
   }();
   return v;
 }
@@ -290,7 +292,7 @@ static void smooth32_fold(ccake_event* ev, dim3 grid, double rc2, double rcs, fl
 {
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, grid, dim3(kWave), 0, ev->stream, ev->cand_s, ev->cand_s32, ev->skey, ev->cell_off,
-                       ev->grid, ev->n, rc2, rcs, hinv2, ev->nslices, ev->part, ev->npad);
+                       ev->grid, ev->n_live, rc2, rcs, hinv2, ev->nslices, ev->part, ev->npad);
   };
   const int f = mp_fold();
   if (f == 64) go(k_smooth_slice32<MODE, U, 64>);
@@ -300,7 +302,7 @@ static void smooth32_fold(ccake_event* ev, dim3 grid, double rc2, double rcs, fl
 
 int launch_smooth_pairs32(ccake_event* ev)
 {
-  const int n = ev->n;
+  const int n = ev->n_live;          // sorted positions of the live events' particles
   const double rc = 2.0 * ev->p.hT;
   const double rcs = search_radius(ev);
   const int ntiles = (n + kTile - 1) / kTile;
@@ -322,7 +324,7 @@ static void grad32_fold(ccake_event* ev, dim3 grid, double rc2, double rcs, floa
 {
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, grid, dim3(kWave), 0, ev->stream, ev->cand_g, ev->cand_g32, ev->skey, ev->cell_off,
-                       ev->grid, ev->n, rc2, rcs, hinv2, ev->nslices, ev->part, ev->npad);
+                       ev->grid, ev->n_live, rc2, rcs, hinv2, ev->nslices, ev->part, ev->npad);
   };
   const int f = mp_fold();
   if (f == 64) go(k_grad_slice32<SHEAR, BULK, 64>);
@@ -332,7 +334,7 @@ static void grad32_fold(ccake_event* ev, dim3 grid, double rc2, double rcs, floa
 
 int launch_grad_pairs32(ccake_event* ev)
 {
-  const int n = ev->n;
+  const int n = ev->n_live;
   const double rc = 2.0 * ev->p.hT;
   const double rcs = search_radius(ev);
   const int ntiles = (n + kTile - 1) / kTile;

@@ -78,7 +78,7 @@ int launch_charge_check(ccake_event* ev, bool* any_charge, bool* any_bulk)
 int launch_stage_gather(ccake_event* ev)
 {
   if (ev->p.dim == 3) return launch3_stage_gather(ev);
-  const int n = ev->n;
+  const int n = ev->n_live;
   hipLaunchKernelGGL(k_stage_gather, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad, n,
                      ev->perm, ev->cand_s, (unsigned long long*)ev->pair_count);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
@@ -284,7 +284,7 @@ static KernelW make_w(double hT)
 
 int launch_gather_smooth(ccake_event* ev)
 {
-  const int n = ev->n;
+  const int n = ev->n_live;
   hipLaunchKernelGGL(k_gather_smooth, dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->F, ev->npad,
                      n, ev->perm, ev->cand_s, ev->p.dim);
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
@@ -295,7 +295,7 @@ int launch_gather_smooth(ccake_event* ev)
 int launch_smooth_pairs(ccake_event* ev, int mode, bool count_pairs)
 {
   if (ev->p.dim == 3) return launch3_smooth_pairs(ev, mode, count_pairs);
-  const int n = ev->n;
+  const int n = ev->n_live;          // sorted positions of the live events' particles
   const double rc = 2.0 * ev->p.hT;
   const double rc2 = rc * rc;
   const double rcs = search_radius(ev);
@@ -326,7 +326,7 @@ int launch_smooth_pairs(ccake_event* ev, int mode, bool count_pairs)
 template <int MODE, int S>
 static void smooth_sum_s(ccake_event* ev, double norm2, int32_t* bt, unsigned long long* pc)
 {
-  const int n = ev->n;
+  const int n = ev->n_live;
   const dim3 grid((n + kSumThreads - 1) / kSumThreads), blk(kSumThreads);
   hipLaunchKernelGGL((k_smooth_sum<MODE, S>), grid, blk, 0, ev->stream, ev->perm, n, norm2, ev->t, ev->part,
                      ev->npad, ev->F, bt, pc);
@@ -399,8 +399,9 @@ int launch_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, s
   int32_t* d_j = static_cast<int32_t*>(b_j.p);
   CCAKE_HIP(hipMemcpyAsync(d_off, off.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
   const double rc = 2.0 * ev->p.hT;
-  hipLaunchKernelGGL(k_walk<3>, dim3((n + kTile - 1) / kTile), dim3(kTile), 0, st, ev->cand_s, ev->skey,
-                     ev->perm, ev->cell_off, ev->grid, n, rc * rc, search_radius(ev), bt, nullptr, d_off,
+  // the live events' rows (an ended event of a batch exports none)
+  hipLaunchKernelGGL(k_walk<3>, dim3((ev->n_live + kTile - 1) / kTile), dim3(kTile), 0, st, ev->cand_s, ev->skey,
+                     ev->perm, ev->cell_off, ev->grid, ev->n_live, rc * rc, search_radius(ev), bt, nullptr, d_off,
                      d_i, d_j);
   CCAKE_HIP(hipGetLastError());
   CCAKE_HIP(hipMemcpyAsync(pi, d_i, total * sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -488,12 +489,13 @@ int launch_export_pairs_noalloc(ccake_event* ev, int32_t* pi, int32_t* pj, size_
   int32_t* ri = reinterpret_cast<int32_t*>(base + kScratchRowsOff);
   hipStream_t st = ev->stream;
   const double rc = 2.0 * ev->p.hT;
-  const int ntiles = (n + kTile - 1) / kTile;
+  const int nl = ev->n_live;            // the walks cover the live events' sorted positions
+  const int ntiles = (nl + kTile - 1) / kTile;
   // counts (self excluded) into scratch, cleared first: the last call's
   // contents are still there
   hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, st, cnt, n, 0);
   hipLaunchKernelGGL(k_walk<2>, dim3(ntiles), dim3(kTile), 0, st, ev->cand_s, ev->skey, ev->perm, ev->cell_off,
-                     ev->grid, n, rc * rc, search_radius(ev), cnt, nullptr, nullptr, nullptr, nullptr);
+                     ev->grid, nl, rc * rc, search_radius(ev), cnt, nullptr, nullptr, nullptr, nullptr);
   hipLaunchKernelGGL(k_row_offsets, dim3(1), dim3(kRowScanThreads), 0, st, cnt, n, off);
   CCAKE_HIP(hipGetLastError());
   int64_t total = 0;
@@ -507,7 +509,7 @@ int launch_export_pairs_noalloc(ccake_event* ev, int32_t* pi, int32_t* pj, size_
   if ((size_t)total > cap) return CCAKE_ERR_CAPACITY;
   int32_t* rj = ri + total;
   hipLaunchKernelGGL(k_walk<3>, dim3(ntiles), dim3(kTile), 0, st, ev->cand_s, ev->skey, ev->perm, ev->cell_off,
-                     ev->grid, n, rc * rc, search_radius(ev), cnt, nullptr, off, ri, rj);
+                     ev->grid, nl, rc * rc, search_radius(ev), cnt, nullptr, off, ri, rj);
   CCAKE_HIP(hipGetLastError());
   if (total) {
     CCAKE_HIP(hipMemcpyAsync(pi, ri, (size_t)total * sizeof(int32_t), hipMemcpyDeviceToHost, st));

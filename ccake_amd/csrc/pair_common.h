@@ -88,8 +88,8 @@ __device__ __forceinline__ void for_spans(const Grid& g, const int32_t* __restri
     const double dy = fmax(0.0, fmax(ylo - s.ymx, s.ymn - yhi));
     if (dy > rcs) continue;
     const double hw = sqrt(fmax(rcs * rcs - dy * dy, 0.0));
-    const int cx0 = cell_of(s.xmn - hw, g.lo[0], g.inv_cell, g.nc[0]);
-    const int cx1 = cell_of(s.xmx + hw, g.lo[0], g.inv_cell, g.nc[0]);
+    const int cx0 = cell_of(s.xmn - hw, g.lo[0], g.inv_cellx, g.nc[0]);
+    const int cx1 = cell_of(s.xmx + hw, g.lo[0], g.inv_cellx, g.nc[0]);
     const int kr = (g.row0 + ry) * g.stride;
     const int beg = uni(cell_off[kr + cx0]);
     const int end = uni(cell_off[kr + cx1 + 1]);
@@ -142,8 +142,8 @@ __device__ __forceinline__ void row_run(const Grid& g, const int32_t* __restrict
   const double dy = fmax(0.0, fmax(ylo - s.ymx, s.ymn - yhi));
   if (dy > rcs) return;
   const double hw = sqrt(fmax(rcs * rcs - dy * dy, 0.0));
-  const int cx0 = cell_of(s.xmn - hw, g.lo[0], g.inv_cell, g.nc[0]);
-  const int cx1 = cell_of(s.xmx + hw, g.lo[0], g.inv_cell, g.nc[0]);
+  const int cx0 = cell_of(s.xmn - hw, g.lo[0], g.inv_cellx, g.nc[0]);
+  const int cx1 = cell_of(s.xmx + hw, g.lo[0], g.inv_cellx, g.nc[0]);
   const int kr = (g.row0 + ry) * g.stride;
   b = cell_off[kr + cx0];
   e = cell_off[kr + cx1 + 1];

@@ -83,6 +83,7 @@ struct Settings {
   double diffusion_regulator_chi0 = 10.0, diffusion_regulator_e0 = 0.1;   // e0, xi0 in GeV/fm^3
   double diffusion_regulator_xi0 = 0.01, diffusion_regulator_rq_max = 1.0;
   int cells_per_rc = 3;
+  int x_subcells = 8;     // x cells per row cell (ccake_params::x_subcells)
   // 0: FP64 pair sums; 1: FP32 pair terms with compensated accumulation
   // (ccake_params::pair_precision; no counterpart in the reference)
   int pair_precision = 0;
@@ -412,6 +413,7 @@ class SystemState {
     }
     p.pair_precision = s.pair_precision;
     p.factorized_kernel = s.factorized_kernel;
+    p.x_subcells = s.x_subcells;
     return p;
   }
 

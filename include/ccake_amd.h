@@ -186,6 +186,11 @@ typedef struct ccake_params {
      Settings::factorized_kernel, include/settings.h:114); 0 is the
      isotropic 3D spline in (x, y, eta) with hT. */
   int    factorized_kernel;
+  /* neighbour search tuning: x cells per cell of a key row (the x cell edge
+     is rc / cells_per_rc / x_subcells; 0 or 1 = square cells).  Finer x
+     cells cut each row's candidate run closer to the cutoff disc without
+     adding rows to walk; the pair set does not depend on it. */
+  int    x_subcells;
 } ccake_params;
 
 /* Description of an equation of state.  For CCAKE_EOS_TABLE the table is a
@@ -367,7 +372,10 @@ CCAKE_API int  ccake_batch_create(ccake_dev* dev, const ccake_params* p, ccake_e
 CCAKE_API int  ccake_batch_events(ccake_event* ev, int* n_events, int* offsets);
 /* ccake_reduce per event: out[e] for each of the n_events events. */
 CCAKE_API int  ccake_batch_reduce(ccake_event* ev, int which, double* out);
-/* continue_evolution per event: keep_going[e] for each event. */
+/* continue_evolution per event: keep_going[e] for each event.  An event
+ * whose flag is 0 has ended (as BSQHydro::run ends for it alone): later
+ * steps of the batch leave it unchanged and cost it nothing, until the next
+ * process_initial_conditions (or initialize) makes every event live again. */
 CCAKE_API int  ccake_batch_continue_evolution(ccake_event* ev, double max_tau, int* keep_going);
 /* Sticky device-side error flags raised by kernels since the last call
  * (negative entropy, EoS failure); clears them. */

@@ -37,7 +37,7 @@ PARAM_FIELDS = [
     ("kappa_over_T2", C.c_double * 9), ("C_B", C.c_double),
     ("diffusion_regulator", C.c_int), ("regulator_chi0", C.c_double), ("regulator_e0", C.c_double),
     ("regulator_xi0", C.c_double), ("regulator_rq_max", C.c_double),
-    ("pair_precision", C.c_int), ("factorized_kernel", C.c_int),
+    ("pair_precision", C.c_int), ("factorized_kernel", C.c_int), ("x_subcells", C.c_int),
 ]
 
 
@@ -149,7 +149,7 @@ def lib():
 def make_params(cfg):
     p = OrcParams()
     for name, typ in PARAM_FIELDS:
-        v = cfg.get(name, 0) if name in ("pair_precision", "factorized_kernel") else cfg[name]
+        v = cfg.get(name, 0) if name in ("pair_precision", "factorized_kernel", "x_subcells") else cfg[name]
         setattr(p, name, typ(*v) if hasattr(typ, "_length_") else v)
     return p
 

@@ -126,6 +126,7 @@ typedef struct orc_params {
   double regulator_chi0, regulator_e0, regulator_xi0, regulator_rq_max;
   int    pair_precision;      /* the product's switch; the oracle is FP64 and ignores it */
   int    factorized_kernel;   /* D = 3: W_2D(r_perp, hT) W_1D(|d eta|, hEta) (settings.h:114) */
+  int    x_subcells;          /* the product's neighbour-search tuning; the oracle ignores it */
 } orc_params;
 
 typedef struct orc_eos_desc {

@@ -3,8 +3,10 @@ infrastructure): three lumpy viscous BSQ events of different sizes and
 seeds on the conformal EoS with charges, initialize and STEPS steps; every
 event's fields within TOL of its own run alone (whole-field relative) and
 its btrack equal; prints "OK".  With --fo the events run to complete
-freeze-out (particlization on) and each event's freeze-out rows are
-compared too.
+freeze-out (particlization on): each event's freeze-out rows are compared,
+and so is its final state, since an event whose continue flag has turned
+false ends in the batch (later batch steps leave it unchanged) as the run
+alone ends.
 
 Usage (through tests/hostemu/emu.sh or on a GPU): run_batch.py SPACING STEPS TOL [--fo] [--table]
 """
@@ -79,6 +81,13 @@ def main():
             ok = ok and same_bt
             print(f"event {e}: btrack equal {same_bt}", flush=True)
         else:
+            for lab in FIELDS:
+                f = ccake_amd.FIELDS["CCAKE_F_" + lab]
+                x, y = b.download(f)[sl], ev.download(f)
+                err = float(np.max(np.abs(x - y)) / (np.max(np.abs(y)) or 1.0))
+                if err > tol:
+                    print(f"event {e}: final {lab} differs by {err:.3g}", flush=True)
+                worst = max(worst, err)
             rb = b.surface_by_event()[e]
             ra = ev.freeze_out_surface()
             same = rb.shape == ra.shape and np.array_equal(rb[:, :2], ra[:, :2])

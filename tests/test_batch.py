@@ -101,7 +101,9 @@ def test_batch_runs_every_event_to_freeze_out(dev, table_eos):
     event continues): each event's freeze-out rows have the ids and steps of
     its run alone and their columns are within 1e-9 of them (whole-column
     relative); the per-event continue flags turn false at the step the run
-    alone ends."""
+    alone ends.  An event ends in the batch when its flag turns false (the
+    later batch steps leave it as it is), so its final fields equal those
+    of its run alone, which ended at that step, within 1e-9."""
     cfg = dict(config.lumpy_params(), particlization=1, e_freeze=table_eos.efreeze(T_FO))
     ics = three_events(0.1)
     b, alone = make(dev, table_eos, cfg, ics)
@@ -133,6 +135,10 @@ def test_batch_runs_every_event_to_freeze_out(dev, table_eos):
         assert rb.shape == ra.shape and np.array_equal(rb[:, :2], ra[:, :2]), e
         for c in range(2, ra.shape[1]):
             assert whole(rb[:, c], ra[:, c]) <= 1e-9, (e, c, whole(rb[:, c], ra[:, c]))
+        sl = b.event_slice(e)
+        for lab in ("X", "Y", "UX", "UY", "SIGMA", "E", "T", "RHOB", "EXT_S", "BULK"):
+            f = ccake_amd.FIELDS["CCAKE_F_" + lab]
+            assert whole(b.download(f)[sl], ev.download(f)) <= 1e-9, (e, lab)
     b.close()
     for ev in alone:
         ev.close()

@@ -99,7 +99,8 @@ def test_mixed_precision_freeze_out_surface(dev, table_eos):
     column within 2e-3 per particle with signed columns measured against
     1e-3 of their largest magnitude, that is within 2e-6 of the largest
     magnitude where a signed field crosses zero.  (Measured on this event
-    with the FP32 blocks folded every 32 candidates: T 1.3e-7, e and s 1e-6
+    with the FP32 blocks folded every 32 candidates, twice the default
+    interval: T 1.3e-7, e and s 1e-6
     at most, medians 1e-8 to 3e-7; the largest errors sit where a signed
     field is near zero (rho_Q 9.6e-4 of the floor, 9.6e-7 of its largest
     magnitude), and the flow's sensitivity amplifies the FP32 terms' 1e-7
