@@ -84,6 +84,7 @@ def parse():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-event measurement")
     ap.add_argument("--no-milne", action="store_true", help="skip the 3+1D Milne event and served 3+1D stream")
+    ap.add_argument("--no-milne-stream", action="store_true", help="skip the served 3+1D stream")
     ap.add_argument("--no-1m", action="store_true", help="skip the 0.01 fm lattice event (FP64 and mixed)")
     ap.add_argument("--batch-events", type=int, default=16)
     ap.add_argument("--large-batch", type=int, default=0,
@@ -735,9 +736,10 @@ def main():
         print("bench: 3+1D Milne event", file=sys.stderr, flush=True)
         out["milne_3d"] = milne_run(dev, eos)
         print(f"bench: milne {json.dumps(out['milne_3d'])}", file=sys.stderr, flush=True)
-        print("bench: served 3+1D stream", file=sys.stderr, flush=True)
-        out["milne_3d"]["served_stream"] = milne_stream(dev, eos)
-        print(f"bench: milne stream {json.dumps(out['milne_3d']['served_stream'])}", file=sys.stderr, flush=True)
+        if not args.no_milne_stream:
+            print("bench: served 3+1D stream", file=sys.stderr, flush=True)
+            out["milne_3d"]["served_stream"] = milne_stream(dev, eos)
+            print(f"bench: milne stream {json.dumps(out['milne_3d']['served_stream'])}", file=sys.stderr, flush=True)
     eos.close()
     if not args.no_serve:
         print(f"bench: serving {args.serve_events} events, {args.serve_slots} slots", file=sys.stderr, flush=True)

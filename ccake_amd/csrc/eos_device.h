@@ -80,16 +80,45 @@ CK_HD void diag_fields(const EosMember& m, const double* x, double* r)
   r[X_TT] = 12.0 * cp * T * T / (T0 * T0 * T0 * T0);
 }
 
+// Swap rows c and pv (pv >= c) of a small matrix with every index known at
+// compile time after unrolling: the pivot row is selected, not addressed,
+// so the matrix stays in registers (an array indexed by a run-time row
+// would live in scratch memory).  Same values as the indexed swap.
+template <int R, int C>
+CK_HD void swap_rows_sel(double (&A)[R][C], int c, int pv)
+{
+  #pragma unroll
+  for (int j = 0; j < C; ++j) {
+    double top = A[c][j];
+    #pragma unroll
+    for (int i = 0; i < R; ++i)
+      if (i > c) top = (pv == i) ? A[i][j] : top;
+    #pragma unroll
+    for (int i = 0; i < R; ++i)
+      if (i > c) A[i][j] = (pv == i) ? A[c][j] : A[i][j];
+    A[c][j] = top;
+  }
+}
+
 // 4x4 linear solve with partial pivoting; false if singular.
 CK_HD bool solve_4x4(const double* J, const double* b, double* x)
 {
   double A[4][5];
-  for (int i = 0; i < 4; ++i) { for (int j = 0; j < 4; ++j) A[i][j] = J[4 * i + j]; A[i][4] = b[i]; }
+  #pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    #pragma unroll
+    for (int j = 0; j < 4; ++j) A[i][j] = J[4 * i + j];
+    A[i][4] = b[i];
+  }
+  #pragma unroll
   for (int c = 0; c < 4; ++c) {
     int pv = c;
-    for (int i = c + 1; i < 4; ++i) if (fabs(A[i][c]) > fabs(A[pv][c])) pv = i;
-    if (A[pv][c] == 0.0 || !__builtin_isfinite(A[pv][c])) return false;
-    if (pv != c) for (int j = 0; j < 5; ++j) { const double t = A[c][j]; A[c][j] = A[pv][j]; A[pv][j] = t; }
+    double best = fabs(A[c][c]);
+    #pragma unroll
+    for (int i = c + 1; i < 4; ++i) if (fabs(A[i][c]) > best) { pv = i; best = fabs(A[i][c]); }
+    swap_rows_sel(A, c, pv);
+    if (A[c][c] == 0.0 || !__builtin_isfinite(A[c][c])) return false;
+    #pragma unroll
     for (int i = c + 1; i < 4; ++i) {
       const double f = A[i][c] / A[c][c];
       for (int j = c; j < 5; ++j) A[i][j] -= f * A[c][j];
@@ -420,10 +449,14 @@ CK_HD double schur3(double lead, double a0, double a1, double a2,
   if (fabs(m11) < EPS) m11 = (m11 >= 0.0 ? EPS : -EPS);
   if (fabs(m22) < EPS) m22 = (m22 >= 0.0 ? EPS : -EPS);
   double A[3][4] = { { m00, m01, m02, b0 }, { m10, m11, m12, b1 }, { m20, m21, m22, b2 } };
+  #pragma unroll
   for (int c = 0; c < 3; ++c) {
     int pv = c;
-    for (int i = c + 1; i < 3; ++i) if (fabs(A[i][c]) > fabs(A[pv][c])) pv = i;
-    if (pv != c) for (int j = 0; j < 4; ++j) { const double t = A[c][j]; A[c][j] = A[pv][j]; A[pv][j] = t; }
+    double best = fabs(A[c][c]);
+    #pragma unroll
+    for (int i = c + 1; i < 3; ++i) if (fabs(A[i][c]) > best) { pv = i; best = fabs(A[i][c]); }
+    swap_rows_sel(A, c, pv);
+    #pragma unroll
     for (int i = c + 1; i < 3; ++i) {
       const double f = A[i][c] / A[c][c];
       for (int j = c; j < 4; ++j) A[i][j] -= f * A[c][j];

@@ -202,8 +202,12 @@ struct HardList {
 // the analytic conformal chain, which then compiles without the table code).
 // MODE 0: the stage (EoS, reset_pi_tensor, transport, gradient records);
 // 1: the initial e inversion; 2: EoS only; 3: MODE 0 after the smoothing sum.
+#ifndef CCAKE_EOS_WAVES_PER_SIMD
+#define CCAKE_EOS_WAVES_PER_SIMD 1
+#endif
 template <int MODE, int KM>
-__global__ __launch_bounds__(256) void k_eos(EosChain ch, double* __restrict__ F, int npad, int n,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CCAKE_EOS_WAVES_PER_SIMD)))
+void k_eos(EosChain ch, double* __restrict__ F, int npad, int n,
                                              int32_t* __restrict__ I, const int32_t* __restrict__ perm,
                                              double t, Transport P, double* __restrict__ cand,
                                              int32_t* __restrict__ err, SmoothSum sm, HardList hl)
