@@ -189,6 +189,16 @@ def bytes_per_particle_eos(nslices):
 
 
 KERNEL_OF_GROUP = {"SMOOTH": "k_smooth_slice", "GRAD": "k_grad_slice", "EOS": "k_eos"}
+# With 4 or more slices per tile the FP64 2+1D pair kernels run a tile's
+# slices as the waves of one workgroup and add them in LDS (k_smooth_tile,
+# k_grad_tile; CCAKE_LDS_SLICES=0 turns that off), and the sum passes read
+# one slice.
+KERNEL_OF_GROUP_LDS = {"SMOOTH": "k_smooth_tile", "GRAD": "k_grad_tile", "EOS": "k_eos"}
+
+
+def lds_tiles(nslices, cfg):
+    return (nslices >= 4 and cfg.get("pair_precision", 0) == 0 and cfg.get("dim", 2) == 2
+            and os.environ.get("CCAKE_LDS_SLICES", "1") != "0")
 
 
 def traffic_of(kernel, workload):
@@ -219,8 +229,10 @@ def make_roofline(prof, pairs, n, cfg, workload, charges, bulk, nslices):
     dom = max(cand, key=lambda g: cand[g][0])
     ms = cand[dom][0] / cand[dom][1]
     sec = max(ms * 1e-3, 1e-12)
+    lds = lds_tiles(nslices, cfg)
+    names = KERNEL_OF_GROUP_LDS if lds else KERNEL_OF_GROUP
     if dom == "EOS":
-        bpp = bytes_per_particle_eos(nslices)
+        bpp = bytes_per_particle_eos(1 if lds else nslices)
         achieved = bpp * n / sec / 1e9
         r = dict(bound="hbm", achieved=round(achieved, 2), peak=PEAK_HBM_GBS, unit="GB/s",
                  frac=round(achieved / PEAK_HBM_GBS, 4),
@@ -232,8 +244,8 @@ def make_roofline(prof, pairs, n, cfg, workload, charges, bulk, nslices):
         r = dict(bound="valu", achieved=round(achieved, 3), peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
                  frac=round(achieved / PEAK_FP64_TFLOPS, 4), engine="FP64 VALU (no matrix work on this path)",
                  work=f"{fpp} flop/pair x {pairs:.0f} pairs per launch")
-    r["traffic"] = traffic_of(KERNEL_OF_GROUP[dom], workload)
-    r["kernel"] = KERNEL_OF_GROUP[dom]
+    r["traffic"] = traffic_of(names[dom], workload)
+    r["kernel"] = names[dom]
     r["avg_launch_ms"] = round(ms, 4)
     return r
 

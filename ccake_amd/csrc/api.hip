@@ -112,7 +112,9 @@ static int fold_sums(const ccake_event* ev)
   if (ev->p.dim == 3) return 0;                 // the 3+1D stage runs its passes as kernels of their own
   static const int env = [] { const char* e = getenv("CCAKE_FOLD_SUMS"); return e ? atoi(e) : -1; }();
   if (env >= 0) return env;
-  return 1 | (ev->p.using_shear ? 0 : 2);
+  // the gradient sums fold into the EoM kernel unless the shear walk's sums
+  // arrive in many slices (their loads would crowd the EoM kernel)
+  return 1 | ((!ev->p.using_shear || lds_slices(ev)) ? 2 : 0);
 }
 
 // Within one stepping call, a stage that keeps the cell sort can be begun by
@@ -741,6 +743,8 @@ static int event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, i
     const int v = std::atoi(ov);
     if (v >= 1 && v <= kMaxSlices) ev->nslices = v;
   }
+  if (const char* ov = std::getenv("CCAKE_LDS_SLICES"))        // tuning override: 0 sums slices through HBM
+    ev->lds_reduce = std::atoi(ov) != 0;
   if (const char* ov = std::getenv("CCAKE_SORT_SKIN")) {       // tuning override, fraction of rc
     const double v = std::atof(ov);
     if (v >= 0.0 && v <= 1.0) ev->skin_frac = v;

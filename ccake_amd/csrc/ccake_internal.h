@@ -208,6 +208,11 @@ struct ccake_event {
   int32_t* live = nullptr;      // device, per event: 1 live, 0 retired (batches only)
   std::vector<int> h_live;
   int n_live = 0;               // particles of the live events (n for a single event)
+  // Slices of the partial sums the sum passes read: nslices, or 1 when the
+  // pair kernel added a tile's slices in LDS (set by each pair launch).
+  int smooth_sum_slices = 1;
+  int grad_sum_slices = 1;
+  bool lds_reduce = true;       // 2+1D FP64 pair kernels reduce a tile's slices in LDS
   int64_t last_pairs = 0;
   size_t bytes = 0;
 };
@@ -250,6 +255,15 @@ inline double search_radius(const ccake_event* ev)
   return r;
 }
 inline int32_t* icol(ccake_event* ev, int f) { return ev->I + (size_t)f * ev->npad; }
+
+// Waves per workgroup of the 2+1D FP64 pair kernels that add a tile's
+// slices in LDS (one wave per slice), or 0 for one single-wave workgroup
+// per (tile, slice) with the slices' sums through HBM.
+inline int lds_slices(const ccake_event* ev)
+{
+  if (ev->p.dim != 2 || ev->p.pair_precision != 0 || !ev->lds_reduce) return 0;
+  return ev->nslices >= 8 ? 8 : (ev->nslices >= 4 ? 4 : 0);
+}
 
 // ---- launchers (one per .hip unit) ----
 // neighbours

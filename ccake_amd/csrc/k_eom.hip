@@ -684,7 +684,8 @@ __global__ __launch_bounds__(256) void k_eom(double* __restrict__ F, int npad, i
   if (!gs.perm && live.retired(i)) return;      // an ended event of a batch keeps its state
   if (gs.perm) {
     i = gs.perm[k];
-    grad_sum_into<SHEAR, 0>(gs.cand, k, i, gs.C, gs.part, gs.nslices, npad, F);
+    if (gs.nslices == 1) grad_sum_into<SHEAR, 1>(gs.cand, k, i, gs.C, gs.part, 1, npad, F);
+    else grad_sum_into<SHEAR, 0>(gs.cand, k, i, gs.C, gs.part, gs.nslices, npad, F);
   }
   const Cols col{ F, npad, i };
   double du[2], ds, dN[3], db, dshv[5] = { 0, 0, 0, 0, 0 }, theta, snu, dez, hn[10], dq[9];
@@ -854,7 +855,7 @@ int launch_eom(ccake_event* ev, int fuse_rk, double dt, double t0, bool fold_gra
     gs.perm = ev->perm;
     gs.cand = ev->cand_g;
     gs.part = ev->part;
-    gs.nslices = ev->nslices;
+    gs.nslices = ev->grad_sum_slices;
     gs.C = grad_sum_scale(ev->p.hT);
     if (begin_next && fuse_rk) {
       gs.next_cand = ev->cand_s;

@@ -385,7 +385,7 @@ int launch_eos(ccake_event* ev, int mode, int fuse_post)
   SmoothSum sm;
   if (md == 3) {
     sm.part = ev->part;
-    sm.nslices = ev->nslices;
+    sm.nslices = ev->smooth_sum_slices;
     sm.norm2 = smooth_norm2(ev->p.hT);
     sm.pair_total = (unsigned long long*)ev->pair_count;
   }

@@ -17,26 +17,20 @@
 
 namespace ccake {
 
-// Pair part: one single-wave workgroup per (tile, slice) unit walks its
-// slice of the tile's candidates and stores the slice's neighbour-side sums;
-// k_grad_sum adds the slices in slice order and applies the owner terms.
+// The walk of one (tile, slice) unit: slice `slice` of `nslices` of the
+// tile's candidates, one wave; fin(acc) receives the neighbour-side sums.
 // BULK false: the event's bulk pressure is zero on every particle (zeta = 0
 // and no initial bulk, checked at initialize), so the grad Pi sums are the
 // exact zeros the full walk would give and are not accumulated.
-template <bool SHEAR, bool BULK>
-__global__ __launch_bounds__(kWave) void k_grad_slice(const double* __restrict__ cand,
-                                                      const int32_t* __restrict__ skey,
-                                                      const int32_t* __restrict__ cell_off,
-                                                      const Grid* __restrict__ gridp, int n,
-                                                      double rc2, double rcs, double hinv2, int nslices,
-                                                      double* __restrict__ part, int npad)
+template <bool SHEAR, bool BULK, class Fin>
+__device__ __forceinline__ void grad_walk(const double* __restrict__ cand, const int32_t* __restrict__ skey,
+                                          const int32_t* __restrict__ cell_off, const Grid* __restrict__ gridp,
+                                          int n, double rc2, double rcs, double hinv2, int tile, int slice,
+                                          int nslices, Fin&& fin)
 {
   constexpr int NACC = grad_nacc<SHEAR>();
   const Grid g0 = *gridp;
-  const int lane = threadIdx.x;
-  const int unit = xcd_tile(blockIdx.x, gridDim.x);
-  const int tile = unit / nslices;
-  const int slice = unit - tile * nslices;
+  const int lane = threadIdx.x & (kWave - 1);
   const int k = tile * kTile + lane;
   const bool act = k < n;
   const int kk = act ? k : 0;
@@ -79,9 +73,61 @@ __global__ __launch_bounds__(kWave) void k_grad_slice(const double* __restrict__
       });
     });
   });
+  fin(acc);
+}
 
-  double* p = part + (size_t)slice * NACC * npad + k;
-  for (int q = 0; q < NACC; ++q) p[(size_t)q * npad] = acc[q];
+// Pair part: one single-wave workgroup per (tile, slice) unit walks its
+// slice of the tile's candidates and stores the slice's neighbour-side sums;
+// k_grad_sum adds the slices in slice order and applies the owner terms.
+template <bool SHEAR, bool BULK>
+__global__ __launch_bounds__(kWave) void k_grad_slice(const double* __restrict__ cand,
+                                                      const int32_t* __restrict__ skey,
+                                                      const int32_t* __restrict__ cell_off,
+                                                      const Grid* __restrict__ gridp, int n,
+                                                      double rc2, double rcs, double hinv2, int nslices,
+                                                      double* __restrict__ part, int npad)
+{
+  constexpr int NACC = grad_nacc<SHEAR>();
+  const int unit = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = unit / nslices;
+  const int slice = unit - tile * nslices;
+  grad_walk<SHEAR, BULK>(cand, skey, cell_off, gridp, n, rc2, rcs, hinv2, tile, slice, nslices,
+                         [&](const double* acc) {
+    double* p = part + (size_t)slice * NACC * npad + tile * kTile + threadIdx.x;
+    for (int q = 0; q < NACC; ++q) p[(size_t)q * npad] = acc[q];
+  });
+}
+
+// The same walk with the W slices of a tile in one workgroup of W waves
+// (wave w walks slice w): the slices' sums are added in slice order in LDS
+// and stored as the sums of a single slice, so the sum pass reads NACC
+// values per particle instead of W * NACC and no partial sum of a slice
+// goes through HBM.
+template <bool SHEAR, bool BULK, int W>
+__global__ __launch_bounds__(kWave * W) void k_grad_tile(const double* __restrict__ cand,
+                                                         const int32_t* __restrict__ skey,
+                                                         const int32_t* __restrict__ cell_off,
+                                                         const Grid* __restrict__ gridp, int n,
+                                                         double rc2, double rcs, double hinv2,
+                                                         double* __restrict__ part, int npad)
+{
+  constexpr int NACC = grad_nacc<SHEAR>();
+  __shared__ double red[W][NACC][kWave];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int slice = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);   // wave-uniform
+  const int lane = threadIdx.x & (kWave - 1);
+  grad_walk<SHEAR, BULK>(cand, skey, cell_off, gridp, n, rc2, rcs, hinv2, tile, slice, W, [&](const double* acc) {
+    #pragma unroll
+    for (int q = 0; q < NACC; ++q) red[slice][q][lane] = acc[q];
+  });
+  __syncthreads();
+  for (int t = threadIdx.x; t < NACC * kWave; t += kWave * W) {
+    const int q = t / kWave, l = t - q * kWave;
+    double s = 0.0;
+    #pragma unroll
+    for (int w = 0; w < W; ++w) s += red[w][q][l];
+    part[(size_t)q * npad + tile * kTile + l] = s;
+  }
 }
 
 // Sum pass: the slices' sums in slice order, scaled by C = kgrad / h^4,
@@ -100,14 +146,33 @@ __global__ __launch_bounds__(256) void k_grad_sum(const double* __restrict__ can
 
 int launch_grad_pairs(ccake_event* ev)
 {
-  if (ev->p.pair_precision == 1) return launch_grad_pairs32(ev);
+  if (ev->p.pair_precision == 1) { ev->grad_sum_slices = ev->nslices; return launch_grad_pairs32(ev); }
   const int n = ev->n_live;          // sorted positions of the live events' particles
   const double h = ev->p.hT;
   const double rc = 2.0 * h;
   const double hinv2 = 1.0 / (h * h);
   const int ntiles = (n + kTile - 1) / kTile;
-  const dim3 grid(ntiles * ev->nslices), blk(kWave);
   const double rcs = search_radius(ev);
+  const int W = lds_slices(ev);
+  if (W) {
+    // the tile's slices reduced in LDS: the sum pass sees one slice
+    ev->grad_sum_slices = 1;
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(ntiles), dim3(kWave * W), 0, ev->stream, ev->cand_g, ev->skey, ev->cell_off,
+                         ev->grid, n, rc * rc, rcs, hinv2, ev->part, ev->npad);
+    };
+    const bool sh = ev->p.using_shear, bu = !ev->no_bulk;
+    if (W == 8) {
+      if (sh) { if (bu) go(k_grad_tile<true, true, 8>); else go(k_grad_tile<true, false, 8>); }
+      else { if (bu) go(k_grad_tile<false, true, 8>); else go(k_grad_tile<false, false, 8>); }
+    } else {
+      if (sh) { if (bu) go(k_grad_tile<true, true, 4>); else go(k_grad_tile<true, false, 4>); }
+      else { if (bu) go(k_grad_tile<false, true, 4>); else go(k_grad_tile<false, false, 4>); }
+    }
+    return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+  }
+  ev->grad_sum_slices = ev->nslices;
+  const dim3 grid(ntiles * ev->nslices), blk(kWave);
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, grid, blk, 0, ev->stream, ev->cand_g, ev->skey, ev->cell_off, ev->grid, n, rc * rc,
                        rcs, hinv2, ev->nslices, ev->part, ev->npad);
@@ -127,7 +192,7 @@ static int grad_sum_dispatch(ccake_event* ev, double C)
     set_error("slice count out of range");
     return CCAKE_ERR_INVALID;
   } else {
-    if (ev->nslices != S) return grad_sum_dispatch<SHEAR, S + 1>(ev, C);
+    if (ev->grad_sum_slices != S) return grad_sum_dispatch<SHEAR, S + 1>(ev, C);
     const int n = ev->n_live;
     hipLaunchKernelGGL((k_grad_sum<SHEAR, S>), dim3((n + 255) / 256), dim3(256), 0, ev->stream, ev->cand_g,
                        ev->perm, n, C, ev->part, ev->npad, ev->F);

@@ -1057,6 +1057,7 @@ double smooth_norm2_event(const ccake_event* ev)
 int launch3_smooth_pairs(ccake_event* ev, int mode, bool count_pairs)
 {
   const int n = ev->n;
+  ev->smooth_sum_slices = ev->nslices;
   const double rc = 2.0 * ev->p.hT;
   const double rc2 = rc * rc;
   const double rcs = search_radius(ev);

@@ -151,20 +151,16 @@ __global__ __launch_bounds__(64) void k_walk(const double* __restrict__ cand,
 template <int MODE>
 constexpr int smooth_nacc() { return (MODE == 0 || MODE == 3) ? 6 : 2; }
 
-template <int MODE>
-__global__ __launch_bounds__(kWave) void k_smooth_slice(const double* __restrict__ cand,
-                                                        const int32_t* __restrict__ skey,
-                                                        const int32_t* __restrict__ cell_off,
-                                                        const Grid* __restrict__ gridp, int n,
-                                                        double rc2, double rcs, double hinv2, int nslices,
-                                                        double* __restrict__ part, int npad)
+// The walk of one (tile, slice) unit (one wave); fin(acc) receives the
+// slice's sums (sigma_lab, the pair count, then s, B, S, Q).
+template <int MODE, class Fin>
+__device__ __forceinline__ void smooth_walk(const double* __restrict__ cand, const int32_t* __restrict__ skey,
+                                            const int32_t* __restrict__ cell_off, const Grid* __restrict__ gridp,
+                                            int n, double rc2, double rcs, double hinv2, int tile, int slice,
+                                            int nslices, Fin&& fin)
 {
-  constexpr int NACC = smooth_nacc<MODE>();
   const Grid g0 = *gridp;
-  const int lane = threadIdx.x;
-  const int unit = xcd_tile(blockIdx.x, gridDim.x);
-  const int tile = unit / nslices;
-  const int slice = unit - tile * nslices;
+  const int lane = threadIdx.x & (kWave - 1);
   const int k = tile * kTile + lane;
   const bool act = k < n;
   const int kk = act ? k : 0;
@@ -199,13 +195,55 @@ __global__ __launch_bounds__(kWave) void k_smooth_slice(const double* __restrict
       });
     });
   });
+  const double acc[6] = { sl, (double)cnt, ss, sB, sS, sQ };
+  fin(acc);
+}
 
-  double* p = part + (size_t)slice * NACC * npad + k;
-  p[0] = sl;
-  p[(size_t)npad] = (double)cnt;
-  if (MODE == 0 || MODE == 3) {
-    p[(size_t)2 * npad] = ss; p[(size_t)3 * npad] = sB;
-    p[(size_t)4 * npad] = sS; p[(size_t)5 * npad] = sQ;
+template <int MODE>
+__global__ __launch_bounds__(kWave) void k_smooth_slice(const double* __restrict__ cand,
+                                                        const int32_t* __restrict__ skey,
+                                                        const int32_t* __restrict__ cell_off,
+                                                        const Grid* __restrict__ gridp, int n,
+                                                        double rc2, double rcs, double hinv2, int nslices,
+                                                        double* __restrict__ part, int npad)
+{
+  constexpr int NACC = smooth_nacc<MODE>();
+  const int unit = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = unit / nslices;
+  const int slice = unit - tile * nslices;
+  smooth_walk<MODE>(cand, skey, cell_off, gridp, n, rc2, rcs, hinv2, tile, slice, nslices, [&](const double* acc) {
+    double* p = part + (size_t)slice * NACC * npad + tile * kTile + threadIdx.x;
+    #pragma unroll
+    for (int q = 0; q < NACC; ++q) p[(size_t)q * npad] = acc[q];
+  });
+}
+
+// The W slices of a tile in one workgroup of W waves, their sums added in
+// slice order in LDS and stored as one slice's (as k_grad_tile).
+template <int MODE, int W>
+__global__ __launch_bounds__(kWave * W) void k_smooth_tile(const double* __restrict__ cand,
+                                                           const int32_t* __restrict__ skey,
+                                                           const int32_t* __restrict__ cell_off,
+                                                           const Grid* __restrict__ gridp, int n,
+                                                           double rc2, double rcs, double hinv2,
+                                                           double* __restrict__ part, int npad)
+{
+  constexpr int NACC = smooth_nacc<MODE>();
+  __shared__ double red[W][NACC][kWave];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int slice = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);   // wave-uniform
+  const int lane = threadIdx.x & (kWave - 1);
+  smooth_walk<MODE>(cand, skey, cell_off, gridp, n, rc2, rcs, hinv2, tile, slice, W, [&](const double* acc) {
+    #pragma unroll
+    for (int q = 0; q < NACC; ++q) red[slice][q][lane] = acc[q];
+  });
+  __syncthreads();
+  for (int t = threadIdx.x; t < NACC * kWave; t += kWave * W) {
+    const int q = t / kWave, l = t - q * kWave;
+    double s = 0.0;
+    #pragma unroll
+    for (int w = 0; w < W; ++w) s += red[w][q][l];
+    part[(size_t)q * npad + tile * kTile + l] = s;
   }
 }
 
@@ -308,9 +346,23 @@ int launch_smooth_pairs(ccake_event* ev, int mode, bool count_pairs)
                        ev->grid, n, rc2, rcs, bt, pc, nullptr, nullptr, nullptr);
     return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
   }
-  if (mode == 0 && ev->p.pair_precision == 1) return launch_smooth_pairs32(ev);
-  const dim3 grid(ntiles * ev->nslices), blk(kWave);
+  if (mode == 0 && ev->p.pair_precision == 1) { ev->smooth_sum_slices = ev->nslices; return launch_smooth_pairs32(ev); }
   const double hinv2 = 1.0 / (ev->p.hT * ev->p.hT);
+  const int W = lds_slices(ev);
+  if (W) {
+    // the tile's slices reduced in LDS: the sum pass sees one slice
+    ev->smooth_sum_slices = 1;
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(ntiles), dim3(kWave * W), 0, st, ev->cand_s, ev->skey, ev->cell_off, ev->grid, n,
+                         rc2, rcs, hinv2, ev->part, ev->npad);
+    };
+    const int m = (mode == 0 && ev->no_charges) ? 3 : mode;
+    if (W == 8) { if (m == 3) go(k_smooth_tile<3, 8>); else if (m == 0) go(k_smooth_tile<0, 8>); else go(k_smooth_tile<1, 8>); }
+    else { if (m == 3) go(k_smooth_tile<3, 4>); else if (m == 0) go(k_smooth_tile<0, 4>); else go(k_smooth_tile<1, 4>); }
+    return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
+  }
+  ev->smooth_sum_slices = ev->nslices;
+  const dim3 grid(ntiles * ev->nslices), blk(kWave);
   if (mode == 0 && ev->no_charges)
     hipLaunchKernelGGL(k_smooth_slice<3>, grid, blk, 0, st, ev->cand_s, ev->skey, ev->cell_off, ev->grid, n, rc2,
                        rcs, hinv2, ev->nslices, ev->part, ev->npad);
@@ -339,7 +391,7 @@ static int smooth_sum_dispatch(ccake_event* ev, double norm2, int32_t* bt, unsig
     set_error("slice count out of range");
     return CCAKE_ERR_INVALID;
   } else {
-    if (ev->nslices == S) { smooth_sum_s<MODE, S>(ev, norm2, bt, pc); return 0; }
+    if (ev->smooth_sum_slices == S) { smooth_sum_s<MODE, S>(ev, norm2, bt, pc); return 0; }
     return smooth_sum_dispatch<MODE, S + 1>(ev, norm2, bt, pc);
   }
 }
