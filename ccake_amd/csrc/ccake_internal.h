@@ -212,7 +212,7 @@ struct ccake_event {
   // pair kernel added a tile's slices in LDS (set by each pair launch).
   int smooth_sum_slices = 1;
   int grad_sum_slices = 1;
-  bool lds_reduce = true;       // 2+1D FP64 pair kernels reduce a tile's slices in LDS
+  bool lds_reduce = true;       // 2+1D pair kernels reduce a tile's slices in LDS
   int64_t last_pairs = 0;
   size_t bytes = 0;
 };
@@ -256,12 +256,12 @@ inline double search_radius(const ccake_event* ev)
 }
 inline int32_t* icol(ccake_event* ev, int f) { return ev->I + (size_t)f * ev->npad; }
 
-// Waves per workgroup of the 2+1D FP64 pair kernels that add a tile's
-// slices in LDS (one wave per slice), or 0 for one single-wave workgroup
-// per (tile, slice) with the slices' sums through HBM.
+// Waves per workgroup of the 2+1D pair kernels that add a tile's slices in
+// LDS (one wave per slice), or 0 for one single-wave workgroup per (tile,
+// slice) with the slices' sums through HBM.
 inline int lds_slices(const ccake_event* ev)
 {
-  if (ev->p.dim != 2 || ev->p.pair_precision != 0 || !ev->lds_reduce) return 0;
+  if (ev->p.dim != 2 || !ev->lds_reduce) return 0;
   return ev->nslices >= 8 ? 8 : (ev->nslices >= 4 ? 4 : 0);
 }
 

@@ -146,7 +146,7 @@ __global__ __launch_bounds__(256) void k_grad_sum(const double* __restrict__ can
 
 int launch_grad_pairs(ccake_event* ev)
 {
-  if (ev->p.pair_precision == 1) { ev->grad_sum_slices = ev->nslices; return launch_grad_pairs32(ev); }
+  if (ev->p.pair_precision == 1) return launch_grad_pairs32(ev);
   const int n = ev->n_live;          // sorted positions of the live events' particles
   const double h = ev->p.hT;
   const double rc = 2.0 * h;

@@ -137,8 +137,11 @@ __device__ __forceinline__ float spline_g32(float q, float iq)
 // Smoothing pairs, MODE 0 (charges) or 3 (no charges: the charge sums are
 // the exact zeros the full walk gives).  Same partial-sum layout as
 // k_smooth_slice: sigma_lab, neighbour count (FP64 predicate), s, B, S, Q.
-template <int MODE, int U, int FOLD>
-__global__ __launch_bounds__(kWave) void k_smooth_slice32(const double* __restrict__ cand64,
+// W = 0: one single-wave workgroup per (tile, slice) unit, each slice's sums
+// stored; W > 0: the W slices of a tile in one workgroup, added in slice
+// order in LDS and stored as one slice's (as k_smooth_tile).
+template <int MODE, int U, int FOLD, int W>
+__global__ __launch_bounds__(kWave * (W > 0 ? W : 1)) void k_smooth_slice32(const double* __restrict__ cand64,
                                                           const uint32_t* __restrict__ cand,
                                                           const int32_t* __restrict__ skey,
                                                           const int32_t* __restrict__ cell_off,
@@ -149,10 +152,17 @@ __global__ __launch_bounds__(kWave) void k_smooth_slice32(const double* __restri
   static_assert(MODE == 0 || MODE == 3, "time-step smoothing modes");
   constexpr int NACC = 6;
   const Grid g0 = *gridp;
-  const int lane = threadIdx.x;
-  const int unit = xcd_tile(blockIdx.x, gridDim.x);
-  const int tile = unit / nslices;
-  const int slice = unit - tile * nslices;
+  const int lane = threadIdx.x & (kWave - 1);
+  int tile, slice;
+  if constexpr (W == 0) {
+    const int unit = xcd_tile(blockIdx.x, gridDim.x);
+    tile = unit / nslices;
+    slice = unit - tile * nslices;
+  } else {
+    tile = xcd_tile(blockIdx.x, gridDim.x);
+    slice = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);   // wave-uniform
+    nslices = W;
+  }
   const int k = tile * kTile + lane;
   const bool act = k < n;
   const int kk = act ? k : 0;
@@ -189,19 +199,32 @@ __global__ __launch_bounds__(kWave) void k_smooth_slice32(const double* __restri
     });
   });
 
-  double* p = part + (size_t)slice * NACC * npad + k;
-  p[0] = sl.value();
-  p[(size_t)npad] = (double)cnt;
-  p[(size_t)2 * npad] = ss.value();
-  p[(size_t)3 * npad] = MODE == 0 ? sB.value() : 0.0;
-  p[(size_t)4 * npad] = MODE == 0 ? sS.value() : 0.0;
-  p[(size_t)5 * npad] = MODE == 0 ? sQ.value() : 0.0;
+  const double v[NACC] = { sl.value(), (double)cnt, ss.value(), MODE == 0 ? sB.value() : 0.0,
+                           MODE == 0 ? sS.value() : 0.0, MODE == 0 ? sQ.value() : 0.0 };
+  if constexpr (W == 0) {
+    double* p = part + (size_t)slice * NACC * npad + k;
+    #pragma unroll
+    for (int q = 0; q < NACC; ++q) p[(size_t)q * npad] = v[q];
+  } else {
+    __shared__ double red[W > 0 ? W : 1][NACC][kWave];
+    #pragma unroll
+    for (int q = 0; q < NACC; ++q) red[slice][q][lane] = v[q];
+    __syncthreads();
+    for (int t = threadIdx.x; t < NACC * kWave; t += kWave * W) {
+      const int q = t / kWave, l = t - q * kWave;
+      double s = 0.0;
+      #pragma unroll
+      for (int w = 0; w < W; ++w) s += red[w][q][l];
+      part[(size_t)q * npad + tile * kTile + l] = s;
+    }
+  }
 }
 
 // Gradient pairs: the 17 (12 without shear) neighbour-side sums of
-// w = m_b grad W_ab / C of k_grad_slice, same partial-sum layout.
-template <bool SHEAR, bool BULK, int FOLD>
-__global__ __launch_bounds__(kWave) void k_grad_slice32(const double* __restrict__ cand64,
+// w = m_b grad W_ab / C of k_grad_slice, same partial-sum layout; W as in
+// k_smooth_slice32.
+template <bool SHEAR, bool BULK, int FOLD, int W>
+__global__ __launch_bounds__(kWave * (W > 0 ? W : 1)) void k_grad_slice32(const double* __restrict__ cand64,
                                                         const uint32_t* __restrict__ cand,
                                                         const int32_t* __restrict__ skey,
                                                         const int32_t* __restrict__ cell_off,
@@ -211,10 +234,17 @@ __global__ __launch_bounds__(kWave) void k_grad_slice32(const double* __restrict
 {
   constexpr int NACC = grad_nacc<SHEAR>();
   const Grid g0 = *gridp;
-  const int lane = threadIdx.x;
-  const int unit = xcd_tile(blockIdx.x, gridDim.x);
-  const int tile = unit / nslices;
-  const int slice = unit - tile * nslices;
+  const int lane = threadIdx.x & (kWave - 1);
+  int tile, slice;
+  if constexpr (W == 0) {
+    const int unit = xcd_tile(blockIdx.x, gridDim.x);
+    tile = unit / nslices;
+    slice = unit - tile * nslices;
+  } else {
+    tile = xcd_tile(blockIdx.x, gridDim.x);
+    slice = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);   // wave-uniform
+    nslices = W;
+  }
   const int k = tile * kTile + lane;
   const bool act = k < n;
   const int kk = act ? k : 0;
@@ -261,9 +291,23 @@ __global__ __launch_bounds__(kWave) void k_grad_slice32(const double* __restrict
     });
   });
 
-  double* p = part + (size_t)slice * NACC * npad + k;
-  #pragma unroll
-  for (int q = 0; q < NACC; ++q) p[(size_t)q * npad] = acc[q].value();
+  if constexpr (W == 0) {
+    double* p = part + (size_t)slice * NACC * npad + k;
+    #pragma unroll
+    for (int q = 0; q < NACC; ++q) p[(size_t)q * npad] = acc[q].value();
+  } else {
+    __shared__ double red[W > 0 ? W : 1][NACC][kWave];
+    #pragma unroll
+    for (int q = 0; q < NACC; ++q) red[slice][q][lane] = acc[q].value();
+    __syncthreads();
+    for (int t = threadIdx.x; t < NACC * kWave; t += kWave * W) {
+      const int q = t / kWave, l = t - q * kWave;
+      double s = 0.0;
+      #pragma unroll
+      for (int w = 0; w < W; ++w) s += red[w][q][l];
+      part[(size_t)q * npad + tile * kTile + l] = s;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -287,17 +331,23 @@ static int mp_smooth_u()
   return v;
 }
 
+// The tile-workgroup form (slices added in LDS) runs with the default fold
+// and records per step; the tuning overrides use one workgroup per slice.
 template <int MODE, int U>
-static void smooth32_fold(ccake_event* ev, dim3 grid, double rc2, double rcs, float hinv2)
+static void smooth32_fold(ccake_event* ev, int ntiles, double rc2, double rcs, float hinv2)
 {
-  auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, grid, dim3(kWave), 0, ev->stream, ev->cand_s, ev->cand_s32, ev->skey, ev->cell_off,
-                       ev->grid, ev->n_live, rc2, rcs, hinv2, ev->nslices, ev->part, ev->npad);
-  };
   const int f = mp_fold();
-  if (f == 64) go(k_smooth_slice32<MODE, U, 64>);
-  else if (f == 32) go(k_smooth_slice32<MODE, U, 32>);
-  else go(k_smooth_slice32<MODE, U, 16>);
+  const int W = (f == kFoldEvery && U == 2) ? lds_slices(ev) : 0;
+  ev->smooth_sum_slices = W ? 1 : ev->nslices;
+  auto go = [&](auto kern, int wg, int waves) {
+    hipLaunchKernelGGL(kern, dim3(wg), dim3(kWave * waves), 0, ev->stream, ev->cand_s, ev->cand_s32, ev->skey,
+                       ev->cell_off, ev->grid, ev->n_live, rc2, rcs, hinv2, ev->nslices, ev->part, ev->npad);
+  };
+  if (W == 8) go(k_smooth_slice32<MODE, U, kFoldEvery, 8>, ntiles, 8);
+  else if (W == 4) go(k_smooth_slice32<MODE, U, kFoldEvery, 4>, ntiles, 4);
+  else if (f == 64) go(k_smooth_slice32<MODE, U, 64, 0>, ntiles * ev->nslices, 1);
+  else if (f == 32) go(k_smooth_slice32<MODE, U, 32, 0>, ntiles * ev->nslices, 1);
+  else go(k_smooth_slice32<MODE, U, 16, 0>, ntiles * ev->nslices, 1);
 }
 
 int launch_smooth_pairs32(ccake_event* ev)
@@ -308,28 +358,31 @@ int launch_smooth_pairs32(ccake_event* ev)
   const int ntiles = (n + kTile - 1) / kTile;
   hipStream_t st = ev->stream;
   hipLaunchKernelGGL(k_records32<0>, dim3((n + 255) / 256), dim3(256), 0, st, ev->cand_s, n, ev->cand_s32);
-  const dim3 grid(ntiles * ev->nslices);
   const float hinv2 = (float)(1.0 / (ev->p.hT * ev->p.hT));
   const bool four = mp_smooth_u() == 4;
   if (ev->no_charges) {
-    if (four) smooth32_fold<3, 4>(ev, grid, rc * rc, rcs, hinv2); else smooth32_fold<3, 2>(ev, grid, rc * rc, rcs, hinv2);
+    if (four) smooth32_fold<3, 4>(ev, ntiles, rc * rc, rcs, hinv2); else smooth32_fold<3, 2>(ev, ntiles, rc * rc, rcs, hinv2);
   } else {
-    if (four) smooth32_fold<0, 4>(ev, grid, rc * rc, rcs, hinv2); else smooth32_fold<0, 2>(ev, grid, rc * rc, rcs, hinv2);
+    if (four) smooth32_fold<0, 4>(ev, ntiles, rc * rc, rcs, hinv2); else smooth32_fold<0, 2>(ev, ntiles, rc * rc, rcs, hinv2);
   }
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
 
 template <bool SHEAR, bool BULK>
-static void grad32_fold(ccake_event* ev, dim3 grid, double rc2, double rcs, float hinv2)
+static void grad32_fold(ccake_event* ev, int ntiles, double rc2, double rcs, float hinv2)
 {
-  auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, grid, dim3(kWave), 0, ev->stream, ev->cand_g, ev->cand_g32, ev->skey, ev->cell_off,
-                       ev->grid, ev->n_live, rc2, rcs, hinv2, ev->nslices, ev->part, ev->npad);
-  };
   const int f = mp_fold();
-  if (f == 64) go(k_grad_slice32<SHEAR, BULK, 64>);
-  else if (f == 32) go(k_grad_slice32<SHEAR, BULK, 32>);
-  else go(k_grad_slice32<SHEAR, BULK, 16>);
+  const int W = f == kFoldEvery ? lds_slices(ev) : 0;
+  ev->grad_sum_slices = W ? 1 : ev->nslices;
+  auto go = [&](auto kern, int wg, int waves) {
+    hipLaunchKernelGGL(kern, dim3(wg), dim3(kWave * waves), 0, ev->stream, ev->cand_g, ev->cand_g32, ev->skey,
+                       ev->cell_off, ev->grid, ev->n_live, rc2, rcs, hinv2, ev->nslices, ev->part, ev->npad);
+  };
+  if (W == 8) go(k_grad_slice32<SHEAR, BULK, kFoldEvery, 8>, ntiles, 8);
+  else if (W == 4) go(k_grad_slice32<SHEAR, BULK, kFoldEvery, 4>, ntiles, 4);
+  else if (f == 64) go(k_grad_slice32<SHEAR, BULK, 64, 0>, ntiles * ev->nslices, 1);
+  else if (f == 32) go(k_grad_slice32<SHEAR, BULK, 32, 0>, ntiles * ev->nslices, 1);
+  else go(k_grad_slice32<SHEAR, BULK, 16, 0>, ntiles * ev->nslices, 1);
 }
 
 int launch_grad_pairs32(ccake_event* ev)
@@ -340,14 +393,13 @@ int launch_grad_pairs32(ccake_event* ev)
   const int ntiles = (n + kTile - 1) / kTile;
   hipStream_t st = ev->stream;
   hipLaunchKernelGGL(k_records32<1>, dim3((n + 255) / 256), dim3(256), 0, st, ev->cand_g, n, ev->cand_g32);
-  const dim3 grid(ntiles * ev->nslices);
   const float hinv2 = (float)(1.0 / (ev->p.hT * ev->p.hT));
   if (ev->p.using_shear) {
-    if (ev->no_bulk) grad32_fold<true, false>(ev, grid, rc * rc, rcs, hinv2);
-    else grad32_fold<true, true>(ev, grid, rc * rc, rcs, hinv2);
+    if (ev->no_bulk) grad32_fold<true, false>(ev, ntiles, rc * rc, rcs, hinv2);
+    else grad32_fold<true, true>(ev, ntiles, rc * rc, rcs, hinv2);
   } else {
-    if (ev->no_bulk) grad32_fold<false, false>(ev, grid, rc * rc, rcs, hinv2);
-    else grad32_fold<false, true>(ev, grid, rc * rc, rcs, hinv2);
+    if (ev->no_bulk) grad32_fold<false, false>(ev, ntiles, rc * rc, rcs, hinv2);
+    else grad32_fold<false, true>(ev, ntiles, rc * rc, rcs, hinv2);
   }
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }

@@ -346,7 +346,7 @@ int launch_smooth_pairs(ccake_event* ev, int mode, bool count_pairs)
                        ev->grid, n, rc2, rcs, bt, pc, nullptr, nullptr, nullptr);
     return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
   }
-  if (mode == 0 && ev->p.pair_precision == 1) { ev->smooth_sum_slices = ev->nslices; return launch_smooth_pairs32(ev); }
+  if (mode == 0 && ev->p.pair_precision == 1) return launch_smooth_pairs32(ev);
   const double hinv2 = 1.0 / (ev->p.hT * ev->p.hT);
   const int W = lds_slices(ev);
   if (W) {
