@@ -574,10 +574,12 @@ def milne_run(dev, eos, steps=20, warmup=2):
     return out
 
 
-def milne_stream(dev, eos, k=8, rate=50.0, slots=2, seed=0x3D5):
-    """Served 3+1D stream: k Milne lumpy events with lognormal sizes (the
-    transverse half-width L ~ LogNormal(ln 3 fm, 0.35) in [2, 5] fm, blob
-    centres N(0, L/3)), arriving as a Poisson stream at `rate` events/s,
+def milne_stream(dev, eos, k=6, rate=50.0, slots=2, seed=0x3D5):
+    """Served 3+1D stream: k Milne lumpy events with lognormal particle
+    counts N in [100k, 2M] (median 450k; the transverse half-width
+    L = 3 fm sqrt(N / 126k), as the 126k-particle event has L = 3 fm; blob
+    centres N(0, L/3), 20 (L / 3 fm)^2 blobs, so that the blobs' density and
+    the particle count per area stay those of that event), arriving as a Poisson stream at `rate` events/s,
     each run to complete freeze-out; served `slots` events at a time (one
     stream each, admitted as slots free up) against one at a time.  The
     served results must equal the sequential ones bit for bit."""
@@ -588,16 +590,19 @@ def milne_stream(dev, eos, k=8, rate=50.0, slots=2, seed=0x3D5):
     arr = serve.poisson_arrivals(k, rate, seed=seed)
     reqs = []
     for rid in range(k):
-        hw = float(np.clip(np.exp(rng.normal(np.log(3.0), 0.35)), 2.0, 5.0))
-        hw = round(hw * 10) / 10
+        npart = 0.0
+        while not 100e3 <= npart <= 2e6:
+            npart = float(np.exp(rng.normal(np.log(450e3), 0.6)))
+        hw = round(3.0 * np.sqrt(npart / 126e3) * 10) / 10
         d = ic.milne_lumpy(seed=0xCCA4E001 + rid, spacing=0.1, half_width=hw, eta_spacing=0.1, eta_max=4.0,
-                           center_sd=hw / 3.0, e_cutoff_gev=0.15)
-        cfg = config.lumpy_params(dim=3, hEta=0.3, cells_per_rc=2, e_freeze=eos.efreeze(T_FO))
+                           center_sd=hw / 3.0, n_blobs=int(round(20 * (hw / 3.0) ** 2)), e_cutoff_gev=0.15)
+        cfg = config.lumpy_params(dim=3, hEta=0.3, cells_per_rc=2, x_subcells=6, e_freeze=eos.efreeze(T_FO))
         reqs.append(serve.Request(rid=rid, ic=d, params=cfg, arrival=float(arr[rid]), max_tau=30.0,
                                   fields=("CCAKE_F_T", "CCAKE_F_E", "CCAKE_F_UX", "CCAKE_F_UY", "CCAKE_F_UETA")))
     n = [len(r.ic["x"]) for r in reqs]
     out = dict(workload=f"{k} 3+1D Milne lumpy events (0.1 fm and 0.1 in eta_s, eta_s in [-4, 4], lognormal "
-                        f"transverse sizes), Poisson arrivals at {rate:g} events/s, each run to freeze-out",
+                        f"particle counts in [100k, 2M], median 450k), Poisson arrivals at {rate:g} events/s, each "
+                        "run to freeze-out",
                particles_min=min(n), particles_max=max(n), particles_median=int(np.median(n)),
                particles_total=int(sum(n)))
     runs = {}

@@ -368,11 +368,24 @@ int launch_smooth_pairs32(ccake_event* ev)
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }
 
+// Waves per tile workgroup of the FP32 gradient walk (at most 4 unless
+// CCAKE_MP_GRAD_WAVES says otherwise: its registers allow 3 waves per SIMD,
+// so a workgroup of 8 would leave one workgroup per CU).
+static int mp_grad_waves(const ccake_event* ev)
+{
+  static const int v = [] {
+    const char* e = std::getenv("CCAKE_MP_GRAD_WAVES");
+    return e ? std::atoi(e) : 4;
+  }();
+  const int w = lds_slices(ev);
+  return v <= 0 ? 0 : (w < v ? w : (v >= 8 ? 8 : 4));
+}
+
 template <bool SHEAR, bool BULK>
 static void grad32_fold(ccake_event* ev, int ntiles, double rc2, double rcs, float hinv2)
 {
   const int f = mp_fold();
-  const int W = f == kFoldEvery ? lds_slices(ev) : 0;
+  const int W = f == kFoldEvery ? mp_grad_waves(ev) : 0;
   ev->grad_sum_slices = W ? 1 : ev->nslices;
   auto go = [&](auto kern, int wg, int waves) {
     hipLaunchKernelGGL(kern, dim3(wg), dim3(kWave * waves), 0, ev->stream, ev->cand_g, ev->cand_g32, ev->skey,
