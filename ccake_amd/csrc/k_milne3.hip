@@ -435,7 +435,11 @@ template <bool SHEAR>
 constexpr int grad3_nacc() { return SHEAR ? 28 : 21; }
 
 template <bool SHEAR, bool FACT>
-__global__ __launch_bounds__(kWave) void k3_grad_slice(const double* __restrict__ cand,
+#ifndef CCAKE_GRAD3_WAVES_PER_SIMD
+#define CCAKE_GRAD3_WAVES_PER_SIMD 1
+#endif
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(CCAKE_GRAD3_WAVES_PER_SIMD)))
+void k3_grad_slice(const double* __restrict__ cand,
                                                        const int32_t* __restrict__ skey,
                                                        const int32_t* __restrict__ cell_off,
                                                        const Grid* __restrict__ gridp, int n, double rc2, double rcs,
