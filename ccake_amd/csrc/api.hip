@@ -753,6 +753,11 @@ static int event_create(ccake_dev* dev, const ccake_params* p, ccake_eos* eos, i
   // partial sums: 17 per slice in 2+1D, 28 in 3+1D (the 3D gradient sums)
   rc |= dalloc(ev, &ev->part, (size_t)ev->nslices_cap * (p->dim == 3 ? 28 : kMaxAcc) * np);
   rc |= dalloc(ev, &ev->hard, np);
+  {
+    // the EoS stage in two kernels (table chains): CCAKE_EOS_SPLIT=0 keeps one
+    const char* es = std::getenv("CCAKE_EOS_SPLIT");
+    if (!(es && std::atoi(es) == 0)) rc |= dalloc(ev, &ev->eos_done, np);
+  }
   rc |= dalloc(ev, &ev->hard_count, 2);
   rc |= dalloc(ev, &ev->cand_s, 8 * np);
   rc |= dalloc(ev, &ev->cand_g, (size_t)(p->dim == 3 ? kCandG3 : 16) * np);
@@ -876,7 +881,7 @@ int ccake_event_destroy(ccake_event* ev)
   void* bufs[] = { ev->F, ev->I, ev->grid, ev->bbox, ev->key, ev->key_a, ev->key_b, ev->val_a, ev->val_b, ev->hist,
                    ev->scan_tmp, ev->cell_off, ev->pair_count, ev->part, ev->hard, ev->hard_count, ev->k4, ev->cand_s, ev->cand_g, ev->red,
                    ev->err, ev->snaps, ev->fo_rows, ev->fo_count, ev->cand_d, ev->cand_s32, ev->cand_g32,
-                   ev->ev_off, ev->eid, ev->red_ev, ev->live };
+                   ev->ev_off, ev->eid, ev->red_ev, ev->live, ev->eos_done };
   for (void* b : bufs) if (b) (void)hipFree(b);
   if (ev->stream) (void)hipStreamDestroy(ev->stream);
   {

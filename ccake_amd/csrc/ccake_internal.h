@@ -213,6 +213,7 @@ struct ccake_event {
   int smooth_sum_slices = 1;
   int grad_sum_slices = 1;
   bool lds_reduce = true;       // 2+1D pair kernels reduce a tile's slices in LDS
+  int32_t* eos_done = nullptr;  // per sorted position: the EoS stage's first attempt decided it (split stage)
   int64_t last_pairs = 0;
   size_t bytes = 0;
 };

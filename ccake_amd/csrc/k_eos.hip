@@ -205,12 +205,19 @@ struct HardList {
 #ifndef CCAKE_EOS_WAVES_PER_SIMD
 #define CCAKE_EOS_WAVES_PER_SIMD 1
 #endif
-template <int MODE, int KM>
+// PART 0: the whole first attempt and tail in one kernel.  PART 1 and 2
+// split it where the live registers are fewest: PART 1 runs the smoothing
+// sum and the Newton attempt and stores the root (T, mu) and member with a
+// per-position flag `done`; PART 2 (after k_eos_hard) evaluates the
+// thermodynamics and the tail for the flagged positions.  The values are
+// those of PART 0: the same operations on the same operands.
+template <int MODE, int KM, int PART>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CCAKE_EOS_WAVES_PER_SIMD)))
 void k_eos(EosChain ch, double* __restrict__ F, int npad, int n,
                                              int32_t* __restrict__ I, const int32_t* __restrict__ perm,
                                              double t, Transport P, double* __restrict__ cand,
-                                             int32_t* __restrict__ err, SmoothSum sm, HardList hl)
+                                             int32_t* __restrict__ err, SmoothSum sm, HardList hl,
+                                             int32_t* __restrict__ done)
 {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int kk = k < n ? k : n - 1;
@@ -221,8 +228,13 @@ void k_eos(EosChain ch, double* __restrict__ F, int npad, int n,
   // so their loads overlap with the partial sums' loads
   double x[4] = { col(CCAKE_F_T), col(CCAKE_F_MUB), col(CCAKE_F_MUQ), col(CCAKE_F_MUS) };
   int member = eos_id[i];
-  if (MODE == 3) smooth_sum_into(sm, F, npad, n, k, i, t, I + (size_t)CCAKE_I_BTRACK * npad);
-  if (k >= n) return;
+  if (PART == 2) {
+    if (k >= n || !done[k]) return;
+  } else {
+    if (MODE == 3) smooth_sum_into(sm, F, npad, n, k, i, t, I + (size_t)CCAKE_I_BTRACK * npad);
+    if (k >= n) return;
+  }
+  if (PART != 2) {
   double tgt[4];
   if (MODE == 1) {
     tgt[0] = col(CCAKE_F_IN_E); tgt[1] = col(CCAKE_F_IN_B); tgt[2] = col(CCAKE_F_IN_SS); tgt[3] = col(CCAKE_F_IN_Q);
@@ -231,7 +243,15 @@ void k_eos(EosChain ch, double* __restrict__ F, int npad, int n,
   }
   if (invert_first<KM>(ch, ch.tab, ch.tab4, MODE == 1 ? 1 : 0, tgt, x, &member, kFastBudget) != 1) {
     hl.idx[atomicAdd(hl.count, 1)] = k;      // the whole inversion in k_eos_hard
+    if (PART == 1) done[k] = 0;
     return;
+  }
+  if (PART == 1) {
+    col(CCAKE_F_T) = x[0]; col(CCAKE_F_MUB) = x[1]; col(CCAKE_F_MUQ) = x[2]; col(CCAKE_F_MUS) = x[3];
+    eos_id[i] = member;
+    done[k] = 1;
+    return;
+  }
   }
   double th[31];
   set_thermo<KM>(ch, ch.tab, member, x, th);
@@ -396,22 +416,32 @@ int launch_eos(ccake_event* ev, int mode, int fuse_post)
   hl.count = ev->hard_count + ev->hard_parity;
   hl.next = ev->hard_count + (1 - ev->hard_parity);
   ev->hard_parity = 1 - ev->hard_parity;
-  auto go = [&](auto kern, auto hard) {
+  auto launch = [&](auto kern, int32_t* done) {
     hipLaunchKernelGGL(kern, g, b, 0, st, ev->eos->chain, ev->F, ev->npad, n, ev->I, ev->perm, ev->t, P,
-                       ev->cand_g, ev->err, sm, hl);
-    hipLaunchKernelGGL(hard, dim3(kHardWaves), dim3(kHardLanes), 0, st, ev->eos->chain, ev->F, ev->npad, n, ev->I,
+                       ev->cand_g, ev->err, sm, hl, done);
+  };
+  auto hard = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(kHardWaves), dim3(kHardLanes), 0, st, ev->eos->chain, ev->F, ev->npad, n, ev->I,
                        ev->perm, ev->t, P, ev->cand_g, ev->err, hl);
   };
+  // The time-step modes with the table chain run split (eos_split): the
+  // Newton part, the hard list, then the thermodynamics and tail.
+  const bool split = ev->eos_done && (md == 0 || md == 3);
   if (chain_kinds(ev->eos->chain) == kConformalChain) {
-    if (md == 1) go(k_eos<1, kConformalChain>, k_eos_hard<1, kConformalChain>);
-    else if (md == 0) go(k_eos<0, kConformalChain>, k_eos_hard<0, kConformalChain>);
-    else if (md == 3) go(k_eos<3, kConformalChain>, k_eos_hard<0, kConformalChain>);
-    else go(k_eos<2, kConformalChain>, k_eos_hard<2, kConformalChain>);
+    if (md == 1) { launch(k_eos<1, kConformalChain, 0>, nullptr); hard(k_eos_hard<1, kConformalChain>); }
+    else if (md == 0) { launch(k_eos<0, kConformalChain, 0>, nullptr); hard(k_eos_hard<0, kConformalChain>); }
+    else if (md == 3) { launch(k_eos<3, kConformalChain, 0>, nullptr); hard(k_eos_hard<0, kConformalChain>); }
+    else { launch(k_eos<2, kConformalChain, 0>, nullptr); hard(k_eos_hard<2, kConformalChain>); }
+  } else if (split) {
+    if (md == 0) launch(k_eos<0, kAllMembers, 1>, ev->eos_done);
+    else launch(k_eos<3, kAllMembers, 1>, ev->eos_done);
+    hard(k_eos_hard<0, kAllMembers>);
+    launch(k_eos<0, kAllMembers, 2>, ev->eos_done);
   } else {
-    if (md == 1) go(k_eos<1, kAllMembers>, k_eos_hard<1, kAllMembers>);
-    else if (md == 0) go(k_eos<0, kAllMembers>, k_eos_hard<0, kAllMembers>);
-    else if (md == 3) go(k_eos<3, kAllMembers>, k_eos_hard<0, kAllMembers>);
-    else go(k_eos<2, kAllMembers>, k_eos_hard<2, kAllMembers>);
+    if (md == 1) { launch(k_eos<1, kAllMembers, 0>, nullptr); hard(k_eos_hard<1, kAllMembers>); }
+    else if (md == 0) { launch(k_eos<0, kAllMembers, 0>, nullptr); hard(k_eos_hard<0, kAllMembers>); }
+    else if (md == 3) { launch(k_eos<3, kAllMembers, 0>, nullptr); hard(k_eos_hard<0, kAllMembers>); }
+    else { launch(k_eos<2, kAllMembers, 0>, nullptr); hard(k_eos_hard<2, kAllMembers>); }
   }
   return hipGetLastError() == hipSuccess ? 0 : CCAKE_ERR_HIP;
 }

@@ -434,10 +434,13 @@ __global__ __launch_bounds__(64) void k3_walk(const double* __restrict__ cand, c
 template <bool SHEAR>
 constexpr int grad3_nacc() { return SHEAR ? 28 : 21; }
 
-template <bool SHEAR, bool FACT>
+// Register budget of the gradient walk: 4 waves per SIMD (128 VGPRs, a few
+// values spilled to scratch) measured 1.115 ms per launch on the 126k event
+// against 1.131 at 3 and 1.402 at the compiler's choice (183 VGPRs, 2 waves).
 #ifndef CCAKE_GRAD3_WAVES_PER_SIMD
-#define CCAKE_GRAD3_WAVES_PER_SIMD 1
+#define CCAKE_GRAD3_WAVES_PER_SIMD 4
 #endif
+template <bool SHEAR, bool FACT>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(CCAKE_GRAD3_WAVES_PER_SIMD)))
 void k3_grad_slice(const double* __restrict__ cand,
                                                        const int32_t* __restrict__ skey,
