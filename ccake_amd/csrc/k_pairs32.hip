@@ -223,8 +223,15 @@ __global__ __launch_bounds__(kWave * (W > 0 ? W : 1)) void k_smooth_slice32(cons
 // Gradient pairs: the 17 (12 without shear) neighbour-side sums of
 // w = m_b grad W_ab / C of k_grad_slice, same partial-sum layout; W as in
 // k_smooth_slice32.
+// Register budget of the FP32 gradient walk: 4 waves per SIMD (128 VGPRs,
+// 11 values spilled) measured 4.36 ms per mixed step on the 200k event
+// against 4.56 at the compiler's choice (147 VGPRs, 3 waves).
+#ifndef CCAKE_GRAD32_WAVES_PER_SIMD
+#define CCAKE_GRAD32_WAVES_PER_SIMD 4
+#endif
 template <bool SHEAR, bool BULK, int FOLD, int W>
-__global__ __launch_bounds__(kWave * (W > 0 ? W : 1)) void k_grad_slice32(const double* __restrict__ cand64,
+__global__ __launch_bounds__(kWave * (W > 0 ? W : 1)) __attribute__((amdgpu_waves_per_eu(CCAKE_GRAD32_WAVES_PER_SIMD)))
+void k_grad_slice32(const double* __restrict__ cand64,
                                                         const uint32_t* __restrict__ cand,
                                                         const int32_t* __restrict__ skey,
                                                         const int32_t* __restrict__ cell_off,
@@ -369,8 +376,8 @@ int launch_smooth_pairs32(ccake_event* ev)
 }
 
 // Waves per tile workgroup of the FP32 gradient walk (at most 4 unless
-// CCAKE_MP_GRAD_WAVES says otherwise: its registers allow 3 waves per SIMD,
-// so a workgroup of 8 would leave one workgroup per CU).
+// CCAKE_MP_GRAD_WAVES says otherwise; 4 and 8 measured 4.36 and 4.40 ms per
+// step, and without the register budget 8-wave workgroups left one per CU).
 static int mp_grad_waves(const ccake_event* ev)
 {
   static const int v = [] {
