@@ -268,6 +268,7 @@ CCAKE_API int  ccake_event_download_int(ccake_event* ev, int ifield, int32_t* ho
 CCAKE_API int  ccake_build_neighbors(ccake_event* ev);
 /* Materialise the pair set (i, j), i != j, |r_i - r_j|^2 <= (2 hT)^2, rows
  * ordered by i then j ascending (parity checks only; never on the hot path).
+ * In a batch, the pairs of the live events (an ended event has none).
  * Replaces reading Cabana::NeighborList (include/utilities.h:57-62). */
 CCAKE_API int  ccake_export_pairs(ccake_event* ev, int32_t* pi, int32_t* pj, size_t cap, size_t* n_pairs);
 /* ccake_export_pairs without any allocation, host or device: no counterpart
