@@ -479,12 +479,14 @@ def batched_run(dev, eos, cfg, k, seed=0xC3):
     return out
 
 
-def served_batch_stream(dev, eos, cfg, ics, tune, plain, rate=50.0, slots=2, window=0.05):
+def served_batch_stream(dev, eos, cfg, ics, tune, plain, rate=50.0, slots=3, window=0.1):
     """The same events as a Poisson stream (rate events/s) served by the
     continuous-batching loop (serve.BatchServer: each free worker forms a
     batch of the requests that have arrived, every event returned as soon as
     it has frozen out), in FP64 and mixed precision, against the plain path's
-    events/s (each event alone, one after the other)."""
+    events/s (each event alone, one after the other).  Three workers and a
+    100 ms window measured best on the 16-event stream (windows of 50, 100,
+    200 and 400 ms with 2 and 3 workers)."""
     import numpy as np
     from ccake_amd import serve
     arr = serve.poisson_arrivals(len(ics), rate)
