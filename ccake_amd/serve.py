@@ -194,7 +194,7 @@ class BatchServer:
     event's run alone to rounding (test_serve.py)."""
 
     def __init__(self, dev, eos, params, slots=2, max_events=64, max_particles=4_000_000, max_tau=30.0,
-                 slices=None, window=0.0):
+                 slices=None, window=0.0, on_result=None):
         if slots < 1 or max_events < 1:
             raise ValueError("slots and max_events must be >= 1")
         if not params.get("particlization", 0):
@@ -203,6 +203,9 @@ class BatchServer:
         self.slots, self.max_events, self.max_particles = int(slots), int(max_events), int(max_particles)
         self.max_tau = float(max_tau)
         self.slices = slices
+        # on_result(result): called (under the server's lock) as each event's
+        # result is ready
+        self.on_result = on_result
         # a free worker waits up to `window` seconds after the first waiting
         # request's arrival for more arrivals before it forms the batch
         self.window = float(window)
@@ -248,6 +251,8 @@ class BatchServer:
         def publish(res):
             with lock:
                 results[res.rid] = res
+                if self.on_result is not None:
+                    self.on_result(res)
 
         def take():
             """The requests that have arrived, up to the batch limits; waits
