@@ -229,13 +229,15 @@ def test_pairs_bit_exact_random_and_boundary(dev, orc):
     ev.close(); eos.close()
 
 
-@pytest.mark.parametrize("cells_per_rc", [1, 2, 3])
-def test_pairs_independent_of_cell_size(dev, orc, cells_per_rc):
+@pytest.mark.parametrize("cells_per_rc,x_subcells", [(1, 1), (2, 1), (3, 1), (3, 8), (4, 3), (4, 8)])
+def test_pairs_independent_of_cell_size(dev, orc, cells_per_rc, x_subcells):
+    """The pair set does not depend on the cell size along y (key rows) or
+    on how finely the rows are cut along x."""
     d = lumpy_small(0.05)
     n = len(d["x"])
     rng = np.random.default_rng(11)
     x = d["x"] + rng.uniform(-0.01, 0.01, n); y = d["y"] + rng.uniform(-0.01, 0.01, n)
-    cfg = config.base_params(hT=0.3, cells_per_rc=cells_per_rc)
+    cfg = config.base_params(hT=0.3, cells_per_rc=cells_per_rc, x_subcells=x_subcells)
     eos = ccake_amd.Eos(dev, 0)
     ev = ccake_amd.Event(dev, eos, cfg, n)
     ev.upload("CCAKE_F_X", x); ev.upload("CCAKE_F_Y", y)
@@ -683,3 +685,43 @@ def test_gubser_full_size_lattice_on_gpu(dev):
     m = r < 2.5
     assert np.max(np.abs(e1[m] - ea[m]) / ea[m]) < 0.07
     eos.close()
+
+
+STATE_FIELDS = ["X", "Y", "UX", "UY", "SIGMA_LAB", "SIGMA", "S", "E", "T", "MUB", "RHOB", "RHOS", "RHOQ", "BULK",
+                "EXT_S", "GRADP0", "GRADE0", "GRADV0", "DUDT0", "DEXT_S"]
+
+
+def run_lumpy_table(dev, table, steps=5):
+    tab, dims, mins, maxs = table
+    eos = ccake_amd.Eos(dev, 1, T_FO, dims, mins, maxs, tab)
+    d = lumpy_small(0.05)
+    cfg = config.lumpy_params(particlization=0)
+    ev = ccake_amd.Event(dev, eos, cfg, len(d["x"]))
+    ccake_amd.load_event(ev, d)
+    ev.initialize()
+    ev.run_steps(cfg["dt"], steps)
+    out = {f: ev.download(ccake_amd.FIELDS["CCAKE_F_" + f]) for f in STATE_FIELDS}
+    ns = ev.nslices()
+    ev.close()
+    eos.close()
+    return out, ns
+
+
+def test_tile_workgroups_and_split_eos_match_the_slice_path(dev, table, monkeypatch):
+    """The tile-workgroup pair kernels (a tile's slices added in LDS) against
+    single-wave slices with their sums through HBM (CCAKE_LDS_SLICES=0), and
+    the EoS stage in two kernels against one (CCAKE_EOS_SPLIT=0), on a
+    40,401-particle lumpy table event after initialization and 5 steps: the
+    split EoS stage bit for bit (the same operations), the slice paths within
+    1e-12 of each other (whole field; only the grouping of the pair sums
+    differs)."""
+    base, ns = run_lumpy_table(dev, table)
+    assert ns >= 4            # the tile workgroups run
+    monkeypatch.setenv("CCAKE_EOS_SPLIT", "0")
+    one, _ = run_lumpy_table(dev, table)
+    monkeypatch.delenv("CCAKE_EOS_SPLIT")
+    monkeypatch.setenv("CCAKE_LDS_SLICES", "0")
+    hbm, _ = run_lumpy_table(dev, table)
+    for f in STATE_FIELDS:
+        assert np.array_equal(base[f], one[f]), f
+        assert rel(base[f], hbm[f]) <= 1e-12, (f, rel(base[f], hbm[f]))
